@@ -1,0 +1,78 @@
+"""``CtrlVar`` defaults read by the uvh assembly.
+
+Úa keeps run-time options in one MATLAB struct, ``CtrlVar``, whose defaults are set
+in ``UaMain/Ua2D_DefaultParameters.m``. This module builds the subset the uvh
+assembly reads, with the same names, nesting and default values (line numbers
+are in ``UaMain/Ua2D_DefaultParameters.m``). ``Struct`` gives MATLAB-like
+attribute access with nested sub-structs.
+"""
+from __future__ import annotations
+
+import copy
+
+
+class Struct:
+    """A minimal MATLAB-struct look-alike: attribute access, nesting, deep copy."""
+
+    def __init__(self, **kw):
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    def __repr__(self):  # pragma: no cover - debugging aid
+        return "Struct(" + ", ".join(f"{k}={v!r}" for k, v in vars(self).items()) + ")"
+
+    def copy(self) -> "Struct":
+        return copy.deepcopy(self)
+
+    def get(self, name, default=None):
+        return getattr(self, name, default)
+
+
+def default_ctrlvar() -> Struct:
+    """Defaults for the options on the uvh assembly path."""
+    S = Struct
+    return S(
+        dt=1.0,                                   # :268
+        theta=0.5,                                # :377
+        SUPG=S(beta0=1.0, beta1=0.0),             # :376
+        uvh=S(SUPG=S(tau="taus", tauMultiplier=1.0)),  # :551, :555
+        SpeedZero=1e-4,                           # :412
+        EpsZero=1e-10,                            # :413
+        etaZero=10.0,                             # :415
+        Czero=0.0,                                # :441
+        kH=1.0,                                   # :2288
+        Hh0=0.0,                                  # :2295
+        ThickMin=1.0,                             # :1469
+        ResetThicknessToMinThickness=False,       # :1472
+        ResetThicknessInNonLinLoop=False,         # :1473
+        Enforce_bAboveB=False,                    # :2643
+        FlowApproximation="SSTREAM",              # :139
+        SlidingLaw="Weertman",                    # :171
+        TriNodes=3,                               # :239
+        uvGroupAssembly=False,                    # :611
+        uvhGroupAssembly=False,                   # :612
+        QuadRules2021=True,                       # :681
+        QuadratureRuleDegree=None,                # :683
+        TG3=False,                                # :388
+        IncludeTG3uvhBoundaryTerm=False,          # :391
+        ThicknessPenalty=False,                   # :1514
+        ThicknessBarrier=False,
+        LevelSetMethod=False,                     # :1969
+        LevelSetMethodAutomaticallyApplyMassBalanceFeedback=True,  # :2005
+        MassBalanceGeometryFeedback=0,            # :2408
+        IncludeMelangeModelPhysics=False,         # :2452
+        Parallel=S(isTest=False, uvhAssembly=S(spmd=S(isOn=False), parfor=S(isOn=False))),  # :2520-2528
+        UseMexFiles=False,                        # :2543
+        uvhMatrixAssembly=S(ZeroFields=False, Ronly=False),  # :2633-2634
+        OnlyCalcBasalDragAndEffectiveViscosity=False,         # :2635
+        Development=S(Pre2025uvhAssembly=False),  # :2640
+        InfoLevelCPU=0,                           # :778
+        InfoLevelNonLinIt=1,
+        GroundingFloatingMaskContains="GF nodes only",  # :2316
+        # Newton-Raphson uvh controls (used by the minimal driver in uasource_amd.newton)
+        NRitmax=50,                               # :597
+        uvhDesiredWorkAndForceTolerances=[float("inf"), 1e-15],   # :496
+        uvhDesiredWorkOrForceTolerances=[float("inf"), 1e-15],    # :498
+        uvhExitBackTrackingStepLength=1e-3,       # :502
+        uvhAcceptableWorkAndForceTolerances=[float("inf"), 1e-6], # :504
+    )
