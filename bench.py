@@ -1,0 +1,179 @@
+"""Benchmark of the uvh R+K assembly on one MI355X.
+
+A step is one full Newton-Raphson R+K assembly (``uvhAssembly`` with
+``Ronly=false``) of the plain FP64 workload: 199,712 six-node triangles,
+401,005 nodes, 7 integration points, with every input already resident in HBM.
+The line printed reports elements assembled per second, the HBM-roofline
+fraction of the assembly, per-kernel times from HIP events on the launch
+stream, and the oracle's C restatement timed on this host's cores beside it.
+
+    python bench.py [--steps K] [--warmup W] [--workload plain_p2] [--ronly]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def algorithmic_bytes(nn: int, nele: int, nod: int, nnz: int, ronly: bool) -> int:
+    """Compulsory DRAM bytes of one assembly: 19 FP64 nodal inputs read once
+    (F1: h,u,v,S,B,rho,C,m,n,A,as,ab; F0: h,u,v,as,ab; x,y), 3 FP64 residual
+    outputs per node, int32 connectivity, and K values written once."""
+    b = nn * (19 * 8 + 3 * 8) + nele * nod * 4
+    if not ronly:
+        b += nnz * 8
+    return b
+
+
+def cpu_baseline(workload: str, M, F0, F1, cv, ronly: bool, budget_s: float):
+    """Oracle O2 (C restatement, OpenMP) on this host: elements/s on the same workload."""
+    from oracle import o2
+    o2.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        avail = os.cpu_count() or 1
+    threads = min(avail, threads) if threads > 0 else min(avail, 16)
+    mesh = o2.O2Mesh(M)
+    o2.assemble(cv, M, F0, F1, mesh=mesh, nthreads=threads, ronly=ronly)  # warm-up
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while True:
+        t0 = time.perf_counter()
+        o2.assemble(cv, M, F0, F1, mesh=mesh, nthreads=threads, ronly=ronly)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end or len(times) >= 50:
+            break
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": M.Nele / med, "unit": "elements/s", "cores": threads, "kind": "port",
+            "sample": f"{workload}: {len(times)} full R{'' if ronly else '+K'} assemblies of {M.Nele} elements "
+                      f"(median {med * 1e3:.1f} ms), oracle O2 C restatement of Úa with {threads} OpenMP threads"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="plain_p2")
+    ap.add_argument("--ronly", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the uvh assembly is built and measured on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; independent meshes shard across GPUs with no exchange).",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from uasource_amd import uvh
+    from uasource_amd.ctrlvar import default_ctrlvar
+    from uasource_amd.fields import WORKLOADS, build_mesh, make_fields
+
+    w = WORKLOADS[args.workload]
+    M = build_mesh(w)
+    F0, F1, _ = make_fields(M, w)
+    cv = default_ctrlvar()
+    dev = torch.device("cuda:0")
+    ctx = uvh.UvhContext(0)
+    ctx.set_mesh(M)
+
+    class Dev:
+        pass
+    d1, d0 = Dev(), Dev()
+    for k in uvh.F1_KEYS:
+        setattr(d1, k, torch.from_numpy(np.ascontiguousarray(getattr(F1, k))).to(dev))
+    for k in uvh.F0_KEYS:
+        setattr(d0, k, torch.from_numpy(np.ascontiguousarray(getattr(F0, k))).to(dev))
+    fields = uvh.make_fields(d0, d1)
+    ctrl = uvh.make_ctrl(cv, F1)
+    R = torch.zeros(3 * M.Nnodes, dtype=torch.float64, device=dev)
+    K = None if args.ronly else torch.zeros(ctx.nnz, dtype=torch.float64, device=dev)
+    flags = uvh.UVH_RONLY if args.ronly else 0
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    def step():
+        ctx.assemble_device(ctrl, fields, flags, R, K, None, None, stream=sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.sync_status()
+
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for _ in range(args.steps):
+        step()
+    t_stop.record(stream)
+    torch.cuda.synchronize()
+    ms_total = t_start.elapsed_time(t_stop)
+    ctx.sync_status()
+    ms_per_step = ms_total / args.steps
+
+    # per-kernel durations: HIP events around each launch, on the launch stream
+    ctx.profile(True)
+    for _ in range(args.steps):
+        step()
+    times = ctx.kernel_times()
+    ctx.profile(False)
+    kernels = {name: tot / max(cnt, 1) for name, (tot, cnt) in times.items()}
+    assembly_ms = sum(kernels.values())
+    dominant = max(kernels, key=kernels.get)
+
+    nbytes = algorithmic_bytes(M.Nnodes, M.Nele, M.nod, ctx.nnz, args.ronly)
+    achieved = nbytes / (assembly_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}{'_ronly' if args.ronly else ''}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_assembly")
+
+    line = {
+        "metric": "uvh elements assembled/s (R+K), HBM GB/s fraction; p50/p99 request latency",
+        "value": M.Nele * args.steps / (ms_total * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered-Delaunay mesh and field recipe)",
+        "config": {"workload": args.workload, "Nele": M.Nele, "Nnodes": M.Nnodes, "nod": M.nod, "nip": M.nip,
+                   "nnz_K": ctx.nnz, "mode": "R only" if args.ronly else "R+K", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "uvh assembly (all kernels of one R+K evaluation)",
+                     "algorithmic_bytes": nbytes, "assembly_us": assembly_ms * 1e3},
+        "kernels_us": {k: v * 1e3 for k, v in kernels.items()},
+        "dominant_kernel": dominant,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.workload, M, F0, F1, cv, args.ronly, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,129 @@
+/*
+ * uvh.h -- C ABI of the MI355X uvh assembly library (libuvh.so).
+ *
+ * Replaces the residual/Jacobian assembly behind Úa's
+ *     [UserVar,RunInfo,R,K] = uvhAssembly(UserVar,RunInfo,CtrlVar,MUA,F0,F1,l1,BCs1)
+ * (UaMain/uvhAssembly.m:1, body UaMain/uvhMatrixAssembly.m:6-585). A MATLAB MEX
+ * gateway (uasource_amd/mex/uvhAssemblyGPU_mex.c) or any other host binds these
+ * entry points; plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Connectivity is 0-based int32, element-major (conn[e*nod + a]); the MEX gateway
+ *    converts MATLAB's 1-based Nele x nod column-major array once per mesh.
+ *  - Nodal arrays are FP64, length nnodes.
+ *  - R is 3*nnodes, ordered [x(1..Nn); y(1..Nn); h(1..Nn)] (uvhMatrixAssembly.m:185-186,425-439),
+ *    R = Tint - Fext (uvhMatrixAssembly.m:455).
+ *  - K is returned as values on a fixed CSR pattern of the 3Nn x 3Nn matrix with blocks
+ *    [Kxu Kxv Kxh; Kyu Kyv Kyh; Khu Khv Khh] (uvhMatrixAssembly.m:25-27,478-502). Row
+ *    (blk, node) holds, in order, the columns adj(node), adj(node)+Nn, adj(node)+2Nn, where
+ *    adj(node) is the sorted list of nodes sharing an element with node (itself included).
+ *    The pattern is structurally symmetric, so the same index arrays describe CSC.
+ *    Entries that sum to exactly zero are kept (MATLAB's sparse would drop them).
+ *  - Status codes mirror the reference's errors: UVH_E_NAN_R is
+ *    error("uvhMatrixAssembly:NaNinR") (uvhMatrixAssembly.m:581-584), UVH_E_NAN_K is
+ *    error(' NaN in K ') (uvhMatrixAssembly.m:575-579).
+ */
+#ifndef UVH_H
+#define UVH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uvh_ctx uvh_ctx;
+
+enum uvh_status {
+    UVH_OK = 0,
+    UVH_E_ARG = 1,          /* bad argument (size, null pointer, unsupported element type) */
+    UVH_E_HIP = 2,          /* HIP runtime error */
+    UVH_E_NAN_R = 3,        /* NaN in R:  uvhMatrixAssembly:NaNinR */
+    UVH_E_NAN_K = 4,        /* NaN on the diagonal of K */
+    UVH_E_UNSUPPORTED = 5,  /* option outside this library: caller falls back to uvhMatrixAssembly */
+    UVH_E_STATE = 6,        /* call out of order (e.g. assemble before set_mesh) */
+    UVH_E_NOMEM = 7         /* device allocation failed */
+};
+
+enum uvh_flags {
+    UVH_RONLY = 1u,         /* CtrlVar.uvhMatrixAssembly.Ronly: R only, K untouched */
+    UVH_ZEROFIELDS = 2u,    /* CtrlVar.uvhMatrixAssembly.ZeroFields (uvhMatrixAssembly.m:48-101) */
+    UVH_FP32 = 4u,          /* precision form: FP32 element arithmetic, FP64 accumulation */
+    UVH_NO_NAN_CHECK = 8u   /* skip the NaN checks of uvhMatrixAssembly.m:575-584 */
+};
+
+enum uvh_supg_tau { UVH_TAU_S = 0, UVH_TAU_T = 1, UVH_TAU_1 = 2, UVH_TAU_2 = 3, UVH_TAU_0 = 4 };
+
+/* Scalars the assembly reads from CtrlVar and F (defaults: UaMain/Ua2D_DefaultParameters.m). */
+typedef struct {
+    double dt;                  /* CtrlVar.dt */
+    double theta;               /* CtrlVar.theta */
+    double supg_beta0;          /* CtrlVar.SUPG.beta0 */
+    double supg_tau_multiplier; /* CtrlVar.uvh.SUPG.tauMultiplier */
+    double speed_zero;          /* CtrlVar.SpeedZero */
+    double eps_zero;            /* CtrlVar.EpsZero */
+    double eta_zero;            /* CtrlVar.etaZero */
+    double czero;               /* CtrlVar.Czero */
+    double kH;                  /* CtrlVar.kH */
+    double Hh0;                 /* CtrlVar.Hh0 */
+    double g;                   /* F.g */
+    double rhow;                /* F.rhow */
+    double alpha;               /* F.alpha */
+    double thick_min;           /* CtrlVar.ThickMin */
+    int32_t supg_tau;           /* CtrlVar.uvh.SUPG.tau as enum uvh_supg_tau */
+    int32_t reset_thickness;    /* ResetThicknessToMinThickness && ResetThicknessInNonLinLoop */
+    int32_t enforce_b_above_B;  /* CtrlVar.Enforce_bAboveB */
+    int32_t reserved;
+} uvh_ctrl;
+
+/* Nodal fields of F1 (current) and F0 (previous time step); each nnodes long.
+ * dadh may be NULL (zero: MassBalanceGeometryFeedback < 2). */
+typedef struct {
+    const double *h, *ub, *vb, *S, *B, *rho, *C, *m, *n, *AGlen, *as, *ab;
+    const double *h0, *ub0, *vb0, *as0, *ab0;
+    const double *dadh;
+} uvh_fields;
+
+/* Context on HIP device `device` (one per process and device; not thread-safe). */
+int uvh_create(uvh_ctx **ctx, int device);
+void uvh_destroy(uvh_ctx *ctx);
+const char *uvh_last_error(const uvh_ctx *ctx);
+
+/* Mesh (MUA) upload and sparsity pattern build, once per mesh.
+ * Replaces the per-mesh part of MUA (connectivity, coordinates, points, weights:
+ * UaMain/CreateMUA.m:31-61) that uvhMatrixAssembly reads; derivatives and detJ
+ * (CalcMuaMeshDerivatives.m) are recomputed on the device from the coordinates.
+ * nod in {3, 6, 10}; 1 <= nip <= 32; pts is nip x 2 (xi, eta), wts is nip. */
+int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const int32_t *conn,
+                 const double *x, const double *y, int32_t nip, const double *pts, const double *wts);
+
+/* Pattern of K: neq = 3*nnodes rows, nnz values. */
+int uvh_pattern_info(const uvh_ctx *ctx, int64_t *neq, int64_t *nnz);
+/* rowptr: neq+1 entries, colind: nnz entries (host arrays). */
+int uvh_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind);
+
+/* Host-buffer assembly: the uvhAssembly drop-in (uvhMatrixAssembly.m:6-585).
+ * Fields are host arrays; R (3Nn) is required; Kval (nnz) may be NULL or is ignored with
+ * UVH_RONLY; Tint and Fext (3Nn) may be NULL. Blocks until the results are on the host. */
+int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *fields, uint32_t flags,
+                 double *R, double *Kval, double *Tint, double *Fext);
+
+/* Device-buffer assembly, enqueued on `stream` (a hipStream_t; NULL = the context's stream).
+ * All pointers are device pointers; nothing is copied to or from the host and the call
+ * returns once the work is enqueued. NaN checks are deferred: uvh_sync_status() waits
+ * for the stream and returns UVH_E_NAN_R / UVH_E_NAN_K if the last assembly met a NaN. */
+int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *dfields, uint32_t flags,
+                        double *dR, double *dKval, double *dTint, double *dFext, void *stream);
+int uvh_sync_status(uvh_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the launch stream (for measurement).
+ * uvh_profile(ctx, 1) starts recording; uvh_kernel_times fills up to `cap` entries of
+ * (name, total ms, launches) accumulated since the last reset, and returns the count. */
+int uvh_profile(uvh_ctx *ctx, int enable);
+int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t *launches, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UVH_H */

@@ -1,0 +1,207 @@
+"""GPU parity of the HIP assembly (through the C ABI) against the oracle.
+
+Bar: the sparsity pattern is bit-exact; every K block and Tint, Fext agree with the
+FP64 oracle to 1e-12 normwise (relative), K entries to 1e-12 of their block's
+largest entry, and R to 1e-12 of max(|Tint|, |Fext|) (R = Tint - Fext cancels).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from uasource_amd import uvh
+from uasource_amd.ctrlvar import default_ctrlvar
+from uasource_amd.fields import WORKLOADS, build_mesh, make_fields
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+_mesh_cache = {}
+
+
+def mesh_fields(name):
+    if name not in _mesh_cache:
+        w = WORKLOADS[name]
+        M = build_mesh(w)
+        F0, F1, BCs = make_fields(M, w)
+        _mesh_cache[name] = (M, F0, F1, BCs)
+    M, F0, F1, BCs = _mesh_cache[name]
+    return M, F0.copy(), F1.copy(), BCs
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = uvh.UvhContext(0)
+    yield c
+    c.close()
+
+
+def assert_parity(ctx, M, gpu, ref, ronly=False):
+    Nn = M.Nnodes
+    scale = max(np.linalg.norm(ref["Tint"]), np.linalg.norm(ref["Fext"]))
+    errR = np.linalg.norm(gpu["R"] - ref["R"]) / scale
+    assert errR <= TOL, f"R rel err {errR:.3e}"
+    if gpu.get("Tint") is not None:
+        for k in ("Tint", "Fext"):
+            for bi in range(3):
+                a, b = gpu[k][bi * Nn:(bi + 1) * Nn], ref[k][bi * Nn:(bi + 1) * Nn]
+                nb = np.linalg.norm(b)
+                if nb > 0:
+                    assert np.linalg.norm(a - b) <= TOL * nb, (k, bi, np.linalg.norm(a - b) / nb)
+    if ronly:
+        return
+    rowptr, colind = ctx.pattern()
+    Kref = ref["K"]
+    assert np.array_equal(rowptr, Kref.indptr.astype(np.int64)), "CSR row pointers differ"
+    assert np.array_equal(colind, Kref.indices.astype(np.int32)), "CSR column indices differ"
+    Kg = gpu["Kval"]
+    Kr = Kref.data
+    rows = np.repeat(np.arange(3 * Nn), np.diff(rowptr)) // Nn
+    cols = colind // Nn
+    worst = 0.0
+    for bi in range(3):
+        for bj in range(3):
+            sel = (rows == bi) & (cols == bj)
+            a, b = Kg[sel], Kr[sel]
+            nb = np.linalg.norm(b)
+            if nb == 0:
+                assert np.all(a == 0)
+                continue
+            rel = np.linalg.norm(a - b) / nb
+            worst = max(worst, rel)
+            assert rel <= TOL, f"block ({bi},{bj}) normwise rel err {rel:.3e}"
+            ent = np.max(np.abs(a - b)) / np.max(np.abs(b))
+            assert ent <= TOL, f"block ({bi},{bj}) entrywise err {ent:.3e}"
+    say(f"   parity ok: R {errR:.2e}, worst K block {worst:.2e}")
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3", "cpu_reference", "medium_p2", "medium_p3"])
+def test_parity_against_oracle(ctx, o2lib, name):
+    M, F0, F1, BCs = mesh_fields(name)
+    cv = default_ctrlvar()
+    say(f"[gpu] parity {name}: Nele={M.Nele} nod={M.nod} nip={M.nip}")
+    ctx.set_mesh(M)
+    gpu = ctx.assemble(cv, F0, F1, want_tf=True)
+    ref = o2lib.assemble(cv, M, F0, F1)
+    assert_parity(ctx, M, gpu, ref)
+
+
+@pytest.mark.parametrize("name", ["tiny_p2", "cpu_reference"])
+def test_parity_against_matlab_order_oracle(ctx, name):
+    from oracle import uvh_oracle as O1
+    M, F0, F1, BCs = mesh_fields(name)
+    cv = default_ctrlvar()
+    say(f"[gpu] O1 parity {name}")
+    ctx.set_mesh(M)
+    gpu = ctx.assemble(cv, F0, F1, want_tf=True)
+    ref = O1.uvh_matrix_assembly(cv, M, F0, F1, BCs)
+    assert_parity(ctx, M, gpu, ref)
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "medium_p2", "medium_p3"])
+def test_ronly_and_zero_fields(ctx, o2lib, name):
+    M, F0, F1, BCs = mesh_fields(name)
+    cv = default_ctrlvar()
+    ctx.set_mesh(M)
+    for zf in (False, True):
+        say(f"[gpu] ronly {name} zero_fields={zf}")
+        gpu = ctx.assemble(cv, F0, F1, ronly=True, zero_fields=zf, want_tf=True)
+        ref = o2lib.assemble(cv, M, F0, F1, ronly=True, zero_fields=zf)
+        assert gpu["Kval"] is None
+        assert_parity(ctx, M, gpu, ref, ronly=True)
+        full = ctx.assemble(cv, F0, F1, zero_fields=zf)
+        assert np.array_equal(full["R"], gpu["R"]), "R-only and R+K residuals differ"
+
+
+def test_deterministic_bitwise(ctx):
+    M, F0, F1, BCs = mesh_fields("medium_p2")
+    cv = default_ctrlvar()
+    ctx.set_mesh(M)
+    say("[gpu] determinism medium_p2")
+    a = ctx.assemble(cv, F0, F1, want_tf=True)
+    b = ctx.assemble(cv, F0, F1, want_tf=True)
+    for k in ("R", "Kval", "Tint", "Fext"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_nan_in_h_raises_reference_error(ctx):
+    M, F0, F1, BCs = mesh_fields("tiny_p2")
+    cv = default_ctrlvar()
+    ctx.set_mesh(M)
+    F1.h = F1.h.copy()
+    F1.h[5] = np.nan
+    say("[gpu] NaN error path")
+    with pytest.raises(uvh.UaError) as ei:
+        ctx.assemble(cv, F0, F1)
+    assert ei.value.identifier in ("uvhMatrixAssembly:NaNinR", "uvhMatrixAssembly:NaNinK")
+    # the context stays usable
+    F1.h[5] = 900.0
+    ctx.assemble(cv, F0, F1)
+
+
+def test_uvhAssembly_dropin_signature(o2lib):
+    from uasource_amd.uvh import uvhAssembly, UaRunInfo
+    M, F0, F1, BCs = mesh_fields("cpu_reference")
+    cv = default_ctrlvar()
+    RunInfo = UaRunInfo()
+    say("[gpu] uvhAssembly drop-in")
+    UserVar, RunInfo, R, K = uvhAssembly(None, RunInfo, cv, M, F0, F1, None, BCs)
+    ref = o2lib.assemble(cv, M, F0, F1)
+    scale = max(np.linalg.norm(ref["Tint"]), np.linalg.norm(ref["Fext"]))
+    assert np.linalg.norm(R - ref["R"]) <= TOL * scale
+    d = (K - ref["K"])
+    assert abs(d).max() <= TOL * abs(ref["K"]).max()
+    assert RunInfo.CPU.Assembly.uvh > 0
+    cv.uvhMatrixAssembly.Ronly = True
+    _, _, R2, K2 = uvhAssembly(None, RunInfo, cv, M, F0, F1, None, BCs)
+    assert K2 is None and np.array_equal(R2, R)
+
+
+def test_device_path_matches_host_path(ctx):
+    import torch
+    M, F0, F1, BCs = mesh_fields("medium_p2")
+    cv = default_ctrlvar()
+    ctx.set_mesh(M)
+    say("[gpu] device path")
+    host = ctx.assemble(cv, F0, F1, want_tf=True)
+    dev = torch.device("cuda:0")
+    tens = {}
+    for k in uvh.F1_KEYS:
+        tens["1" + k] = torch.from_numpy(np.ascontiguousarray(getattr(F1, k))).to(dev)
+    for k in uvh.F0_KEYS:
+        tens["0" + k] = torch.from_numpy(np.ascontiguousarray(getattr(F0, k))).to(dev)
+
+    class FS:
+        pass
+    f1, f0 = FS(), FS()
+    for k in uvh.F1_KEYS:
+        setattr(f1, k, tens["1" + k])
+    for k in uvh.F0_KEYS:
+        setattr(f0, k, tens["0" + k])
+    fields = uvh.make_fields(f0, f1)
+    ctrl = uvh.make_ctrl(cv, F1)
+    R = torch.zeros(3 * M.Nnodes, dtype=torch.float64, device=dev)
+    K = torch.zeros(ctx.nnz, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    ctx.assemble_device(ctrl, fields, 0, R, K, None, None, stream=stream)
+    ctx.sync_status()
+    assert np.array_equal(R.cpu().numpy(), host["R"])
+    assert np.array_equal(K.cpu().numpy(), host["Kval"])
+
+
+def test_full_size_plain_p2_parity(ctx, o2lib):
+    """The measured workload itself (199,712 six-node elements, 7 points) against O2."""
+    M, F0, F1, BCs = mesh_fields("plain_p2")
+    cv = default_ctrlvar()
+    say(f"[gpu] full-size parity plain_p2 Nele={M.Nele}")
+    ctx.set_mesh(M)
+    gpu = ctx.assemble(cv, F0, F1, want_tf=True)
+    ref = o2lib.assemble(cv, M, F0, F1, nthreads=16)
+    assert_parity(ctx, M, gpu, ref)

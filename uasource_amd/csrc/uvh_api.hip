@@ -1,0 +1,379 @@
+// C ABI of libuvh (declared in include/uvh.h): context, mesh upload, assembly entry points.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/uvh.h"
+#include "uvh_internal.h"
+
+namespace uvh {
+hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s);
+hipError_t launch_clear_status(int *status, hipStream_t s);
+hipError_t launch_element(int nod, int nele, int nip, const int32_t *conn, const double *x, const double *y,
+                          const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p, bool ronly,
+                          double *EB, double *RE, hipStream_t s);
+hipError_t launch_rows(int nod, int nn, const int32_t *inc_ptr, const int32_t *inc, const uint8_t *pos,
+                       const int64_t *rowptr_n, int64_t nnzN, const double *EB, const double *RE, double *R,
+                       double *Tint, double *Fext, double *Kval, int check_nan, int *status, hipStream_t s);
+}  // namespace uvh
+
+namespace {
+
+struct KernelTimer {
+    const char *name;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct PendingEvent {
+    int slot;
+    hipEvent_t a, b;
+};
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T));
+}
+
+}  // namespace
+
+struct uvh_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    std::string err;
+    bool has_mesh = false;
+    uvh::MeshHost mesh;
+    // device mesh
+    int32_t *d_conn = nullptr;
+    double *d_x = nullptr, *d_y = nullptr;
+    uvh::ShapeTables *d_tab = nullptr;
+    int64_t *d_rowptr_n = nullptr;
+    int32_t *d_inc_ptr = nullptr, *d_inc = nullptr;
+    uint8_t *d_pos = nullptr;
+    // work
+    double *d_b = nullptr, *d_RE = nullptr, *d_EB = nullptr;
+    int *d_status = nullptr;
+    // host-path staging
+    double *d_fields = nullptr;  // 18 * nn
+    double *d_R = nullptr, *d_T = nullptr, *d_F = nullptr, *d_K = nullptr;
+    // profiling
+    bool profiling = false;
+    std::vector<KernelTimer> timers;
+    std::vector<PendingEvent> pending;
+
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char *what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return UVH_E_HIP;
+    }
+    void free_mesh() {
+        hipFree(d_conn); hipFree(d_x); hipFree(d_y); hipFree(d_tab); hipFree(d_rowptr_n);
+        hipFree(d_inc_ptr); hipFree(d_inc); hipFree(d_pos); hipFree(d_b); hipFree(d_RE); hipFree(d_EB);
+        hipFree(d_fields); hipFree(d_R); hipFree(d_T); hipFree(d_F); hipFree(d_K);
+        d_conn = nullptr; d_x = d_y = nullptr; d_tab = nullptr; d_rowptr_n = nullptr;
+        d_inc_ptr = d_inc = nullptr; d_pos = nullptr; d_b = d_RE = d_EB = nullptr;
+        d_fields = d_R = d_T = d_F = d_K = nullptr;
+        has_mesh = false;
+    }
+    int timer_slot(const char *name) {
+        for (size_t k = 0; k < timers.size(); ++k)
+            if (std::strcmp(timers[k].name, name) == 0) return int(k);
+        timers.push_back(KernelTimer{name});
+        return int(timers.size() - 1);
+    }
+    // Brackets one launch with events when profiling is on.
+    template <class F>
+    hipError_t timed(const char *name, hipStream_t s, F &&launch) {
+        if (!profiling) return launch();
+        PendingEvent pe{timer_slot(name), nullptr, nullptr};
+        hipEventCreate(&pe.a);
+        hipEventCreate(&pe.b);
+        hipEventRecord(pe.a, s);
+        hipError_t e = launch();
+        hipEventRecord(pe.b, s);
+        pending.push_back(pe);
+        return e;
+    }
+    void resolve_timers() {
+        for (auto &pe : pending) {
+            hipEventSynchronize(pe.b);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, pe.a, pe.b);
+            timers[pe.slot].total_ms += ms;
+            timers[pe.slot].launches += 1;
+            hipEventDestroy(pe.a);
+            hipEventDestroy(pe.b);
+        }
+        pending.clear();
+    }
+};
+
+namespace {
+
+uvh::Params make_params(const uvh_ctrl &c, uint32_t flags) {
+    uvh::Params p{};
+    p.dt = c.dt; p.theta = c.theta; p.beta0 = c.supg_beta0; p.tau_mult = c.supg_tau_multiplier;
+    p.uz = c.speed_zero; p.eps0 = c.eps_zero; p.eta0 = c.eta_zero; p.czero = c.czero;
+    p.kH = c.kH; p.Hh0 = c.Hh0; p.g = c.g; p.rhow = c.rhow;
+    p.ca = std::cos(c.alpha); p.sa = std::sin(c.alpha);
+    p.thick_min = c.thick_min;
+    p.tau_opt = c.supg_tau;
+    p.zero_fields = (flags & UVH_ZEROFIELDS) ? 1 : 0;
+    p.reset_thickness = c.reset_thickness;
+    p.enforce_b_above = c.enforce_b_above_B;
+    return p;
+}
+
+bool fields_complete(const uvh_fields &f) {
+    const double *req[] = {f.h, f.ub, f.vb, f.S, f.B, f.rho, f.C, f.m, f.n, f.AGlen, f.as, f.ab,
+                           f.h0, f.ub0, f.vb0, f.as0, f.ab0};
+    for (auto p : req)
+        if (!p) return false;
+    return true;
+}
+
+uvh::FieldPtrs to_ptrs(const uvh_fields &f) {
+    return uvh::FieldPtrs{f.h, f.ub, f.vb, f.S, f.B, f.rho, f.C, f.m, f.n, f.AGlen, f.as, f.ab,
+                          f.h0, f.ub0, f.vb0, f.as0, f.ab0, f.dadh};
+}
+
+}  // namespace
+
+extern "C" {
+
+int uvh_create(uvh_ctx **out, int device) {
+    if (!out) return UVH_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return UVH_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return UVH_E_HIP;
+    auto *ctx = new uvh_ctx();
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
+    if (e != hipSuccess) {
+        delete ctx;
+        return UVH_E_HIP;
+    }
+    *out = ctx;
+    return UVH_OK;
+}
+
+void uvh_destroy(uvh_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->own_stream) hipStreamSynchronize(ctx->own_stream);
+    ctx->resolve_timers();
+    ctx->free_mesh();
+    hipFree(ctx->d_status);
+    if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+const char *uvh_last_error(const uvh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const int32_t *conn, const double *x,
+                 const double *y, int32_t nip, const double *pts, const double *wts) {
+    if (!ctx) return UVH_E_ARG;
+    if (nele <= 0 || nnodes <= 0 || !conn || !x || !y || !pts || !wts)
+        return ctx->fail(UVH_E_ARG, "uvh_set_mesh: empty mesh or null pointer");
+    if (nod != 3 && nod != 6 && nod != 10) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: nod must be 3, 6 or 10");
+    if (nip < 1 || nip > uvh::kMaxNip) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: nip must be in 1..32");
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->own_stream);
+    ctx->free_mesh();
+    uvh::MeshHost &m = ctx->mesh;
+    m = uvh::MeshHost{};
+    m.nele = nele; m.nod = nod; m.nn = nnodes; m.nip = nip;
+    m.conn.assign(conn, conn + size_t(nele) * nod);
+    m.x.assign(x, x + nnodes);
+    m.y.assign(y, y + nnodes);
+    std::string msg = uvh::fill_shape_tables(m.tab, nod, nip, pts, wts);
+    if (msg.empty()) msg = uvh::build_mesh_host(m);
+    if (!msg.empty()) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: " + msg);
+    hipError_t e = hipSuccess;
+    const size_t ne = size_t(nele), nn = size_t(nnodes);
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    chk(dalloc(&ctx->d_conn, ne * nod));
+    chk(dalloc(&ctx->d_x, nn));
+    chk(dalloc(&ctx->d_y, nn));
+    chk(dalloc(&ctx->d_tab, 1));
+    chk(dalloc(&ctx->d_rowptr_n, nn + 1));
+    chk(dalloc(&ctx->d_inc_ptr, nn + 1));
+    chk(dalloc(&ctx->d_inc, ne * nod));
+    chk(dalloc(&ctx->d_pos, ne * nod * nod));
+    chk(dalloc(&ctx->d_b, nn));
+    chk(dalloc(&ctx->d_RE, ne * nod * 6));
+    if (e != hipSuccess) {
+        ctx->free_mesh();
+        return ctx->fail(UVH_E_NOMEM, std::string("uvh_set_mesh: device allocation failed: ") + hipGetErrorString(e));
+    }
+    chk(hipMemcpy(ctx->d_conn, m.conn.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_x, m.x.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_y, m.y.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_tab, &m.tab, sizeof(uvh::ShapeTables), hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_rowptr_n, m.rowptr_n.data(), (nn + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_inc_ptr, m.inc_ptr.data(), (nn + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_inc, m.inc.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_pos, m.pos.data(), ne * nod * nod, hipMemcpyHostToDevice));
+    if (e != hipSuccess) {
+        ctx->free_mesh();
+        return ctx->hip_fail(e, "uvh_set_mesh: upload");
+    }
+    ctx->has_mesh = true;
+    return UVH_OK;
+}
+
+int uvh_pattern_info(const uvh_ctx *ctx, int64_t *neq, int64_t *nnz) {
+    if (!ctx || !ctx->has_mesh) return UVH_E_STATE;
+    if (neq) *neq = 3 * int64_t(ctx->mesh.nn);
+    if (nnz) *nnz = 9 * ctx->mesh.nnzN();
+    return UVH_OK;
+}
+
+int uvh_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind) {
+    if (!ctx || !ctx->has_mesh) return UVH_E_STATE;
+    if (!rowptr || !colind) return UVH_E_ARG;
+    const uvh::MeshHost &m = ctx->mesh;
+    const int64_t nn = m.nn, nnzN = m.nnzN();
+    for (int bi = 0; bi < 3; ++bi)
+        for (int64_t k = 0; k < nn; ++k) {
+            const int64_t r = bi * nn + k;
+            const int64_t start = int64_t(bi) * 3 * nnzN + 3 * m.rowptr_n[k];
+            const int64_t deg = m.rowptr_n[k + 1] - m.rowptr_n[k];
+            rowptr[r] = start;
+            for (int bj = 0; bj < 3; ++bj)
+                for (int64_t t = 0; t < deg; ++t)
+                    colind[start + bj * deg + t] = int32_t(m.colind_n[m.rowptr_n[k] + t] + bj * nn);
+        }
+    rowptr[3 * nn] = 9 * nnzN;
+    return UVH_OK;
+}
+
+int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
+                        double *dK, double *dT, double *dF, void *stream) {
+    if (!ctx) return UVH_E_ARG;
+    if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_assemble: no mesh (call uvh_set_mesh first)");
+    if (!ctrl || !df || !dR) return ctx->fail(UVH_E_ARG, "uvh_assemble: null ctrl, fields or R");
+    if (!fields_complete(*df)) return ctx->fail(UVH_E_ARG, "uvh_assemble: a required field pointer is null");
+    if (flags & UVH_FP32) return ctx->fail(UVH_E_UNSUPPORTED, "uvh_assemble: UVH_FP32 is not available in this build");
+    if (ctrl->supg_tau < 0 || ctrl->supg_tau > 4) return ctx->fail(UVH_E_ARG, "uvh_assemble: supg_tau out of range");
+    const bool ronly = (flags & UVH_RONLY) != 0;
+    if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_assemble: K output required unless UVH_RONLY");
+    hipSetDevice(ctx->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    const uvh::MeshHost &m = ctx->mesh;
+    if (!ronly && !ctx->d_EB) {
+        if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * 9 * m.nod) != hipSuccess)
+            return ctx->fail(UVH_E_NOMEM, "uvh_assemble: cannot allocate element rows");
+    }
+    const uvh::Params p = make_params(*ctrl, flags);
+    const uvh::FieldPtrs f = to_ptrs(*df);
+    const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
+    hipError_t e = ctx->timed("clear_status", s, [&] { return uvh::launch_clear_status(ctx->d_status, s); });
+    if (e == hipSuccess)
+        e = ctx->timed("nodal_b", s, [&] { return uvh::launch_nodal_b(m.nn, f, p, ctx->d_b, s); });
+    if (e == hipSuccess)
+        e = ctx->timed(ronly ? "element_r" : "element_rk", s, [&] {
+            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->d_conn, ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab,
+                                       p, ronly, ctx->d_EB, ctx->d_RE, s);
+        });
+    if (e == hipSuccess)
+        e = ctx->timed(ronly ? "rows_r" : "rows_rk", s, [&] {
+            return uvh::launch_rows(m.nod, m.nn, ctx->d_inc_ptr, ctx->d_inc, ctx->d_pos, ctx->d_rowptr_n, m.nnzN(),
+                                    ctx->d_EB, ctx->d_RE, dR, dT, dF, ronly ? nullptr : dK, check, ctx->d_status, s);
+        });
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: kernel launch");
+    return UVH_OK;
+}
+
+int uvh_sync_status(uvh_ctx *ctx) {
+    if (!ctx) return UVH_E_ARG;
+    hipSetDevice(ctx->device);
+    int st = 0;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(&st, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_sync_status");
+    if (st & 2) return ctx->fail(UVH_E_NAN_K, " NaN in K ");
+    if (st & 1) return ctx->fail(UVH_E_NAN_R, "uvhMatrixAssembly:NaNinR  NaN in R ");
+    return UVH_OK;
+}
+
+int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *hf, uint32_t flags, double *R, double *Kval,
+                 double *Tint, double *Fext) {
+    if (!ctx) return UVH_E_ARG;
+    if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_assemble: no mesh (call uvh_set_mesh first)");
+    if (!ctrl || !hf || !R) return ctx->fail(UVH_E_ARG, "uvh_assemble: null ctrl, fields or R");
+    if (!fields_complete(*hf)) return ctx->fail(UVH_E_ARG, "uvh_assemble: a required field pointer is null");
+    const bool ronly = (flags & UVH_RONLY) != 0;
+    hipSetDevice(ctx->device);
+    const uvh::MeshHost &m = ctx->mesh;
+    const size_t nn = size_t(m.nn);
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    if (!ctx->d_fields) chk(dalloc(&ctx->d_fields, 18 * nn));
+    if (!ctx->d_R) chk(dalloc(&ctx->d_R, 3 * nn));
+    if (!ctx->d_T) chk(dalloc(&ctx->d_T, 3 * nn));
+    if (!ctx->d_F) chk(dalloc(&ctx->d_F, 3 * nn));
+    if (!ronly && Kval && !ctx->d_K) chk(dalloc(&ctx->d_K, size_t(9) * m.nnzN()));
+    if (e != hipSuccess) return ctx->fail(UVH_E_NOMEM, "uvh_assemble: device allocation failed");
+    const double *src[18] = {hf->h, hf->ub, hf->vb, hf->S, hf->B, hf->rho, hf->C, hf->m, hf->n, hf->AGlen,
+                             hf->as, hf->ab, hf->h0, hf->ub0, hf->vb0, hf->as0, hf->ab0, hf->dadh};
+    const double *dst[18];
+    hipStream_t s = ctx->own_stream;
+    for (int k = 0; k < 18; ++k) {
+        double *d = ctx->d_fields + k * nn;
+        dst[k] = src[k] ? d : nullptr;
+        if (src[k]) chk(hipMemcpyAsync(d, src[k], nn * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: field upload");
+    uvh_fields df{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8], dst[9], dst[10],
+                  dst[11], dst[12], dst[13], dst[14], dst[15], dst[16], dst[17]};
+    double *dK = (!ronly && Kval) ? ctx->d_K : nullptr;
+    if (!ronly && !Kval) flags |= UVH_RONLY;
+    int st = uvh_assemble_device(ctx, ctrl, &df, flags, ctx->d_R, dK, ctx->d_T, ctx->d_F, s);
+    if (st != UVH_OK) return st;
+    chk(hipMemcpyAsync(R, ctx->d_R, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (Tint) chk(hipMemcpyAsync(Tint, ctx->d_T, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (Fext) chk(hipMemcpyAsync(Fext, ctx->d_F, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (dK) chk(hipMemcpyAsync(Kval, dK, size_t(9) * m.nnzN() * sizeof(double), hipMemcpyDeviceToHost, s));
+    chk(hipStreamSynchronize(s));
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: download");
+    if (flags & UVH_NO_NAN_CHECK) return UVH_OK;
+    return uvh_sync_status(ctx);
+}
+
+int uvh_profile(uvh_ctx *ctx, int enable) {
+    if (!ctx) return UVH_E_ARG;
+    hipSetDevice(ctx->device);
+    ctx->resolve_timers();
+    ctx->profiling = enable != 0;
+    for (auto &t : ctx->timers) { t.total_ms = 0; t.launches = 0; }
+    return UVH_OK;
+}
+
+int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t *launches, int cap) {
+    if (!ctx) return UVH_E_ARG;
+    hipSetDevice(ctx->device);
+    ctx->resolve_timers();
+    int n = 0;
+    for (auto &t : ctx->timers) {
+        if (n >= cap) break;
+        if (names) names[n] = t.name;
+        if (total_ms) total_ms[n] = t.total_ms;
+        if (launches) launches[n] = t.launches;
+        ++n;
+    }
+    return n;
+}
+
+}  // extern "C"
