@@ -108,7 +108,7 @@ int uvh_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind);
 int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *fields, uint32_t flags,
                  double *R, double *Kval, double *Tint, double *Fext);
 
-/* Device-buffer assembly, enqueued on `stream` (a hipStream_t; NULL = the context's stream).
+/* Device-buffer assembly, enqueued on `stream` (a hipStream_t; NULL is the HIP null stream).
  * All pointers are device pointers; nothing is copied to or from the host and the call
  * returns once the work is enqueued. NaN checks are deferred: uvh_sync_status() waits
  * for the stream and returns UVH_E_NAN_R / UVH_E_NAN_K if the last assembly met a NaN. */

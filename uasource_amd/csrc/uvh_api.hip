@@ -171,7 +171,7 @@ int uvh_create(uvh_ctx **out, int device) {
 void uvh_destroy(uvh_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
-    if (ctx->own_stream) hipStreamSynchronize(ctx->own_stream);
+    hipDeviceSynchronize();
     ctx->resolve_timers();
     ctx->free_mesh();
     hipFree(ctx->d_status);
@@ -189,7 +189,7 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     if (nod != 3 && nod != 6 && nod != 10) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: nod must be 3, 6 or 10");
     if (nip < 1 || nip > uvh::kMaxNip) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: nip must be in 1..32");
     hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->own_stream);
+    hipDeviceSynchronize();  // buffers may be in use on any stream the caller passed
     ctx->free_mesh();
     uvh::MeshHost &m = ctx->mesh;
     m = uvh::MeshHost{};
@@ -270,7 +270,7 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     const bool ronly = (flags & UVH_RONLY) != 0;
     if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_assemble: K output required unless UVH_RONLY");
     hipSetDevice(ctx->device);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL is the HIP null stream
     const uvh::MeshHost &m = ctx->mesh;
     if (!ronly && !ctx->d_EB) {
         if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * 9 * m.nod) != hipSuccess)

@@ -74,11 +74,14 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
                            const double *__restrict__ tN, const double *__restrict__ tXi,
                            const double *__restrict__ tEta, double wq, const Params &p,
                            double *__restrict__ c) {
-    // Jacobian and inverse (UaMain/derivVector.m:96-107)
+    // Jacobian and inverse (UaMain/derivVector.m:96-107). Coordinates are taken relative to
+    // the element's first node: sum_i dN_i = 0, so J is unchanged, and the differences are
+    // exact, which avoids cancelling ~1e5 m coordinates down to ~1e2 m element sizes.
     double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+    const double xo = nv[NV_X * NOD], yo = nv[NV_Y * NOD];
 #pragma unroll
-    for (int i = 0; i < NOD; ++i) {
-        const double xi = nv[NV_X * NOD + i], yi = nv[NV_Y * NOD + i];
+    for (int i = 1; i < NOD; ++i) {
+        const double xi = nv[NV_X * NOD + i] - xo, yi = nv[NV_Y * NOD + i] - yo;
         J11 += tXi[i] * xi; J12 += tXi[i] * yi;
         J21 += tEta[i] * xi; J22 += tEta[i] * yi;
     }

@@ -75,6 +75,14 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise UvhLibraryError(f"{path} is missing: build it with `python -m uasource_amd.csrc.build`")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64 and asks for it by
+    # its unversioned name, so if libuvh (which asks for the soname libamdhip64.so.7) were
+    # loaded first, torch would load a second runtime and find no device. Loading torch
+    # first makes libuvh bind to the runtime torch already holds.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.c_void_p
     lib.uvh_create.argtypes = [C.POINTER(vp), C.c_int]
