@@ -214,3 +214,21 @@ def test_full_size_plain_p2_parity(ctx, o2lib):
     gpu = ctx.assemble(cv, F0, F1, want_tf=True)
     ref = o2lib.assemble(cv, M, F0, F1, nthreads=16)
     assert_parity(ctx, M, gpu, ref)
+
+
+def test_two_pass_path_matches_fused_path(o2lib, monkeypatch):
+    """A/B check in the manner of the reference's CtrlVar.Parallel.isTest: the element-rows-
+    through-HBM path (UVH_TWO_PASS=1) and the fused LDS-window path both meet the oracle."""
+    M, F0, F1, BCs = mesh_fields("medium_p2")
+    cv = default_ctrlvar()
+    ref = o2lib.assemble(cv, M, F0, F1)
+    monkeypatch.setenv("UVH_TWO_PASS", "1")
+    c2 = uvh.UvhContext(0)
+    monkeypatch.delenv("UVH_TWO_PASS")
+    try:
+        say("[gpu] two-pass path medium_p2")
+        c2.set_mesh(M)
+        out2 = c2.assemble(cv, F0, F1, want_tf=True)
+        assert_parity(c2, M, out2, ref)
+    finally:
+        c2.close()

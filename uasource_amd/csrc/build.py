@@ -16,7 +16,7 @@ OUT = os.path.join(PKG, "libuvh.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["uvh_kernels.hip", "uvh_api.hip"]
-CXX_SOURCES = ["uvh_mesh.cpp"]
+CXX_SOURCES = ["uvh_mesh.cpp", "uvh_schedule.cpp", "uvh_host_api.cpp"]
 HEADERS = ["uvh_internal.h", os.path.join("..", "..", "include", "uvh.h")]
 
 ARCH = os.environ.get("UVH_OFFLOAD_ARCH", "gfx950")

@@ -3,8 +3,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/uvh.h"
@@ -19,6 +21,8 @@ hipError_t launch_element(int nod, int nele, int nip, const int32_t *conn, const
 hipError_t launch_rows(int nod, int nn, const int32_t *inc_ptr, const int32_t *inc, const uint8_t *pos,
                        const int64_t *rowptr_n, int64_t nnzN, const double *EB, const double *RE, double *R,
                        double *Tint, double *Fext, double *Kval, int check_nan, int *status, hipStream_t s);
+hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
+                        int64_t nnzN, int nn, bool ronly, int check_nan, hipStream_t s);
 }  // namespace uvh
 
 namespace {
@@ -55,7 +59,23 @@ struct uvh_ctx {
     uvh::ShapeTables *d_tab = nullptr;
     int64_t *d_rowptr_n = nullptr;
     int32_t *d_inc_ptr = nullptr, *d_inc = nullptr;
-    uint8_t *d_pos = nullptr;
+    uint8_t *d_pos = nullptr, *d_diag = nullptr;
+    // fused-assembly schedule
+    bool two_pass = false;  // UVH_TWO_PASS=1: element rows through HBM + per-node sums (A/B checks)
+    uvh::Schedule sched;
+    uvh::FusedLayout layout{};
+    int32_t *d_pbp = nullptr, *d_belem = nullptr, *d_easlot = nullptr, *d_zptr = nullptr, *d_zslot = nullptr;
+    int32_t *d_zlen = nullptr, *d_fptr = nullptr, *d_fnode = nullptr, *d_fslot = nullptr;
+    uint8_t *d_earank = nullptr, *d_bmax = nullptr;
+    uvh::MeshArgs *d_margs = nullptr;
+    // per-call argument blocks: a ring of pinned host copies and device copies; a slot is
+    // reused only after the event recorded behind its upload has completed
+    static constexpr int kRing = 8;
+    uvh::CallArgs *h_cargs = nullptr, *d_cargs = nullptr;
+    hipEvent_t ring_ev[kRing] = {};
+    bool ring_used[kRing] = {};
+    int ring_pos = 0, last_slot = -1;
+    uvh::CallArgs last_args{};
     // work
     double *d_b = nullptr, *d_RE = nullptr, *d_EB = nullptr;
     int *d_status = nullptr;
@@ -79,6 +99,13 @@ struct uvh_ctx {
         hipFree(d_conn); hipFree(d_x); hipFree(d_y); hipFree(d_tab); hipFree(d_rowptr_n);
         hipFree(d_inc_ptr); hipFree(d_inc); hipFree(d_pos); hipFree(d_b); hipFree(d_RE); hipFree(d_EB);
         hipFree(d_fields); hipFree(d_R); hipFree(d_T); hipFree(d_F); hipFree(d_K);
+        hipFree(d_diag); hipFree(d_pbp); hipFree(d_belem); hipFree(d_easlot); hipFree(d_zptr); hipFree(d_zslot);
+        hipFree(d_zlen); hipFree(d_fptr); hipFree(d_fnode); hipFree(d_fslot); hipFree(d_earank); hipFree(d_bmax);
+        hipFree(d_margs);
+        d_margs = nullptr;
+        last_slot = -1;
+        d_diag = d_earank = d_bmax = nullptr;
+        d_pbp = d_belem = d_easlot = d_zptr = d_zslot = d_zlen = d_fptr = d_fnode = d_fslot = nullptr;
         d_conn = nullptr; d_x = d_y = nullptr; d_tab = nullptr; d_rowptr_n = nullptr;
         d_inc_ptr = d_inc = nullptr; d_pos = nullptr; d_b = d_RE = d_EB = nullptr;
         d_fields = d_R = d_T = d_F = d_K = nullptr;
@@ -158,8 +185,15 @@ int uvh_create(uvh_ctx **out, int device) {
     if (hipSetDevice(device) != hipSuccess) return UVH_E_HIP;
     auto *ctx = new uvh_ctx();
     ctx->device = device;
+    const char *tp = std::getenv("UVH_TWO_PASS");
+    ctx->two_pass = tp && tp[0] == '1';
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_cargs), sizeof(uvh::CallArgs) * uvh_ctx::kRing, 0);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_cargs), sizeof(uvh::CallArgs) * uvh_ctx::kRing);
+    for (int k = 0; e == hipSuccess && k < uvh_ctx::kRing; ++k) e = hipEventCreateWithFlags(&ctx->ring_ev[k], hipEventDisableTiming);
     if (e != hipSuccess) {
         delete ctx;
         return UVH_E_HIP;
@@ -175,6 +209,10 @@ void uvh_destroy(uvh_ctx *ctx) {
     ctx->resolve_timers();
     ctx->free_mesh();
     hipFree(ctx->d_status);
+    for (int k = 0; k < uvh_ctx::kRing; ++k)
+        if (ctx->ring_ev[k]) hipEventDestroy(ctx->ring_ev[k]);
+    hipHostFree(ctx->h_cargs);
+    hipFree(ctx->d_cargs);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -199,6 +237,12 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     m.y.assign(y, y + nnodes);
     std::string msg = uvh::fill_shape_tables(m.tab, nod, nip, pts, wts);
     if (msg.empty()) msg = uvh::build_mesh_host(m);
+    ctx->layout = uvh::fused_layout(nod, nip);
+    if (msg.empty()) {
+        const char *pe = std::getenv("UVH_PATCHES");
+        const int target = pe ? std::max(1, std::atoi(pe)) : 256;
+        msg = uvh::build_schedule(m, ctx->layout.epb, ctx->layout.window_doubles, target, ctx->sched);
+    }
     if (!msg.empty()) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: " + msg);
     hipError_t e = hipSuccess;
     const size_t ne = size_t(nele), nn = size_t(nnodes);
@@ -212,7 +256,20 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(dalloc(&ctx->d_inc, ne * nod));
     chk(dalloc(&ctx->d_pos, ne * nod * nod));
     chk(dalloc(&ctx->d_b, nn));
-    chk(dalloc(&ctx->d_RE, ne * nod * 6));
+    chk(dalloc(&ctx->d_diag, nn));
+    const uvh::Schedule &S = ctx->sched;
+    chk(dalloc(&ctx->d_pbp, S.patch_batch_ptr.size()));
+    chk(dalloc(&ctx->d_belem, S.batch_elem.size()));
+    chk(dalloc(&ctx->d_easlot, S.ea_slot.size()));
+    chk(dalloc(&ctx->d_earank, S.ea_rank.size()));
+    chk(dalloc(&ctx->d_bmax, S.batch_maxrank.size()));
+    chk(dalloc(&ctx->d_zptr, S.zero_ptr.size()));
+    chk(dalloc(&ctx->d_zslot, std::max<size_t>(1, S.zero_slot.size())));
+    chk(dalloc(&ctx->d_zlen, std::max<size_t>(1, S.zero_len.size())));
+    chk(dalloc(&ctx->d_fptr, S.flush_ptr.size()));
+    chk(dalloc(&ctx->d_fnode, std::max<size_t>(1, S.flush_node.size())));
+    chk(dalloc(&ctx->d_fslot, std::max<size_t>(1, S.flush_slot.size())));
+    if (ctx->two_pass) chk(dalloc(&ctx->d_RE, ne * nod * 6));
     if (e != hipSuccess) {
         ctx->free_mesh();
         return ctx->fail(UVH_E_NOMEM, std::string("uvh_set_mesh: device allocation failed: ") + hipGetErrorString(e));
@@ -225,6 +282,23 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(hipMemcpy(ctx->d_inc_ptr, m.inc_ptr.data(), (nn + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_inc, m.inc.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_pos, m.pos.data(), ne * nod * nod, hipMemcpyHostToDevice));
+    chk(hipMemcpy(ctx->d_diag, m.diag.data(), nn, hipMemcpyHostToDevice));
+    auto up = [&](auto *dst, const auto &v) {
+        if (!v.empty()) chk(hipMemcpy(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+    };
+    up(ctx->d_pbp, S.patch_batch_ptr); up(ctx->d_belem, S.batch_elem); up(ctx->d_easlot, S.ea_slot);
+    up(ctx->d_earank, S.ea_rank); up(ctx->d_bmax, S.batch_maxrank); up(ctx->d_zptr, S.zero_ptr);
+    up(ctx->d_zslot, S.zero_slot); up(ctx->d_zlen, S.zero_len); up(ctx->d_fptr, S.flush_ptr);
+    up(ctx->d_fnode, S.flush_node); up(ctx->d_fslot, S.flush_slot);
+    chk(dalloc(&ctx->d_margs, 1));
+    if (e == hipSuccess) {
+        uvh::MeshArgs ma{};
+        ma.s = uvh::SchedPtrs{ctx->d_pbp, ctx->d_belem, ctx->d_easlot, ctx->d_earank, ctx->d_bmax, ctx->d_zptr,
+                              ctx->d_zslot, ctx->d_zlen, ctx->d_fptr, ctx->d_fnode, ctx->d_fslot};
+        ma.pos = ctx->d_pos; ma.diag = ctx->d_diag; ma.conn = ctx->d_conn; ma.x = ctx->d_x; ma.y = ctx->d_y;
+        ma.tab = ctx->d_tab; ma.rowptr_n = ctx->d_rowptr_n; ma.b = ctx->d_b; ma.status = ctx->d_status;
+        chk(hipMemcpy(ctx->d_margs, &ma, sizeof(ma), hipMemcpyHostToDevice));
+    }
     if (e != hipSuccess) {
         ctx->free_mesh();
         return ctx->hip_fail(e, "uvh_set_mesh: upload");
@@ -272,7 +346,7 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     hipSetDevice(ctx->device);
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL is the HIP null stream
     const uvh::MeshHost &m = ctx->mesh;
-    if (!ronly && !ctx->d_EB) {
+    if (ctx->two_pass && !ronly && !ctx->d_EB) {
         if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * 9 * m.nod) != hipSuccess)
             return ctx->fail(UVH_E_NOMEM, "uvh_assemble: cannot allocate element rows");
     }
@@ -282,6 +356,29 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     hipError_t e = ctx->timed("clear_status", s, [&] { return uvh::launch_clear_status(ctx->d_status, s); });
     if (e == hipSuccess)
         e = ctx->timed("nodal_b", s, [&] { return uvh::launch_nodal_b(m.nn, f, p, ctx->d_b, s); });
+    if (e == hipSuccess && !ctx->two_pass) {
+        uvh::CallArgs ca{};
+        ca.f = f; ca.p = p; ca.R = dR; ca.Tint = dT; ca.Fext = dF; ca.Kval = ronly ? nullptr : dK;
+        int slot = ctx->last_slot;
+        if (slot < 0 || std::memcmp(&ca, &ctx->last_args, sizeof(ca)) != 0) {
+            slot = ctx->ring_pos;
+            ctx->ring_pos = (ctx->ring_pos + 1) % uvh_ctx::kRing;
+            if (ctx->ring_used[slot]) hipEventSynchronize(ctx->ring_ev[slot]);
+            ctx->h_cargs[slot] = ca;
+            e = hipMemcpyAsync(ctx->d_cargs + slot, ctx->h_cargs + slot, sizeof(ca), hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ring_ev[slot], s);
+            ctx->ring_used[slot] = true;
+            ctx->last_args = ca;
+            ctx->last_slot = slot;
+        }
+        if (e == hipSuccess)
+            e = ctx->timed(ronly ? "fused_r" : "fused_rk", s, [&] {
+                return uvh::launch_fused(m.nod, m.nip, ctx->sched.npatch, ctx->layout, ctx->d_margs,
+                                         ctx->d_cargs + slot, m.nnzN(), m.nn, ronly, check, s);
+            });
+        if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: kernel launch");
+        return UVH_OK;
+    }
     if (e == hipSuccess)
         e = ctx->timed(ronly ? "element_r" : "element_rk", s, [&] {
             return uvh::launch_element(m.nod, m.nele, m.nip, ctx->d_conn, ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab,

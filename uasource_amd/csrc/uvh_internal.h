@@ -32,14 +32,71 @@ struct MeshHost {
     std::vector<int32_t> inc_ptr;     // nn+1: node -> incident (element, local node)
     std::vector<int32_t> inc;         // e*nod + a, in element order per node
     std::vector<uint8_t> pos;         // nele*nod*nod: pos[(e*nod+a)*nod+b] = index of conn[e][b] in adj(conn[e][a])
+    std::vector<uint8_t> diag;        // nn: index of node k in adj(k)
     ShapeTables tab{};
     int64_t nnzN() const { return rowptr_n.empty() ? 0 : rowptr_n.back(); }
 };
+
+// Fused-assembly schedule (see uvh_schedule.cpp): patches of elements, batches of `epb`
+// elements per patch, LDS window slots (offsets in doubles) of the owned node rows.
+struct Schedule {
+    int epb = 0;
+    int64_t window = 0;                 // LDS window capacity in doubles
+    int32_t npatch = 0;
+    std::vector<int32_t> patch_batch_ptr;  // npatch+1
+    std::vector<int32_t> batch_elem;       // nbatch*epb, -1 padded
+    std::vector<int32_t> ea_slot;          // nbatch*epb*nod: slot of (element, local node), -1 if not owned
+    std::vector<uint8_t> ea_rank;          // nbatch*epb*nod: order among the batch's elements on that node
+    std::vector<uint8_t> batch_maxrank;    // nbatch
+    std::vector<int32_t> zero_ptr;         // nbatch+1
+    std::vector<int32_t> zero_slot, zero_len;
+    std::vector<int32_t> flush_ptr;        // nbatch+1
+    std::vector<int32_t> flush_node, flush_slot;
+    int64_t peak_window = 0;               // largest window use over the patches (doubles)
+    int64_t patch_work = 0;                // element visits (>= nele: ghosts are visited twice)
+    int32_t nbatch() const { return int32_t(batch_maxrank.size()); }
+};
+
+// Row slot size in doubles: 9*deg K values, then Tx, Ty, Th, Fx, Fy, Fh.
+int64_t slot_doubles(int64_t deg);
+// Builds the schedule for `target_patches` patches; patches whose window would exceed
+// `window` doubles are split further. Needs rowptr_n, inc_ptr and inc.
+std::string build_schedule(MeshHost &m, int epb, int64_t window, int target_patches, Schedule &S);
 
 // Builds rowptr_n/colind_n, inc_ptr/inc and pos from conn; returns "" or an error message.
 std::string build_mesh_host(MeshHost &m);
 // Fills the shape tables for nod nodes at the nip points pts (nip x 2) with weights wts.
 std::string fill_shape_tables(ShapeTables &t, int nod, int nip, const double *pts, const double *wts);
+
+// Schedule tables of the fused kernel (device pointers; see uvh_schedule.cpp).
+struct SchedPtrs {
+    const int32_t *patch_batch_ptr, *batch_elem, *ea_slot;
+    const uint8_t *ea_rank, *batch_maxrank;
+    const int32_t *zero_ptr, *zero_slot, *zero_len, *flush_ptr, *flush_node, *flush_slot;
+};
+
+// Argument blocks of the fused kernel, kept in device memory and read through one pointer
+// each: as kernel arguments the ~45 pointers would not fit in scalar registers.
+struct MeshArgs {             // fixed per mesh
+    SchedPtrs s;
+    const uint8_t *pos, *diag;
+    const int32_t *conn;
+    const double *x, *y;
+    const ShapeTables *tab;
+    const int64_t *rowptr_n;
+    double *b;                // nodal lower surface, written by the pre-pass
+    int *status;
+};
+
+// LDS budget of the fused kernel per workgroup (gfx950: 160 KiB per workgroup).
+constexpr int64_t kFusedLdsBytes = 160 * 1024;
+
+// LDS layout of the fused kernel (in doubles) and its elements per batch.
+struct FusedLayout {
+    int table_doubles, work_doubles, epb;
+    int64_t window_doubles;
+};
+FusedLayout fused_layout(int nod, int nip);
 
 // Scalars of the assembly, device-ready (cos/sin of alpha precomputed).
 struct Params {
@@ -52,6 +109,13 @@ struct FieldPtrs {
     const double *h, *u, *v, *S, *B, *rho, *C, *m, *n, *A, *as1, *ab1;
     const double *h0, *u0, *v0, *as0, *ab0;
     const double *dadh;
+};
+
+// Per-assembly argument block of the fused kernel (device memory; see MeshArgs).
+struct CallArgs {             // per assembly
+    FieldPtrs f;
+    Params p;
+    double *R, *Tint, *Fext, *Kval;
 };
 
 }  // namespace uvh

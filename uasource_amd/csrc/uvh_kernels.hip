@@ -9,12 +9,13 @@
 //                the CSR rows of K (UaMain/uvhMatrixAssembly.m:416-509); incident elements are
 //                summed in ascending element order, so results are bitwise reproducible.
 //
-// Lane mapping of k_element: a group of G lanes (G = 4, 8, 16 or 32, a power of two >=
-// max(nod, nip)) owns one element. Lanes q < nip compute the physics of integration point q
-// and publish a record of per-point coefficients in LDS; lanes i < nod then accumulate row i
-// of the element (6 residual terms, 9 x nod Jacobian terms) over all points in registers.
+// Thread mapping of k_element: a 256-thread block owns `epb` elements. Threads first gather
+// (element, node) pairs into LDS, then compute the physics of (element, integration point)
+// pairs and publish a record of per-point coefficients in LDS, then accumulate (element, row)
+// pairs: 6 residual terms and 9 x nod Jacobian terms per row, over all points, in registers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -68,6 +69,18 @@ __device__ __forceinline__ void corner_ids(int &c0, int &c1, int &c2) {
     else { c0 = 0; c1 = 3; c2 = 6; }
 }
 
+// Smooth step and its derivative at x = k (z - z0) (HeavisideApprox.m:101, DiracDelta.m:29):
+//   H = 1/2 + 1/2 tanh(x),   D = k/2 sech^2(x).
+// One expm1 gives both: with t = exp(-2|x|), tanh|x| = (1 - t)/(1 + t) and
+// sech^2 x = 4t/(1 + t)^2, each to a few ulp for every x.
+__device__ __forceinline__ void soft_step(double x, double k, double &H, double &D) {
+    const double em = expm1(-2.0 * fabs(x));  // t - 1, in (-1, 0]
+    const double den = 2.0 + em;              // 1 + t
+    const double th = copysign(-em / den, x);
+    H = 0.5 + 0.5 * th;
+    D = 0.5 * k * (4.0 * (1.0 + em) / (den * den));
+}
+
 // Physics of one integration point: fills the coefficient record c[kNC].
 template <int NOD>
 __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in LDS
@@ -94,7 +107,7 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     double as0 = 0, ab0 = 0, as1 = 0, ab1 = 0, dadh = 0, Bq = 0, Sq = 0, rho = 0;
     double dhdx = 0, dhdy = 0, dh0dx = 0, dh0dy = 0, exx0 = 0, eyy0 = 0, dbdx = 0, dbdy = 0;
     double drhodx = 0, drhody = 0, exx = 0, eyy = 0, exy2 = 0;
-#pragma unroll
+#pragma unroll 1
     for (int i = 0; i < NOD; ++i) {
         const double Ni = tN[i];
         const double nx = i11 * tXi[i] + i12 * tEta[i];
@@ -121,33 +134,40 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     }
     const double exy = 0.5 * exy2;
     const double a1 = as1 + ab1, a0 = as0 + ab0;
+    const double rrhow = 1.0 / p.rhow;
 
-    // floatation (uvhAssemblyIntPointImplicitSUPG.m:209-243; HeavisideApprox.m, DiracDelta.m)
+    // floatation (uvhAssemblyIntPointImplicitSUPG.m:209-243): He = H(h - hf), HE = H(hf - h),
+    // delta, Delta their derivatives; with Hh0 = 0 the second pair mirrors the first.
     const double H = Sq - Bq;
     const double hf = p.rhow * H / rho;
-    const double x1 = p.kH * ((h - hf) - p.Hh0);
-    const double x2 = p.kH * ((hf - h) - p.Hh0);
-    const double He = 0.5 + 0.5 * tanh(x1);
-    const double HE = 0.5 + 0.5 * tanh(x2);
-    const double s1c = 1.0 / cosh(x1), s2c = 1.0 / cosh(x2);
-    const double delta = 0.5 * p.kH * (s1c * s1c);
-    const double Delta = 0.5 * p.kH * (s2c * s2c);
-    const double Hpos = (0.5 + 0.5 * tanh(p.kH * (H - p.Hh0))) * H;
-    const double d = HE * rho * h / p.rhow + He * Hpos;
-    const double dd = HE * rho / p.rhow - Delta * h * rho / p.rhow + delta * Hpos;
+    double He, delta, HE, Delta, HH, unused;
+    soft_step(p.kH * ((h - hf) - p.Hh0), p.kH, He, delta);
+    if (p.Hh0 == 0.0) {
+        HE = 1.0 - He;
+        Delta = delta;
+    } else {
+        soft_step(p.kH * ((hf - h) - p.Hh0), p.kH, HE, Delta);
+    }
+    soft_step(p.kH * (H - p.Hh0), p.kH, HH, unused);
+    const double Hpos = HH * H;
+    const double d = HE * rho * h * rrhow + He * Hpos;
+    const double dd = HE * rho * rrhow - Delta * h * rho * rrhow + delta * Hpos;
 
-    // Glen's flow law (EffectiveViscositySSTREAM.m:7-9)
-    const double e = sqrt(p.eps0 * p.eps0 + exx * exx + eyy * eyy + exx * eyy + exy * exy);
-    const double Am = pow(A, -1.0 / n);
-    const double eta = 0.5 * Am * pow(e, (1.0 - n) / n) + p.eta0;
-    const double E = (1.0 - n) / (4.0 * n) * Am * pow(e, (1.0 - 3.0 * n) / n);
+    // Glen's flow law (EffectiveViscositySSTREAM.m:7-9) from logarithms:
+    //   A^(-1/n) e^((1-n)/n) = exp(-log(A)/n + (1-n)/n * log(e)),  E = (1-n)/(4n) * that / e^2
+    const double e2 = p.eps0 * p.eps0 + exx * exx + eyy * eyy + exx * eyy + exy * exy;
+    const double rn = 1.0 / n;
+    const double visc = exp(-log(A) * rn + (1.0 - n) * rn * (0.5 * log(e2)));
+    const double eta = 0.5 * visc + p.eta0;
+    const double E = 0.25 * (1.0 - n) * rn * visc / e2;
 
-    // Weertman sliding (BasalDrag.m:54-77,410-443)
+    // Weertman sliding (BasalDrag.m:54-77,410-443):
+    //   beta2 = (C+C0)^(-1/m) U^(1/m-1),  Dbeta2 = (1/m-1) beta2 / U^2
     const double Hec = He < 0 ? 0.0 : He, dlc = delta < 0 ? 0.0 : delta;
     const double U2 = u * u + v * v + p.uz * p.uz;
-    const double Cm = pow(C + p.czero, -1.0 / m);
-    const double beta2 = Cm * pow(sqrt(U2), 1.0 / m - 1.0);
-    const double Db = (1.0 / m - 1.0) * Cm * pow(U2, (1.0 - 3.0 * m) / (2.0 * m));
+    const double rm = 1.0 / m;
+    const double beta2 = exp(-log(C + p.czero) * rm + (rm - 1.0) * (0.5 * log(U2)));
+    const double Db = (rm - 1.0) * beta2 / U2;
     const double taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
     const double dtxdu = Hec * (beta2 + Db * u * u), dtydv = Hec * (beta2 + Db * v * v);
     const double dtxdv = Hec * Db * u * v;
@@ -214,79 +234,40 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     c[C_FH] = W * (dt * rho * ((1.0 - th) * a0 + th * a1) - dt * (th * qx1 + (1.0 - th) * qx0 + th * qy1 + (1.0 - th) * qy0));
 }
 
-template <int NOD, int G, int BLOCK, bool RONLY>
-__global__ __launch_bounds__(BLOCK) void k_element(int nele, int nip, const int32_t *__restrict__ conn,
-                                                 const double *__restrict__ xn, const double *__restrict__ yn,
-                                                 FieldPtrs f, const double *__restrict__ bnod,
-                                                 const ShapeTables *__restrict__ tab, Params p,
-                                                 double *__restrict__ EB, double *__restrict__ RE) {
-    constexpr int EPB = BLOCK / G;
-    extern __shared__ double lds[];
-    const int tstride = nip * NOD;
-    double *sN = lds;                       // [nip][NOD]
-    double *sXi = sN + tstride;
-    double *sEta = sXi + tstride;
-    double *sW = sEta + tstride;            // [nip]
-    double *work = lds + ((3 * tstride + nip + 1) & ~1);  // 16-byte aligned union:
-                                            // nodal [EPB][NV_COUNT][NOD] / coef [EPB][nip][kNC]
-
-    const int tid = threadIdx.x;
-    for (int k = tid; k < nip * NOD; k += BLOCK) {
-        const int q = k / NOD, i = k % NOD;
-        sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
+// Gather of node a of element e into the LDS staging record nv[NV_COUNT][NOD].
+template <int NOD>
+__device__ __forceinline__ void gather_node(const int32_t *__restrict__ conn, const double *__restrict__ xn,
+                                            const double *__restrict__ yn, const FieldPtrs &f,
+                                            const double *__restrict__ bnod, const Params &p, int64_t e, int a,
+                                            double *__restrict__ nv) {
+    const int k = conn[e * NOD + a];
+    double h = f.h[k], u = f.u[k], v = f.v[k], u0 = f.u0[k], v0 = f.v0[k];
+    double as1 = f.as1[k], ab1 = f.ab1[k], as0 = f.as0[k], ab0 = f.ab0[k];
+    const double h0 = f.h0[k];
+    if (p.zero_fields) {  // uvhMatrixAssembly.m:73-87
+        u = 0.0; v = 0.0; u0 = 0.0; v0 = 0.0; h = h0;
+        as1 = fabs(ab1) + 1.0; ab1 = fabs(ab1);
+        as0 = fabs(ab0) + 1.0; ab0 = fabs(ab0);
     }
-    if (tid < nip) sW[tid] = tab->w[tid];
+    nv[NV_H * NOD + a] = h; nv[NV_U * NOD + a] = u; nv[NV_V * NOD + a] = v;
+    nv[NV_S * NOD + a] = f.S[k]; nv[NV_B * NOD + a] = f.B[k]; nv[NV_RHO * NOD + a] = f.rho[k];
+    nv[NV_C * NOD + a] = f.C[k]; nv[NV_M * NOD + a] = f.m[k]; nv[NV_N * NOD + a] = f.n[k];
+    nv[NV_A * NOD + a] = f.A[k]; nv[NV_AS1 * NOD + a] = as1; nv[NV_AB1 * NOD + a] = ab1;
+    nv[NV_H0 * NOD + a] = h0; nv[NV_U0 * NOD + a] = u0; nv[NV_V0 * NOD + a] = v0;
+    nv[NV_AS0 * NOD + a] = as0; nv[NV_AB0 * NOD + a] = ab0;
+    nv[NV_DADH * NOD + a] = f.dadh ? f.dadh[k] : 0.0;
+    nv[NV_BL * NOD + a] = bnod[k];
+    nv[NV_X * NOD + a] = xn[k]; nv[NV_Y * NOD + a] = yn[k];
+}
 
-    const int el = tid / G, lane = tid % G;
-    const int e = blockIdx.x * EPB + el;
-    const bool valid = e < nele;
-
-    // stage 1: gather nodal values (lane i loads node i of its element)
-    double *nv = work + el * (NV_COUNT * NOD);
-    if (valid && lane < NOD) {
-        const int k = conn[(int64_t)e * NOD + lane];
-        double h = f.h[k], u = f.u[k], v = f.v[k], u0 = f.u0[k], v0 = f.v0[k];
-        double as1 = f.as1[k], ab1 = f.ab1[k], as0 = f.as0[k], ab0 = f.ab0[k];
-        const double h0 = f.h0[k];
-        if (p.zero_fields) {  // uvhMatrixAssembly.m:73-87
-            u = 0.0; v = 0.0; u0 = 0.0; v0 = 0.0; h = h0;
-            as1 = fabs(ab1) + 1.0; ab1 = fabs(ab1);
-            as0 = fabs(ab0) + 1.0; ab0 = fabs(ab0);
-        }
-        nv[NV_H * NOD + lane] = h; nv[NV_U * NOD + lane] = u; nv[NV_V * NOD + lane] = v;
-        nv[NV_S * NOD + lane] = f.S[k]; nv[NV_B * NOD + lane] = f.B[k]; nv[NV_RHO * NOD + lane] = f.rho[k];
-        nv[NV_C * NOD + lane] = f.C[k]; nv[NV_M * NOD + lane] = f.m[k]; nv[NV_N * NOD + lane] = f.n[k];
-        nv[NV_A * NOD + lane] = f.A[k]; nv[NV_AS1 * NOD + lane] = as1; nv[NV_AB1 * NOD + lane] = ab1;
-        nv[NV_H0 * NOD + lane] = h0; nv[NV_U0 * NOD + lane] = u0; nv[NV_V0 * NOD + lane] = v0;
-        nv[NV_AS0 * NOD + lane] = as0; nv[NV_AB0 * NOD + lane] = ab0;
-        nv[NV_DADH * NOD + lane] = f.dadh ? f.dadh[k] : 0.0;
-        nv[NV_BL * NOD + lane] = bnod[k];
-        nv[NV_X * NOD + lane] = xn[k]; nv[NV_Y * NOD + lane] = yn[k];
-    }
-    __syncthreads();
-
-    // stage 2: physics of integration point q = lane
-    double c[kNC];
-    const bool phys = valid && lane < nip;
-    if (phys) ip_physics<NOD>(nv, sN + lane * NOD, sXi + lane * NOD, sEta + lane * NOD, sW[lane], p, c);
-    __syncthreads();  // nodal staging is dead from here; the coefficient records reuse it
-    double *cr = work + el * (nip * kNC);
-    if (phys) {
-#pragma unroll
-        for (int k = 0; k < kNC; k += 2) {
-            double2 t;
-            t.x = c[k];
-            t.y = (k + 1 < C_COUNT) ? c[k + 1] : 0.0;
-            *reinterpret_cast<double2 *>(cr + lane * kNC + k) = t;
-        }
-    }
-    __syncthreads();
-
-    // stage 3: row lane i accumulates element row i over the integration points
-    if (!valid || lane >= NOD) return;
-    const int i = lane;
+// Row i of the element: 6 residual terms and (unless RONLY) 9 x NOD Jacobian terms,
+// accumulated over the integration points from the coefficient records cr[nip][kNC].
+template <int NOD, bool RONLY>
+__device__ __forceinline__ void element_row(int i, int nip, const double *__restrict__ cr,
+                                            const double *__restrict__ sN, const double *__restrict__ sXi,
+                                            const double *__restrict__ sEta, double (&Rr)[6],
+                                            double (&K)[RONLY ? 1 : 9][NOD]) {
     double Tx = 0, Fx = 0, Ty = 0, Fy = 0, Th = 0, Fh = 0;
-    double K[RONLY ? 1 : 9][NOD];
     if (!RONLY) {
 #pragma unroll
         for (int b = 0; b < 9; ++b)
@@ -320,7 +301,8 @@ __global__ __launch_bounds__(BLOCK) void k_element(int nele, int nip, const int3
             const double Xi = r[C_PX] * Nxi + r[C_PXY] * Nyi + r[C_PCX] * Ni;
             const double Yi = r[C_PY] * Nyi + r[C_PXY] * Nxi + r[C_PCY] * Ni;
             const double Wpsi = r[C_W] * psi;
-            const double AX = r[C_AX], AY = r[C_AY], RH = r[C_RH], GAM = r[C_GAM], RU = r[C_RU], RV = r[C_RV];
+            const double hAX = Wpsi * r[C_AX], hAY = Wpsi * r[C_AY], hRH = Wpsi * r[C_RH];
+            const double hG = Wpsi * r[C_GAM], hRU = Wpsi * r[C_RU], hRV = Wpsi * r[C_RV];
 #pragma unroll
             for (int j = 0; j < NOD; ++j) {
                 const double Nj = qN[j];
@@ -334,15 +316,78 @@ __global__ __launch_bounds__(BLOCK) void k_element(int nele, int nip, const int3
                 K[3][j] += o1 * Nxj + m2 * Nyj + m3 * aj + n4 * Nj;   // Kyu
                 K[4][j] += m1 * Nyj + m2 * Nxj + m3 * bj + m4 * Nj;   // Kyv
                 K[5][j] += Yi * Nj;                                    // Kyh
-                K[6][j] += Wpsi * (AX * Nj + RH * Nxj);                // Khu
-                K[7][j] += Wpsi * (AY * Nj + RH * Nyj);                // Khv
-                K[8][j] += Wpsi * (GAM * Nj + RU * Nxj + RV * Nyj);    // Khh
+                K[6][j] += hAX * Nj + hRH * Nxj;                       // Khu
+                K[7][j] += hAY * Nj + hRH * Nyj;                       // Khv
+                K[8][j] += hG * Nj + hRU * Nxj + hRV * Nyj;            // Khh
             }
         }
     }
-    const int64_t row = (int64_t)e * NOD + i;
+    Rr[0] = Tx; Rr[1] = Fx; Rr[2] = Ty; Rr[3] = Fy; Rr[4] = Th; Rr[5] = Fh;
+}
+
+// Per-element kernel: EPB elements per 256-thread block (runtime), dense thread mappings
+// for the three stages: (element, node) gathers, (element, point) physics, (element, row)
+// rows. The caller guarantees epb*NOD <= 256 and epb*nip <= 256.
+template <int NOD, bool RONLY>
+__global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, const int32_t *__restrict__ conn,
+                                                 const double *__restrict__ xn, const double *__restrict__ yn,
+                                                 FieldPtrs f, const double *__restrict__ bnod,
+                                                 const ShapeTables *__restrict__ tab, Params p,
+                                                 double *__restrict__ EB, double *__restrict__ RE) {
+    extern __shared__ double lds[];
+    const int tstride = nip * NOD;
+    double *sN = lds;                       // [nip][NOD]
+    double *sXi = sN + tstride;
+    double *sEta = sXi + tstride;
+    double *sW = sEta + tstride;            // [nip]
+    double *work = lds + ((3 * tstride + nip + 1) & ~1);  // 16-byte aligned union:
+                                            // nodal [epb][NV_COUNT][NOD] / coef [epb][nip][kNC]
+    const int tid = threadIdx.x;
+    for (int k = tid; k < tstride; k += 256) {
+        const int q = k / NOD, i = k - q * NOD;
+        sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
+    }
+    if (tid < nip) sW[tid] = tab->w[tid];
+    const int64_t e0 = (int64_t)blockIdx.x * epb;
+
+    // stage 1: gathers, one (element, node) per thread
+    if (tid < epb * NOD) {
+        const int el = tid / NOD, a = tid - el * NOD;
+        if (e0 + el < nele) gather_node<NOD>(conn, xn, yn, f, bnod, p, e0 + el, a, work + el * (NV_COUNT * NOD));
+    }
+    __syncthreads();
+
+    // stage 2: physics, one (element, integration point) per thread
+    double c[kNC];
+    const int pel = tid / nip, pq = tid - pel * nip;
+    const bool phys = tid < epb * nip && e0 + pel < nele;
+    if (phys) ip_physics<NOD>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
+                              sW[pq], p, c);
+    __syncthreads();  // the nodal staging is dead from here; the coefficient records reuse it
+    if (phys) {
+        double *dst = work + (pel * nip + pq) * kNC;
+#pragma unroll
+        for (int k = 0; k < kNC; k += 2) {
+            double2 t;
+            t.x = c[k];
+            t.y = (k + 1 < C_COUNT) ? c[k + 1] : 0.0;
+            *reinterpret_cast<double2 *>(dst + k) = t;
+        }
+    }
+    __syncthreads();
+
+    // stage 3: rows, one (element, row) per thread
+    if (tid >= epb * NOD) return;
+    const int el = tid / NOD, i = tid - el * NOD;
+    const int64_t e = e0 + el;
+    if (e >= nele) return;
+    double Rr[6];
+    double K[RONLY ? 1 : 9][NOD];
+    element_row<NOD, RONLY>(i, nip, work + el * nip * kNC, sN, sXi, sEta, Rr, K);
+    const int64_t row = e * NOD + i;
     double *re = RE + row * 6;
-    re[0] = Tx; re[1] = Fx; re[2] = Ty; re[3] = Fy; re[4] = Th; re[5] = Fh;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) re[k] = Rr[k];
     if (!RONLY) {
         double *eb = EB + row * (9 * NOD);
 #pragma unroll
@@ -352,7 +397,9 @@ __global__ __launch_bounds__(BLOCK) void k_element(int nele, int nip, const int3
     }
 }
 
-// One thread per node: R, Tint, Fext and the three CSR rows (x, y, h) of K.
+// Node rows: one wave per node. Lanes cover the node's 9*deg CSR entries (three rows of
+// 3*deg); each entry sums its incident elements' contributions in ascending element
+// order, so the result is bitwise reproducible. Writes are coalesced along the rows.
 template <int NOD>
 __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict__ inc_ptr,
                                               const int32_t *__restrict__ inc, const uint8_t *__restrict__ pos,
@@ -361,99 +408,221 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
                                               double *__restrict__ R, double *__restrict__ Tint,
                                               double *__restrict__ Fext, double *__restrict__ Kval, int check_nan,
                                               int *__restrict__ status) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (k >= nn) return;
     const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
-    double T0 = 0, T1 = 0, T2 = 0, F0 = 0, F1 = 0, F2 = 0;
-    for (int t = t0; t < t1; ++t) {
-        const double *re = RE + (int64_t)inc[t] * 6;
-        T0 += re[0]; F0 += re[1]; T1 += re[2]; F1 += re[3]; T2 += re[4]; F2 += re[5];
-    }
-    const double r0 = T0 - F0, r1 = T1 - F1, r2 = T2 - F2;
-    R[k] = r0; R[nn + k] = r1; R[2 * (int64_t)nn + k] = r2;
-    if (Tint) { Tint[k] = T0; Tint[nn + k] = T1; Tint[2 * (int64_t)nn + k] = T2; }
-    if (Fext) { Fext[k] = F0; Fext[nn + k] = F1; Fext[2 * (int64_t)nn + k] = F2; }
     int bad = 0;
-    if (check_nan && (isnan(r0) || isnan(r1) || isnan(r2))) bad |= 1;
+    if (lane < 3) {
+        double T = 0, F = 0;
+        for (int t = t0; t < t1; ++t) {
+            const double *re = RE + (int64_t)inc[t] * 6 + 2 * lane;
+            T += re[0];
+            F += re[1];
+        }
+        const double r = T - F;
+        R[(int64_t)lane * nn + k] = r;
+        if (Tint) Tint[(int64_t)lane * nn + k] = T;
+        if (Fext) Fext[(int64_t)lane * nn + k] = F;
+        if (check_nan && isnan(r)) bad |= 1;
+    }
     if (Kval) {
         const int64_t rp = rowptr_n[k];
         const int deg = (int)(rowptr_n[k + 1] - rp);
-        double *rows[3];
+        const int diag_pos = pos[(int64_t)inc[t0] * NOD + inc[t0] % NOD];
+        for (int idx = lane; idx < 9 * deg; idx += 64) {
+            const int bi = idx / (3 * deg);
+            const int rem = idx - bi * 3 * deg;
+            const int bj = rem / deg;
+            const int col = rem - bj * deg;
+            double s = 0.0;
+            for (int t = t0; t < t1; ++t) {
+                const int ea = inc[t];
+                const uint8_t *ps = pos + (int64_t)ea * NOD;
 #pragma unroll
-        for (int bi = 0; bi < 3; ++bi) {
-            rows[bi] = Kval + (int64_t)bi * 3 * nnzN + 3 * rp;
-            for (int c = 0; c < 3 * deg; ++c) rows[bi][c] = 0.0;
-        }
-        int diag = 0;
-        for (int t = t0; t < t1; ++t) {
-            const int ea = inc[t];
-            const uint8_t *ps = pos + (int64_t)ea * NOD;
-            const double *eb = EB + (int64_t)ea * (9 * NOD);
-            if (t == t0) diag = ps[ea % NOD];
-#pragma unroll
-            for (int bi = 0; bi < 3; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 3; ++bj)
-#pragma unroll
-                    for (int j = 0; j < NOD; ++j) rows[bi][bj * deg + ps[j]] += eb[(bi * 3 + bj) * NOD + j];
-        }
-        if (check_nan) {
-#pragma unroll
-            for (int bi = 0; bi < 3; ++bi)
-                if (isnan(rows[bi][bi * deg + diag])) bad |= 2;
+                for (int b = 0; b < NOD; ++b)
+                    if (ps[b] == col) s += EB[(int64_t)ea * (9 * NOD) + (bi * 3 + bj) * NOD + b];
+            }
+            Kval[(int64_t)bi * 3 * nnzN + 3 * rp + rem] = s;
+            if (check_nan && bj == bi && col == diag_pos && isnan(s)) bad |= 2;
         }
     }
     if (bad) atomicOr(status, bad);
 }
 
+// Fused assembly: one workgroup per patch. Per batch of `epb` elements: zero the window
+// slots of rows that start here, gather + physics + element rows (as k_element), add the
+// rows of owned nodes into their window slots in rank order (one rank per barrier phase,
+// so no two threads add to the same slot at once and every sum has a fixed order), then
+// write the rows whose last element was in this batch to K, R, Tint, Fext.
+template <int NOD, bool RONLY>
+__global__ __launch_bounds__(256) void k_fused(const MeshArgs *__restrict__ MA, const CallArgs *__restrict__ CA,
+                                               int nip, int epb, int work_doubles, int64_t nnzN, int nn,
+                                               int check_nan) {
+    extern __shared__ double lds[];
+    const int tstride = nip * NOD;
+    double *sN = lds;
+    double *sXi = sN + tstride;
+    double *sEta = sXi + tstride;
+    double *sW = sEta + tstride;
+    double *work = lds + ((3 * tstride + nip + 1) & ~1);
+    double *win = work + work_doubles;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        const ShapeTables *tab = MA->tab;
+        for (int k = tid; k < tstride; k += 256) {
+            const int q = k / NOD, i = k - q * NOD;
+            sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
+        }
+        if (tid < nip) sW[tid] = tab->w[tid];
+    }
+    int bad = 0;
+    const int b0 = MA->s.patch_batch_ptr[blockIdx.x], b1 = MA->s.patch_batch_ptr[blockIdx.x + 1];
+    for (int b = b0; b < b1; ++b) {
+        // rows that start in this batch: zero their slots (one wave per row)
+        {
+            const int z0 = MA->s.zero_ptr[b], z1 = MA->s.zero_ptr[b + 1];
+            for (int z = z0 + wave; z < z1; z += 4) {
+                double *w = win + MA->s.zero_slot[z];
+                const int len = MA->s.zero_len[z];
+                for (int k = lane; k < len; k += 64) w[k] = 0.0;
+            }
+        }
+        // gathers
+        const int32_t *belem = MA->s.batch_elem + (int64_t)b * epb;
+        if (tid < epb * NOD) {
+            const int el = tid / NOD, a = tid - el * NOD;
+            const int e = belem[el];
+            if (e >= 0)
+                gather_node<NOD>(MA->conn, MA->x, MA->y, CA->f, MA->b, CA->p, e, a, work + el * (NV_COUNT * NOD));
+        }
+        __syncthreads();
+        // physics
+        double c[kNC];
+        const int pel = tid / nip, pq = tid - pel * nip;
+        const bool phys = tid < epb * nip && belem[pel < epb ? pel : 0] >= 0;
+        if (phys) {
+            const Params p = CA->p;
+            ip_physics<NOD>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD, sW[pq],
+                            p, c);
+        }
+        __syncthreads();
+        if (phys) {
+            double *dst = work + (pel * nip + pq) * kNC;
+#pragma unroll
+            for (int k = 0; k < kNC; k += 2) {
+                double2 t;
+                t.x = c[k];
+                t.y = (k + 1 < C_COUNT) ? c[k + 1] : 0.0;
+                *reinterpret_cast<double2 *>(dst + k) = t;
+            }
+        }
+        __syncthreads();
+        // element rows of owned nodes
+        const int el = tid / NOD, i = tid - el * NOD;
+        int slot = -1, rank = 0, e = -1, deg = 0;
+        if (tid < epb * NOD) {
+            e = belem[el];
+            const int64_t ea = ((int64_t)b * epb + el) * NOD + i;
+            slot = e >= 0 ? MA->s.ea_slot[ea] : -1;
+            rank = MA->s.ea_rank[ea];
+        }
+        double Rr[6];
+        double K[RONLY ? 1 : 9][NOD];
+        if (slot >= 0) {
+            element_row<NOD, RONLY>(i, nip, work + el * nip * kNC, sN, sXi, sEta, Rr, K);
+            const int node = MA->conn[(int64_t)e * NOD + i];
+            deg = (int)(MA->rowptr_n[node + 1] - MA->rowptr_n[node]);
+        }
+        const int maxrank = MA->s.batch_maxrank[b];
+        const uint8_t *ps = MA->pos + ((int64_t)(e < 0 ? 0 : e) * NOD + i) * NOD;
+        for (int r = 0; r <= maxrank; ++r) {
+            if (slot >= 0 && rank == r) {
+                double *w = win + slot;
+                if (!RONLY) {
+#pragma unroll
+                    for (int j = 0; j < NOD; ++j) {
+                        const int cj = ps[j];
+#pragma unroll
+                        for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 3; ++bj) w[bi * 3 * deg + bj * deg + cj] += K[bi * 3 + bj][j];
+                    }
+                }
+                double *wr = w + 9 * deg;
+                wr[0] += Rr[0]; wr[1] += Rr[2]; wr[2] += Rr[4];   // Tx Ty Th
+                wr[3] += Rr[1]; wr[4] += Rr[3]; wr[5] += Rr[5];   // Fx Fy Fh
+            }
+            __syncthreads();
+        }
+        // rows whose last element was in this batch: write out (one wave per row)
+        {
+            const int f0 = MA->s.flush_ptr[b], f1 = MA->s.flush_ptr[b + 1];
+            double *Kval = CA->Kval;
+            for (int fl = f0 + wave; fl < f1; fl += 4) {
+                const int node = MA->s.flush_node[fl];
+                const double *w = win + MA->s.flush_slot[fl];
+                const int64_t rp = MA->rowptr_n[node];
+                const int dg = (int)(MA->rowptr_n[node + 1] - rp);
+                if (!RONLY && Kval) {
+#pragma unroll
+                    for (int bi = 0; bi < 3; ++bi) {
+                        double *dst = Kval + (int64_t)bi * 3 * nnzN + 3 * rp;
+                        const double *src = w + bi * 3 * dg;
+                        for (int k = lane; k < 3 * dg; k += 64) dst[k] = src[k];
+                    }
+                    // diagonal entry of block (lane, lane) (uvhMatrixAssembly.m:575-579)
+                    if (check_nan && lane < 3 && isnan(w[lane * 3 * dg + lane * dg + MA->diag[node]])) bad |= 2;
+                }
+                if (lane < 3) {
+                    const double T = w[9 * dg + lane], F = w[9 * dg + 3 + lane];
+                    const double r = T - F;
+                    CA->R[(int64_t)lane * nn + node] = r;
+                    if (CA->Tint) CA->Tint[(int64_t)lane * nn + node] = T;
+                    if (CA->Fext) CA->Fext[(int64_t)lane * nn + node] = F;
+                    if (check_nan && isnan(r)) bad |= 1;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (bad) atomicOr(MA->status, bad);
+}
+
 // ------------------------------------------------------------------ launchers
 
 namespace {
-template <int NOD, int G>
-size_t element_lds_bytes(int nip, int block) {
-    const int EPB = block / G;
-    const size_t tables = size_t((3 * nip * NOD + nip + 1) & ~1) * sizeof(double);
-    const size_t nodal = size_t(EPB) * NV_COUNT * NOD * sizeof(double);
-    const size_t coef = size_t(EPB) * nip * kNC * sizeof(double);
-    return tables + (nodal > coef ? nodal : coef);
-}
-
 constexpr size_t kLdsLimit = 64 * 1024;
 
-template <int NOD, int G>
-hipError_t launch_element_g(int nele, int nip, const int32_t *conn, const double *x, const double *y,
-                            const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
-                            bool ronly, double *EB, double *RE, hipStream_t s) {
-    if (element_lds_bytes<NOD, G>(nip, 256) <= kLdsLimit) {
-        constexpr int EPB = 256 / G;
-        const int grid = (nele + EPB - 1) / EPB;
-        const size_t lds = element_lds_bytes<NOD, G>(nip, 256);
-        if (ronly)
-            hipLaunchKernelGGL((k_element<NOD, G, 256, true>), dim3(grid), dim3(256), lds, s, nele, nip, conn, x, y, f, b, tab, p, EB, RE);
-        else
-            hipLaunchKernelGGL((k_element<NOD, G, 256, false>), dim3(grid), dim3(256), lds, s, nele, nip, conn, x, y, f, b, tab, p, EB, RE);
-    } else {
-        constexpr int EPB = 128 / G;
-        const int grid = (nele + EPB - 1) / EPB;
-        const size_t lds = element_lds_bytes<NOD, G>(nip, 128);
-        if (lds > kLdsLimit) return hipErrorInvalidConfiguration;
-        if (ronly)
-            hipLaunchKernelGGL((k_element<NOD, G, 128, true>), dim3(grid), dim3(128), lds, s, nele, nip, conn, x, y, f, b, tab, p, EB, RE);
-        else
-            hipLaunchKernelGGL((k_element<NOD, G, 128, false>), dim3(grid), dim3(128), lds, s, nele, nip, conn, x, y, f, b, tab, p, EB, RE);
-    }
-    return hipGetLastError();
+// Elements per 256-thread block: every stage maps at most one item per thread, and the
+// LDS (shape tables + the larger of nodal staging and coefficient records) stays <= 64 KiB.
+int element_epb(int nod, int nip) {
+    const size_t tables = size_t((3 * nip * nod + nip + 1) & ~1) * sizeof(double);
+    const size_t per_el = std::max(size_t(NV_COUNT) * nod, size_t(nip) * kNC) * sizeof(double);
+    int epb = std::min(256 / nip, 256 / nod);
+    epb = std::min<int>(epb, int((kLdsLimit - tables) / per_el));
+    return epb;
+}
+
+size_t element_lds_bytes(int nod, int nip, int epb) {
+    const size_t tables = size_t((3 * nip * nod + nip + 1) & ~1) * sizeof(double);
+    const size_t per_el = std::max(size_t(NV_COUNT) * nod, size_t(nip) * kNC) * sizeof(double);
+    return tables + per_el * epb;
 }
 
 template <int NOD>
 hipError_t launch_element_nod(int nele, int nip, const int32_t *conn, const double *x, const double *y,
                               const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
                               bool ronly, double *EB, double *RE, hipStream_t s) {
-    const int need = nip > NOD ? nip : NOD;
-    if (need <= 4) return launch_element_g<NOD, (NOD <= 4 ? 4 : 32)>(nele, nip, conn, x, y, f, b, tab, p, ronly, EB, RE, s);
-    if (need <= 8) return launch_element_g<NOD, (NOD <= 8 ? 8 : 32)>(nele, nip, conn, x, y, f, b, tab, p, ronly, EB, RE, s);
-    if (need <= 16) return launch_element_g<NOD, 16>(nele, nip, conn, x, y, f, b, tab, p, ronly, EB, RE, s);
-    return launch_element_g<NOD, 32>(nele, nip, conn, x, y, f, b, tab, p, ronly, EB, RE, s);
+    const int epb = element_epb(NOD, nip);
+    if (epb < 1) return hipErrorInvalidConfiguration;
+    const int grid = (nele + epb - 1) / epb;
+    const size_t lds = element_lds_bytes(NOD, nip, epb);
+    if (ronly)
+        hipLaunchKernelGGL((k_element<NOD, true>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, EB, RE);
+    else
+        hipLaunchKernelGGL((k_element<NOD, false>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, EB, RE);
+    return hipGetLastError();
 }
 }  // namespace
 
@@ -481,7 +650,7 @@ hipError_t launch_element(int nod, int nele, int nip, const int32_t *conn, const
 hipError_t launch_rows(int nod, int nn, const int32_t *inc_ptr, const int32_t *inc, const uint8_t *pos,
                        const int64_t *rowptr_n, int64_t nnzN, const double *EB, const double *RE, double *R,
                        double *Tint, double *Fext, double *Kval, int check_nan, int *status, hipStream_t s) {
-    const dim3 grid((nn + 255) / 256), block(256);
+    const dim3 grid((nn + 3) / 4), block(256);
     switch (nod) {
         case 3: hipLaunchKernelGGL(k_rows<3>, grid, block, 0, s, nn, inc_ptr, inc, pos, rowptr_n, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
         case 6: hipLaunchKernelGGL(k_rows<6>, grid, block, 0, s, nn, inc_ptr, inc, pos, rowptr_n, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
@@ -491,4 +660,37 @@ hipError_t launch_rows(int nod, int nn, const int32_t *inc_ptr, const int32_t *i
     return hipGetLastError();
 }
 
+// LDS layout of the fused kernel: shape tables, per-batch work area, row window.
+FusedLayout fused_layout(int nod, int nip) {
+    FusedLayout L{};
+    L.table_doubles = (3 * nip * nod + nip + 1) & ~1;
+    L.epb = element_epb(nod, nip);
+    const int per_el = std::max(NV_COUNT * nod, nip * kNC);
+    L.work_doubles = (L.epb * per_el + 1) & ~1;
+    const int64_t total = kFusedLdsBytes / int64_t(sizeof(double));
+    L.window_doubles = ((total - L.table_doubles - L.work_doubles) / 2) * 2;
+    return L;
+}
+
+hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
+                        int64_t nnzN, int nn, bool ronly, int check_nan, hipStream_t s) {
+    if (npatch <= 0) return hipSuccess;
+    const size_t lds = size_t(L.table_doubles + L.work_doubles + L.window_doubles) * sizeof(double);
+    const dim3 grid(npatch), block(256);
+#define UVH_FUSED(NODV, RO)                                                                                     \
+    do {                                                                                                        \
+        auto kern = k_fused<NODV, RO>;                                                                          \
+        hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                            int(lds));                                                                          \
+        hipLaunchKernelGGL(kern, grid, block, lds, s, MA, CA, nip, L.epb, L.work_doubles, nnzN, nn, check_nan);  \
+    } while (0)
+    switch (nod) {
+        case 3: if (ronly) UVH_FUSED(3, true); else UVH_FUSED(3, false); break;
+        case 6: if (ronly) UVH_FUSED(6, true); else UVH_FUSED(6, false); break;
+        case 10: if (ronly) UVH_FUSED(10, true); else UVH_FUSED(10, false); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef UVH_FUSED
+    return hipGetLastError();
+}
 }  // namespace uvh

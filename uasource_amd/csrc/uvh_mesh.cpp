@@ -125,6 +125,12 @@ std::string build_mesh_host(MeshHost &m) {
         adjacency_of(k, buf);
         std::copy(buf.begin(), buf.end(), m.colind_n.begin() + m.rowptr_n[k]);
     });
+    m.diag.assign(size_t(nn), 0);
+    for (int32_t k = 0; k < nn; ++k) {
+        const int32_t *c0 = m.colind_n.data() + m.rowptr_n[k];
+        const int32_t *c1 = m.colind_n.data() + m.rowptr_n[size_t(k) + 1];
+        m.diag[k] = uint8_t(std::lower_bound(c0, c1, k) - c0);
+    }
     // placement: column position of node b within row conn[e][a]
     m.pos.assign(size_t(nele) * nod * nod, 0);
     {
