@@ -216,19 +216,22 @@ def test_full_size_plain_p2_parity(ctx, o2lib):
     assert_parity(ctx, M, gpu, ref)
 
 
-def test_two_pass_path_matches_fused_path(o2lib, monkeypatch):
-    """A/B check in the manner of the reference's CtrlVar.Parallel.isTest: the element-rows-
-    through-HBM path (UVH_TWO_PASS=1) and the fused LDS-window path both meet the oracle."""
-    M, F0, F1, BCs = mesh_fields("medium_p2")
+@pytest.mark.parametrize("name", ["tiny_p1", "medium_p2", "medium_p3", "plain_p2"])
+def test_fused_path_matches_oracle(o2lib, monkeypatch, name):
+    """A/B check in the manner of the reference's CtrlVar.Parallel.isTest: the fused
+    LDS-window path (UVH_ASSEMBLY=fused) meets the oracle like the default two-pass path."""
+    M, F0, F1, BCs = mesh_fields(name)
     cv = default_ctrlvar()
-    ref = o2lib.assemble(cv, M, F0, F1)
-    monkeypatch.setenv("UVH_TWO_PASS", "1")
+    ref = o2lib.assemble(cv, M, F0, F1, nthreads=16)
+    monkeypatch.setenv("UVH_ASSEMBLY", "fused")
     c2 = uvh.UvhContext(0)
-    monkeypatch.delenv("UVH_TWO_PASS")
+    monkeypatch.delenv("UVH_ASSEMBLY")
     try:
-        say("[gpu] two-pass path medium_p2")
+        say(f"[gpu] fused path {name}")
         c2.set_mesh(M)
         out2 = c2.assemble(cv, F0, F1, want_tf=True)
         assert_parity(c2, M, out2, ref)
+        r_only = c2.assemble(cv, F0, F1, ronly=True)
+        assert np.array_equal(r_only["R"], out2["R"])
     finally:
         c2.close()

@@ -16,11 +16,12 @@ namespace uvh {
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s);
 hipError_t launch_clear_status(int *status, hipStream_t s);
 hipError_t launch_element(int nod, int nele, int nip, const int32_t *conn, const double *x, const double *y,
-                          const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p, bool ronly,
-                          double *EB, double *RE, hipStream_t s);
-hipError_t launch_rows(int nod, int nn, const int32_t *inc_ptr, const int32_t *inc, const uint8_t *pos,
-                       const int64_t *rowptr_n, int64_t nnzN, const double *EB, const double *RE, double *R,
-                       double *Tint, double *Fext, double *Kval, int check_nan, int *status, hipStream_t s);
+                          const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
+                          const int32_t *inc_pos, bool ronly, double *EB, double *RE, hipStream_t s);
+hipError_t launch_rows(int nod, int nn, int maxdeg, const int32_t *inc_ptr, const int32_t *inc,
+                       const uint8_t *pos, const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN,
+                       const double *EB, const double *RE, double *R, double *Tint, double *Fext, double *Kval,
+                       int check_nan, int *status, hipStream_t s);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
                         int64_t nnzN, int nn, bool ronly, int check_nan, hipStream_t s);
 }  // namespace uvh
@@ -59,9 +60,10 @@ struct uvh_ctx {
     uvh::ShapeTables *d_tab = nullptr;
     int64_t *d_rowptr_n = nullptr;
     int32_t *d_inc_ptr = nullptr, *d_inc = nullptr;
-    uint8_t *d_pos = nullptr, *d_diag = nullptr;
+    uint8_t *d_pos = nullptr, *d_diag = nullptr, *d_pos_inc = nullptr;
+    int32_t *d_inc_pos = nullptr;
     // fused-assembly schedule
-    bool two_pass = false;  // UVH_TWO_PASS=1: element rows through HBM + per-node sums (A/B checks)
+    bool two_pass = true;   // false: fused LDS-window kernel (UVH_ASSEMBLY=fused)
     uvh::Schedule sched;
     uvh::FusedLayout layout{};
     int32_t *d_pbp = nullptr, *d_belem = nullptr, *d_easlot = nullptr, *d_zptr = nullptr, *d_zslot = nullptr;
@@ -101,8 +103,8 @@ struct uvh_ctx {
         hipFree(d_fields); hipFree(d_R); hipFree(d_T); hipFree(d_F); hipFree(d_K);
         hipFree(d_diag); hipFree(d_pbp); hipFree(d_belem); hipFree(d_easlot); hipFree(d_zptr); hipFree(d_zslot);
         hipFree(d_zlen); hipFree(d_fptr); hipFree(d_fnode); hipFree(d_fslot); hipFree(d_earank); hipFree(d_bmax);
-        hipFree(d_margs);
-        d_margs = nullptr;
+        hipFree(d_margs); hipFree(d_pos_inc); hipFree(d_inc_pos);
+        d_margs = nullptr; d_pos_inc = nullptr; d_inc_pos = nullptr;
         last_slot = -1;
         d_diag = d_earank = d_bmax = nullptr;
         d_pbp = d_belem = d_easlot = d_zptr = d_zslot = d_zlen = d_fptr = d_fnode = d_fslot = nullptr;
@@ -185,8 +187,10 @@ int uvh_create(uvh_ctx **out, int device) {
     if (hipSetDevice(device) != hipSuccess) return UVH_E_HIP;
     auto *ctx = new uvh_ctx();
     ctx->device = device;
-    const char *tp = std::getenv("UVH_TWO_PASS");
-    ctx->two_pass = tp && tp[0] == '1';
+    // UVH_ASSEMBLY=fused selects the single-kernel LDS-window assembly; the default is the
+    // two-pass path (element rows through HBM, then one wave per node).
+    const char *am = std::getenv("UVH_ASSEMBLY");
+    ctx->two_pass = !(am && std::strcmp(am, "fused") == 0);
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
     if (e == hipSuccess)
@@ -269,7 +273,11 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(dalloc(&ctx->d_fptr, S.flush_ptr.size()));
     chk(dalloc(&ctx->d_fnode, std::max<size_t>(1, S.flush_node.size())));
     chk(dalloc(&ctx->d_fslot, std::max<size_t>(1, S.flush_slot.size())));
-    if (ctx->two_pass) chk(dalloc(&ctx->d_RE, ne * nod * 6));
+    if (ctx->two_pass) {
+        chk(dalloc(&ctx->d_RE, ne * nod * 6));
+        chk(dalloc(&ctx->d_inc_pos, ne * nod));
+        chk(dalloc(&ctx->d_pos_inc, ne * nod * nod));
+    }
     if (e != hipSuccess) {
         ctx->free_mesh();
         return ctx->fail(UVH_E_NOMEM, std::string("uvh_set_mesh: device allocation failed: ") + hipGetErrorString(e));
@@ -283,6 +291,10 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(hipMemcpy(ctx->d_inc, m.inc.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_pos, m.pos.data(), ne * nod * nod, hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_diag, m.diag.data(), nn, hipMemcpyHostToDevice));
+    if (ctx->two_pass) {
+        chk(hipMemcpy(ctx->d_inc_pos, m.inc_pos.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
+        chk(hipMemcpy(ctx->d_pos_inc, m.pos_inc.data(), ne * nod * nod, hipMemcpyHostToDevice));
+    }
     auto up = [&](auto *dst, const auto &v) {
         if (!v.empty()) chk(hipMemcpy(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
     };
@@ -382,12 +394,13 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     if (e == hipSuccess)
         e = ctx->timed(ronly ? "element_r" : "element_rk", s, [&] {
             return uvh::launch_element(m.nod, m.nele, m.nip, ctx->d_conn, ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab,
-                                       p, ronly, ctx->d_EB, ctx->d_RE, s);
+                                       p, ctx->d_inc_pos, ronly, ctx->d_EB, ctx->d_RE, s);
         });
     if (e == hipSuccess)
         e = ctx->timed(ronly ? "rows_r" : "rows_rk", s, [&] {
-            return uvh::launch_rows(m.nod, m.nn, ctx->d_inc_ptr, ctx->d_inc, ctx->d_pos, ctx->d_rowptr_n, m.nnzN(),
-                                    ctx->d_EB, ctx->d_RE, dR, dT, dF, ronly ? nullptr : dK, check, ctx->d_status, s);
+            return uvh::launch_rows(m.nod, m.nn, m.maxdeg, ctx->d_inc_ptr, ctx->d_inc, ctx->d_pos_inc, ctx->d_rowptr_n,
+                                    ctx->d_diag, m.nnzN(), ctx->d_EB, ctx->d_RE, dR, dT, dF, ronly ? nullptr : dK,
+                                    check, ctx->d_status, s);
         });
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: kernel launch");
     return UVH_OK;

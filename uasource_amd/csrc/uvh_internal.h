@@ -33,6 +33,9 @@ struct MeshHost {
     std::vector<int32_t> inc;         // e*nod + a, in element order per node
     std::vector<uint8_t> pos;         // nele*nod*nod: pos[(e*nod+a)*nod+b] = index of conn[e][b] in adj(conn[e][a])
     std::vector<uint8_t> diag;        // nn: index of node k in adj(k)
+    std::vector<int32_t> inc_pos;     // nele*nod: position of (e, a) in the incidence order (inverse of inc)
+    std::vector<uint8_t> pos_inc;     // nele*nod*nod: pos rows in the incidence order
+    int32_t maxdeg = 0;               // largest node degree
     ShapeTables tab{};
     int64_t nnzN() const { return rowptr_n.empty() ? 0 : rowptr_n.back(); }
 };

@@ -69,6 +69,14 @@ __device__ __forceinline__ void corner_ids(int &c0, int &c1, int &c2) {
     else { c0 = 0; c1 = 3; c2 = 6; }
 }
 
+// LDS add without returning the old value (ds_add_f64). Callers guarantee that the lanes of
+// one call never target the same address and that successive calls to one address come
+// from the same wave in program order or are separated by a barrier, so the order of the
+// additions, and the rounded result, is fixed.
+__device__ __forceinline__ void lds_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Smooth step and its derivative at x = k (z - z0) (HeavisideApprox.m:101, DiracDelta.m:29):
 //   H = 1/2 + 1/2 tanh(x),   D = k/2 sech^2(x).
 // One expm1 gives both: with t = exp(-2|x|), tanh|x| = (1 - t)/(1 + t) and
@@ -325,6 +333,86 @@ __device__ __forceinline__ void element_row(int i, int nip, const double *__rest
     Rr[0] = Tx; Rr[1] = Fx; Rr[2] = Ty; Rr[3] = Fy; Rr[4] = Th; Rr[5] = Fh;
 }
 
+// Row i, block row BI (0: x, 1: y, 2: h) of the element: T and F of that row and its
+// 3 x NOD Jacobian blocks, accumulated over the integration points.
+template <int NOD, int BI>
+__device__ __forceinline__ void element_row_b(int i, int nip, const double *__restrict__ cr,
+                                              const double *__restrict__ sN, const double *__restrict__ sXi,
+                                              const double *__restrict__ sEta, double &Tr, double &Fr,
+                                              double (&K)[3][NOD]) {
+    double T = 0, F = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int j = 0; j < NOD; ++j) K[b][j] = 0.0;
+    for (int q = 0; q < nip; ++q) {
+        const double *r = cr + q * kNC;
+        const double *qN = sN + q * NOD, *qXi = sXi + q * NOD, *qEta = sEta + q * NOD;
+        const double I11 = r[C_I11], I12 = r[C_I12], I21 = r[C_I21], I22 = r[C_I22];
+        const double S1 = r[C_S1], S2 = r[C_S2], EXY = r[C_EXY];
+        const double Ni = qN[i];
+        const double Nxi = I11 * qXi[i] + I12 * qEta[i];
+        const double Nyi = I21 * qXi[i] + I22 * qEta[i];
+        const double wheta = r[C_WHETA];
+        if (BI == 0) {
+            const double ai = S1 * Nxi + EXY * Nyi;
+            T += 2.0 * wheta * ai + r[C_WTX] * Ni;
+            F += r[C_WF1X] * Ni + r[C_WF2] * Nxi;
+            const double whE2 = r[C_WHE2];
+            const double l1 = 4.0 * wheta * Nxi, l2 = wheta * Nyi, l3 = whE2 * ai, l4 = r[C_WDTXDU] * Ni;
+            const double n1 = 2.0 * wheta * Nxi, n4 = r[C_WDTXDV] * Ni;
+            const double Xi = r[C_PX] * Nxi + r[C_PXY] * Nyi + r[C_PCX] * Ni;
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) {
+                const double Nj = qN[j];
+                const double Nxj = I11 * qXi[j] + I12 * qEta[j];
+                const double Nyj = I21 * qXi[j] + I22 * qEta[j];
+                const double aj = S1 * Nxj + EXY * Nyj;
+                const double bj = S2 * Nyj + EXY * Nxj;
+                K[0][j] += l1 * Nxj + l2 * Nyj + l3 * aj + l4 * Nj;
+                K[1][j] += n1 * Nyj + l2 * Nxj + l3 * bj + n4 * Nj;
+                K[2][j] += Xi * Nj;
+            }
+        } else if (BI == 1) {
+            const double bi = S2 * Nyi + EXY * Nxi;
+            T += 2.0 * wheta * bi + r[C_WTY] * Ni;
+            F += r[C_WF1Y] * Ni + r[C_WF2] * Nyi;
+            const double whE2 = r[C_WHE2];
+            const double m1 = 4.0 * wheta * Nyi, m2 = wheta * Nxi, m3 = whE2 * bi, m4 = r[C_WDTYDV] * Ni;
+            const double o1 = 2.0 * wheta * Nyi, n4 = r[C_WDTXDV] * Ni;
+            const double Yi = r[C_PY] * Nyi + r[C_PXY] * Nxi + r[C_PCY] * Ni;
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) {
+                const double Nj = qN[j];
+                const double Nxj = I11 * qXi[j] + I12 * qEta[j];
+                const double Nyj = I21 * qXi[j] + I22 * qEta[j];
+                const double aj = S1 * Nxj + EXY * Nyj;
+                const double bj = S2 * Nyj + EXY * Nxj;
+                K[0][j] += o1 * Nxj + m2 * Nyj + m3 * aj + n4 * Nj;
+                K[1][j] += m1 * Nyj + m2 * Nxj + m3 * bj + m4 * Nj;
+                K[2][j] += Yi * Nj;
+            }
+        } else {
+            const double psi = Ni + r[C_TU] * Nxi + r[C_TV] * Nyi;
+            T += r[C_TH] * psi;
+            F += r[C_FH] * psi;
+            const double Wpsi = r[C_W] * psi;
+            const double hAX = Wpsi * r[C_AX], hAY = Wpsi * r[C_AY], hRH = Wpsi * r[C_RH];
+            const double hG = Wpsi * r[C_GAM], hRU = Wpsi * r[C_RU], hRV = Wpsi * r[C_RV];
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) {
+                const double Nj = qN[j];
+                const double Nxj = I11 * qXi[j] + I12 * qEta[j];
+                const double Nyj = I21 * qXi[j] + I22 * qEta[j];
+                K[0][j] += hAX * Nj + hRH * Nxj;
+                K[1][j] += hAY * Nj + hRH * Nyj;
+                K[2][j] += hG * Nj + hRU * Nxj + hRV * Nyj;
+            }
+        }
+    }
+    Tr = T; Fr = F;
+}
+
 // Per-element kernel: EPB elements per 256-thread block (runtime), dense thread mappings
 // for the three stages: (element, node) gathers, (element, point) physics, (element, row)
 // rows. The caller guarantees epb*NOD <= 256 and epb*nip <= 256.
@@ -333,6 +421,7 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
                                                  const double *__restrict__ xn, const double *__restrict__ yn,
                                                  FieldPtrs f, const double *__restrict__ bnod,
                                                  const ShapeTables *__restrict__ tab, Params p,
+                                                 const int32_t *__restrict__ inc_pos,
                                                  double *__restrict__ EB, double *__restrict__ RE) {
     extern __shared__ double lds[];
     const int tstride = nip * NOD;
@@ -381,84 +470,154 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     const int el = tid / NOD, i = tid - el * NOD;
     const int64_t e = e0 + el;
     if (e >= nele) return;
-    double Rr[6];
-    double K[RONLY ? 1 : 9][NOD];
-    element_row<NOD, RONLY>(i, nip, work + el * nip * kNC, sN, sXi, sEta, Rr, K);
-    const int64_t row = e * NOD + i;
+    // the three block rows one after the other: 3*NOD accumulators instead of 9*NOD;
+    // the row is stored at its position in the node-incidence order
+    const int64_t row = inc_pos[e * NOD + i];
     double *re = RE + row * 6;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) re[k] = Rr[k];
+    double *eb = EB + row * (9 * NOD);
+    const double *crr = work + el * nip * kNC;
+    double T, F, Kb[3][NOD];
+    element_row_b<NOD, 0>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+    re[0] = T; re[1] = F;
     if (!RONLY) {
-        double *eb = EB + row * (9 * NOD);
 #pragma unroll
-        for (int b = 0; b < 9; ++b)
+        for (int b = 0; b < 3; ++b)
 #pragma unroll
-            for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = K[b][j];
+            for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = Kb[b][j];
+    }
+    element_row_b<NOD, 1>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+    re[2] = T; re[3] = F;
+    if (!RONLY) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) eb[(3 + b) * NOD + j] = Kb[b][j];
+    }
+    element_row_b<NOD, 2>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+    re[4] = T; re[5] = F;
+    if (!RONLY) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) eb[(6 + b) * NOD + j] = Kb[b][j];
     }
 }
 
-// Node rows: one wave per node. Lanes cover the node's 9*deg CSR entries (three rows of
-// 3*deg); each entry sums its incident elements' contributions in ascending element
-// order, so the result is bitwise reproducible. Writes are coalesced along the rows.
+// Node rows: one wave per node. The incident element rows (stored contiguously per node, in
+// incidence order, by k_element) are added, in ascending element order, into a per-wave LDS buffer that
+// holds the node's three CSR rows (9*deg values); within one element row no two values
+// share a column, so every sum has a fixed order and the result is bitwise reproducible.
+// The buffer is then written to K coalesced along the rows.
 template <int NOD>
-__global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict__ inc_ptr,
+__global__ __launch_bounds__(256) void k_rows(int nn, int maxdeg, const int32_t *__restrict__ inc_ptr,
                                               const int32_t *__restrict__ inc, const uint8_t *__restrict__ pos,
-                                              const int64_t *__restrict__ rowptr_n, int64_t nnzN,
-                                              const double *__restrict__ EB, const double *__restrict__ RE,
-                                              double *__restrict__ R, double *__restrict__ Tint,
-                                              double *__restrict__ Fext, double *__restrict__ Kval, int check_nan,
-                                              int *__restrict__ status) {
-    const int lane = threadIdx.x & 63;
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (k >= nn) return;
+                                              const int64_t *__restrict__ rowptr_n, const uint8_t *__restrict__ diag,
+                                              int64_t nnzN, const double *__restrict__ EB,
+                                              const double *__restrict__ RE, double *__restrict__ R,
+                                              double *__restrict__ Tint, double *__restrict__ Fext,
+                                              double *__restrict__ Kval, int check_nan, int *__restrict__ status) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 4 + wave;
+    if (k >= nn) return;  // whole wave; no block-wide barriers below
+    double *buf = lds + wave * 9 * maxdeg;
     const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
     int bad = 0;
-    if (lane < 3) {
-        double T = 0, F = 0;
-        for (int t = t0; t < t1; ++t) {
-            const double *re = RE + (int64_t)inc[t] * 6 + 2 * lane;
-            T += re[0];
-            F += re[1];
+    if (lane < 6) {
+        // lane 2*bi: T of block row bi, lane 2*bi+1: F
+        double v = 0;
+        for (int t = t0; t < t1; ++t) v += RE[(int64_t)t * 6 + lane];
+        const int bi = lane >> 1;
+        if (lane & 1) {
+            if (Fext) Fext[(int64_t)bi * nn + k] = v;
+        } else {
+            if (Tint) Tint[(int64_t)bi * nn + k] = v;
         }
-        const double r = T - F;
-        R[(int64_t)lane * nn + k] = r;
-        if (Tint) Tint[(int64_t)lane * nn + k] = T;
-        if (Fext) Fext[(int64_t)lane * nn + k] = F;
-        if (check_nan && isnan(r)) bad |= 1;
+        const double other = __shfl_xor(v, 1, 64);
+        if (!(lane & 1)) {
+            const double r = v - other;
+            R[(int64_t)bi * nn + k] = r;
+            if (check_nan && isnan(r)) bad |= 1;
+        }
     }
     if (Kval) {
         const int64_t rp = rowptr_n[k];
         const int deg = (int)(rowptr_n[k + 1] - rp);
-        const int diag_pos = pos[(int64_t)inc[t0] * NOD + inc[t0] % NOD];
-        for (int idx = lane; idx < 9 * deg; idx += 64) {
-            const int bi = idx / (3 * deg);
-            const int rem = idx - bi * 3 * deg;
-            const int bj = rem / deg;
-            const int col = rem - bj * deg;
-            double s = 0.0;
-            for (int t = t0; t < t1; ++t) {
-                const int ea = inc[t];
-                const uint8_t *ps = pos + (int64_t)ea * NOD;
+        for (int v = lane; v < 9 * deg; v += 64) buf[v] = 0.0;
+        // the node's element rows t0..t1-1 are contiguous (incidence order); up to 8 at a
+        // time: all loads first, then the adds in ascending element order
+        constexpr int NV = 9 * NOD;               // values per element row
+        constexpr int VPL = (NV + 63) / 64;       // values per lane
+        for (int c0 = t0; c0 < t1; c0 += 8) {
+            const int cnt = min(8, t1 - c0);
+            double vals[8][VPL];
+            int cols[8][VPL];
 #pragma unroll
-                for (int b = 0; b < NOD; ++b)
-                    if (ps[b] == col) s += EB[(int64_t)ea * (9 * NOD) + (bi * 3 + bj) * NOD + b];
+            for (int c = 0; c < 8; ++c) {
+#pragma unroll
+                for (int u = 0; u < VPL; ++u) {
+                    const int v = lane + 64 * u;
+                    if (c < cnt && v < NV) {
+                        const int bk = v / NOD, j = v - bk * NOD;
+                        vals[c][u] = EB[(int64_t)(c0 + c) * NV + v];
+                        cols[c][u] = pos[(int64_t)(c0 + c) * NOD + j];
+                    }
+                }
             }
-            Kval[(int64_t)bi * 3 * nnzN + 3 * rp + rem] = s;
-            if (check_nan && bj == bi && col == diag_pos && isnan(s)) bad |= 2;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+#pragma unroll
+                for (int u = 0; u < VPL; ++u) {
+                    const int v = lane + 64 * u;
+                    if (c < cnt && v < NV) {
+                        const int bk = v / NOD;
+                        const int bi = bk / 3, bj = bk - bi * 3;
+                        lds_add(buf + bi * 3 * deg + bj * deg + cols[c][u], vals[c][u]);
+                    }
+                }
+            }
         }
+        for (int v = lane; v < 9 * deg; v += 64) {
+            const int bi = v / (3 * deg);
+            Kval[(int64_t)bi * 3 * nnzN + 3 * rp + (v - bi * 3 * deg)] = buf[v];
+        }
+        if (check_nan && lane < 3 && isnan(buf[lane * 3 * deg + lane * deg + diag[k]])) bad |= 2;
     }
     if (bad) atomicOr(status, bad);
 }
 
-// Fused assembly: one workgroup per patch. Per batch of `epb` elements: zero the window
-// slots of rows that start here, gather + physics + element rows (as k_element), add the
-// rows of owned nodes into their window slots in rank order (one rank per barrier phase,
-// so no two threads add to the same slot at once and every sum has a fixed order), then
-// write the rows whose last element was in this batch to K, R, Tint, Fext.
+// Residual only: one thread per node sums its incident element residual rows (contiguous).
+__global__ __launch_bounds__(256) void k_rows_r(int nn, const int32_t *__restrict__ inc_ptr,
+                                                const int32_t *__restrict__ inc, const double *__restrict__ RE,
+                                                double *__restrict__ R, double *__restrict__ Tint,
+                                                double *__restrict__ Fext, int check_nan, int *__restrict__ status) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    double T0 = 0, T1 = 0, T2 = 0, F0 = 0, F1 = 0, F2 = 0;
+    for (int t = inc_ptr[k]; t < inc_ptr[k + 1]; ++t) {
+        const double *re = RE + (int64_t)t * 6;
+        T0 += re[0]; F0 += re[1]; T1 += re[2]; F1 += re[3]; T2 += re[4]; F2 += re[5];
+    }
+    const double r0 = T0 - F0, r1 = T1 - F1, r2 = T2 - F2;
+    R[k] = r0; R[nn + k] = r1; R[2 * (int64_t)nn + k] = r2;
+    if (Tint) { Tint[k] = T0; Tint[nn + k] = T1; Tint[2 * (int64_t)nn + k] = T2; }
+    if (Fext) { Fext[k] = F0; Fext[nn + k] = F1; Fext[2 * (int64_t)nn + k] = F2; }
+    if (check_nan && (isnan(r0) || isnan(r1) || isnan(r2))) atomicOr(status, 1);
+}
+
+// Fused assembly: one 512-thread workgroup per patch. Per batch of `epb` elements: zero the
+// window slots of rows that start here, gather nodal values, compute the integration-point
+// physics, compute the element rows as (element, node, block row) items, add the rows of
+// owned nodes into their window slots in rank order (one rank per barrier phase, so no two
+// threads add to the same slot at once and every sum has a fixed order), then write the rows
+// whose last element was in this batch to K, R, Tint and Fext.
+constexpr int kFusedThreads = 512;
+constexpr int kFusedWaves = kFusedThreads / 64;
+
 template <int NOD, bool RONLY>
-__global__ __launch_bounds__(256) void k_fused(const MeshArgs *__restrict__ MA, const CallArgs *__restrict__ CA,
-                                               int nip, int epb, int work_doubles, int64_t nnzN, int nn,
-                                               int check_nan) {
+__global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restrict__ MA,
+                                                         const CallArgs *__restrict__ CA, int nip, int epb,
+                                                         int work_doubles, int64_t nnzN, int nn, int check_nan) {
     extern __shared__ double lds[];
     const int tstride = nip * NOD;
     double *sN = lds;
@@ -470,19 +629,20 @@ __global__ __launch_bounds__(256) void k_fused(const MeshArgs *__restrict__ MA, 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     {
         const ShapeTables *tab = MA->tab;
-        for (int k = tid; k < tstride; k += 256) {
+        for (int k = tid; k < tstride; k += kFusedThreads) {
             const int q = k / NOD, i = k - q * NOD;
             sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
         }
         if (tid < nip) sW[tid] = tab->w[tid];
     }
     int bad = 0;
+    const int nrow = epb * NOD;  // (element, node) pairs per batch
     const int b0 = MA->s.patch_batch_ptr[blockIdx.x], b1 = MA->s.patch_batch_ptr[blockIdx.x + 1];
     for (int b = b0; b < b1; ++b) {
         // rows that start in this batch: zero their slots (one wave per row)
         {
             const int z0 = MA->s.zero_ptr[b], z1 = MA->s.zero_ptr[b + 1];
-            for (int z = z0 + wave; z < z1; z += 4) {
+            for (int z = z0 + wave; z < z1; z += kFusedWaves) {
                 double *w = win + MA->s.zero_slot[z];
                 const int len = MA->s.zero_len[z];
                 for (int k = lane; k < len; k += 64) w[k] = 0.0;
@@ -490,7 +650,7 @@ __global__ __launch_bounds__(256) void k_fused(const MeshArgs *__restrict__ MA, 
         }
         // gathers
         const int32_t *belem = MA->s.batch_elem + (int64_t)b * epb;
-        if (tid < epb * NOD) {
+        if (tid < nrow) {
             const int el = tid / NOD, a = tid - el * NOD;
             const int e = belem[el];
             if (e >= 0)
@@ -518,19 +678,23 @@ __global__ __launch_bounds__(256) void k_fused(const MeshArgs *__restrict__ MA, 
             }
         }
         __syncthreads();
-        // element rows of owned nodes
-        const int el = tid / NOD, i = tid - el * NOD;
+        // element rows of owned nodes: item = bi * nrow + (el * NOD + i), block row bi
+        const int bi = tid / nrow;
+        const int rr = tid - bi * nrow;
+        const int el = rr / NOD, i = rr - el * NOD;
         int slot = -1, rank = 0, e = -1, deg = 0;
-        if (tid < epb * NOD) {
+        if (bi < 3) {
             e = belem[el];
             const int64_t ea = ((int64_t)b * epb + el) * NOD + i;
             slot = e >= 0 ? MA->s.ea_slot[ea] : -1;
             rank = MA->s.ea_rank[ea];
         }
-        double Rr[6];
-        double K[RONLY ? 1 : 9][NOD];
+        double T = 0, F = 0, Kb[3][NOD];
         if (slot >= 0) {
-            element_row<NOD, RONLY>(i, nip, work + el * nip * kNC, sN, sXi, sEta, Rr, K);
+            const double *crr = work + el * nip * kNC;
+            if (bi == 0) element_row_b<NOD, 0>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+            else if (bi == 1) element_row_b<NOD, 1>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+            else element_row_b<NOD, 2>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
             const int node = MA->conn[(int64_t)e * NOD + i];
             deg = (int)(MA->rowptr_n[node + 1] - MA->rowptr_n[node]);
         }
@@ -540,18 +704,16 @@ __global__ __launch_bounds__(256) void k_fused(const MeshArgs *__restrict__ MA, 
             if (slot >= 0 && rank == r) {
                 double *w = win + slot;
                 if (!RONLY) {
+                    double *wb = w + bi * 3 * deg;
 #pragma unroll
                     for (int j = 0; j < NOD; ++j) {
                         const int cj = ps[j];
 #pragma unroll
-                        for (int bi = 0; bi < 3; ++bi)
-#pragma unroll
-                            for (int bj = 0; bj < 3; ++bj) w[bi * 3 * deg + bj * deg + cj] += K[bi * 3 + bj][j];
+                        for (int bj = 0; bj < 3; ++bj) lds_add(wb + bj * deg + cj, Kb[bj][j]);
                     }
                 }
-                double *wr = w + 9 * deg;
-                wr[0] += Rr[0]; wr[1] += Rr[2]; wr[2] += Rr[4];   // Tx Ty Th
-                wr[3] += Rr[1]; wr[4] += Rr[3]; wr[5] += Rr[5];   // Fx Fy Fh
+                lds_add(w + 9 * deg + bi, T);       // Tx Ty Th
+                lds_add(w + 9 * deg + 3 + bi, F);   // Fx Fy Fh
             }
             __syncthreads();
         }
@@ -559,28 +721,28 @@ __global__ __launch_bounds__(256) void k_fused(const MeshArgs *__restrict__ MA, 
         {
             const int f0 = MA->s.flush_ptr[b], f1 = MA->s.flush_ptr[b + 1];
             double *Kval = CA->Kval;
-            for (int fl = f0 + wave; fl < f1; fl += 4) {
+            for (int fl = f0 + wave; fl < f1; fl += kFusedWaves) {
                 const int node = MA->s.flush_node[fl];
                 const double *w = win + MA->s.flush_slot[fl];
                 const int64_t rp = MA->rowptr_n[node];
                 const int dg = (int)(MA->rowptr_n[node + 1] - rp);
                 if (!RONLY && Kval) {
 #pragma unroll
-                    for (int bi = 0; bi < 3; ++bi) {
-                        double *dst = Kval + (int64_t)bi * 3 * nnzN + 3 * rp;
-                        const double *src = w + bi * 3 * dg;
+                    for (int bk = 0; bk < 3; ++bk) {
+                        double *dst = Kval + (int64_t)bk * 3 * nnzN + 3 * rp;
+                        const double *src = w + bk * 3 * dg;
                         for (int k = lane; k < 3 * dg; k += 64) dst[k] = src[k];
                     }
                     // diagonal entry of block (lane, lane) (uvhMatrixAssembly.m:575-579)
                     if (check_nan && lane < 3 && isnan(w[lane * 3 * dg + lane * dg + MA->diag[node]])) bad |= 2;
                 }
                 if (lane < 3) {
-                    const double T = w[9 * dg + lane], F = w[9 * dg + 3 + lane];
-                    const double r = T - F;
-                    CA->R[(int64_t)lane * nn + node] = r;
-                    if (CA->Tint) CA->Tint[(int64_t)lane * nn + node] = T;
-                    if (CA->Fext) CA->Fext[(int64_t)lane * nn + node] = F;
-                    if (check_nan && isnan(r)) bad |= 1;
+                    const double Tv = w[9 * dg + lane], Fv = w[9 * dg + 3 + lane];
+                    const double rv = Tv - Fv;
+                    CA->R[(int64_t)lane * nn + node] = rv;
+                    if (CA->Tint) CA->Tint[(int64_t)lane * nn + node] = Tv;
+                    if (CA->Fext) CA->Fext[(int64_t)lane * nn + node] = Fv;
+                    if (check_nan && isnan(rv)) bad |= 1;
                 }
             }
         }
@@ -613,15 +775,15 @@ size_t element_lds_bytes(int nod, int nip, int epb) {
 template <int NOD>
 hipError_t launch_element_nod(int nele, int nip, const int32_t *conn, const double *x, const double *y,
                               const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
-                              bool ronly, double *EB, double *RE, hipStream_t s) {
+                              const int32_t *inc_pos, bool ronly, double *EB, double *RE, hipStream_t s) {
     const int epb = element_epb(NOD, nip);
     if (epb < 1) return hipErrorInvalidConfiguration;
     const int grid = (nele + epb - 1) / epb;
     const size_t lds = element_lds_bytes(NOD, nip, epb);
     if (ronly)
-        hipLaunchKernelGGL((k_element<NOD, true>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, EB, RE);
+        hipLaunchKernelGGL((k_element<NOD, true>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, EB, RE);
     else
-        hipLaunchKernelGGL((k_element<NOD, false>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, EB, RE);
+        hipLaunchKernelGGL((k_element<NOD, false>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, EB, RE);
     return hipGetLastError();
 }
 }  // namespace
@@ -637,24 +799,31 @@ hipError_t launch_clear_status(int *status, hipStream_t s) {
 }
 
 hipError_t launch_element(int nod, int nele, int nip, const int32_t *conn, const double *x, const double *y,
-                          const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p, bool ronly,
-                          double *EB, double *RE, hipStream_t s) {
+                          const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
+                          const int32_t *inc_pos, bool ronly, double *EB, double *RE, hipStream_t s) {
     switch (nod) {
-        case 3: return launch_element_nod<3>(nele, nip, conn, x, y, f, b, tab, p, ronly, EB, RE, s);
-        case 6: return launch_element_nod<6>(nele, nip, conn, x, y, f, b, tab, p, ronly, EB, RE, s);
-        case 10: return launch_element_nod<10>(nele, nip, conn, x, y, f, b, tab, p, ronly, EB, RE, s);
+        case 3: return launch_element_nod<3>(nele, nip, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s);
+        case 6: return launch_element_nod<6>(nele, nip, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s);
+        case 10: return launch_element_nod<10>(nele, nip, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_rows(int nod, int nn, const int32_t *inc_ptr, const int32_t *inc, const uint8_t *pos,
-                       const int64_t *rowptr_n, int64_t nnzN, const double *EB, const double *RE, double *R,
-                       double *Tint, double *Fext, double *Kval, int check_nan, int *status, hipStream_t s) {
+hipError_t launch_rows(int nod, int nn, int maxdeg, const int32_t *inc_ptr, const int32_t *inc,
+                       const uint8_t *pos, const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN,
+                       const double *EB, const double *RE, double *R, double *Tint, double *Fext, double *Kval,
+                       int check_nan, int *status, hipStream_t s) {
+    if (!Kval) {
+        hipLaunchKernelGGL(k_rows_r, dim3((nn + 255) / 256), dim3(256), 0, s, nn, inc_ptr, inc, RE, R, Tint, Fext,
+                           check_nan, status);
+        return hipGetLastError();
+    }
     const dim3 grid((nn + 3) / 4), block(256);
+    const size_t lds = size_t(4) * 9 * maxdeg * sizeof(double);
     switch (nod) {
-        case 3: hipLaunchKernelGGL(k_rows<3>, grid, block, 0, s, nn, inc_ptr, inc, pos, rowptr_n, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
-        case 6: hipLaunchKernelGGL(k_rows<6>, grid, block, 0, s, nn, inc_ptr, inc, pos, rowptr_n, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
-        case 10: hipLaunchKernelGGL(k_rows<10>, grid, block, 0, s, nn, inc_ptr, inc, pos, rowptr_n, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
+        case 3: hipLaunchKernelGGL(k_rows<3>, grid, block, lds, s, nn, maxdeg, inc_ptr, inc, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
+        case 6: hipLaunchKernelGGL(k_rows<6>, grid, block, lds, s, nn, maxdeg, inc_ptr, inc, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
+        case 10: hipLaunchKernelGGL(k_rows<10>, grid, block, lds, s, nn, maxdeg, inc_ptr, inc, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -664,7 +833,9 @@ hipError_t launch_rows(int nod, int nn, const int32_t *inc_ptr, const int32_t *i
 FusedLayout fused_layout(int nod, int nip) {
     FusedLayout L{};
     L.table_doubles = (3 * nip * nod + nip + 1) & ~1;
-    L.epb = element_epb(nod, nip);
+    const int per_el_bytes = std::max(NV_COUNT * nod, nip * kNC) * int(sizeof(double));
+    L.epb = std::min(kFusedThreads / nip, kFusedThreads / (3 * nod));
+    L.epb = std::min<int>(L.epb, int((kLdsLimit - L.table_doubles * sizeof(double)) / per_el_bytes));
     const int per_el = std::max(NV_COUNT * nod, nip * kNC);
     L.work_doubles = (L.epb * per_el + 1) & ~1;
     const int64_t total = kFusedLdsBytes / int64_t(sizeof(double));
@@ -676,7 +847,7 @@ hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, cons
                         int64_t nnzN, int nn, bool ronly, int check_nan, hipStream_t s) {
     if (npatch <= 0) return hipSuccess;
     const size_t lds = size_t(L.table_doubles + L.work_doubles + L.window_doubles) * sizeof(double);
-    const dim3 grid(npatch), block(256);
+    const dim3 grid(npatch), block(kFusedThreads);
 #define UVH_FUSED(NODV, RO)                                                                                     \
     do {                                                                                                        \
         auto kern = k_fused<NODV, RO>;                                                                          \

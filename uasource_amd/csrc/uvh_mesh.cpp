@@ -119,6 +119,7 @@ std::string build_mesh_host(MeshHost &m) {
     for (int32_t k = 0; k < nn; ++k) {
         if (deg[size_t(k) + 1] > 255) return "node degree above 255 (per-element placement uses 8 bits)";
         m.rowptr_n[size_t(k) + 1] = m.rowptr_n[k] + deg[size_t(k) + 1];
+        m.maxdeg = std::max<int32_t>(m.maxdeg, int32_t(deg[size_t(k) + 1]));
     }
     m.colind_n.assign(size_t(m.rowptr_n[nn]), 0);
     parallel([&](int32_t k, std::vector<int32_t> &buf) {
@@ -149,6 +150,16 @@ std::string build_mesh_host(MeshHost &m) {
                     }
             });
         for (auto &x : th) x.join();
+    }
+    // incidence-ordered copies: the element row (e, a) is stored at inc_pos[e*nod + a], so the
+    // rows of each node are contiguous
+    m.inc_pos.assign(size_t(nele) * nod, 0);
+    m.pos_inc.assign(size_t(nele) * nod * nod, 0);
+    for (int64_t t = 0; t < int64_t(nele) * nod; ++t) {
+        const int32_t ea = m.inc[t];
+        m.inc_pos[ea] = int32_t(t);
+        std::copy(m.pos.begin() + size_t(ea) * nod, m.pos.begin() + size_t(ea + 1) * nod,
+                  m.pos_inc.begin() + size_t(t) * nod);
     }
     return "";
 }

@@ -25,7 +25,8 @@ import scipy.sparse as sp
 from .ctrlvar import Struct
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libuvh.so")
+# UVH_LIBRARY overrides the library path (e.g. to A/B two builds of the kernels).
+LIB_PATH = os.environ.get("UVH_LIBRARY") or os.path.join(_HERE, "libuvh.so")
 
 UVH_OK, UVH_E_ARG, UVH_E_HIP, UVH_E_NAN_R, UVH_E_NAN_K, UVH_E_UNSUPPORTED, UVH_E_STATE, UVH_E_NOMEM = range(8)
 UVH_RONLY, UVH_ZEROFIELDS, UVH_FP32, UVH_NO_NAN_CHECK = 1, 2, 4, 8
