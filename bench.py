@@ -69,7 +69,20 @@ def main(argv=None):
     ap.add_argument("--ronly", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes three times (HBM counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the uvh assembly is built and measured on a single MI355X "
               f"(--gpus {args.gpus} is not supported; independent meshes shard across GPUs with no exchange).",
@@ -142,7 +155,10 @@ def main(argv=None):
     tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}{'_ronly' if args.ronly else ''}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_assembly")
+            tinfo = json.load(fh)
+        # the committed measurement applies only to the kernels it was taken on
+        if sorted(tinfo.get("kernels", [])) == sorted(kernels):
+            traffic = tinfo.get("hbm_bytes_per_assembly")
 
     line = {
         "metric": "uvh elements assembled/s (R+K), HBM GB/s fraction; p50/p99 request latency",

@@ -359,7 +359,8 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL is the HIP null stream
     const uvh::MeshHost &m = ctx->mesh;
     if (ctx->two_pass && !ronly && !ctx->d_EB) {
-        if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * 9 * m.nod) != hipSuccess)
+        const size_t stride = size_t((9 * m.nod + 15) / 16) * 16;  // rows padded to 128 bytes
+        if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * stride) != hipSuccess)
             return ctx->fail(UVH_E_NOMEM, "uvh_assemble: cannot allocate element rows");
     }
     const uvh::Params p = make_params(*ctrl, flags);

@@ -44,6 +44,10 @@ enum : int {
 };
 constexpr int kNC = C_COUNT + (C_COUNT & 1);  // even, for 16-byte LDS reads
 
+// Element rows in HBM (two-pass path) are padded to whole 128-byte lines.
+template <int NOD>
+constexpr int eb_stride() { return ((9 * NOD + 15) / 16) * 16; }
+
 __global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nn) return;
@@ -75,6 +79,15 @@ __device__ __forceinline__ void corner_ids(int &c0, int &c1, int &c2) {
 // additions, and the rounded result, is fixed.
 __device__ __forceinline__ void lds_add(double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each
+// XCD has its own L2. Neighbouring element ranges share nodes, so give every XCD one
+// contiguous eighth of the ranges: logical block = (b % 8) * ceil(G / 8) + b / 8.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int grid) {
+    const int per = (grid + kXcds - 1) / kXcds;
+    return (b % kXcds) * per + b / kXcds;
 }
 
 // Smooth step and its derivative at x = k (z - z0) (HeavisideApprox.m:101, DiracDelta.m:29):
@@ -437,7 +450,10 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
         sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
     }
     if (tid < nip) sW[tid] = tab->w[tid];
-    const int64_t e0 = (int64_t)blockIdx.x * epb;
+    const int nblocks = (nele + epb - 1) / epb;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    if (lb >= nblocks) return;  // whole block: no barrier has been reached
+    const int64_t e0 = (int64_t)lb * epb;
 
     // stage 1: gathers, one (element, node) per thread
     if (tid < epb * NOD) {
@@ -474,7 +490,7 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     // the row is stored at its position in the node-incidence order
     const int64_t row = inc_pos[e * NOD + i];
     double *re = RE + row * 6;
-    double *eb = EB + row * (9 * NOD);
+    double *eb = EB + row * eb_stride<NOD>();
     const double *crr = work + el * nip * kNC;
     double T, F, Kb[3][NOD];
     element_row_b<NOD, 0>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
@@ -518,7 +534,7 @@ __global__ __launch_bounds__(256) void k_rows(int nn, int maxdeg, const int32_t 
                                               double *__restrict__ Kval, int check_nan, int *__restrict__ status) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int k = blockIdx.x * 4 + wave;
+    const int k = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
     if (k >= nn) return;  // whole wave; no block-wide barriers below
     double *buf = lds + wave * 9 * maxdeg;
     const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
@@ -559,7 +575,7 @@ __global__ __launch_bounds__(256) void k_rows(int nn, int maxdeg, const int32_t 
                     const int v = lane + 64 * u;
                     if (c < cnt && v < NV) {
                         const int bk = v / NOD, j = v - bk * NOD;
-                        vals[c][u] = EB[(int64_t)(c0 + c) * NV + v];
+                        vals[c][u] = EB[(int64_t)(c0 + c) * eb_stride<NOD>() + v];
                         cols[c][u] = pos[(int64_t)(c0 + c) * NOD + j];
                     }
                 }
@@ -778,7 +794,8 @@ hipError_t launch_element_nod(int nele, int nip, const int32_t *conn, const doub
                               const int32_t *inc_pos, bool ronly, double *EB, double *RE, hipStream_t s) {
     const int epb = element_epb(NOD, nip);
     if (epb < 1) return hipErrorInvalidConfiguration;
-    const int grid = (nele + epb - 1) / epb;
+    const int nblocks = (nele + epb - 1) / epb;
+    const int grid = ((nblocks + kXcds - 1) / kXcds) * kXcds;  // whole rounds over the XCDs
     const size_t lds = element_lds_bytes(NOD, nip, epb);
     if (ronly)
         hipLaunchKernelGGL((k_element<NOD, true>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, EB, RE);
@@ -818,7 +835,8 @@ hipError_t launch_rows(int nod, int nn, int maxdeg, const int32_t *inc_ptr, cons
                            check_nan, status);
         return hipGetLastError();
     }
-    const dim3 grid((nn + 3) / 4), block(256);
+    const int nblocks = (nn + 3) / 4;
+    const dim3 grid(((nblocks + kXcds - 1) / kXcds) * kXcds), block(256);
     const size_t lds = size_t(4) * 9 * maxdeg * sizeof(double);
     switch (nod) {
         case 3: hipLaunchKernelGGL(k_rows<3>, grid, block, lds, s, nn, maxdeg, inc_ptr, inc, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;

@@ -1,0 +1,134 @@
+"""Summaries of rocprofv3 runs of ``bench.py``: per-kernel durations and HBM bytes.
+
+Two subcommands, both reading rocprofv3 CSV output and writing short text under
+``profiles/``:
+
+``stats <trace_dir> <workload> [--ronly] [--command CMD]``
+    turns ``rocprofv3 --kernel-trace --stats`` output (``run_kernel_stats.csv``) into
+    ``profiles/kernel_stats_<workload>[_ronly].md``.
+
+``traffic <dir> <workload> [--ronly]``
+    turns four ``rocprofv3 --pmc`` passes into ``profiles/traffic_<workload>[_ronly].json``
+    (HBM bytes per assembly, per kernel). ``<dir>`` holds ``fetch/`` and ``write/`` (FETCH_SIZE
+    and WRITE_SIZE over ``bench.py``) and ``cfetch/`` and ``cwrite/`` (the same counters over
+    ``bench.py --copy-calibration 1073741824``, a plain device copy of 1 GiB). The copy reads
+    and writes exactly 1 GiB, so counter-KiB x 1024 / 2^30 is the counter's bytes per true
+    byte; every kernel's counter is divided by that factor. The passes are run as::
+
+      rocprofv3 --pmc FETCH_SIZE -d <dir>/fetch -o run --output-format csv -- python3 bench.py --steps 3 --warmup 1 --no-cpu-baseline
+      rocprofv3 --pmc WRITE_SIZE -d <dir>/write -o run --output-format csv -- python3 bench.py --steps 3 --warmup 1 --no-cpu-baseline
+      rocprofv3 --pmc FETCH_SIZE -d <dir>/cfetch -o run --output-format csv -- python3 bench.py --copy-calibration 1073741824
+      rocprofv3 --pmc WRITE_SIZE -d <dir>/cwrite -o run --output-format csv -- python3 bench.py --copy-calibration 1073741824
+"""
+from __future__ import annotations
+
+import collections
+import csv
+import json
+import os
+import sys
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROFILES = os.path.join(REPO, "profiles")
+
+# kernel base name -> the name bench.py's per-kernel HIP-event timers use (without the _rk/_r suffix)
+MODE_KERNELS = {"k_element": "element", "k_element_f32": "element", "k_rows": "rows", "k_rows_r": "rows",
+                "k_fused": "fused"}
+PLAIN_KERNELS = {"k_nodal_b": "nodal_b", "k_clear_status": "clear_status"}
+COPY_BYTES = 1 << 30
+
+
+def base_name(full: str) -> str:
+    return full.split("(")[0].split("<")[0].replace("void ", "").replace("uvh::", "").strip()
+
+
+def timer_name(base: str, ronly: bool) -> str | None:
+    if base in MODE_KERNELS:
+        return MODE_KERNELS[base] + ("_r" if ronly else "_rk")
+    return PLAIN_KERNELS.get(base)
+
+
+def per_kernel(path: str, counter: str) -> dict:
+    """Average counter value (KiB) per dispatch, keyed by the kernel's full name."""
+    with open(os.path.join(path, "run_counter_collection.csv")) as fh:
+        rows = list(csv.DictReader(fh))
+    tot = collections.defaultdict(float)
+    cnt = collections.defaultdict(set)
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+        tot[name] += float(r["Counter_Value"])
+        cnt[name].add(r["Dispatch_Id"])
+    return {k: tot[k] / len(cnt[k]) for k in tot}
+
+
+def traffic(d: str, workload: str, ronly: bool) -> dict:
+    fetch = per_kernel(os.path.join(d, "fetch"), "FETCH_SIZE")
+    write = per_kernel(os.path.join(d, "write"), "WRITE_SIZE")
+    cf = per_kernel(os.path.join(d, "cfetch"), "FETCH_SIZE")
+    cw = per_kernel(os.path.join(d, "cwrite"), "WRITE_SIZE")
+    copy_kernel = max(cf, key=cf.get)
+    fscale = cf[copy_kernel] * 1024 / COPY_BYTES
+    wscale = cw[copy_kernel] * 1024 / COPY_BYTES
+    out = {"workload": workload, "mode": "R only" if ronly else "R+K",
+           "copy_calibration": {"kernel": copy_kernel, "bytes": COPY_BYTES, "fetch_scale": fscale,
+                                "write_scale": wscale},
+           "per_kernel_bytes": {}}
+    total = 0.0
+    names = set()
+    for k in fetch:
+        t = timer_name(base_name(k), ronly)
+        if t is None:
+            continue
+        fb = fetch[k] * 1024 / fscale
+        wb = write.get(k, 0.0) * 1024 / wscale
+        out["per_kernel_bytes"][k] = {"fetch": fb, "write": wb}
+        total += fb + wb
+        names.add(t)
+    out["hbm_bytes_per_assembly"] = total
+    out["kernels"] = sorted(names)
+    return out
+
+
+def stats_md(trace_dir: str, workload: str, command: str) -> str:
+    with open(os.path.join(trace_dir, "run_kernel_stats.csv")) as fh:
+        rows = list(csv.DictReader(fh))
+    lines = [f"# Kernel durations: uvh assembly, {workload}", "",
+             "Command (repository root on the GPU host):", "", "```", command, "```", "",
+             "| Kernel | Calls | Average (us) | Min (us) | Max (us) | Share |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        name = r["Name"].split("(")[0].strip()
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f}% |")
+    return "\n".join(lines) + "\n"
+
+
+def main(argv=None) -> int:
+    argv = list(sys.argv[1:] if argv is None else argv)
+    if len(argv) < 3 or argv[0] not in ("stats", "traffic"):
+        print(__doc__)
+        return 2
+    cmd, d, workload = argv[:3]
+    ronly = "--ronly" in argv
+    suffix = "_ronly" if ronly else ""
+    if cmd == "traffic":
+        out = traffic(d, workload, ronly)
+        fn = os.path.join(PROFILES, f"traffic_{workload}{suffix}.json")
+        with open(fn, "w") as fh:
+            json.dump(out, fh, indent=1)
+        print(json.dumps(out, indent=1))
+        return 0
+    command = ""
+    if "--command" in argv:
+        command = argv[argv.index("--command") + 1]
+    text = stats_md(d, workload, command)
+    fn = os.path.join(PROFILES, f"kernel_stats_{workload}{suffix}.md")
+    with open(fn, "w") as fh:
+        fh.write(text)
+    print(text)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
