@@ -67,6 +67,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain_p2")
     ap.add_argument("--ronly", action="store_true")
+    ap.add_argument("--fp32", action="store_true",
+                    help="precision form: FP32 element arithmetic, FP64 accumulation (reports its error vs FP64)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -115,7 +117,7 @@ def main(argv=None):
     ctrl = uvh.make_ctrl(cv, F1)
     R = torch.zeros(3 * M.Nnodes, dtype=torch.float64, device=dev)
     K = None if args.ronly else torch.zeros(ctx.nnz, dtype=torch.float64, device=dev)
-    flags = uvh.UVH_RONLY if args.ronly else 0
+    flags = (uvh.UVH_RONLY if args.ronly else 0) | (uvh.UVH_FP32 if args.fp32 else 0)
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
 
@@ -139,6 +141,26 @@ def main(argv=None):
     ctx.sync_status()
     ms_per_step = ms_total / args.steps
 
+    precision = None
+    if args.fp32:
+        # error of the precision form against the FP64 path on the same inputs (the FP64 path
+        # is the one held to 1e-12 of the oracle)
+        R64 = torch.zeros_like(R)
+        K64 = None if K is None else torch.zeros_like(K)
+        T32, F32 = torch.zeros_like(R), torch.zeros_like(R)
+        T64, F64 = torch.zeros_like(R), torch.zeros_like(R)
+        ctx.assemble_device(ctrl, fields, flags, R, K, T32, F32, stream=sptr)
+        ctx.assemble_device(ctrl, fields, flags & ~uvh.UVH_FP32, R64, K64, T64, F64, stream=sptr)
+        ctx.sync_status()
+        scale = max(torch.linalg.norm(T64).item(), torch.linalg.norm(F64).item())
+        precision = {"R_rel_err": torch.linalg.norm(R - R64).item() / scale,
+                     "Tint_rel_err": (torch.linalg.norm(T32 - T64) / torch.linalg.norm(T64)).item(),
+                     "Fext_rel_err": (torch.linalg.norm(F32 - F64) / torch.linalg.norm(F64)).item(),
+                     "reference": "FP64 path of this library on the same inputs"}
+        if K is not None:
+            precision["K_rel_err"] = (torch.linalg.norm(K - K64) / torch.linalg.norm(K64)).item()
+        del R64, K64, T32, F32, T64, F64
+
     # per-kernel durations: HIP events around each launch, on the launch stream
     ctx.profile(True)
     for _ in range(args.steps):
@@ -152,7 +174,8 @@ def main(argv=None):
     nbytes = algorithmic_bytes(M.Nnodes, M.Nele, M.nod, ctx.nnz, args.ronly)
     achieved = nbytes / (assembly_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}{'_ronly' if args.ronly else ''}.json")
+    tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}{'_fp32' if args.fp32 else ''}"
+                                           f"{'_ronly' if args.ronly else ''}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             tinfo = json.load(fh)
@@ -171,7 +194,7 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 element arithmetic, f64 accumulation" if args.fp32 else "f64",
         "data": "synthetic (seeded jittered-Delaunay mesh and field recipe)",
         "config": {"workload": args.workload, "Nele": M.Nele, "Nnodes": M.Nnodes, "nod": M.nod, "nip": M.nip,
                    "nnz_K": ctx.nnz, "mode": "R only" if args.ronly else "R+K", "parallelism": "1 GPU"},
@@ -180,6 +203,7 @@ def main(argv=None):
                      "kernel": "uvh assembly (all kernels of one R+K evaluation)",
                      "algorithmic_bytes": nbytes, "assembly_us": assembly_ms * 1e3},
         "kernels_us": {k: v * 1e3 for k, v in kernels.items()},
+        "precision": precision,
         "dominant_kernel": dominant,
     }
     if not args.no_cpu_baseline:

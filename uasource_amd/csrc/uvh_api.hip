@@ -15,13 +15,16 @@
 namespace uvh {
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s);
 hipError_t launch_clear_status(int *status, hipStream_t s);
-hipError_t launch_element(int nod, int nele, int nip, const int32_t *conn, const double *x, const double *y,
-                          const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
-                          const int32_t *inc_pos, bool ronly, double *EB, double *RE, hipStream_t s);
-hipError_t launch_rows(int nod, int nn, int maxdeg, const int32_t *inc_ptr, const int32_t *inc,
-                       const uint8_t *pos, const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN,
-                       const double *EB, const double *RE, double *R, double *Tint, double *Fext, double *Kval,
-                       int check_nan, int *status, hipStream_t s);
+hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, const int32_t *conn, const double *x,
+                          const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
+                          const Params &p, const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s);
+hipError_t launch_rows(int nod, int nn, int maxdeg, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
+                       const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
+                       double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
+                       hipStream_t s);
+int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
+int element_epb_default(int nod, int nip, bool fp32);
+size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
                         int64_t nnzN, int nn, bool ronly, int check_nan, hipStream_t s);
 }  // namespace uvh
@@ -80,6 +83,7 @@ struct uvh_ctx {
     uvh::CallArgs last_args{};
     // work
     double *d_b = nullptr, *d_RE = nullptr, *d_EB = nullptr;
+    int epb[2] = {0, 0};    // elements per block of the element kernel: [FP64, FP32]
     int *d_status = nullptr;
     // host-path staging
     double *d_fields = nullptr;  // 18 * nn
@@ -315,6 +319,8 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
         ctx->free_mesh();
         return ctx->hip_fail(e, "uvh_set_mesh: upload");
     }
+    ctx->epb[0] = uvh::element_epb_default(nod, nip, false);
+    ctx->epb[1] = uvh::element_epb_default(nod, nip, true);
     ctx->has_mesh = true;
     return UVH_OK;
 }
@@ -351,7 +357,9 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_assemble: no mesh (call uvh_set_mesh first)");
     if (!ctrl || !df || !dR) return ctx->fail(UVH_E_ARG, "uvh_assemble: null ctrl, fields or R");
     if (!fields_complete(*df)) return ctx->fail(UVH_E_ARG, "uvh_assemble: a required field pointer is null");
-    if (flags & UVH_FP32) return ctx->fail(UVH_E_UNSUPPORTED, "uvh_assemble: UVH_FP32 is not available in this build");
+    const bool fp32 = (flags & UVH_FP32) != 0;
+    if (fp32 && !ctx->two_pass)
+        return ctx->fail(UVH_E_UNSUPPORTED, "uvh_assemble: UVH_FP32 needs the two-pass assembly");
     if (ctrl->supg_tau < 0 || ctrl->supg_tau > 4) return ctx->fail(UVH_E_ARG, "uvh_assemble: supg_tau out of range");
     const bool ronly = (flags & UVH_RONLY) != 0;
     if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_assemble: K output required unless UVH_RONLY");
@@ -359,7 +367,8 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL is the HIP null stream
     const uvh::MeshHost &m = ctx->mesh;
     if (ctx->two_pass && !ronly && !ctx->d_EB) {
-        const size_t stride = size_t((9 * m.nod + 15) / 16) * 16;  // rows padded to 128 bytes
+        // FP64 rows padded to 128 bytes; the FP32 rows of the precision form fit in the same buffer
+        const size_t stride = uvh::element_row_stride(m.nod, false);
         if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * stride) != hipSuccess)
             return ctx->fail(UVH_E_NOMEM, "uvh_assemble: cannot allocate element rows");
     }
@@ -393,13 +402,14 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
         return UVH_OK;
     }
     if (e == hipSuccess)
-        e = ctx->timed(ronly ? "element_r" : "element_rk", s, [&] {
-            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->d_conn, ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab,
-                                       p, ctx->d_inc_pos, ronly, ctx->d_EB, ctx->d_RE, s);
+        e = ctx->timed(fp32 ? (ronly ? "element32_r" : "element32_rk") : (ronly ? "element_r" : "element_rk"), s, [&] {
+            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[fp32 ? 1 : 0], fp32, ctx->d_conn, ctx->d_x,
+                                       ctx->d_y, f, ctx->d_b, ctx->d_tab, p, ctx->d_inc_pos, ronly, ctx->d_EB,
+                                       ctx->d_RE, s);
         });
     if (e == hipSuccess)
-        e = ctx->timed(ronly ? "rows_r" : "rows_rk", s, [&] {
-            return uvh::launch_rows(m.nod, m.nn, m.maxdeg, ctx->d_inc_ptr, ctx->d_inc, ctx->d_pos_inc, ctx->d_rowptr_n,
+        e = ctx->timed(fp32 ? (ronly ? "rows32_r" : "rows32_rk") : (ronly ? "rows_r" : "rows_rk"), s, [&] {
+            return uvh::launch_rows(m.nod, m.nn, m.maxdeg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
                                     ctx->d_diag, m.nnzN(), ctx->d_EB, ctx->d_RE, dR, dT, dF, ronly ? nullptr : dK,
                                     check, ctx->d_status, s);
         });

@@ -45,8 +45,11 @@ enum : int {
 constexpr int kNC = C_COUNT + (C_COUNT & 1);  // even, for 16-byte LDS reads
 
 // Element rows in HBM (two-pass path) are padded to whole 128-byte lines.
-template <int NOD>
-constexpr int eb_stride() { return ((9 * NOD + 15) / 16) * 16; }
+template <int NOD, typename T>
+constexpr int eb_stride() {
+    constexpr int per_line = 128 / int(sizeof(T));
+    return ((9 * NOD + per_line - 1) / per_line) * per_line;
+}
 
 __global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,155 +93,208 @@ __device__ __forceinline__ int xcd_block(int b, int grid) {
     return (b % kXcds) * per + b / kXcds;
 }
 
+// Transcendentals in the element arithmetic type: double for the plain form, float for the
+// precision form.
+template <typename T>
+struct Fn;
+template <>
+struct Fn<double> {
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double expm1(double x) { return ::expm1(x); }
+    static __device__ __forceinline__ double tanh(double x) { return ::tanh(x); }
+    static __device__ __forceinline__ double abs(double x) { return ::fabs(x); }
+};
+template <>
+struct Fn<float> {
+    static __device__ __forceinline__ float exp(float x) { return ::expf(x); }
+    static __device__ __forceinline__ float log(float x) { return ::logf(x); }
+    static __device__ __forceinline__ float sqrt(float x) { return ::sqrtf(x); }
+    static __device__ __forceinline__ float expm1(float x) { return ::expm1f(x); }
+    static __device__ __forceinline__ float tanh(float x) { return ::tanhf(x); }
+    static __device__ __forceinline__ float abs(float x) { return ::fabsf(x); }
+};
+
 // Smooth step and its derivative at x = k (z - z0) (HeavisideApprox.m:101, DiracDelta.m:29):
 //   H = 1/2 + 1/2 tanh(x),   D = k/2 sech^2(x).
 // One expm1 gives both: with t = exp(-2|x|), tanh|x| = (1 - t)/(1 + t) and
 // sech^2 x = 4t/(1 + t)^2, each to a few ulp for every x.
-__device__ __forceinline__ void soft_step(double x, double k, double &H, double &D) {
-    const double em = expm1(-2.0 * fabs(x));  // t - 1, in (-1, 0]
-    const double den = 2.0 + em;              // 1 + t
-    const double th = copysign(-em / den, x);
-    H = 0.5 + 0.5 * th;
-    D = 0.5 * k * (4.0 * (1.0 + em) / (den * den));
+template <typename T>
+__device__ __forceinline__ void soft_step(T x, T k, T &H, T &D) {
+    const T em = Fn<T>::expm1(T(-2) * Fn<T>::abs(x));  // t - 1, in (-1, 0]
+    const T den = T(2) + em;                             // 1 + t
+    const T th = copysign(-em / den, x);
+    H = T(0.5) + T(0.5) * th;
+    D = T(0.5) * k * (T(4) * (T(1) + em) / (den * den));
 }
 
-// Physics of one integration point: fills the coefficient record c[kNC].
-template <int NOD>
+// Floatation terms at one point (uvhAssemblyIntPointImplicitSUPG.m:209-245):
+// He = H(h - hf), HE = H(hf - h), delta and Delta their derivatives, H+ = H(H) H with
+// H = S - B, the water depth d = HE rho h / rhow + He H+ and dd = d(d)/dh.
+template <typename T>
+__device__ __forceinline__ void floatation(T h, T hf, T H, T rho, const Params &p, T &He, T &delta, T &d, T &dd) {
+    const T kH = T(p.kH), Hh0 = T(p.Hh0), rrhow = T(1) / T(p.rhow);
+    T HE, Delta, HH, unused;
+    soft_step<T>(kH * ((h - hf) - Hh0), kH, He, delta);
+    if (p.Hh0 == 0.0) {
+        HE = T(1) - He;
+        Delta = delta;
+    } else {
+        soft_step<T>(kH * ((hf - h) - Hh0), kH, HE, Delta);
+    }
+    soft_step<T>(kH * (H - Hh0), kH, HH, unused);
+    const T Hpos = HH * H;
+    d = HE * rho * h * rrhow + He * Hpos;
+    dd = HE * rho * rrhow - Delta * h * rho * rrhow + delta * Hpos;
+}
+
+// Physics of one integration point in arithmetic type T: fills the coefficient record c[kNC].
+// The nodal values and shape tables are FP64. For T = float (precision form) the thickness,
+// surface, bed and density are still interpolated in FP64, gradients are taken from node
+// differences (exact in FP64), and the floatation terms and the hydrostatic combinations
+// built on them are evaluated in FP64 at every point: on floating ice rho h - rhow d cancels
+// to zero, which FP32 cannot resolve.
+template <int NOD, typename T>
 __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in LDS
                            const double *__restrict__ tN, const double *__restrict__ tXi,
                            const double *__restrict__ tEta, double wq, const Params &p,
-                           double *__restrict__ c) {
+                           T *__restrict__ c) {
     // Jacobian and inverse (UaMain/derivVector.m:96-107). Coordinates are taken relative to
     // the element's first node: sum_i dN_i = 0, so J is unchanged, and the differences are
     // exact, which avoids cancelling ~1e5 m coordinates down to ~1e2 m element sizes.
-    double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+    T J11 = 0, J12 = 0, J21 = 0, J22 = 0;
     const double xo = nv[NV_X * NOD], yo = nv[NV_Y * NOD];
 #pragma unroll
     for (int i = 1; i < NOD; ++i) {
-        const double xi = nv[NV_X * NOD + i] - xo, yi = nv[NV_Y * NOD + i] - yo;
-        J11 += tXi[i] * xi; J12 += tXi[i] * yi;
-        J21 += tEta[i] * xi; J22 += tEta[i] * yi;
+        const T xi = T(nv[NV_X * NOD + i] - xo), yi = T(nv[NV_Y * NOD + i] - yo);
+        const T dxi = T(tXi[i]), deta = T(tEta[i]);
+        J11 += dxi * xi; J12 += dxi * yi;
+        J21 += deta * xi; J22 += deta * yi;
     }
-    const double det = J11 * J22 - J12 * J21;
-    const double rdet = 1.0 / det;
-    const double i11 = J22 * rdet, i12 = -J12 * rdet, i21 = -J21 * rdet, i22 = J11 * rdet;
+    const T det = J11 * J22 - J12 * J21;
+    const T rdet = T(1) / det;
+    const T i11 = J22 * rdet, i12 = -J12 * rdet, i21 = -J21 * rdet, i22 = J11 * rdet;
 
-    // interpolation and gradients (uvhAssemblyIntPointImplicitSUPG.m:62-126,206-208,272-300)
-    double h = 0, u = 0, v = 0, C = 0, m = 0, A = 0, n = 0, h0 = 0, u0 = 0, v0 = 0;
-    double as0 = 0, ab0 = 0, as1 = 0, ab1 = 0, dadh = 0, Bq = 0, Sq = 0, rho = 0;
-    double dhdx = 0, dhdy = 0, dh0dx = 0, dh0dy = 0, exx0 = 0, eyy0 = 0, dbdx = 0, dbdy = 0;
-    double drhodx = 0, drhody = 0, exx = 0, eyy = 0, exy2 = 0;
+    // interpolation and gradients (uvhAssemblyIntPointImplicitSUPG.m:62-126,206-208,272-300);
+    // gradients of h, h0, b and rho from differences to node 0 (sum_i dN_i = 0)
+    double hd = 0, h0d = 0, Bd = 0, Sd = 0, rhod = 0;
+    T u = 0, v = 0, C = 0, m = 0, A = 0, n = 0, u0 = 0, v0 = 0;
+    T as0 = 0, ab0 = 0, as1 = 0, ab1 = 0, dadh = 0;
+    T dhdx = 0, dhdy = 0, dh0dx = 0, dh0dy = 0, exx0 = 0, eyy0 = 0, dbdx = 0, dbdy = 0;
+    T drhodx = 0, drhody = 0, exx = 0, eyy = 0, exy2 = 0;
+    const double hr = nv[NV_H * NOD], h0r = nv[NV_H0 * NOD], br = nv[NV_BL * NOD], rhor = nv[NV_RHO * NOD];
 #pragma unroll 1
     for (int i = 0; i < NOD; ++i) {
-        const double Ni = tN[i];
-        const double nx = i11 * tXi[i] + i12 * tEta[i];
-        const double ny = i21 * tXi[i] + i22 * tEta[i];
-        const double hi = nv[NV_H * NOD + i], ui = nv[NV_U * NOD + i], vi = nv[NV_V * NOD + i];
-        const double h0i = nv[NV_H0 * NOD + i], u0i = nv[NV_U0 * NOD + i], v0i = nv[NV_V0 * NOD + i];
-        const double rhoi = nv[NV_RHO * NOD + i], bi = nv[NV_BL * NOD + i];
-        h += Ni * hi; u += Ni * ui; v += Ni * vi;
-        C += Ni * nv[NV_C * NOD + i]; m += Ni * nv[NV_M * NOD + i];
-        A += Ni * nv[NV_A * NOD + i]; n += Ni * nv[NV_N * NOD + i];
-        h0 += Ni * h0i; u0 += Ni * u0i; v0 += Ni * v0i;
-        as0 += Ni * nv[NV_AS0 * NOD + i]; ab0 += Ni * nv[NV_AB0 * NOD + i];
-        as1 += Ni * nv[NV_AS1 * NOD + i]; ab1 += Ni * nv[NV_AB1 * NOD + i];
-        dadh += Ni * nv[NV_DADH * NOD + i];
-        Bq += Ni * nv[NV_B * NOD + i]; Sq += Ni * nv[NV_S * NOD + i];
-        rho += Ni * rhoi;
-        dhdx += nx * hi; dhdy += ny * hi;
-        dh0dx += nx * h0i; dh0dy += ny * h0i;
+        const double Nd = tN[i];
+        const T Ni = T(Nd);
+        const T nx = i11 * T(tXi[i]) + i12 * T(tEta[i]);
+        const T ny = i21 * T(tXi[i]) + i22 * T(tEta[i]);
+        const double hid = nv[NV_H * NOD + i], h0id = nv[NV_H0 * NOD + i], rhoid = nv[NV_RHO * NOD + i];
+        const T dh = T(hid - hr), dh0 = T(h0id - h0r), dbl = T(nv[NV_BL * NOD + i] - br), drho = T(rhoid - rhor);
+        const T ui = T(nv[NV_U * NOD + i]), vi = T(nv[NV_V * NOD + i]);
+        const T u0i = T(nv[NV_U0 * NOD + i]), v0i = T(nv[NV_V0 * NOD + i]);
+        hd += Nd * hid; h0d += Nd * h0id; u += Ni * ui; v += Ni * vi;
+        C += Ni * T(nv[NV_C * NOD + i]); m += Ni * T(nv[NV_M * NOD + i]);
+        A += Ni * T(nv[NV_A * NOD + i]); n += Ni * T(nv[NV_N * NOD + i]);
+        u0 += Ni * u0i; v0 += Ni * v0i;
+        as0 += Ni * T(nv[NV_AS0 * NOD + i]); ab0 += Ni * T(nv[NV_AB0 * NOD + i]);
+        as1 += Ni * T(nv[NV_AS1 * NOD + i]); ab1 += Ni * T(nv[NV_AB1 * NOD + i]);
+        dadh += Ni * T(nv[NV_DADH * NOD + i]);
+        Bd += Nd * nv[NV_B * NOD + i]; Sd += Nd * nv[NV_S * NOD + i];
+        rhod += Nd * rhoid;
+        dhdx += nx * dh; dhdy += ny * dh;
+        dh0dx += nx * dh0; dh0dy += ny * dh0;
         exx0 += nx * u0i; eyy0 += ny * v0i;
-        dbdx += nx * bi; dbdy += ny * bi;
-        drhodx += nx * rhoi; drhody += ny * rhoi;
+        dbdx += nx * dbl; dbdy += ny * dbl;
+        drhodx += nx * drho; drhody += ny * drho;
         exx += nx * ui; eyy += ny * vi;
         exy2 += nx * vi + ny * ui;
     }
-    const double exy = 0.5 * exy2;
-    const double a1 = as1 + ab1, a0 = as0 + ab0;
-    const double rrhow = 1.0 / p.rhow;
+    const T h = T(hd), h0 = T(h0d), rho = T(rhod);
+    const T exy = T(0.5) * exy2;
+    const T a1 = as1 + ab1, a0 = as0 + ab0;
 
-    // floatation (uvhAssemblyIntPointImplicitSUPG.m:209-243): He = H(h - hf), HE = H(hf - h),
-    // delta, Delta their derivatives; with Hh0 = 0 the second pair mirrors the first.
-    const double H = Sq - Bq;
-    const double hf = p.rhow * H / rho;
-    double He, delta, HE, Delta, HH, unused;
-    soft_step(p.kH * ((h - hf) - p.Hh0), p.kH, He, delta);
-    if (p.Hh0 == 0.0) {
-        HE = 1.0 - He;
-        Delta = delta;
-    } else {
-        soft_step(p.kH * ((hf - h) - p.Hh0), p.kH, HE, Delta);
-    }
-    soft_step(p.kH * (H - p.Hh0), p.kH, HH, unused);
-    const double Hpos = HH * H;
-    const double d = HE * rho * h * rrhow + He * Hpos;
-    const double dd = HE * rho * rrhow - Delta * h * rho * rrhow + delta * Hpos;
+    // floatation (uvhAssemblyIntPointImplicitSUPG.m:209-245), FP64, and the combinations of
+    // the water depth d that enter the driving stress and its derivative
+    const double Hd = Sd - Bd;
+    const double hfd = p.rhow * Hd / rhod;
+    double He64, delta64, d64, dd64;
+    floatation<double>(hd, hfd, Hd, rhod, p, He64, delta64, d64, dd64);
+    const T He = T(He64), delta = T(delta64);
+    const T hyd1 = T(rhod * hd - p.rhow * d64);             // rho h - rhow d
+    const T hyd2 = T(rhod * hd * hd - p.rhow * d64 * d64);  // rho h^2 - rhow d^2
+    const T hyd3 = T(rhod - p.rhow * dd64);                 // rho - rhow dd
+    const T hyd4 = T(rhod * hd - p.rhow * d64 * dd64);      // rho h - rhow d dd
+    const T dh0h = T(h0d - hd);                             // h0 - h
 
     // Glen's flow law (EffectiveViscositySSTREAM.m:7-9) from logarithms:
     //   A^(-1/n) e^((1-n)/n) = exp(-log(A)/n + (1-n)/n * log(e)),  E = (1-n)/(4n) * that / e^2
-    const double e2 = p.eps0 * p.eps0 + exx * exx + eyy * eyy + exx * eyy + exy * exy;
-    const double rn = 1.0 / n;
-    const double visc = exp(-log(A) * rn + (1.0 - n) * rn * (0.5 * log(e2)));
-    const double eta = 0.5 * visc + p.eta0;
-    const double E = 0.25 * (1.0 - n) * rn * visc / e2;
+    const T eps0 = T(p.eps0);
+    const T e2 = eps0 * eps0 + exx * exx + eyy * eyy + exx * eyy + exy * exy;
+    const T rn = T(1) / n;
+    const T visc = Fn<T>::exp(-Fn<T>::log(A) * rn + (T(1) - n) * rn * (T(0.5) * Fn<T>::log(e2)));
+    const T eta = T(0.5) * visc + T(p.eta0);
+    const T E = T(0.25) * (T(1) - n) * rn * visc / e2;
 
     // Weertman sliding (BasalDrag.m:54-77,410-443):
     //   beta2 = (C+C0)^(-1/m) U^(1/m-1),  Dbeta2 = (1/m-1) beta2 / U^2
-    const double Hec = He < 0 ? 0.0 : He, dlc = delta < 0 ? 0.0 : delta;
-    const double U2 = u * u + v * v + p.uz * p.uz;
-    const double rm = 1.0 / m;
-    const double beta2 = exp(-log(C + p.czero) * rm + (rm - 1.0) * (0.5 * log(U2)));
-    const double Db = (rm - 1.0) * beta2 / U2;
-    const double taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
-    const double dtxdu = Hec * (beta2 + Db * u * u), dtydv = Hec * (beta2 + Db * v * v);
-    const double dtxdv = Hec * Db * u * v;
-    const double dtxdh = dlc * beta2 * u, dtydh = dlc * beta2 * v;
+    const T Hec = He < T(0) ? T(0) : He, dlc = delta < T(0) ? T(0) : delta;
+    const T uz = T(p.uz);
+    const T U2 = u * u + v * v + uz * uz;
+    const T rm = T(1) / m;
+    const T beta2 = Fn<T>::exp(-Fn<T>::log(C + T(p.czero)) * rm + (rm - T(1)) * (T(0.5) * Fn<T>::log(U2)));
+    const T Db = (rm - T(1)) * beta2 / U2;
+    const T taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
+    const T dtxdu = Hec * (beta2 + Db * u * u), dtydv = Hec * (beta2 + Db * v * v);
+    const T dtxdv = Hec * Db * u * v;
+    const T dtxdh = dlc * beta2 * u, dtydh = dlc * beta2 * v;
 
     // SUPG (uvhAssemblyIntPointImplicitSUPG.m:323-390, SUPGtau.m:24-77)
     int c0i, c1i, c2i;
     corner_ids<NOD>(c0i, c1i, c2i);
     const double X1 = nv[NV_X * NOD + c0i], X2 = nv[NV_X * NOD + c1i], X3 = nv[NV_X * NOD + c2i];
     const double Y1 = nv[NV_Y * NOD + c0i], Y2 = nv[NV_Y * NOD + c1i], Y3 = nv[NV_Y * NOD + c2i];
-    const double area = 0.5 * fabs((X1 - X3) * (Y2 - Y1) - (X1 - X2) * (Y3 - Y1));
-    const double l = sqrt(2.0 * area);
-    const double speed0 = sqrt(u0 * u0 + v0 * v0 + p.uz * p.uz);
-    const double taut = p.dt / 2.0 + 2.220446049250313e-16;
-    const double taus = l / (2.0 * speed0 + p.uz);
-    double tau;
+    const T area = T(0.5 * fabs((X1 - X3) * (Y2 - Y1) - (X1 - X2) * (Y3 - Y1)));
+    const T l = Fn<T>::sqrt(T(2) * area);
+    const T speed0 = Fn<T>::sqrt(u0 * u0 + v0 * v0 + uz * uz);
+    const T taut = T(p.dt / 2.0 + 2.220446049250313e-16);
+    const T taus = l / (T(2) * speed0 + uz);
+    T tau;
     switch (p.tau_opt) {
         case 1: tau = taut; break;
         case 2: {
-            const double ecn = taut / taus;
-            const double K = ecn < 1e-6 ? ecn / 3.0 : 1.0 / tanh(ecn) - 1.0 / ecn;
+            const T ecn = taut / taus;
+            const T K = ecn < T(1e-6) ? ecn / T(3) : T(1) / Fn<T>::tanh(ecn) - T(1) / ecn;
             tau = K * taus;
         } break;
-        case 3: tau = (p.dt / 2.0) * 1.0 / (1.0 + taut / taus); break;
-        case 4: tau = 0.0; break;
+        case 3: tau = T(p.dt / 2.0) * T(1) / (T(1) + taut / taus); break;
+        case 4: tau = T(0); break;
         default: tau = taus;
     }
-    const double tau0 = p.beta0 * (p.tau_mult * tau);
-    const double W = det * wq;
+    const T tau0 = T(p.beta0) * (T(p.tau_mult) * tau);
+    const T W = det * T(wq);
 
     // fluxes (uvhNodalLoopSSTREAM.m:39-44)
-    const double qx1 = rho * exx * h + rho * u * dhdx + drhodx * u * h;
-    const double qy1 = rho * eyy * h + rho * v * dhdy + drhody * v * h;
-    const double qx0 = rho * exx0 * h0 + rho * u0 * dh0dx + drhodx * u0 * h0;
-    const double qy0 = rho * eyy0 * h0 + rho * v0 * dh0dy + drhody * v0 * h0;
-    const double th = p.theta, dt = p.dt, g = p.g, ca = p.ca, sa = p.sa, rhow = p.rhow;
-    const double dtth = dt * th;
+    const T qx1 = rho * exx * h + rho * u * dhdx + drhodx * u * h;
+    const T qy1 = rho * eyy * h + rho * v * dhdy + drhody * v * h;
+    const T qx0 = rho * exx0 * h0 + rho * u0 * dh0dx + drhodx * u0 * h0;
+    const T qy0 = rho * eyy0 * h0 + rho * v0 * dh0dy + drhody * v0 * h0;
+    const T th = T(p.theta), dt = T(p.dt), g = T(p.g), ca = T(p.ca), sa = T(p.sa);
+    const T dtth = dt * th;
 
     c[C_I11] = i11; c[C_I12] = i12; c[C_I21] = i21; c[C_I22] = i22;
-    c[C_TU] = 0.5 * tau0 * u0; c[C_TV] = 0.5 * tau0 * v0;
-    c[C_WHETA] = W * h * eta; c[C_WHE2] = 2.0 * W * h * E;
-    c[C_S1] = 2.0 * exx + eyy; c[C_S2] = 2.0 * eyy + exx; c[C_EXY] = exy;
+    c[C_TU] = T(0.5) * tau0 * u0; c[C_TV] = T(0.5) * tau0 * v0;
+    c[C_WHETA] = W * h * eta; c[C_WHE2] = T(2) * W * h * E;
+    c[C_S1] = T(2) * exx + eyy; c[C_S2] = T(2) * eyy + exx; c[C_EXY] = exy;
     c[C_WDTXDU] = W * dtxdu; c[C_WDTYDV] = W * dtydv; c[C_WDTXDV] = W * dtxdv;
-    const double c2 = ca * g * (rho * h - rhow * d * dd);
-    const double cx1 = dtxdh + ca * g * (rho - rhow * dd) * dbdx - sa * g * rho;
-    const double cy1 = dtydh + ca * g * (rho - rhow * dd) * dbdy;
-    c[C_PX] = W * (eta * (4.0 * exx + 2.0 * eyy) - c2);
-    c[C_PXY] = W * (eta * 2.0 * exy);
+    const T c2 = ca * g * hyd4;
+    const T cx1 = dtxdh + ca * g * hyd3 * dbdx - sa * g * rho;
+    const T cy1 = dtydh + ca * g * hyd3 * dbdy;
+    c[C_PX] = W * (eta * (T(4) * exx + T(2) * eyy) - c2);
+    c[C_PXY] = W * (eta * T(2) * exy);
     c[C_PCX] = W * cx1;
-    c[C_PY] = W * (eta * (4.0 * eyy + 2.0 * exx) - c2);
+    c[C_PY] = W * (eta * (T(4) * eyy + T(2) * exx) - c2);
     c[C_PCY] = W * cy1;
     c[C_W] = W;
     c[C_AX] = dtth * (rho * dhdx + drhodx * h);
@@ -248,11 +304,11 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     c[C_RU] = dtth * (rho * u);
     c[C_RV] = dtth * (rho * v);
     c[C_WTX] = W * taux; c[C_WTY] = W * tauy;
-    c[C_WF1X] = W * (-ca * g * (rho * h - rhow * d) * dbdx + rho * g * h * sa);
-    c[C_WF1Y] = W * (-ca * g * (rho * h - rhow * d) * dbdy);
-    c[C_WF2] = W * (0.5 * ca * g * (rho * h * h - rhow * d * d));
-    c[C_TH] = -W * (rho * (h0 - h));
-    c[C_FH] = W * (dt * rho * ((1.0 - th) * a0 + th * a1) - dt * (th * qx1 + (1.0 - th) * qx0 + th * qy1 + (1.0 - th) * qy0));
+    c[C_WF1X] = W * (-ca * g * hyd1 * dbdx + rho * g * h * sa);
+    c[C_WF1Y] = W * (-ca * g * hyd1 * dbdy);
+    c[C_WF2] = W * (T(0.5) * ca * g * hyd2);
+    c[C_TH] = -W * (rho * dh0h);
+    c[C_FH] = W * (dt * rho * ((T(1) - th) * a0 + th * a1) - dt * (th * qx1 + (T(1) - th) * qx0 + th * qy1 + (T(1) - th) * qy0));
 }
 
 // Gather of node a of element e into the LDS staging record nv[NV_COUNT][NOD].
@@ -281,173 +337,131 @@ __device__ __forceinline__ void gather_node(const int32_t *__restrict__ conn, co
     nv[NV_X * NOD + a] = xn[k]; nv[NV_Y * NOD + a] = yn[k];
 }
 
-// Row i of the element: 6 residual terms and (unless RONLY) 9 x NOD Jacobian terms,
-// accumulated over the integration points from the coefficient records cr[nip][kNC].
-template <int NOD, bool RONLY>
-__device__ __forceinline__ void element_row(int i, int nip, const double *__restrict__ cr,
-                                            const double *__restrict__ sN, const double *__restrict__ sXi,
-                                            const double *__restrict__ sEta, double (&Rr)[6],
-                                            double (&K)[RONLY ? 1 : 9][NOD]) {
-    double Tx = 0, Fx = 0, Ty = 0, Fy = 0, Th = 0, Fh = 0;
-    if (!RONLY) {
-#pragma unroll
-        for (int b = 0; b < 9; ++b)
-#pragma unroll
-            for (int j = 0; j < NOD; ++j) K[b][j] = 0.0;
-    }
-    for (int q = 0; q < nip; ++q) {
-        const double *r = cr + q * kNC;
-        const double *qN = sN + q * NOD, *qXi = sXi + q * NOD, *qEta = sEta + q * NOD;
-        const double I11 = r[C_I11], I12 = r[C_I12], I21 = r[C_I21], I22 = r[C_I22];
-        const double S1 = r[C_S1], S2 = r[C_S2], EXY = r[C_EXY];
-        const double Ni = qN[i];
-        const double Nxi = I11 * qXi[i] + I12 * qEta[i];
-        const double Nyi = I21 * qXi[i] + I22 * qEta[i];
-        const double ai = S1 * Nxi + EXY * Nyi;
-        const double bi = S2 * Nyi + EXY * Nxi;
-        const double psi = Ni + r[C_TU] * Nxi + r[C_TV] * Nyi;
-        const double wheta = r[C_WHETA];
-        Tx += 2.0 * wheta * ai + r[C_WTX] * Ni;
-        Ty += 2.0 * wheta * bi + r[C_WTY] * Ni;
-        Fx += r[C_WF1X] * Ni + r[C_WF2] * Nxi;
-        Fy += r[C_WF1Y] * Ni + r[C_WF2] * Nyi;
-        Th += r[C_TH] * psi;
-        Fh += r[C_FH] * psi;
-        if (!RONLY) {
-            const double whE2 = r[C_WHE2];
-            const double l1 = 4.0 * wheta * Nxi, l2 = wheta * Nyi, l3 = whE2 * ai, l4 = r[C_WDTXDU] * Ni;
-            const double m1 = 4.0 * wheta * Nyi, m2 = wheta * Nxi, m3 = whE2 * bi, m4 = r[C_WDTYDV] * Ni;
-            const double n1 = 2.0 * wheta * Nxi, n4 = r[C_WDTXDV] * Ni;
-            const double o1 = 2.0 * wheta * Nyi;
-            const double Xi = r[C_PX] * Nxi + r[C_PXY] * Nyi + r[C_PCX] * Ni;
-            const double Yi = r[C_PY] * Nyi + r[C_PXY] * Nxi + r[C_PCY] * Ni;
-            const double Wpsi = r[C_W] * psi;
-            const double hAX = Wpsi * r[C_AX], hAY = Wpsi * r[C_AY], hRH = Wpsi * r[C_RH];
-            const double hG = Wpsi * r[C_GAM], hRU = Wpsi * r[C_RU], hRV = Wpsi * r[C_RV];
-#pragma unroll
-            for (int j = 0; j < NOD; ++j) {
-                const double Nj = qN[j];
-                const double Nxj = I11 * qXi[j] + I12 * qEta[j];
-                const double Nyj = I21 * qXi[j] + I22 * qEta[j];
-                const double aj = S1 * Nxj + EXY * Nyj;
-                const double bj = S2 * Nyj + EXY * Nxj;
-                K[0][j] += l1 * Nxj + l2 * Nyj + l3 * aj + l4 * Nj;   // Kxu
-                K[1][j] += n1 * Nyj + l2 * Nxj + l3 * bj + n4 * Nj;   // Kxv
-                K[2][j] += Xi * Nj;                                    // Kxh
-                K[3][j] += o1 * Nxj + m2 * Nyj + m3 * aj + n4 * Nj;   // Kyu
-                K[4][j] += m1 * Nyj + m2 * Nxj + m3 * bj + m4 * Nj;   // Kyv
-                K[5][j] += Yi * Nj;                                    // Kyh
-                K[6][j] += hAX * Nj + hRH * Nxj;                       // Khu
-                K[7][j] += hAY * Nj + hRH * Nyj;                       // Khv
-                K[8][j] += hG * Nj + hRU * Nxj + hRV * Nyj;            // Khh
-            }
-        }
-    }
-    Rr[0] = Tx; Rr[1] = Fx; Rr[2] = Ty; Rr[3] = Fy; Rr[4] = Th; Rr[5] = Fh;
-}
-
 // Row i, block row BI (0: x, 1: y, 2: h) of the element: T and F of that row and its
-// 3 x NOD Jacobian blocks, accumulated over the integration points.
-template <int NOD, int BI>
-__device__ __forceinline__ void element_row_b(int i, int nip, const double *__restrict__ cr,
-                                              const double *__restrict__ sN, const double *__restrict__ sXi,
-                                              const double *__restrict__ sEta, double &Tr, double &Fr,
-                                              double (&K)[3][NOD]) {
-    double T = 0, F = 0;
+// 3 x NOD Jacobian blocks, accumulated over the integration points in type T.
+// The terms of uvhNodalLoopSSTREAM.m:62-179 (2025 form) are regrouped per (i, point) so that
+// each (i, j, point) costs three FMAs per uv block: with a_j = S1 Nx_j + EXY Ny_j and
+// b_j = S2 Ny_j + EXY Nx_j,
+//   Kxu_ij = (4 wh eta Nx_i + wE a_i S1) Nx_j + (wh eta Ny_i + wE a_i EXY) Ny_j + W dtaux/du N_i N_j
+//   Kxv_ij = (2 wh eta Nx_i + wE a_i S2) Ny_j + (wh eta Ny_i + wE a_i EXY) Nx_j + W dtaux/dv N_i N_j
+//   Kyu_ij = (2 wh eta Ny_i + wE b_i S1) Nx_j + (wh eta Nx_i + wE b_i EXY) Ny_j + W dtaux/dv N_i N_j
+//   Kyv_ij = (4 wh eta Ny_i + wE b_i S2) Ny_j + (wh eta Nx_i + wE b_i EXY) Nx_j + W dtauy/dv N_i N_j
+// (wh eta = W h eta, wE = 2 W h E), summed over the points.
+template <int NOD, int BI, typename T>
+__device__ __forceinline__ void element_row_b(int i, int nip, const T *__restrict__ cr, const T *__restrict__ sN,
+                                              const T *__restrict__ sXi, const T *__restrict__ sEta, T &Tr, T &Fr,
+                                              T (&K)[3][NOD]) {
+    T Ts = 0, Fs = 0;
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int j = 0; j < NOD; ++j) K[b][j] = 0.0;
+        for (int j = 0; j < NOD; ++j) K[b][j] = T(0);
     for (int q = 0; q < nip; ++q) {
-        const double *r = cr + q * kNC;
-        const double *qN = sN + q * NOD, *qXi = sXi + q * NOD, *qEta = sEta + q * NOD;
-        const double I11 = r[C_I11], I12 = r[C_I12], I21 = r[C_I21], I22 = r[C_I22];
-        const double S1 = r[C_S1], S2 = r[C_S2], EXY = r[C_EXY];
-        const double Ni = qN[i];
-        const double Nxi = I11 * qXi[i] + I12 * qEta[i];
-        const double Nyi = I21 * qXi[i] + I22 * qEta[i];
-        const double wheta = r[C_WHETA];
-        if (BI == 0) {
-            const double ai = S1 * Nxi + EXY * Nyi;
-            T += 2.0 * wheta * ai + r[C_WTX] * Ni;
-            F += r[C_WF1X] * Ni + r[C_WF2] * Nxi;
-            const double whE2 = r[C_WHE2];
-            const double l1 = 4.0 * wheta * Nxi, l2 = wheta * Nyi, l3 = whE2 * ai, l4 = r[C_WDTXDU] * Ni;
-            const double n1 = 2.0 * wheta * Nxi, n4 = r[C_WDTXDV] * Ni;
-            const double Xi = r[C_PX] * Nxi + r[C_PXY] * Nyi + r[C_PCX] * Ni;
-#pragma unroll
-            for (int j = 0; j < NOD; ++j) {
-                const double Nj = qN[j];
-                const double Nxj = I11 * qXi[j] + I12 * qEta[j];
-                const double Nyj = I21 * qXi[j] + I22 * qEta[j];
-                const double aj = S1 * Nxj + EXY * Nyj;
-                const double bj = S2 * Nyj + EXY * Nxj;
-                K[0][j] += l1 * Nxj + l2 * Nyj + l3 * aj + l4 * Nj;
-                K[1][j] += n1 * Nyj + l2 * Nxj + l3 * bj + n4 * Nj;
-                K[2][j] += Xi * Nj;
+        const T *r = cr + q * kNC;
+        const T *qN = sN + q * NOD, *qXi = sXi + q * NOD, *qEta = sEta + q * NOD;
+        const T I11 = r[C_I11], I12 = r[C_I12], I21 = r[C_I21], I22 = r[C_I22];
+        const T Ni = qN[i];
+        const T Nxi = I11 * qXi[i] + I12 * qEta[i];
+        const T Nyi = I21 * qXi[i] + I22 * qEta[i];
+        if (BI < 2) {
+            const T S1 = r[C_S1], S2 = r[C_S2], EXY = r[C_EXY];
+            const T wheta = r[C_WHETA], whE2 = r[C_WHE2];
+            T cxx, cyy, cmix, cn0, cn1, ch;
+            if (BI == 0) {
+                const T ai = S1 * Nxi + EXY * Nyi;
+                Ts += T(2) * wheta * ai + r[C_WTX] * Ni;
+                Fs += r[C_WF1X] * Ni + r[C_WF2] * Nxi;
+                const T wa = whE2 * ai;
+                cxx = T(4) * wheta * Nxi + wa * S1;    // Kxu: Nx_j
+                cmix = wheta * Nyi + wa * EXY;         // Kxu: Ny_j and Kxv: Nx_j
+                cyy = T(2) * wheta * Nxi + wa * S2;    // Kxv: Ny_j
+                cn0 = r[C_WDTXDU] * Ni;
+                cn1 = r[C_WDTXDV] * Ni;
+                ch = r[C_PX] * Nxi + r[C_PXY] * Nyi + r[C_PCX] * Ni;
+            } else {
+                const T bi = S2 * Nyi + EXY * Nxi;
+                Ts += T(2) * wheta * bi + r[C_WTY] * Ni;
+                Fs += r[C_WF1Y] * Ni + r[C_WF2] * Nyi;
+                const T wb = whE2 * bi;
+                cxx = T(2) * wheta * Nyi + wb * S1;    // Kyu: Nx_j
+                cmix = wheta * Nxi + wb * EXY;         // Kyu: Ny_j and Kyv: Nx_j
+                cyy = T(4) * wheta * Nyi + wb * S2;    // Kyv: Ny_j
+                cn0 = r[C_WDTXDV] * Ni;
+                cn1 = r[C_WDTYDV] * Ni;
+                ch = r[C_PY] * Nyi + r[C_PXY] * Nxi + r[C_PCY] * Ni;
             }
-        } else if (BI == 1) {
-            const double bi = S2 * Nyi + EXY * Nxi;
-            T += 2.0 * wheta * bi + r[C_WTY] * Ni;
-            F += r[C_WF1Y] * Ni + r[C_WF2] * Nyi;
-            const double whE2 = r[C_WHE2];
-            const double m1 = 4.0 * wheta * Nyi, m2 = wheta * Nxi, m3 = whE2 * bi, m4 = r[C_WDTYDV] * Ni;
-            const double o1 = 2.0 * wheta * Nyi, n4 = r[C_WDTXDV] * Ni;
-            const double Yi = r[C_PY] * Nyi + r[C_PXY] * Nxi + r[C_PCY] * Ni;
 #pragma unroll
             for (int j = 0; j < NOD; ++j) {
-                const double Nj = qN[j];
-                const double Nxj = I11 * qXi[j] + I12 * qEta[j];
-                const double Nyj = I21 * qXi[j] + I22 * qEta[j];
-                const double aj = S1 * Nxj + EXY * Nyj;
-                const double bj = S2 * Nyj + EXY * Nxj;
-                K[0][j] += o1 * Nxj + m2 * Nyj + m3 * aj + n4 * Nj;
-                K[1][j] += m1 * Nyj + m2 * Nxj + m3 * bj + m4 * Nj;
-                K[2][j] += Yi * Nj;
+                const T Nj = qN[j];
+                const T Nxj = I11 * qXi[j] + I12 * qEta[j];
+                const T Nyj = I21 * qXi[j] + I22 * qEta[j];
+                K[0][j] += cxx * Nxj + cmix * Nyj + cn0 * Nj;
+                K[1][j] += cyy * Nyj + cmix * Nxj + cn1 * Nj;
+                K[2][j] += ch * Nj;
             }
         } else {
-            const double psi = Ni + r[C_TU] * Nxi + r[C_TV] * Nyi;
-            T += r[C_TH] * psi;
-            F += r[C_FH] * psi;
-            const double Wpsi = r[C_W] * psi;
-            const double hAX = Wpsi * r[C_AX], hAY = Wpsi * r[C_AY], hRH = Wpsi * r[C_RH];
-            const double hG = Wpsi * r[C_GAM], hRU = Wpsi * r[C_RU], hRV = Wpsi * r[C_RV];
+            const T psi = Ni + r[C_TU] * Nxi + r[C_TV] * Nyi;
+            Ts += r[C_TH] * psi;
+            Fs += r[C_FH] * psi;
+            const T Wpsi = r[C_W] * psi;
+            const T hAX = Wpsi * r[C_AX], hAY = Wpsi * r[C_AY], hRH = Wpsi * r[C_RH];
+            const T hG = Wpsi * r[C_GAM], hRU = Wpsi * r[C_RU], hRV = Wpsi * r[C_RV];
 #pragma unroll
             for (int j = 0; j < NOD; ++j) {
-                const double Nj = qN[j];
-                const double Nxj = I11 * qXi[j] + I12 * qEta[j];
-                const double Nyj = I21 * qXi[j] + I22 * qEta[j];
+                const T Nj = qN[j];
+                const T Nxj = I11 * qXi[j] + I12 * qEta[j];
+                const T Nyj = I21 * qXi[j] + I22 * qEta[j];
                 K[0][j] += hAX * Nj + hRH * Nxj;
                 K[1][j] += hAY * Nj + hRH * Nyj;
                 K[2][j] += hG * Nj + hRU * Nxj + hRV * Nyj;
             }
         }
     }
-    Tr = T; Fr = F;
+    Tr = Ts; Fr = Fs;
+}
+
+// Shape tables in LDS for the row stage, in type T: the FP64 tables themselves for the plain
+// form, FP32 copies for the precision form.
+template <typename T>
+__device__ __forceinline__ int row_table_doubles(int tstride) {
+    return sizeof(T) == sizeof(double) ? 0 : (3 * tstride + 1) / 2;
 }
 
 // Per-element kernel: EPB elements per 256-thread block (runtime), dense thread mappings
 // for the three stages: (element, node) gathers, (element, point) physics, (element, row)
-// rows. The caller guarantees epb*NOD <= 256 and epb*nip <= 256.
-template <int NOD, bool RONLY>
+// rows. The caller guarantees epb*NOD <= 256 and epb*nip <= 256. T is the arithmetic type of
+// the physics and the element rows (double, or float for the precision form); the rows are
+// stored in T at their position in the node-incidence order.
+template <int NOD, typename T, bool RONLY>
 __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, const int32_t *__restrict__ conn,
                                                  const double *__restrict__ xn, const double *__restrict__ yn,
                                                  FieldPtrs f, const double *__restrict__ bnod,
                                                  const ShapeTables *__restrict__ tab, Params p,
                                                  const int32_t *__restrict__ inc_pos,
-                                                 double *__restrict__ EB, double *__restrict__ RE) {
+                                                 T *__restrict__ EB, T *__restrict__ RE) {
     extern __shared__ double lds[];
     const int tstride = nip * NOD;
     double *sN = lds;                       // [nip][NOD]
     double *sXi = sN + tstride;
     double *sEta = sXi + tstride;
     double *sW = sEta + tstride;            // [nip]
-    double *work = lds + ((3 * tstride + nip + 1) & ~1);  // 16-byte aligned union:
+    T *rN = reinterpret_cast<T *>(sN), *rXi = reinterpret_cast<T *>(sXi), *rEta = reinterpret_cast<T *>(sEta);
+    const int ttab = (3 * tstride + nip + 1) & ~1;
+    if (sizeof(T) != sizeof(double)) {
+        rN = reinterpret_cast<T *>(lds + ttab);
+        rXi = rN + tstride;
+        rEta = rXi + tstride;
+    }
+    double *work = lds + ttab + ((row_table_doubles<T>(tstride) + 1) & ~1);  // 16-byte aligned union:
                                             // nodal [epb][NV_COUNT][NOD] / coef [epb][nip][kNC]
     const int tid = threadIdx.x;
     for (int k = tid; k < tstride; k += 256) {
         const int q = k / NOD, i = k - q * NOD;
-        sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
+        const double vN = tab->N[q][i], vXi = tab->dxi[q][i], vEta = tab->deta[q][i];
+        sN[k] = vN; sXi[k] = vXi; sEta[k] = vEta;
+        if (sizeof(T) != sizeof(double)) {
+            rN[k] = T(vN); rXi[k] = T(vXi); rEta[k] = T(vEta);
+        }
     }
     if (tid < nip) sW[tid] = tab->w[tid];
     const int nblocks = (nele + epb - 1) / epb;
@@ -463,21 +477,19 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     __syncthreads();
 
     // stage 2: physics, one (element, integration point) per thread
-    double c[kNC];
+    T c[kNC];
     const int pel = tid / nip, pq = tid - pel * nip;
     const bool phys = tid < epb * nip && e0 + pel < nele;
-    if (phys) ip_physics<NOD>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
-                              sW[pq], p, c);
+    if (phys) ip_physics<NOD, T>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
+                                 sW[pq], p, c);
     __syncthreads();  // the nodal staging is dead from here; the coefficient records reuse it
+    T *rec = reinterpret_cast<T *>(work);
     if (phys) {
-        double *dst = work + (pel * nip + pq) * kNC;
+        T *dst = rec + (pel * nip + pq) * kNC;
 #pragma unroll
-        for (int k = 0; k < kNC; k += 2) {
-            double2 t;
-            t.x = c[k];
-            t.y = (k + 1 < C_COUNT) ? c[k + 1] : 0.0;
-            *reinterpret_cast<double2 *>(dst + k) = t;
-        }
+        for (int k = C_COUNT; k < kNC; ++k) c[k] = T(0);
+#pragma unroll
+        for (int k = 0; k < kNC; ++k) dst[k] = c[k];
     }
     __syncthreads();
 
@@ -489,28 +501,28 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     // the three block rows one after the other: 3*NOD accumulators instead of 9*NOD;
     // the row is stored at its position in the node-incidence order
     const int64_t row = inc_pos[e * NOD + i];
-    double *re = RE + row * 6;
-    double *eb = EB + row * eb_stride<NOD>();
-    const double *crr = work + el * nip * kNC;
-    double T, F, Kb[3][NOD];
-    element_row_b<NOD, 0>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
-    re[0] = T; re[1] = F;
+    T *re = RE + row * 6;
+    T *eb = EB + row * eb_stride<NOD, T>();
+    const T *crr = rec + el * nip * kNC;
+    T Tr, Fr, Kb[3][NOD];
+    element_row_b<NOD, 0, T>(i, nip, crr, rN, rXi, rEta, Tr, Fr, Kb);
+    re[0] = Tr; re[1] = Fr;
     if (!RONLY) {
 #pragma unroll
         for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = Kb[b][j];
     }
-    element_row_b<NOD, 1>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
-    re[2] = T; re[3] = F;
+    element_row_b<NOD, 1, T>(i, nip, crr, rN, rXi, rEta, Tr, Fr, Kb);
+    re[2] = Tr; re[3] = Fr;
     if (!RONLY) {
 #pragma unroll
         for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int j = 0; j < NOD; ++j) eb[(3 + b) * NOD + j] = Kb[b][j];
     }
-    element_row_b<NOD, 2>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
-    re[4] = T; re[5] = F;
+    element_row_b<NOD, 2, T>(i, nip, crr, rN, rXi, rEta, Tr, Fr, Kb);
+    re[4] = Tr; re[5] = Fr;
     if (!RONLY) {
 #pragma unroll
         for (int b = 0; b < 3; ++b)
@@ -519,100 +531,109 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     }
 }
 
-// Node rows: one wave per node. The incident element rows (stored contiguously per node, in
-// incidence order, by k_element) are added, in ascending element order, into a per-wave LDS buffer that
-// holds the node's three CSR rows (9*deg values); within one element row no two values
-// share a column, so every sum has a fixed order and the result is bitwise reproducible.
-// The buffer is then written to K coalesced along the rows.
-template <int NOD>
+// Node rows: kRowLanes lanes per node, 64 / kRowLanes nodes per wave. The incident element
+// rows (stored contiguously per node, in incidence order, by k_element) are added, in
+// ascending element order, into an LDS buffer that holds the node's three CSR rows
+// (9*deg values); within one element row no two values share a column, and the adds of one
+// lane group are issued in program order, so every sum has a fixed order and the result is
+// bitwise reproducible. The buffer is then written to K, contiguous along the rows. Element
+// rows of type TE (float in the precision form) are accumulated in FP64.
+constexpr int kRowLanes = 16;
+constexpr int kRowNodesPerWave = 64 / kRowLanes;
+constexpr int kRowNodesPerBlock = 4 * kRowNodesPerWave;
+
+template <int NOD, typename TE>
 __global__ __launch_bounds__(256) void k_rows(int nn, int maxdeg, const int32_t *__restrict__ inc_ptr,
-                                              const int32_t *__restrict__ inc, const uint8_t *__restrict__ pos,
+                                              const uint8_t *__restrict__ pos,
                                               const int64_t *__restrict__ rowptr_n, const uint8_t *__restrict__ diag,
-                                              int64_t nnzN, const double *__restrict__ EB,
-                                              const double *__restrict__ RE, double *__restrict__ R,
+                                              int64_t nnzN, const TE *__restrict__ EB,
+                                              const TE *__restrict__ RE, double *__restrict__ R,
                                               double *__restrict__ Tint, double *__restrict__ Fext,
                                               double *__restrict__ Kval, int check_nan, int *__restrict__ status) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int k = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
-    if (k >= nn) return;  // whole wave; no block-wide barriers below
-    double *buf = lds + wave * 9 * maxdeg;
+    const int sub = lane / kRowLanes, gl = lane - sub * kRowLanes;
+    const int slot = wave * kRowNodesPerWave + sub;
+    const int k = xcd_block(blockIdx.x, gridDim.x) * kRowNodesPerBlock + slot;
+    if (k >= nn) return;  // whole lane group; no barriers or cross-group shuffles below
+    double *buf = lds + slot * 9 * maxdeg;
     const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
+    const int64_t rp = rowptr_n[k];
+    const int deg = (int)(rowptr_n[k + 1] - rp);
     int bad = 0;
-    if (lane < 6) {
+    if (gl < 6) {
         // lane 2*bi: T of block row bi, lane 2*bi+1: F
         double v = 0;
-        for (int t = t0; t < t1; ++t) v += RE[(int64_t)t * 6 + lane];
-        const int bi = lane >> 1;
-        if (lane & 1) {
+        for (int t = t0; t < t1; ++t) v += double(RE[(int64_t)t * 6 + gl]);
+        const int bi = gl >> 1;
+        if (gl & 1) {
             if (Fext) Fext[(int64_t)bi * nn + k] = v;
         } else {
             if (Tint) Tint[(int64_t)bi * nn + k] = v;
         }
         const double other = __shfl_xor(v, 1, 64);
-        if (!(lane & 1)) {
+        if (!(gl & 1)) {
             const double r = v - other;
             R[(int64_t)bi * nn + k] = r;
             if (check_nan && isnan(r)) bad |= 1;
         }
     }
-    if (Kval) {
-        const int64_t rp = rowptr_n[k];
-        const int deg = (int)(rowptr_n[k + 1] - rp);
-        for (int v = lane; v < 9 * deg; v += 64) buf[v] = 0.0;
-        // the node's element rows t0..t1-1 are contiguous (incidence order); up to 8 at a
-        // time: all loads first, then the adds in ascending element order
-        constexpr int NV = 9 * NOD;               // values per element row
-        constexpr int VPL = (NV + 63) / 64;       // values per lane
-        for (int c0 = t0; c0 < t1; c0 += 8) {
-            const int cnt = min(8, t1 - c0);
-            double vals[8][VPL];
-            int cols[8][VPL];
+    for (int v = gl; v < 9 * deg; v += kRowLanes) buf[v] = 0.0;
+    // the node's element rows t0..t1-1 are contiguous (incidence order); up to 4 at a time:
+    // all loads first, then the adds in ascending element order
+    constexpr int NV = 9 * NOD;                           // values per element row
+    constexpr int VPL = (NV + kRowLanes - 1) / kRowLanes; // values per lane
+    constexpr int CH = 4;
+    for (int c0 = t0; c0 < t1; c0 += CH) {
+        const int cnt = min(CH, t1 - c0);
+        TE vals[CH][VPL];
+        int cols[CH][VPL];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
+        for (int c = 0; c < CH; ++c) {
 #pragma unroll
-                for (int u = 0; u < VPL; ++u) {
-                    const int v = lane + 64 * u;
-                    if (c < cnt && v < NV) {
-                        const int bk = v / NOD, j = v - bk * NOD;
-                        vals[c][u] = EB[(int64_t)(c0 + c) * eb_stride<NOD>() + v];
-                        cols[c][u] = pos[(int64_t)(c0 + c) * NOD + j];
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-#pragma unroll
-                for (int u = 0; u < VPL; ++u) {
-                    const int v = lane + 64 * u;
-                    if (c < cnt && v < NV) {
-                        const int bk = v / NOD;
-                        const int bi = bk / 3, bj = bk - bi * 3;
-                        lds_add(buf + bi * 3 * deg + bj * deg + cols[c][u], vals[c][u]);
-                    }
+            for (int u = 0; u < VPL; ++u) {
+                const int v = gl + kRowLanes * u;
+                if (c < cnt && v < NV) {
+                    const int bk = v / NOD, j = v - bk * NOD;
+                    vals[c][u] = EB[(int64_t)(c0 + c) * eb_stride<NOD, TE>() + v];
+                    cols[c][u] = pos[(int64_t)(c0 + c) * NOD + j];
                 }
             }
         }
-        for (int v = lane; v < 9 * deg; v += 64) {
-            const int bi = v / (3 * deg);
-            Kval[(int64_t)bi * 3 * nnzN + 3 * rp + (v - bi * 3 * deg)] = buf[v];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+#pragma unroll
+            for (int u = 0; u < VPL; ++u) {
+                const int v = gl + kRowLanes * u;
+                if (c < cnt && v < NV) {
+                    const int bk = v / NOD;
+                    const int bi = bk / 3, bj = bk - bi * 3;
+                    lds_add(buf + bi * 3 * deg + bj * deg + cols[c][u], double(vals[c][u]));
+                }
+            }
         }
-        if (check_nan && lane < 3 && isnan(buf[lane * 3 * deg + lane * deg + diag[k]])) bad |= 2;
     }
+    for (int v = gl; v < 9 * deg; v += kRowLanes) {
+        const int bi = v / (3 * deg);
+        Kval[(int64_t)bi * 3 * nnzN + 3 * rp + (v - bi * 3 * deg)] = buf[v];
+    }
+    if (check_nan && gl < 3 && isnan(buf[gl * 3 * deg + gl * deg + diag[k]])) bad |= 2;
     if (bad) atomicOr(status, bad);
 }
 
 // Residual only: one thread per node sums its incident element residual rows (contiguous).
+template <typename TE>
 __global__ __launch_bounds__(256) void k_rows_r(int nn, const int32_t *__restrict__ inc_ptr,
-                                                const int32_t *__restrict__ inc, const double *__restrict__ RE,
-                                                double *__restrict__ R, double *__restrict__ Tint,
-                                                double *__restrict__ Fext, int check_nan, int *__restrict__ status) {
+                                                const TE *__restrict__ RE, double *__restrict__ R,
+                                                double *__restrict__ Tint, double *__restrict__ Fext, int check_nan,
+                                                int *__restrict__ status) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nn) return;
     double T0 = 0, T1 = 0, T2 = 0, F0 = 0, F1 = 0, F2 = 0;
     for (int t = inc_ptr[k]; t < inc_ptr[k + 1]; ++t) {
-        const double *re = RE + (int64_t)t * 6;
-        T0 += re[0]; F0 += re[1]; T1 += re[2]; F1 += re[3]; T2 += re[4]; F2 += re[5];
+        const TE *re = RE + (int64_t)t * 6;
+        T0 += double(re[0]); F0 += double(re[1]); T1 += double(re[2]);
+        F1 += double(re[3]); T2 += double(re[4]); F2 += double(re[5]);
     }
     const double r0 = T0 - F0, r1 = T1 - F1, r2 = T2 - F2;
     R[k] = r0; R[nn + k] = r1; R[2 * (int64_t)nn + k] = r2;
@@ -679,8 +700,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restr
         const bool phys = tid < epb * nip && belem[pel < epb ? pel : 0] >= 0;
         if (phys) {
             const Params p = CA->p;
-            ip_physics<NOD>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD, sW[pq],
-                            p, c);
+            ip_physics<NOD, double>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
+                                    sW[pq], p, c);
         }
         __syncthreads();
         if (phys) {
@@ -708,9 +729,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restr
         double T = 0, F = 0, Kb[3][NOD];
         if (slot >= 0) {
             const double *crr = work + el * nip * kNC;
-            if (bi == 0) element_row_b<NOD, 0>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
-            else if (bi == 1) element_row_b<NOD, 1>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
-            else element_row_b<NOD, 2>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+            if (bi == 0) element_row_b<NOD, 0, double>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+            else if (bi == 1) element_row_b<NOD, 1, double>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+            else element_row_b<NOD, 2, double>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
             const int node = MA->conn[(int64_t)e * NOD + i];
             deg = (int)(MA->rowptr_n[node + 1] - MA->rowptr_n[node]);
         }
@@ -772,38 +793,48 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restr
 namespace {
 constexpr size_t kLdsLimit = 64 * 1024;
 
-// Elements per 256-thread block: every stage maps at most one item per thread, and the
-// LDS (shape tables + the larger of nodal staging and coefficient records) stays <= 64 KiB.
-int element_epb(int nod, int nip) {
-    const size_t tables = size_t((3 * nip * nod + nip + 1) & ~1) * sizeof(double);
-    const size_t per_el = std::max(size_t(NV_COUNT) * nod, size_t(nip) * kNC) * sizeof(double);
-    int epb = std::min(256 / nip, 256 / nod);
-    epb = std::min<int>(epb, int((kLdsLimit - tables) / per_el));
-    return epb;
+size_t element_table_bytes(int nod, int nip, bool fp32) {
+    const int tstride = nip * nod;
+    size_t d = size_t((3 * tstride + nip + 1) & ~1);
+    if (fp32) d += size_t(((3 * tstride + 1) / 2 + 1) & ~1);
+    return d * sizeof(double);
 }
 
-size_t element_lds_bytes(int nod, int nip, int epb) {
-    const size_t tables = size_t((3 * nip * nod + nip + 1) & ~1) * sizeof(double);
-    const size_t per_el = std::max(size_t(NV_COUNT) * nod, size_t(nip) * kNC) * sizeof(double);
-    return tables + per_el * epb;
+size_t element_per_el_bytes(int nod, int nip, bool fp32) {
+    return std::max(size_t(NV_COUNT) * nod * sizeof(double), size_t(nip) * kNC * (fp32 ? sizeof(float) : sizeof(double)));
 }
 
-template <int NOD>
-hipError_t launch_element_nod(int nele, int nip, const int32_t *conn, const double *x, const double *y,
-                              const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
-                              const int32_t *inc_pos, bool ronly, double *EB, double *RE, hipStream_t s) {
-    const int epb = element_epb(NOD, nip);
-    if (epb < 1) return hipErrorInvalidConfiguration;
+template <int NOD, typename T>
+hipError_t launch_element_t(int nele, int nip, int epb, const int32_t *conn, const double *x, const double *y,
+                            const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
+                            const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
     const int nblocks = (nele + epb - 1) / epb;
     const int grid = ((nblocks + kXcds - 1) / kXcds) * kXcds;  // whole rounds over the XCDs
-    const size_t lds = element_lds_bytes(NOD, nip, epb);
-    if (ronly)
-        hipLaunchKernelGGL((k_element<NOD, true>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, EB, RE);
-    else
-        hipLaunchKernelGGL((k_element<NOD, false>), dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, EB, RE);
+    const bool fp32 = sizeof(T) == sizeof(float);
+    const size_t lds = element_table_bytes(NOD, nip, fp32) + element_per_el_bytes(NOD, nip, fp32) * epb;
+    auto kr = k_element<NOD, T, true>;
+    auto kk = k_element<NOD, T, false>;
+    auto kern = ronly ? kr : kk;
+    if (lds > kLdsLimit)
+        hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos,
+                       static_cast<T *>(EB), static_cast<T *>(RE));
     return hipGetLastError();
 }
 }  // namespace
+
+// Elements per 256-thread block: every stage maps at most one item per thread and the LDS
+// (shape tables + the larger of nodal staging and coefficient records) stays within
+// `lds_cap` bytes (at most 160 KiB, the gfx950 workgroup limit).
+int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap) {
+    int epb = std::min(256 / nip, 256 / nod);
+    const size_t tables = element_table_bytes(nod, nip, fp32);
+    if (lds_cap <= tables) return 0;
+    epb = std::min<int>(epb, int((lds_cap - tables) / element_per_el_bytes(nod, nip, fp32)));
+    return epb;
+}
+
+int element_epb_default(int nod, int nip, bool fp32) { return element_epb_max(nod, nip, fp32, kLdsLimit); }
 
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s) {
     hipLaunchKernelGGL(k_nodal_b, dim3((nn + 255) / 256), dim3(256), 0, s, nn, f, p, b);
@@ -815,35 +846,61 @@ hipError_t launch_clear_status(int *status, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_element(int nod, int nele, int nip, const int32_t *conn, const double *x, const double *y,
-                          const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
-                          const int32_t *inc_pos, bool ronly, double *EB, double *RE, hipStream_t s) {
+hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, const int32_t *conn, const double *x,
+                          const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
+                          const Params &p, const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
+    if (epb < 1 || epb * nip > 256 || epb * nod > 256) return hipErrorInvalidConfiguration;
+#define UVH_EL(NODV)                                                                                           \
+    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s) \
+                : launch_element_t<NODV, double>(nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s)
     switch (nod) {
-        case 3: return launch_element_nod<3>(nele, nip, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s);
-        case 6: return launch_element_nod<6>(nele, nip, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s);
-        case 10: return launch_element_nod<10>(nele, nip, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s);
+        case 3: UVH_EL(3);
+        case 6: UVH_EL(6);
+        case 10: UVH_EL(10);
         default: return hipErrorInvalidValue;
     }
+#undef UVH_EL
 }
 
-hipError_t launch_rows(int nod, int nn, int maxdeg, const int32_t *inc_ptr, const int32_t *inc,
-                       const uint8_t *pos, const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN,
-                       const double *EB, const double *RE, double *R, double *Tint, double *Fext, double *Kval,
-                       int check_nan, int *status, hipStream_t s) {
+size_t element_row_stride(int nod, bool fp32) {
+    const int per_line = fp32 ? 32 : 16;
+    return size_t((9 * nod + per_line - 1) / per_line) * per_line;
+}
+
+hipError_t launch_rows(int nod, int nn, int maxdeg, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
+                       const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
+                       double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
+                       hipStream_t s) {
     if (!Kval) {
-        hipLaunchKernelGGL(k_rows_r, dim3((nn + 255) / 256), dim3(256), 0, s, nn, inc_ptr, inc, RE, R, Tint, Fext,
-                           check_nan, status);
+        if (fp32)
+            hipLaunchKernelGGL(k_rows_r<float>, dim3((nn + 255) / 256), dim3(256), 0, s, nn, inc_ptr,
+                               static_cast<const float *>(RE), R, Tint, Fext, check_nan, status);
+        else
+            hipLaunchKernelGGL(k_rows_r<double>, dim3((nn + 255) / 256), dim3(256), 0, s, nn, inc_ptr,
+                               static_cast<const double *>(RE), R, Tint, Fext, check_nan, status);
         return hipGetLastError();
     }
-    const int nblocks = (nn + 3) / 4;
+    const int nblocks = (nn + kRowNodesPerBlock - 1) / kRowNodesPerBlock;
     const dim3 grid(((nblocks + kXcds - 1) / kXcds) * kXcds), block(256);
-    const size_t lds = size_t(4) * 9 * maxdeg * sizeof(double);
+    const size_t lds = size_t(kRowNodesPerBlock) * 9 * maxdeg * sizeof(double);
+    if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
+#define UVH_ROWS(NODV, TE)                                                                                     \
+    do {                                                                                                       \
+        auto kern = k_rows<NODV, TE>;                                                                          \
+        if (lds > kLdsLimit)                                                                                   \
+            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                int(lds));                                                                     \
+        hipLaunchKernelGGL(kern, grid, block, lds, s, nn, maxdeg, inc_ptr, pos, rowptr_n, diag, nnzN,           \
+                           static_cast<const TE *>(EB), static_cast<const TE *>(RE), R, Tint, Fext, Kval,      \
+                           check_nan, status);                                                                 \
+    } while (0)
     switch (nod) {
-        case 3: hipLaunchKernelGGL(k_rows<3>, grid, block, lds, s, nn, maxdeg, inc_ptr, inc, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
-        case 6: hipLaunchKernelGGL(k_rows<6>, grid, block, lds, s, nn, maxdeg, inc_ptr, inc, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
-        case 10: hipLaunchKernelGGL(k_rows<10>, grid, block, lds, s, nn, maxdeg, inc_ptr, inc, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan, status); break;
+        case 3: if (fp32) UVH_ROWS(3, float); else UVH_ROWS(3, double); break;
+        case 6: if (fp32) UVH_ROWS(6, float); else UVH_ROWS(6, double); break;
+        case 10: if (fp32) UVH_ROWS(10, float); else UVH_ROWS(10, double); break;
         default: return hipErrorInvalidValue;
     }
+#undef UVH_ROWS
     return hipGetLastError();
 }
 

@@ -3,11 +3,11 @@
 Two subcommands, both reading rocprofv3 CSV output and writing short text under
 ``profiles/``:
 
-``stats <trace_dir> <workload> [--ronly] [--command CMD]``
+``stats <trace_dir> <workload> [--fp32] [--ronly] [--command CMD]``
     turns ``rocprofv3 --kernel-trace --stats`` output (``run_kernel_stats.csv``) into
     ``profiles/kernel_stats_<workload>[_ronly].md``.
 
-``traffic <dir> <workload> [--ronly]``
+``traffic <dir> <workload> [--fp32] [--ronly]``
     turns four ``rocprofv3 --pmc`` passes into ``profiles/traffic_<workload>[_ronly].json``
     (HBM bytes per assembly, per kernel). ``<dir>`` holds ``fetch/`` and ``write/`` (FETCH_SIZE
     and WRITE_SIZE over ``bench.py``) and ``cfetch/`` and ``cwrite/`` (the same counters over
@@ -42,9 +42,11 @@ def base_name(full: str) -> str:
     return full.split("(")[0].split("<")[0].replace("void ", "").replace("uvh::", "").strip()
 
 
-def timer_name(base: str, ronly: bool) -> str | None:
+def timer_name(full: str, ronly: bool) -> str | None:
+    base = base_name(full)
     if base in MODE_KERNELS:
-        return MODE_KERNELS[base] + ("_r" if ronly else "_rk")
+        fp32 = "float" in full.split("(")[0] and base in ("k_element", "k_rows", "k_rows_r")
+        return MODE_KERNELS[base] + ("32" if fp32 else "") + ("_r" if ronly else "_rk")
     return PLAIN_KERNELS.get(base)
 
 
@@ -63,7 +65,7 @@ def per_kernel(path: str, counter: str) -> dict:
     return {k: tot[k] / len(cnt[k]) for k in tot}
 
 
-def traffic(d: str, workload: str, ronly: bool) -> dict:
+def traffic(d: str, workload: str, ronly: bool, fp32: bool = False) -> dict:
     fetch = per_kernel(os.path.join(d, "fetch"), "FETCH_SIZE")
     write = per_kernel(os.path.join(d, "write"), "WRITE_SIZE")
     cf = per_kernel(os.path.join(d, "cfetch"), "FETCH_SIZE")
@@ -78,9 +80,9 @@ def traffic(d: str, workload: str, ronly: bool) -> dict:
     total = 0.0
     names = set()
     for k in fetch:
-        t = timer_name(base_name(k), ronly)
-        if t is None:
-            continue
+        t = timer_name(k, ronly)
+        if t is None or (t.startswith(("element", "rows")) and ("32" in t) != fp32):
+            continue  # the precision-form run also makes one FP64 assembly for its error check
         fb = fetch[k] * 1024 / fscale
         wb = write.get(k, 0.0) * 1024 / wscale
         out["per_kernel_bytes"][k] = {"fetch": fb, "write": wb}
@@ -111,9 +113,9 @@ def main(argv=None) -> int:
         return 2
     cmd, d, workload = argv[:3]
     ronly = "--ronly" in argv
-    suffix = "_ronly" if ronly else ""
+    suffix = ("_fp32" if "--fp32" in argv else "") + ("_ronly" if ronly else "")
     if cmd == "traffic":
-        out = traffic(d, workload, ronly)
+        out = traffic(d, workload, ronly, "--fp32" in argv)
         fn = os.path.join(PROFILES, f"traffic_{workload}{suffix}.json")
         with open(fn, "w") as fh:
             json.dump(out, fh, indent=1)

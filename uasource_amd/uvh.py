@@ -267,13 +267,14 @@ class UvhContext:
         return self._pattern
 
     def assemble(self, CtrlVar, F0, F1, ronly=False, zero_fields=False, want_tf=False, dadh=None,
-                 nan_check=True):
-        """Host-array assembly; returns dict(R, Kval or None, Tint, Fext)."""
+                 nan_check=True, fp32=False):
+        """Host-array assembly; returns dict(R, Kval or None, Tint, Fext). ``fp32`` selects the
+        precision form (FP32 element arithmetic, FP64 accumulation)."""
         ctrl = make_ctrl(CtrlVar, F1)
         keep = []
         fields = make_fields(F0, F1, dadh, keep)
         flags = (UVH_RONLY if ronly else 0) | (UVH_ZEROFIELDS if zero_fields else 0) | \
-                (0 if nan_check else UVH_NO_NAN_CHECK)
+                (0 if nan_check else UVH_NO_NAN_CHECK) | (UVH_FP32 if fp32 else 0)
         R = np.zeros(3 * self.nn)
         Kval = None if ronly else np.zeros(self.nnz)
         Tint = np.zeros(3 * self.nn) if want_tf else None
