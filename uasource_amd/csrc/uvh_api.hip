@@ -17,8 +17,9 @@ hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b
 hipError_t launch_clear_status(int *status, hipStream_t s);
 hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, const int32_t *conn, const double *x,
                           const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
-                          const Params &p, const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s);
-hipError_t launch_rows(int nod, int nn, int maxdeg, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
+                          const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly, void *EB,
+                          void *RE, hipStream_t s);
+hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
                        hipStream_t s);
@@ -61,6 +62,7 @@ struct uvh_ctx {
     int32_t *d_conn = nullptr;
     double *d_x = nullptr, *d_y = nullptr;
     uvh::ShapeTables *d_tab = nullptr;
+    uvh::ShapeTables32 *d_tab32 = nullptr;
     int64_t *d_rowptr_n = nullptr;
     int32_t *d_inc_ptr = nullptr, *d_inc = nullptr;
     uint8_t *d_pos = nullptr, *d_diag = nullptr, *d_pos_inc = nullptr;
@@ -102,7 +104,8 @@ struct uvh_ctx {
         return UVH_E_HIP;
     }
     void free_mesh() {
-        hipFree(d_conn); hipFree(d_x); hipFree(d_y); hipFree(d_tab); hipFree(d_rowptr_n);
+        hipFree(d_conn); hipFree(d_x); hipFree(d_y); hipFree(d_tab); hipFree(d_tab32); hipFree(d_rowptr_n);
+        d_tab32 = nullptr;
         hipFree(d_inc_ptr); hipFree(d_inc); hipFree(d_pos); hipFree(d_b); hipFree(d_RE); hipFree(d_EB);
         hipFree(d_fields); hipFree(d_R); hipFree(d_T); hipFree(d_F); hipFree(d_K);
         hipFree(d_diag); hipFree(d_pbp); hipFree(d_belem); hipFree(d_easlot); hipFree(d_zptr); hipFree(d_zslot);
@@ -259,6 +262,7 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(dalloc(&ctx->d_x, nn));
     chk(dalloc(&ctx->d_y, nn));
     chk(dalloc(&ctx->d_tab, 1));
+    chk(dalloc(&ctx->d_tab32, 1));
     chk(dalloc(&ctx->d_rowptr_n, nn + 1));
     chk(dalloc(&ctx->d_inc_ptr, nn + 1));
     chk(dalloc(&ctx->d_inc, ne * nod));
@@ -290,6 +294,16 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(hipMemcpy(ctx->d_x, m.x.data(), nn * sizeof(double), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_y, m.y.data(), nn * sizeof(double), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_tab, &m.tab, sizeof(uvh::ShapeTables), hipMemcpyHostToDevice));
+    {
+        uvh::ShapeTables32 t32{};
+        for (int q = 0; q < uvh::kMaxNip; ++q)
+            for (int a = 0; a < uvh::kMaxNod; ++a) {
+                t32.N[q][a] = float(m.tab.N[q][a]);
+                t32.dxi[q][a] = float(m.tab.dxi[q][a]);
+                t32.deta[q][a] = float(m.tab.deta[q][a]);
+            }
+        chk(hipMemcpy(ctx->d_tab32, &t32, sizeof(t32), hipMemcpyHostToDevice));
+    }
     chk(hipMemcpy(ctx->d_rowptr_n, m.rowptr_n.data(), (nn + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_inc_ptr, m.inc_ptr.data(), (nn + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_inc, m.inc.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -404,12 +418,13 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     if (e == hipSuccess)
         e = ctx->timed(fp32 ? (ronly ? "element32_r" : "element32_rk") : (ronly ? "element_r" : "element_rk"), s, [&] {
             return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[fp32 ? 1 : 0], fp32, ctx->d_conn, ctx->d_x,
-                                       ctx->d_y, f, ctx->d_b, ctx->d_tab, p, ctx->d_inc_pos, ronly, ctx->d_EB,
+                                       ctx->d_y, f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly,
+                                       ctx->d_EB,
                                        ctx->d_RE, s);
         });
     if (e == hipSuccess)
         e = ctx->timed(fp32 ? (ronly ? "rows32_r" : "rows32_rk") : (ronly ? "rows_r" : "rows_rk"), s, [&] {
-            return uvh::launch_rows(m.nod, m.nn, m.maxdeg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
+            return uvh::launch_rows(m.nod, m.nn, m.max_block_deg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
                                     ctx->d_diag, m.nnzN(), ctx->d_EB, ctx->d_RE, dR, dT, dF, ronly ? nullptr : dK,
                                     check, ctx->d_status, s);
         });

@@ -9,6 +9,8 @@ namespace uvh {
 
 constexpr int kMaxNod = 10;
 constexpr int kMaxNip = 32;
+// Nodes per workgroup of the row-scatter kernel (consecutive node numbers).
+constexpr int kRowNodesPerBlock = 16;
 
 // Shape-function values and local derivatives at the integration points, plus the
 // weights, for one element type and quadrature rule. Filled on the host from the
@@ -19,6 +21,13 @@ struct ShapeTables {
     double dxi[kMaxNip][kMaxNod];
     double deta[kMaxNip][kMaxNod];
     double w[kMaxNip];
+};
+
+// FP32 copies of the shape tables for the precision form's element rows.
+struct ShapeTables32 {
+    float N[kMaxNip][kMaxNod];
+    float dxi[kMaxNip][kMaxNod];
+    float deta[kMaxNip][kMaxNod];
 };
 
 // Per-mesh host data: the node graph that fixes the CSR pattern of K and the
@@ -36,6 +45,7 @@ struct MeshHost {
     std::vector<int32_t> inc_pos;     // nele*nod: position of (e, a) in the incidence order (inverse of inc)
     std::vector<uint8_t> pos_inc;     // nele*nod*nod: pos rows in the incidence order
     int32_t maxdeg = 0;               // largest node degree
+    int64_t max_block_deg = 0;        // largest sum of degrees over kRowNodesPerBlock consecutive nodes
     ShapeTables tab{};
     int64_t nnzN() const { return rowptr_n.empty() ? 0 : rowptr_n.back(); }
 };

@@ -347,9 +347,20 @@ __device__ __forceinline__ void gather_node(const int32_t *__restrict__ conn, co
 //   Kyu_ij = (2 wh eta Ny_i + wE b_i S1) Nx_j + (wh eta Nx_i + wE b_i EXY) Ny_j + W dtaux/dv N_i N_j
 //   Kyv_ij = (4 wh eta Ny_i + wE b_i S2) Ny_j + (wh eta Nx_i + wE b_i EXY) Nx_j + W dtauy/dv N_i N_j
 // (wh eta = W h eta, wE = 2 W h E), summed over the points.
+// The row's own shape values (index i, different in every lane) come from the LDS tables
+// sN/sXi/sEta ([nip][NOD]); the column values (index j, the same in every lane) are read
+// from the global tables g, which the compiler turns into scalar loads.
+template <typename T>
+struct RowTab;
+template <>
+struct RowTab<double> { using type = ShapeTables; };
+template <>
+struct RowTab<float> { using type = ShapeTables32; };
+
 template <int NOD, int BI, typename T>
 __device__ __forceinline__ void element_row_b(int i, int nip, const T *__restrict__ cr, const T *__restrict__ sN,
-                                              const T *__restrict__ sXi, const T *__restrict__ sEta, T &Tr, T &Fr,
+                                              const T *__restrict__ sXi, const T *__restrict__ sEta,
+                                              const typename RowTab<T>::type *__restrict__ g, T &Tr, T &Fr,
                                               T (&K)[3][NOD]) {
     T Ts = 0, Fs = 0;
 #pragma unroll
@@ -358,11 +369,11 @@ __device__ __forceinline__ void element_row_b(int i, int nip, const T *__restric
         for (int j = 0; j < NOD; ++j) K[b][j] = T(0);
     for (int q = 0; q < nip; ++q) {
         const T *r = cr + q * kNC;
-        const T *qN = sN + q * NOD, *qXi = sXi + q * NOD, *qEta = sEta + q * NOD;
+        const T *qN = g->N[q], *qXi = g->dxi[q], *qEta = g->deta[q];
         const T I11 = r[C_I11], I12 = r[C_I12], I21 = r[C_I21], I22 = r[C_I22];
-        const T Ni = qN[i];
-        const T Nxi = I11 * qXi[i] + I12 * qEta[i];
-        const T Nyi = I21 * qXi[i] + I22 * qEta[i];
+        const T Ni = sN[q * NOD + i];
+        const T Nxi = I11 * sXi[q * NOD + i] + I12 * sEta[q * NOD + i];
+        const T Nyi = I21 * sXi[q * NOD + i] + I22 * sEta[q * NOD + i];
         if (BI < 2) {
             const T S1 = r[C_S1], S2 = r[C_S2], EXY = r[C_EXY];
             const T wheta = r[C_WHETA], whE2 = r[C_WHE2];
@@ -436,7 +447,8 @@ template <int NOD, typename T, bool RONLY>
 __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, const int32_t *__restrict__ conn,
                                                  const double *__restrict__ xn, const double *__restrict__ yn,
                                                  FieldPtrs f, const double *__restrict__ bnod,
-                                                 const ShapeTables *__restrict__ tab, Params p,
+                                                 const ShapeTables *__restrict__ tab,
+                                                 const typename RowTab<T>::type *__restrict__ rtab, Params p,
                                                  const int32_t *__restrict__ inc_pos,
                                                  T *__restrict__ EB, T *__restrict__ RE) {
     extern __shared__ double lds[];
@@ -493,41 +505,31 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     }
     __syncthreads();
 
-    // stage 3: rows, one (element, row) per thread
-    if (tid >= epb * NOD) return;
-    const int el = tid / NOD, i = tid - el * NOD;
-    const int64_t e = e0 + el;
-    if (e >= nele) return;
-    // the three block rows one after the other: 3*NOD accumulators instead of 9*NOD;
-    // the row is stored at its position in the node-incidence order
-    const int64_t row = inc_pos[e * NOD + i];
-    T *re = RE + row * 6;
-    T *eb = EB + row * eb_stride<NOD, T>();
-    const T *crr = rec + el * nip * kNC;
-    T Tr, Fr, Kb[3][NOD];
-    element_row_b<NOD, 0, T>(i, nip, crr, rN, rXi, rEta, Tr, Fr, Kb);
-    re[0] = Tr; re[1] = Fr;
-    if (!RONLY) {
+    // stage 3: element rows, as (block row, element, row) items dealt round-robin to the
+    // threads; each item accumulates 3 x NOD Jacobian terms and its T and F over the points
+    const int nrow = epb * NOD;
+    for (int it = tid; it < 3 * nrow; it += 256) {
+        const int bi = it / nrow;
+        const int rr = it - bi * nrow;
+        const int el = rr / NOD, i = rr - el * NOD;
+        const int64_t e = e0 + el;
+        if (e >= nele) continue;
+        // the row is stored at its position in the node-incidence order
+        const int64_t row = inc_pos[e * NOD + i];
+        const T *crr = rec + el * nip * kNC;
+        T Tr, Fr, Kb[3][NOD];
+        if (bi == 0) element_row_b<NOD, 0, T>(i, nip, crr, rN, rXi, rEta, rtab, Tr, Fr, Kb);
+        else if (bi == 1) element_row_b<NOD, 1, T>(i, nip, crr, rN, rXi, rEta, rtab, Tr, Fr, Kb);
+        else element_row_b<NOD, 2, T>(i, nip, crr, rN, rXi, rEta, rtab, Tr, Fr, Kb);
+        RE[row * 6 + 2 * bi] = Tr;
+        RE[row * 6 + 2 * bi + 1] = Fr;
+        if (!RONLY) {
+            T *eb = EB + row * eb_stride<NOD, T>() + bi * 3 * NOD;
 #pragma unroll
-        for (int b = 0; b < 3; ++b)
+            for (int b = 0; b < 3; ++b)
 #pragma unroll
-            for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = Kb[b][j];
-    }
-    element_row_b<NOD, 1, T>(i, nip, crr, rN, rXi, rEta, Tr, Fr, Kb);
-    re[2] = Tr; re[3] = Fr;
-    if (!RONLY) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-#pragma unroll
-            for (int j = 0; j < NOD; ++j) eb[(3 + b) * NOD + j] = Kb[b][j];
-    }
-    element_row_b<NOD, 2, T>(i, nip, crr, rN, rXi, rEta, Tr, Fr, Kb);
-    re[4] = Tr; re[5] = Fr;
-    if (!RONLY) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-#pragma unroll
-            for (int j = 0; j < NOD; ++j) eb[(6 + b) * NOD + j] = Kb[b][j];
+                for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = Kb[b][j];
+        }
     }
 }
 
@@ -540,10 +542,10 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
 // rows of type TE (float in the precision form) are accumulated in FP64.
 constexpr int kRowLanes = 16;
 constexpr int kRowNodesPerWave = 64 / kRowLanes;
-constexpr int kRowNodesPerBlock = 4 * kRowNodesPerWave;
+static_assert(kRowNodesPerBlock == 4 * kRowNodesPerWave, "row kernel: 4 waves of lane groups per block");
 
 template <int NOD, typename TE>
-__global__ __launch_bounds__(256) void k_rows(int nn, int maxdeg, const int32_t *__restrict__ inc_ptr,
+__global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict__ inc_ptr,
                                               const uint8_t *__restrict__ pos,
                                               const int64_t *__restrict__ rowptr_n, const uint8_t *__restrict__ diag,
                                               int64_t nnzN, const TE *__restrict__ EB,
@@ -554,11 +556,14 @@ __global__ __launch_bounds__(256) void k_rows(int nn, int maxdeg, const int32_t 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / kRowLanes, gl = lane - sub * kRowLanes;
     const int slot = wave * kRowNodesPerWave + sub;
-    const int k = xcd_block(blockIdx.x, gridDim.x) * kRowNodesPerBlock + slot;
+    const int k0 = xcd_block(blockIdx.x, gridDim.x) * kRowNodesPerBlock;  // the block's nodes are consecutive
+    const int k = k0 + slot;
     if (k >= nn) return;  // whole lane group; no barriers or cross-group shuffles below
-    double *buf = lds + slot * 9 * maxdeg;
-    const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
     const int64_t rp = rowptr_n[k];
+    // the block's row buffers are packed: node k's starts at 9 * (rowptr_n[k] - rowptr_n[k0]);
+    // the launch provides the largest such total over the blocks
+    double *buf = lds + 9 * (rp - rowptr_n[k0]);
+    const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
     const int deg = (int)(rowptr_n[k + 1] - rp);
     int bad = 0;
     if (gl < 6) {
@@ -729,9 +734,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restr
         double T = 0, F = 0, Kb[3][NOD];
         if (slot >= 0) {
             const double *crr = work + el * nip * kNC;
-            if (bi == 0) element_row_b<NOD, 0, double>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
-            else if (bi == 1) element_row_b<NOD, 1, double>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
-            else element_row_b<NOD, 2, double>(i, nip, crr, sN, sXi, sEta, T, F, Kb);
+            if (bi == 0) element_row_b<NOD, 0, double>(i, nip, crr, sN, sXi, sEta, MA->tab, T, F, Kb);
+            else if (bi == 1) element_row_b<NOD, 1, double>(i, nip, crr, sN, sXi, sEta, MA->tab, T, F, Kb);
+            else element_row_b<NOD, 2, double>(i, nip, crr, sN, sXi, sEta, MA->tab, T, F, Kb);
             const int node = MA->conn[(int64_t)e * NOD + i];
             deg = (int)(MA->rowptr_n[node + 1] - MA->rowptr_n[node]);
         }
@@ -792,6 +797,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restr
 
 namespace {
 constexpr size_t kLdsLimit = 64 * 1024;
+// Default LDS cap of the element kernel: two 256-thread workgroups per CU (160 KiB per CU).
+constexpr size_t kElementLds = 80 * 1024;
 
 size_t element_table_bytes(int nod, int nip, bool fp32) {
     const int tstride = nip * nod;
@@ -806,8 +813,11 @@ size_t element_per_el_bytes(int nod, int nip, bool fp32) {
 
 template <int NOD, typename T>
 hipError_t launch_element_t(int nele, int nip, int epb, const int32_t *conn, const double *x, const double *y,
-                            const FieldPtrs &f, const double *b, const ShapeTables *tab, const Params &p,
-                            const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
+                            const FieldPtrs &f, const double *b, const ShapeTables *tab, const ShapeTables32 *tab32,
+                            const Params &p, const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
+    const typename RowTab<T>::type *rtab;
+    if constexpr (sizeof(T) == sizeof(double)) rtab = tab;
+    else rtab = tab32;
     const int nblocks = (nele + epb - 1) / epb;
     const int grid = ((nblocks + kXcds - 1) / kXcds) * kXcds;  // whole rounds over the XCDs
     const bool fp32 = sizeof(T) == sizeof(float);
@@ -817,7 +827,7 @@ hipError_t launch_element_t(int nele, int nip, int epb, const int32_t *conn, con
     auto kern = ronly ? kr : kk;
     if (lds > kLdsLimit)
         hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
                        static_cast<T *>(EB), static_cast<T *>(RE));
     return hipGetLastError();
 }
@@ -834,7 +844,7 @@ int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap) {
     return epb;
 }
 
-int element_epb_default(int nod, int nip, bool fp32) { return element_epb_max(nod, nip, fp32, kLdsLimit); }
+int element_epb_default(int nod, int nip, bool fp32) { return element_epb_max(nod, nip, fp32, kElementLds); }
 
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s) {
     hipLaunchKernelGGL(k_nodal_b, dim3((nn + 255) / 256), dim3(256), 0, s, nn, f, p, b);
@@ -848,11 +858,12 @@ hipError_t launch_clear_status(int *status, hipStream_t s) {
 
 hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, const int32_t *conn, const double *x,
                           const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
-                          const Params &p, const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
+                          const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly, void *EB,
+                          void *RE, hipStream_t s) {
     if (epb < 1 || epb * nip > 256 || epb * nod > 256) return hipErrorInvalidConfiguration;
 #define UVH_EL(NODV)                                                                                           \
-    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s) \
-                : launch_element_t<NODV, double>(nele, nip, epb, conn, x, y, f, b, tab, p, inc_pos, ronly, EB, RE, s)
+    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s) \
+                : launch_element_t<NODV, double>(nele, nip, epb, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s)
     switch (nod) {
         case 3: UVH_EL(3);
         case 6: UVH_EL(6);
@@ -867,7 +878,7 @@ size_t element_row_stride(int nod, bool fp32) {
     return size_t((9 * nod + per_line - 1) / per_line) * per_line;
 }
 
-hipError_t launch_rows(int nod, int nn, int maxdeg, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
+hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
                        hipStream_t s) {
@@ -882,7 +893,7 @@ hipError_t launch_rows(int nod, int nn, int maxdeg, bool fp32, const int32_t *in
     }
     const int nblocks = (nn + kRowNodesPerBlock - 1) / kRowNodesPerBlock;
     const dim3 grid(((nblocks + kXcds - 1) / kXcds) * kXcds), block(256);
-    const size_t lds = size_t(kRowNodesPerBlock) * 9 * maxdeg * sizeof(double);
+    const size_t lds = size_t(9) * size_t(block_vals) * sizeof(double);
     if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
 #define UVH_ROWS(NODV, TE)                                                                                     \
     do {                                                                                                       \
@@ -890,7 +901,7 @@ hipError_t launch_rows(int nod, int nn, int maxdeg, bool fp32, const int32_t *in
         if (lds > kLdsLimit)                                                                                   \
             hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, \
                                 int(lds));                                                                     \
-        hipLaunchKernelGGL(kern, grid, block, lds, s, nn, maxdeg, inc_ptr, pos, rowptr_n, diag, nnzN,           \
+        hipLaunchKernelGGL(kern, grid, block, lds, s, nn, inc_ptr, pos, rowptr_n, diag, nnzN,                   \
                            static_cast<const TE *>(EB), static_cast<const TE *>(RE), R, Tint, Fext, Kval,      \
                            check_nan, status);                                                                 \
     } while (0)
