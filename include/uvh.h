@@ -102,6 +102,12 @@ int uvh_pattern_info(const uvh_ctx *ctx, int64_t *neq, int64_t *nnz);
 /* rowptr: neq+1 entries, colind: nnz entries (host arrays). */
 int uvh_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind);
 
+/* CSC order of K, for hosts that store sparse matrices by column (MATLAB's sparse, the
+ * reference's K: UaMain/uvhMatrixAssembly.m:509 via sparseUA.m:17). The pattern is structurally
+ * symmetric, so the CSC column pointers and row indices are the CSR rowptr and colind of
+ * uvh_get_pattern; the values are permuted: csc_val[k] = Kval[map[k]], map has nnz entries. */
+int uvh_get_csc_map(const uvh_ctx *ctx, int64_t *map);
+
 /* Host-buffer assembly: the uvhAssembly drop-in (uvhMatrixAssembly.m:6-585).
  * Fields are host arrays; R (3Nn) is required; Kval (nnz) may be NULL or is ignored with
  * UVH_RONLY; Tint and Fext (3Nn) may be NULL. Blocks until the results are on the host. */

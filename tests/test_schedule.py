@@ -171,3 +171,48 @@ def test_full_size_schedule_fits_and_ghost_overhead_is_small():
     overhead = S["patch_work"] / M.Nele - 1
     print(f"plain_p2: {S['npatch']} patches, peak window {S['peak_window']} doubles, ghost overhead {overhead:.3f}")
     assert overhead < 0.3
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p3", "medium_p2"])
+def test_csc_map_matches_scipy_transpose(name):
+    """K in CSC (MATLAB's sparse layout) is the CSR values permuted by the map, on the same
+    index arrays; checked against scipy's own CSR -> CSC conversion of random values."""
+    import scipy.sparse as sp
+    lib = host_lib()
+    lib.uvhh_csc_map.restype = None
+    lib.uvhh_csc_map.argtypes = [C.c_void_p, C.c_void_p]
+    M = build_mesh(WORKLOADS[name])
+    conn = np.ascontiguousarray(M.connectivity, dtype=np.int32)
+    x = np.ascontiguousarray(M.coordinates[:, 0])
+    y = np.ascontiguousarray(M.coordinates[:, 1])
+    err = C.create_string_buffer(256)
+    h = lib.uvhh_build(M.Nele, M.nod, M.Nnodes, conn.ctypes.data, x.ctypes.data, y.ctypes.data, 0, 0, 0, err, 256)
+    assert h, err.value.decode()
+    try:
+        rp = np.zeros(lib.uvhh_size(h, 0), dtype=np.int64)
+        lib.uvhh_copy(h, 0, rp.ctypes.data)
+        ci = np.zeros(lib.uvhh_size(h, 1), dtype=np.int32)
+        lib.uvhh_copy(h, 1, ci.ctypes.data)
+        nnzN, Nn = int(rp[-1]), M.Nnodes
+        # the full 3Nn x 3Nn pattern as uvh_get_pattern lays it out
+        rowptr = np.zeros(3 * Nn + 1, dtype=np.int64)
+        colind = np.zeros(9 * nnzN, dtype=np.int32)
+        for bi in range(3):
+            for k in range(Nn):
+                start = bi * 3 * nnzN + 3 * rp[k]
+                deg = rp[k + 1] - rp[k]
+                rowptr[bi * Nn + k] = start
+                for bj in range(3):
+                    colind[start + bj * deg:start + (bj + 1) * deg] = ci[rp[k]:rp[k + 1]] + bj * Nn
+        rowptr[-1] = 9 * nnzN
+        cmap = np.zeros(9 * nnzN, dtype=np.int64)
+        lib.uvhh_csc_map(h, cmap.ctypes.data)
+    finally:
+        lib.uvhh_free(h)
+    vals = np.random.default_rng(5).standard_normal(9 * nnzN)
+    K = sp.csr_matrix((vals, colind, rowptr), shape=(3 * Nn, 3 * Nn))
+    Kc = K.tocsc()
+    Kc.sort_indices()
+    assert np.array_equal(Kc.indptr, rowptr) and np.array_equal(Kc.indices, colind)
+    assert np.array_equal(Kc.data, vals[cmap])
+    assert np.array_equal(np.sort(cmap), np.arange(9 * nnzN))  # a permutation

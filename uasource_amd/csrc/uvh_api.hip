@@ -365,6 +365,13 @@ int uvh_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind) {
     return UVH_OK;
 }
 
+int uvh_get_csc_map(const uvh_ctx *ctx, int64_t *map) {
+    if (!ctx || !ctx->has_mesh) return UVH_E_STATE;
+    if (!map) return UVH_E_ARG;
+    uvh::build_csc_map(ctx->mesh, map);
+    return UVH_OK;
+}
+
 int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
                         double *dK, double *dT, double *dF, void *stream) {
     if (!ctx) return UVH_E_ARG;

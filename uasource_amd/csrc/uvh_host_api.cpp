@@ -95,6 +95,8 @@ void uvhh_copy(void *h, int32_t which, void *dst) {
     }
 }
 
+void uvhh_csc_map(void *h, int64_t *map) { uvh::build_csc_map(static_cast<Built *>(h)->m, map); }
+
 void uvhh_free(void *h) { delete static_cast<Built *>(h); }
 
 }  // extern "C"

@@ -78,6 +78,9 @@ std::string build_schedule(MeshHost &m, int epb, int64_t window, int target_patc
 
 // Builds rowptr_n/colind_n, inc_ptr/inc and pos from conn; returns "" or an error message.
 std::string build_mesh_host(MeshHost &m);
+// CSC value order of K: csc[k] = csr[map[k]] with CSC colptr/rowind equal to CSR rowptr/colind
+// (the pattern is structurally symmetric). map has 9 * nnzN entries.
+void build_csc_map(const MeshHost &m, int64_t *map);
 // Fills the shape tables for nod nodes at the nip points pts (nip x 2) with weights wts.
 std::string fill_shape_tables(ShapeTables &t, int nod, int nip, const double *pts, const double *wts);
 

@@ -167,4 +167,32 @@ std::string build_mesh_host(MeshHost &m) {
     return "";
 }
 
+void build_csc_map(const MeshHost &m, int64_t *map) {
+    // CSR slot idx = bi*3*nnzN + 3*rowptr_n[n] + bj*deg(n) + t holds K((bi,n),(bj,m)) with
+    // m = adj(n)[t]. Read as CSC (same index arrays), the slot is column (bi,n), row (bj,m),
+    // and must hold K((bj,m),(bi,n)), which CSR keeps at bj*3*nnzN + 3*rowptr_n[m] + bi*deg(m)
+    // + (position of n in adj(m)).
+    const int64_t nn = m.nn, nnzN = m.nnzN();
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nthreads; ++w)
+        th.emplace_back([&, w] {
+            for (int64_t n = w; n < nn; n += nthreads) {
+                const int64_t rp = m.rowptr_n[n], deg = m.rowptr_n[n + 1] - rp;
+                for (int64_t t = 0; t < deg; ++t) {
+                    const int32_t mm = m.colind_n[rp + t];
+                    const int32_t *c0 = m.colind_n.data() + m.rowptr_n[mm];
+                    const int32_t *c1 = m.colind_n.data() + m.rowptr_n[mm + 1];
+                    const int64_t tpos = std::lower_bound(c0, c1, int32_t(n)) - c0;
+                    const int64_t degm = m.rowptr_n[mm + 1] - m.rowptr_n[mm];
+                    for (int bi = 0; bi < 3; ++bi)
+                        for (int bj = 0; bj < 3; ++bj)
+                            map[bi * 3 * nnzN + 3 * rp + bj * deg + t] =
+                                bj * 3 * nnzN + 3 * m.rowptr_n[mm] + bi * degm + tpos;
+                }
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
 }  // namespace uvh

@@ -34,7 +34,7 @@ TAU_OPTIONS = {"taus": 0, "taut": 1, "tau1": 2, "tau2": 3, "tau0": 4}
 
 # Every entry point declared in include/uvh.h.
 EXPORTED = ("uvh_create", "uvh_destroy", "uvh_last_error", "uvh_set_mesh", "uvh_pattern_info",
-            "uvh_get_pattern", "uvh_assemble", "uvh_assemble_device", "uvh_sync_status",
+            "uvh_get_pattern", "uvh_get_csc_map", "uvh_assemble", "uvh_assemble_device", "uvh_sync_status",
             "uvh_profile", "uvh_kernel_times")
 
 
@@ -98,6 +98,8 @@ def load_library(path: str = LIB_PATH):
     lib.uvh_pattern_info.restype = C.c_int
     lib.uvh_get_pattern.argtypes = [vp, vp, vp]
     lib.uvh_get_pattern.restype = C.c_int
+    lib.uvh_get_csc_map.argtypes = [vp, vp]
+    lib.uvh_get_csc_map.restype = C.c_int
     lib.uvh_assemble.argtypes = [vp, C.POINTER(UvhCtrl), C.POINTER(UvhFields), C.c_uint32, dp, dp, dp, dp]
     lib.uvh_assemble.restype = C.c_int
     lib.uvh_assemble_device.argtypes = [vp, C.POINTER(UvhCtrl), C.POINTER(UvhFields), C.c_uint32, dp, dp, dp, dp, vp]
@@ -217,6 +219,7 @@ class UvhContext:
         self.nn = self.nele = self.nod = self.nip = 0
         self.neq = self.nnz = 0
         self._pattern = None
+        self._csc_map = None
 
     def close(self):
         if self.h:
@@ -255,6 +258,7 @@ class UvhContext:
         self._check(self.lib.uvh_pattern_info(self.h, C.byref(neq), C.byref(nnz)), "uvh_pattern_info")
         self.neq, self.nnz = neq.value, nnz.value
         self._pattern = None
+        self._csc_map = None
         self.mesh_key = mesh_key(MUA)
 
     def pattern(self):
@@ -305,10 +309,21 @@ class UvhContext:
         n = self.lib.uvh_kernel_times(self.h, names, tot, cnt, cap)
         return {names[k].decode(): (tot[k], cnt[k]) for k in range(n)}
 
+    def csc_map(self):
+        """int64[nnz]: CSC values of K are ``Kval[map]`` on the CSR index arrays (symmetric pattern)."""
+        if self._csc_map is None:
+            m = np.zeros(self.nnz, dtype=np.int64)
+            self._check(self.lib.uvh_get_csc_map(self.h, _ptr(m)), "uvh_get_csc_map")
+            self._csc_map = m
+        return self._csc_map
+
     def K_matrix(self, Kval, fmt="csr"):
+        """K as a scipy sparse matrix; ``fmt="csc"`` permutes the values through ``csc_map``
+        (the layout of MATLAB's sparse) without a sort."""
         rowptr, colind = self.pattern()
-        K = sp.csr_matrix((Kval, colind, rowptr), shape=(self.neq, self.neq))
-        return K.tocsc() if fmt == "csc" else K
+        if fmt == "csc":
+            return sp.csc_matrix((Kval[self.csc_map()], colind, rowptr), shape=(self.neq, self.neq))
+        return sp.csr_matrix((Kval, colind, rowptr), shape=(self.neq, self.neq))
 
 
 def mesh_key(MUA):
