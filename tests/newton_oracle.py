@@ -1,0 +1,68 @@
+"""Oracle-driven assembler for the Newton driver (test infrastructure): the same interface
+as ``uasource_amd.newton.GpuAssembler``, with R and K from oracle O2 on the host."""
+import numpy as np
+
+EPS = np.finfo(float).eps
+
+
+class OracleAssembler:
+    def __init__(self, o2, CtrlVar, MUA, F0, F1, nthreads=8):
+        self.o2, self.cv, self.M, self.F0 = o2, CtrlVar, MUA, F0
+        self.F1 = F1.copy()
+        self.mesh = o2.O2Mesh(MUA)
+        self.nthreads = nthreads
+        self.nn = MUA.Nnodes
+        self.x = np.concatenate([F1.ub, F1.vb, F1.h]).astype(float)
+        self.d = np.zeros_like(self.x)
+        self.R0 = None
+        self.n_rk = self.n_r = 0
+        self.free = None
+
+    def set_fixed(self, fixed):
+        self.free = None
+        if fixed is not None and len(fixed):
+            self.free = np.ones(3 * self.nn)
+            self.free[fixed] = 0.0
+
+    def _F(self, x):
+        F = self.F1.copy()
+        n = self.nn
+        F.ub, F.vb, F.h = x[:n].copy(), x[n:2 * n].copy(), x[2 * n:].copy()
+        return F
+
+    def _assemble(self, x, ronly, zero_fields=False):
+        return self.o2.assemble(self.cv, self.M, self.F0, self._F(x), mesh=self.mesh, nthreads=self.nthreads,
+                                ronly=ronly, zero_fields=zero_fields)
+
+    def normalisation(self):
+        R = self._assemble(self.x, True, True)["R"]
+        self.n_r += 1
+        return float(R @ R) + 1000 * EPS
+
+    def assemble_rk(self, fp32, fp32_residual):
+        assert not fp32, "the oracle is FP64"
+        out = self._assemble(self.x, False)
+        self.n_rk += 1
+        self.R0 = out["R"]
+        return out["K"].tocsr(), -self.R0
+
+    def set_step(self, d):
+        self.d = np.asarray(d, dtype=float).copy()
+
+    def cost(self, gamma, fp32, N):
+        if gamma == 0.0:
+            R = self.R0
+        else:
+            R = self._assemble(self.x + gamma * self.d, True)["R"]
+            self.n_r += 1
+        if self.free is not None:
+            R = R * self.free
+        D2 = float(-R @ self.d)
+        return float(R @ R) / N, D2 * D2, D2
+
+    def accept(self, gamma):
+        self.x = self.x + gamma * self.d
+
+    def state(self):
+        n = self.nn
+        return self.x[:n].copy(), self.x[n:2 * n].copy(), self.x[2 * n:].copy()

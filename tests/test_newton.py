@@ -1,0 +1,86 @@
+"""Newton-Raphson uvh solve (uasource_amd.newton): the driver restates
+SSTREAM_TransientImplicit.m:224-418. CPU tests run the driver on oracle O2; GPU tests run it on
+the HIP assembly and require the same behaviour: both converge to Úa's default tolerance
+(rForce < 1e-15), the iteration counts agree within one, and the converged u, v, h agree.
+The precision form (FP32 Jacobian, FP32 residual until rForce < 1e-10, then FP64) must reach
+the FP64 solution to 1e-6."""
+import numpy as np
+import pytest
+
+from newton_oracle import OracleAssembler
+from uasource_amd import newton
+from uasource_amd.ctrlvar import default_ctrlvar
+from uasource_amd.fields import WORKLOADS, build_mesh, newton_problem
+
+_ref = {}
+
+
+def oracle_solution(o2lib, name):
+    if name not in _ref:
+        M = build_mesh(WORKLOADS[name])
+        F0, F1, BCs, fixed = newton_problem(M, WORKLOADS[name])
+        cv = default_ctrlvar()
+        res = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, fixed=fixed)
+        _ref[name] = (M, F0, F1, fixed, res)
+    return _ref[name]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_oracle_newton_converges_quadratically(o2lib):
+    M, F0, F1, fixed, res = oracle_solution(o2lib, "cpu_reference")
+    assert res.converged and res.rForce < 1e-15, (res.reason, res.rForce)
+    # quadratic convergence at the end: the last full steps square the residual
+    last = [h[2] for h in res.history if h[1] == 1.0][-3:]
+    assert len(last) == 3 and last[2] < 1e-3 * last[1] and last[1] < 1e-2 * last[0]
+    # the fixed degrees of freedom did not move
+    x0 = np.concatenate([F1.ub, F1.vb, F1.h])
+    x1 = np.concatenate([res.ub, res.vb, res.h])
+    assert np.array_equal(x0[fixed], x1[fixed])
+
+
+def test_line_search_accepts_full_step_and_backtracks():
+    cv = default_ctrlvar()
+    # full step below NewtonAcceptRatio * r0
+    g, best, n = newton._line_search(cv, lambda g: (1.0, 0, 0), 1.0, (0.1, 0.0, 0.0))
+    assert g == 1.0 and n == 0
+    # f(g) = (1 - 2g)^2 + 0.2 g^2... a parabola with its minimum at g ~ 0.45: backtracking finds a reduction
+    f = lambda g: ((1 - 2 * g) ** 2 + 0.2 * g * g, 0.0, 0.0)
+    g, best, n = newton._line_search(cv, f, 1.0, f(1.0))
+    assert 0.25 <= g < 1.0 and best[0] < 0.5 and n >= 1
+    # no reduction anywhere: gamma = 0
+    g, best, n = newton._line_search(cv, lambda g: (2.0, 0, 0), 1.0, (2.0, 0.0, 0.0))
+    assert g == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cpu_reference", "medium_p2"])
+def test_gpu_newton_matches_oracle_newton(o2lib, name):
+    import sys
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, name)
+    cv = default_ctrlvar()
+    print(f"[gpu] newton {name} (oracle: {ref.iterations} iterations, {ref.reason})", flush=True)
+    sys.stdout.flush()
+    gpu = newton.solve(newton.GpuAssembler(cv, M, F0, F1), cv, fixed=fixed)
+    print(f"   gpu: {gpu.iterations} iterations, {gpu.reason}, rForce {gpu.rForce:.2e}", flush=True)
+    assert ref.converged and gpu.converged, (ref.reason, gpu.reason)
+    assert abs(gpu.iterations - ref.iterations) <= 1
+    assert gpu.rForce < 1e-15
+    for a, b in ((gpu.ub, ref.ub), (gpu.vb, ref.vb), (gpu.h, ref.h)):
+        assert rel(a, b) < 1e-9
+
+
+@pytest.mark.gpu
+def test_gpu_newton_precision_form_reaches_fp64_solution(o2lib):
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "medium_p2")
+    cv = default_ctrlvar()
+    print("[gpu] newton precision form medium_p2", flush=True)
+    res = newton.solve(newton.GpuAssembler(cv, M, F0, F1), cv, fixed=fixed, fp32=True)
+    errs = [rel(a, b) for a, b in ((res.ub, ref.ub), (res.vb, ref.vb), (res.h, ref.h))]
+    print(f"   fp32 form: {res.iterations} iterations ({ref.iterations} FP64), {res.reason}, "
+          f"u/v/h rel err {errs[0]:.1e}/{errs[1]:.1e}/{errs[2]:.1e}", flush=True)
+    assert res.converged and res.rForce < 1e-15
+    assert res.history[-1][4], "the last step used an FP64 residual"
+    assert max(errs) < 1e-6

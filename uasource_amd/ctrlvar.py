@@ -69,8 +69,17 @@ def default_ctrlvar() -> Struct:
         InfoLevelCPU=0,                           # :778
         InfoLevelNonLinIt=1,
         GroundingFloatingMaskContains="GF nodes only",  # :2316
-        # Newton-Raphson uvh controls (used by the minimal driver in uasource_amd.newton)
+        # Newton-Raphson uvh controls (used by the driver in uasource_amd.newton)
         NRitmax=50,                               # :597
+        NRitmin=1,                                # :600
+        NewtonAcceptRatio=0.5,                    # :601
+        BacktrackingGammaMin=1e-10,               # :604
+        BackTrackBeta=0.1,                        # :616
+        BackTrackMaxIterations=50,                # :617
+        BackTrackGuardLower=0.25,                 # :625
+        BackTrackGuardUpper=0.95,                 # :626
+        uvhMinimisationQuantity="Force Residuals",  # :540
+        uvhAcceptableWorkOrForceTolerances=[1.0, 1e-8],  # :505
         uvhDesiredWorkAndForceTolerances=[float("inf"), 1e-15],   # :496
         uvhDesiredWorkOrForceTolerances=[float("inf"), 1e-15],    # :498
         uvhExitBackTrackingStepLength=1e-3,       # :502

@@ -117,3 +117,27 @@ def make_fields(MUA: MUA, w: Workload, seed_offset: int = 0):
     F1 = UaFields(Nn, h=h.copy(), ub=u1, vb=v1, **{k: v.copy() for k, v in common.items()})
     F0 = UaFields(Nn, h=h.copy(), ub=u0, vb=v0, **{k: v.copy() for k, v in common.items()})
     return F0, F1, BoundaryConditions()
+
+
+def newton_problem(MUA: MUA, w: Workload):
+    """``(F0, F1, BCs, fixed)`` of the Newton-solve workload on ``MUA``: the geometry and
+    physics of ``make_fields``, a smooth start (``ub = 100 + 400 x/Lx`` m/a, ``vb = 0``, the
+    same in F0), and Dirichlet conditions: ub, vb and h held at the inflow boundary x = 0 and
+    vb held on the sides y = 0 and y = Ly. ``fixed`` indexes those degrees of freedom in
+    ``[ub; vb; h]``. (The assembly workloads' velocities are uniform random; Newton from such
+    a state, or with no inflow condition on h, needs Úa's Cauchy-step fall-backs.)"""
+    F0, F1, _ = make_fields(MUA, w)
+    x = MUA.coordinates[:, 0]
+    y = MUA.coordinates[:, 1]
+    Nn = MUA.Nnodes
+    F1.ub = 100.0 + 400.0 * (x - x.min()) / (x.max() - x.min())
+    F1.vb = np.zeros(Nn)
+    F0.ub, F0.vb = F1.ub.copy(), F1.vb.copy()
+    tol = 1e-6 * max(x.max() - x.min(), y.max() - y.min())
+    inflow = np.where(x <= x.min() + tol)[0]
+    side = np.where((y <= y.min() + tol) | (y >= y.max() - tol))[0]
+    fixed = np.unique(np.concatenate([inflow, Nn + inflow, 2 * Nn + inflow, Nn + side]))
+    BCs = BoundaryConditions(ubFixedNode=inflow, ubFixedValue=F1.ub[inflow], vbFixedNode=np.union1d(inflow, side),
+                             vbFixedValue=np.zeros(len(np.union1d(inflow, side))), hFixedNode=inflow,
+                             hFixedValue=F1.h[inflow])
+    return F0, F1, BCs, fixed

@@ -1,0 +1,287 @@
+"""Newton-Raphson solve of the implicit uvh system around the GPU assembly.
+
+Restates the uvh iteration of ``UaMain/SSTREAM_TransientImplicit.m:224-418`` for problems
+without boundary-condition constraints (``L`` empty):
+
+* normalisation ``Fext0 = R`` of a zero-fields, R-only assembly (``:224-227``);
+* the cost of ``UaMain/CalcCostFunctionNRuvh.m:22-98`` at ``x + gamma d``:
+  ``frhs = -R``, ``rForce = frhs'frhs / (Fext0'Fext0 + 1000 eps)``, ``D2 = frhs'd``,
+  ``rWork = D2^2``;
+* the exit tests of ``:269-350`` (desired/acceptable work-and-force tolerances depending
+  on the last step length, ``NRitmax``, backtracking failure, and stagnation of ``r/r0``
+  over three iterations);
+* the Newton step ``K d = -R`` (``:356-372``) and the line search in the Newton direction of
+  ``UaMain/rLineminUa.m:60-135`` with the backtracking of ``UaMain/BackTracking.m``
+  (full step accepted below ``NewtonAcceptRatio * r0``, else guarded quadratic fits down to
+  a step of 1e-3). The Cauchy and steepest-descent fall-backs of ``rLineminUa`` are not
+  restated: the iteration stops as "backtracking stagnated" where they would be tried.
+
+The residual cost is device-resident (the R-only assemblies, the axpy to the trial state
+and the dot products stay in HBM; three scalars come back per evaluation) and the residual
+at ``gamma = 0`` is taken from the R+K assembly instead of being assembled again. The linear
+solve is on the host (SciPy's SuperLU), as in Úa, which calls ``solveKApe`` on the CPU.
+
+Precision form: with ``fp32=True`` the Jacobian is always assembled with FP32 element
+arithmetic, and so is the residual while ``rForce`` is above ``fp64_switch``; below it the
+residual (and with it the step's right-hand side, the line search and the convergence test)
+is FP64, so the iteration converges to the FP64 solution and the final convergence check is
+FP64.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from . import uvh
+
+EPS = np.finfo(float).eps
+
+
+@dataclass
+class NewtonResult:
+    converged: bool
+    iterations: int
+    reason: str
+    rForce: float
+    rWork: float
+    history: list = field(default_factory=list)     # (iteration, gamma, rForce, rWork, fp64 residual)
+    t_assembly: float = 0.0
+    t_solve: float = 0.0
+    t_linesearch: float = 0.0
+    n_rk: int = 0
+    n_r: int = 0
+    ub: np.ndarray | None = None
+    vb: np.ndarray | None = None
+    h: np.ndarray | None = None
+
+
+def superlu_solve(K: sp.csr_matrix, rhs: np.ndarray) -> np.ndarray:
+    """Host direct solve (SciPy SuperLU, COLAMD ordering)."""
+    return spla.splu(K.tocsc(), permc_spec="COLAMD").solve(rhs)
+
+
+class GpuAssembler:
+    """R and K of one mesh on the GPU, with F0, F1 and the Newton state resident in HBM.
+
+    The interface the driver uses: ``normalisation()``, ``assemble_rk(fp32)``,
+    ``set_step(d)``, ``cost(gamma, fp32)``, ``accept(gamma)``, ``state()``."""
+
+    def __init__(self, CtrlVar, MUA, F0, F1, device: int = 0, ctx: uvh.UvhContext | None = None):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(f"cuda:{device}")
+        uvh.check_supported(CtrlVar)
+        self.ctx = ctx or uvh.UvhContext(device)
+        self.ctx.set_mesh(MUA)
+        self.nn = MUA.Nnodes
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.dev)
+        self.f1 = {k: t(getattr(F1, k)) for k in uvh.F1_KEYS}
+        self.f0 = {k: t(getattr(F0, k)) for k in uvh.F0_KEYS}
+        # Newton state x = [ub; vb; h] and the trial state the assembly reads
+        self.x = torch.cat([self.f1["ub"], self.f1["vb"], self.f1["h"]])
+        self.xt = self.x.clone()
+        n = self.nn
+        trial = {"ub": self.xt[:n], "vb": self.xt[n:2 * n], "h": self.xt[2 * n:]}
+
+        class _F:
+            pass
+        fd1, fd0 = _F(), _F()
+        for k in uvh.F1_KEYS:
+            setattr(fd1, k, trial.get(k, self.f1[k]))
+        for k in uvh.F0_KEYS:
+            setattr(fd0, k, self.f0[k])
+        self.fields = uvh.make_fields(fd0, fd1)
+        self.ctrl = uvh.make_ctrl(CtrlVar, F1)
+        self.R = torch.zeros(3 * n, dtype=torch.float64, device=self.dev)
+        self.R0 = torch.zeros_like(self.R)
+        self.K = torch.zeros(self.ctx.nnz, dtype=torch.float64, device=self.dev)
+        self.d = torch.zeros_like(self.R)
+        self.stream = torch.cuda.current_stream(self.dev).cuda_stream
+        self.n_rk = self.n_r = 0
+        self.free = None
+
+    def set_fixed(self, fixed):
+        """Residual rows of fixed degrees of freedom are left out of the cost."""
+        if fixed is None or len(fixed) == 0:
+            self.free = None
+        else:
+            m = np.ones(3 * self.nn)
+            m[fixed] = 0.0
+            self.free = self.torch.from_numpy(m).to(self.dev)
+
+    def _assemble(self, flags, K=None):
+        self.ctx.assemble_device(self.ctrl, self.fields, flags, self.R, K, None, None, stream=self.stream)
+        self.ctx.sync_status()
+
+    def normalisation(self) -> float:
+        """Fext0'Fext0 + 1000 eps: R of the zero-fields, R-only assembly at the current state."""
+        self.xt.copy_(self.x)
+        self._assemble(uvh.UVH_RONLY | uvh.UVH_ZEROFIELDS)
+        self.n_r += 1
+        return float(self.torch.dot(self.R, self.R).item()) + 1000 * EPS
+
+    def assemble_rk(self, fp32: bool, fp32_residual: bool):
+        """K (host CSR) at the current state; R(gamma = 0) is kept on the device. With an FP32
+        Jacobian and an FP64 residual, the residual comes from an extra R-only assembly."""
+        self.xt.copy_(self.x)
+        self._assemble(uvh.UVH_FP32 if fp32 else 0, self.K)
+        self.n_rk += 1
+        if fp32 and not fp32_residual:
+            self._assemble(uvh.UVH_RONLY)
+            self.n_r += 1
+        self.R0.copy_(self.R)
+        return self.ctx.K_matrix(self.K.cpu().numpy()), (-self.R0).cpu().numpy()
+
+    def set_step(self, d: np.ndarray):
+        self.d.copy_(self.torch.from_numpy(d))
+
+    def cost(self, gamma: float, fp32: bool, N: float):
+        """(rForce, rWork, D2) at x + gamma d; the residual at gamma = 0 is the cached R0."""
+        if gamma == 0.0:
+            R = self.R0
+        else:
+            self.torch.add(self.x, self.d, alpha=gamma, out=self.xt)
+            self._assemble(uvh.UVH_RONLY | (uvh.UVH_FP32 if fp32 else 0))
+            self.n_r += 1
+            R = self.R
+        if self.free is not None:
+            R = R * self.free
+        frhs_frhs = self.torch.dot(R, R)
+        D2 = -self.torch.dot(R, self.d)
+        v = self.torch.stack([frhs_frhs, D2]).cpu().numpy()
+        return v[0] / N, v[1] * v[1], v[1]
+
+    def accept(self, gamma: float):
+        self.x.add_(self.d, alpha=gamma)
+
+    def state(self):
+        x = self.x.cpu().numpy()
+        n = self.nn
+        return x[:n].copy(), x[n:2 * n].copy(), x[2 * n:].copy()
+
+
+def _criteria(CtrlVar, gamma, rWork, rForce, iteration):
+    """Exit test of SSTREAM_TransientImplicit.m:269-287."""
+    if gamma > max(CtrlVar.uvhExitBackTrackingStepLength, CtrlVar.BacktrackingGammaMin):
+        aw, af = CtrlVar.uvhDesiredWorkAndForceTolerances
+        ow, of = CtrlVar.uvhDesiredWorkOrForceTolerances
+    else:
+        aw, af = CtrlVar.uvhAcceptableWorkAndForceTolerances
+        ow, of = CtrlVar.uvhAcceptableWorkOrForceTolerances
+    return (rWork < aw and rForce < af) and (rWork < ow or rForce < of) and iteration >= CtrlVar.NRitmin
+
+
+def _line_search(CtrlVar, cost, r0, r1):
+    """Newton-direction backtracking (rLineminUa.m:60-135, BackTracking.m): returns
+    (gamma, (rForce, rWork, D2) at gamma, number of trial evaluations)."""
+    slope0 = -2.0 * r0
+    target = max(CtrlVar.NewtonAcceptRatio * r0, min(r0 + CtrlVar.BackTrackBeta * slope0, CtrlVar.NewtonAcceptRatio * r0))
+    best_g, best = 1.0, r1
+    if r1[0] < target:
+        return 1.0, r1, 0
+    gamma_min = 1e-3                      # rLineminUa.m:90 in the Newton direction
+    lower, upper = CtrlVar.BackTrackGuardLower, CtrlVar.BackTrackGuardUpper
+    b, fb = 1.0, r1[0]
+    evals = 0
+    for _ in range(CtrlVar.BackTrackMaxIterations):
+        # minimum of the quadratic through f(0) = r0, f'(0) = slope0 and f(b) = fb (BackTracking.m:268)
+        den = 2.0 * ((fb - r0) / b - slope0)
+        g = -b * slope0 / den if den > 0 else lower * b
+        g = min(max(g, lower * b), upper * b)
+        if g < gamma_min:
+            break
+        rg = cost(g)
+        evals += 1
+        if rg[0] < best[0]:
+            best_g, best = g, rg
+        if rg[0] < target:
+            break
+        b, fb = g, rg[0]
+    if best[0] >= r0:
+        return 0.0, (r0, float("nan"), float("nan")), evals
+    return best_g, best, evals
+
+
+def eliminate(linear_solver, fixed: np.ndarray | None):
+    """Wraps a linear solver for Dirichlet conditions on the degrees of freedom in ``fixed``
+    (indices into [ub; vb; h]): the step is zero there and the rows and columns are removed.
+    This is what Úa's Lagrange-multiplier form (``solveKApe`` with ``L``) gives at a feasible
+    point, whose multipliers absorb the residual of the fixed rows."""
+    if fixed is None or len(fixed) == 0:
+        return linear_solver
+
+    def solve_free(K, rhs):
+        free = np.ones(K.shape[0], dtype=bool)
+        free[fixed] = False
+        d = np.zeros(K.shape[0])
+        d[free] = linear_solver(K[free][:, free], rhs[free])
+        return d
+    return solve_free
+
+
+def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, fp64_switch: float = 1e-10,
+          fixed: np.ndarray | None = None, verbose: bool = False) -> NewtonResult:
+    """Newton-Raphson uvh iteration (SSTREAM_TransientImplicit.m:224-418). ``fixed``: degrees of
+    freedom held at their current values (their residual rows are left out of the cost)."""
+    linear_solver = eliminate(linear_solver, fixed)
+    assembler.set_fixed(fixed)
+    N = assembler.normalisation()
+    res = NewtonResult(False, 0, "", float("inf"), float("nan"))
+    iteration, gamma = 0, 1.0
+    r, r0 = float("inf"), float("inf")
+    rForce, rWork = float("inf"), float("nan")
+    residual64 = not fp32
+    ratios = np.zeros(3)
+    backtrack_ok = True
+    while True:
+        rw = 0.0 if np.isnan(rWork) else rWork
+        if residual64 and _criteria(CtrlVar, gamma, rw, rForce, iteration):
+            res.converged, res.reason = True, "converged"
+            break
+        if iteration > CtrlVar.NRitmax:
+            res.reason = "NRitmax"
+            break
+        if not backtrack_ok or gamma == 0:
+            res.reason = "backtracking stagnated"
+            break
+        ratios[iteration % 3] = r / r0 if np.isfinite(r0) and r0 > 0 else 0.0
+        if np.all(ratios > 0.98):
+            res.converged = rw < 1e-14
+            res.reason = "stagnated"
+            break
+        iteration += 1
+        if fp32 and not residual64 and rForce < fp64_switch:
+            residual64 = True   # from here on the residual, the step and the exit test are FP64
+        fp32_res = fp32 and not residual64
+        t0 = time.perf_counter()
+        K, rhs = assembler.assemble_rk(fp32, fp32_res)
+        t1 = time.perf_counter()
+        d = linear_solver(K, rhs)
+        t2 = time.perf_counter()
+        assembler.set_step(d)
+        cost = lambda g: assembler.cost(g, fp32_res, N)
+        c0 = cost(0.0)
+        c1 = cost(1.0)
+        gamma, best, _ = _line_search(CtrlVar, cost, c0[0], c1)
+        backtrack_ok = gamma > 0
+        if gamma > 0:
+            assembler.accept(gamma)
+            rForce, rWork = best[0], best[1]
+            r = rForce if CtrlVar.uvhMinimisationQuantity == "Force Residuals" else rWork
+        r0 = c0[0] if CtrlVar.uvhMinimisationQuantity == "Force Residuals" else c0[1]
+        t3 = time.perf_counter()
+        res.t_assembly += t1 - t0
+        res.t_solve += t2 - t1
+        res.t_linesearch += t3 - t2
+        res.history.append((iteration, gamma, rForce, rWork, not fp32_res))
+        if verbose:
+            print(f"  it {iteration}: gamma {gamma:.3g} rForce {rForce:.3e} rWork {rWork:.3e}"
+                  f"{'' if not fp32 else (' (fp64 R)' if not fp32_res else ' (fp32 R)')}", flush=True)
+    res.iterations, res.rForce, res.rWork = iteration, rForce, rWork
+    res.n_rk, res.n_r = assembler.n_rk, assembler.n_r
+    res.ub, res.vb, res.h = assembler.state()
+    return res
