@@ -128,6 +128,38 @@ int uvh_sync_status(uvh_ctx *ctx);
 int uvh_profile(uvh_ctx *ctx, int enable);
 int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t *launches, int cap);
 
+/* ---------------------------------------------------------------- batched assembly
+ * Many independent meshes (ensemble members with perturbed C and AGlen, mixed 3-, 6- and
+ * 10-node elements) assembled together: one persistent element launch and one persistent
+ * row launch over a flattened (member, block) work list. Replaces N separate uvhAssembly
+ * calls (UaMain/uvhAssembly.m:1), one per member; Úa itself spreads such work over MATLAB
+ * workers (uvhMatrixAssemblySSTREAM_SPMD.m), which here share one GPU instead.
+ * Each member's results are those of uvh_assemble on that member alone to rounding (the
+ * kernels are compiled separately, so FMA contraction may differ) and bit for bit the same
+ * from run to run. */
+typedef struct uvh_batch uvh_batch;
+
+int uvhb_create(uvh_batch **batch, int device);
+void uvhb_destroy(uvh_batch *batch);
+const char *uvhb_last_error(const uvh_batch *batch);
+/* Adds a mesh (arguments as uvh_set_mesh); *member receives its index. */
+int uvhb_add_mesh(uvh_batch *batch, int32_t nele, int32_t nod, int32_t nnodes, const int32_t *conn,
+                  const double *x, const double *y, int32_t nip, const double *pts, const double *wts,
+                  int32_t *member);
+int uvhb_size(const uvh_batch *batch);
+/* The member's context (owned by the batch): pattern, CSC map and single-mesh assembly. */
+uvh_ctx *uvhb_member(uvh_batch *batch, int32_t member);
+/* Element tile (elements per 256-thread workgroup) of every member with `nod` nodes, for
+ * the FP64 (fp32 = 0) or FP32 element arithmetic; the autotuner's output. */
+int uvhb_set_tile(uvh_batch *batch, int32_t nod, int32_t fp32, int32_t epb);
+int uvhb_get_tile(const uvh_batch *batch, int32_t nod, int32_t fp32);
+/* Device-buffer assembly of every member: dfields, dR and dK are arrays of uvh_size()
+ * entries (dK may be NULL with UVH_RONLY). Enqueued on `stream`; uvhb_sync_status waits and
+ * fills member_status (may be NULL) with UVH_OK / UVH_E_NAN_R / UVH_E_NAN_K per member. */
+int uvhb_assemble_device(uvh_batch *batch, const uvh_ctrl *ctrl, const uvh_fields *dfields, uint32_t flags,
+                         double *const *dR, double *const *dK, void *stream);
+int uvhb_sync_status(uvh_batch *batch, int32_t *member_status);
+
 #ifdef __cplusplus
 }
 #endif

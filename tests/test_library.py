@@ -14,11 +14,12 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_entry_points():
     text = open(os.path.join(REPO, "include", "uvh.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(uvh_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(uvhb?_[a-z_]+)\s*\(", text)))
 
 
 def test_header_and_binding_list_agree():
-    assert declared_entry_points() == sorted(uvh.EXPORTED)
+    from uasource_amd.batch import EXPORTED_BATCH
+    assert declared_entry_points() == sorted(uvh.EXPORTED + EXPORTED_BATCH)
 
 
 def test_library_exports_every_declared_symbol():

@@ -24,6 +24,11 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
                        hipStream_t s);
 int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
+size_t element_lds_bytes(int nod, int nip, int epb, bool fp32);
+hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
+                        const int2 *elem_items, int n_elem, const int2 *row_items, int n_row, size_t elem_lds,
+                        size_t row_lds, int persistent_blocks, bool fp32, bool ronly, int check_nan, int *ctr,
+                        int *status, const Params &p, hipStream_t s);
 int element_epb_default(int nod, int nip, bool fp32);
 size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
@@ -249,7 +254,8 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     std::string msg = uvh::fill_shape_tables(m.tab, nod, nip, pts, wts);
     if (msg.empty()) msg = uvh::build_mesh_host(m);
     ctx->layout = uvh::fused_layout(nod, nip);
-    if (msg.empty()) {
+    ctx->sched = uvh::Schedule{};
+    if (msg.empty() && !ctx->two_pass) {  // the fused path's patch schedule
         const char *pe = std::getenv("UVH_PATCHES");
         const int target = pe ? std::max(1, std::atoi(pe)) : 256;
         msg = uvh::build_schedule(m, ctx->layout.epb, ctx->layout.window_doubles, target, ctx->sched);
@@ -517,6 +523,241 @@ int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t
         ++n;
     }
     return n;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batched assembly
+struct uvh_batch {
+    int device = 0;
+    std::string err;
+    std::vector<uvh_ctx *> members;
+    // work lists for the FP64 [0] and FP32 [1] element tiles
+    bool built[2] = {false, false};
+    int2 *d_items[2][3] = {};   // nodal, element, row items
+    int n_items[2][3] = {};
+    size_t elem_lds[2] = {0, 0}, row_lds = 0;
+    int persistent_blocks[2] = {0, 0};
+    uvh::BatchMember *d_mem = nullptr, *h_mem = nullptr;  // device array and its pinned host copy
+    std::vector<uvh::BatchMember> last;
+    hipEvent_t upload_done = nullptr;
+    bool upload_pending = false;
+    int *d_ctr = nullptr, *d_status = nullptr;
+    int fail(int code, const std::string &m) { err = m; return code; }
+    void free_lists() {
+        for (int t = 0; t < 2; ++t) {
+            for (int k = 0; k < 3; ++k) { hipFree(d_items[t][k]); d_items[t][k] = nullptr; n_items[t][k] = 0; }
+            built[t] = false;
+        }
+    }
+};
+
+namespace {
+
+// Work lists of one element tile choice: (member, block) pairs in member order.
+int batch_build_lists(uvh_batch *b, int t) {
+    const bool fp32 = t == 1;
+    std::vector<int2> items[3];
+    size_t elds = 0, rlds = 0;
+    for (int mi = 0; mi < int(b->members.size()); ++mi) {
+        const uvh_ctx *c = b->members[mi];
+        const uvh::MeshHost &m = c->mesh;
+        const int epb = c->epb[t];
+        for (int k = 0; k < (m.nn + 255) / 256; ++k) items[0].push_back(make_int2(mi, k));
+        for (int k = 0; k < (m.nele + epb - 1) / epb; ++k) items[1].push_back(make_int2(mi, k));
+        for (int k = 0; k < (m.nn + uvh::kRowNodesPerBlock - 1) / uvh::kRowNodesPerBlock; ++k)
+            items[2].push_back(make_int2(mi, k));
+        elds = std::max(elds, uvh::element_lds_bytes(m.nod, m.nip, epb, fp32));
+        rlds = std::max(rlds, size_t(9) * size_t(m.max_block_deg) * sizeof(double));
+    }
+    for (int k = 0; k < 3; ++k) {
+        hipFree(b->d_items[t][k]);
+        b->d_items[t][k] = nullptr;
+        b->n_items[t][k] = int(items[k].size());
+        if (dalloc(&b->d_items[t][k], items[k].size()) != hipSuccess ||
+            hipMemcpy(b->d_items[t][k], items[k].data(), items[k].size() * sizeof(int2), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return b->fail(UVH_E_NOMEM, "uvhb: cannot upload work lists");
+    }
+    b->elem_lds[t] = elds;
+    b->row_lds = rlds;
+    // persistent grid: every CU holds as many element workgroups as LDS and registers allow
+    int cus = 0;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+    const int by_lds = std::max<int>(1, int((160 * 1024) / std::max<size_t>(elds, 1)));
+    b->persistent_blocks[t] = std::max(1, cus) * std::min(by_lds, 4);
+    b->built[t] = true;
+    return UVH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uvhb_create(uvh_batch **out, int device) {
+    if (!out) return UVH_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return UVH_E_HIP;
+    hipSetDevice(device);
+    auto *b = new uvh_batch();
+    b->device = device;
+    if (hipMalloc(reinterpret_cast<void **>(&b->d_ctr), 2 * sizeof(int)) != hipSuccess ||
+        hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return UVH_E_HIP;
+    }
+    *out = b;
+    return UVH_OK;
+}
+
+void uvhb_destroy(uvh_batch *b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    hipDeviceSynchronize();
+    b->free_lists();
+    for (auto *c : b->members) uvh_destroy(c);
+    hipFree(b->d_mem);
+    hipHostFree(b->h_mem);
+    hipFree(b->d_ctr);
+    hipFree(b->d_status);
+    if (b->upload_done) hipEventDestroy(b->upload_done);
+    delete b;
+}
+
+const char *uvhb_last_error(const uvh_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int uvhb_add_mesh(uvh_batch *b, int32_t nele, int32_t nod, int32_t nnodes, const int32_t *conn, const double *x,
+                  const double *y, int32_t nip, const double *pts, const double *wts, int32_t *member) {
+    if (!b) return UVH_E_ARG;
+    uvh_ctx *c = nullptr;
+    int st = uvh_create(&c, b->device);
+    if (st != UVH_OK) return b->fail(st, "uvhb_add_mesh: cannot create a member context");
+    c->two_pass = true;
+    st = uvh_set_mesh(c, nele, nod, nnodes, conn, x, y, nip, pts, wts);
+    if (st != UVH_OK) {
+        b->err = std::string("uvhb_add_mesh: ") + c->err;
+        uvh_destroy(c);
+        return st;
+    }
+    const uvh::MeshHost &m = c->mesh;
+    if (dalloc(&c->d_EB, size_t(m.nele) * m.nod * uvh::element_row_stride(m.nod, false)) != hipSuccess) {
+        uvh_destroy(c);
+        return b->fail(UVH_E_NOMEM, "uvhb_add_mesh: cannot allocate element rows");
+    }
+    if (member) *member = int32_t(b->members.size());
+    b->members.push_back(c);
+    b->free_lists();
+    hipFree(b->d_mem); b->d_mem = nullptr;
+    hipHostFree(b->h_mem); b->h_mem = nullptr;
+    hipFree(b->d_status); b->d_status = nullptr;
+    b->last.clear();
+    return UVH_OK;
+}
+
+int uvhb_size(const uvh_batch *b) { return b ? int(b->members.size()) : 0; }
+
+uvh_ctx *uvhb_member(uvh_batch *b, int32_t member) {
+    if (!b || member < 0 || member >= int(b->members.size())) return nullptr;
+    return b->members[member];
+}
+
+int uvhb_set_tile(uvh_batch *b, int32_t nod, int32_t fp32, int32_t epb) {
+    if (!b) return UVH_E_ARG;
+    for (auto *c : b->members) {
+        if (c->mesh.nod != nod) continue;
+        if (epb < 1 || epb * c->mesh.nip > 256 || epb * nod > 256 ||
+            uvh::element_lds_bytes(nod, c->mesh.nip, epb, fp32 != 0) > 156 * 1024)
+            return b->fail(UVH_E_ARG, "uvhb_set_tile: tile does not fit (epb*nip, epb*nod <= 256, LDS <= 156 KiB)");
+    }
+    for (auto *c : b->members)
+        if (c->mesh.nod == nod) c->epb[fp32 ? 1 : 0] = epb;
+    b->built[fp32 ? 1 : 0] = false;
+    return UVH_OK;
+}
+
+int uvhb_get_tile(const uvh_batch *b, int32_t nod, int32_t fp32) {
+    if (!b) return 0;
+    for (auto *c : b->members)
+        if (c->mesh.nod == nod) return c->epb[fp32 ? 1 : 0];
+    return 0;
+}
+
+int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *dfields, uint32_t flags,
+                         double *const *dR, double *const *dK, void *stream) {
+    if (!b) return UVH_E_ARG;
+    const int nm = int(b->members.size());
+    if (nm == 0) return b->fail(UVH_E_STATE, "uvhb_assemble: empty batch");
+    if (!ctrl || !dfields || !dR) return b->fail(UVH_E_ARG, "uvhb_assemble: null ctrl, fields or R");
+    const bool ronly = (flags & UVH_RONLY) != 0, fp32 = (flags & UVH_FP32) != 0;
+    const int t = fp32 ? 1 : 0;
+    if (!ronly && !dK) return b->fail(UVH_E_ARG, "uvhb_assemble: K outputs required unless UVH_RONLY");
+    for (int i = 0; i < nm; ++i) {
+        if (!fields_complete(dfields[i]) || !dR[i] || (!ronly && !dK[i]))
+            return b->fail(UVH_E_ARG, "uvhb_assemble: member " + std::to_string(i) + ": null field or output");
+    }
+    hipSetDevice(b->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!b->d_mem) {
+        if (dalloc(&b->d_mem, size_t(nm)) != hipSuccess || dalloc(&b->d_status, size_t(nm)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void **>(&b->h_mem), sizeof(uvh::BatchMember) * nm, 0) != hipSuccess)
+            return b->fail(UVH_E_NOMEM, "uvhb_assemble: cannot allocate the member table");
+    }
+    if (!b->built[t]) {
+        int st = batch_build_lists(b, t);
+        if (st != UVH_OK) return st;
+    }
+    // member table: uploaded only when a pointer differs from the last upload
+    std::vector<uvh::BatchMember> cur(nm);
+    for (int i = 0; i < nm; ++i) {
+        const uvh_ctx *c = b->members[i];
+        const uvh::MeshHost &m = c->mesh;
+        uvh::BatchMember &M = cur[i];
+        std::memset(&M, 0, sizeof(M));
+        M.conn = c->d_conn; M.x = c->d_x; M.y = c->d_y; M.tab = c->d_tab; M.tab32 = c->d_tab32;
+        M.inc_ptr = c->d_inc_ptr; M.inc_pos = c->d_inc_pos; M.pos_inc = c->d_pos_inc; M.diag = c->d_diag;
+        M.rowptr_n = c->d_rowptr_n; M.b = c->d_b; M.EB = c->d_EB; M.RE = c->d_RE;
+        M.nnzN = m.nnzN(); M.nele = m.nele; M.nn = m.nn; M.nod = m.nod; M.nip = m.nip; M.epb = c->epb[t];
+        M.f = to_ptrs(dfields[i]);
+        M.R = dR[i]; M.Tint = nullptr; M.Fext = nullptr; M.Kval = ronly ? nullptr : dK[i];
+    }
+    hipError_t e = hipSuccess;
+    if (b->last.size() != cur.size() || std::memcmp(b->last.data(), cur.data(), sizeof(uvh::BatchMember) * nm) != 0) {
+        if (b->upload_pending) hipEventSynchronize(b->upload_done);  // the pinned copy may still be in flight
+        std::memcpy(b->h_mem, cur.data(), sizeof(uvh::BatchMember) * nm);
+        e = hipMemcpyAsync(b->d_mem, b->h_mem, sizeof(uvh::BatchMember) * nm, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(b->upload_done, s);
+        b->upload_pending = true;
+        b->last = cur;
+    }
+    if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_assemble: upload: ") + hipGetErrorString(e));
+    const uvh::Params p = make_params(*ctrl, flags);
+    const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
+    e = uvh::launch_batch(b->d_mem, nm, b->d_items[t][0], b->n_items[t][0], b->d_items[t][1], b->n_items[t][1],
+                          b->d_items[t][2], b->n_items[t][2], b->elem_lds[t], b->row_lds, b->persistent_blocks[t],
+                          fp32, ronly, check, b->d_ctr, b->d_status, p, s);
+    if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_assemble: launch: ") + hipGetErrorString(e));
+    return UVH_OK;
+}
+
+int uvhb_sync_status(uvh_batch *b, int32_t *member_status) {
+    if (!b) return UVH_E_ARG;
+    hipSetDevice(b->device);
+    const int nm = int(b->members.size());
+    std::vector<int> st(nm, 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && b->d_status) e = hipMemcpy(st.data(), b->d_status, nm * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_sync_status: ") + hipGetErrorString(e));
+    int first = UVH_OK;
+    for (int i = 0; i < nm; ++i) {
+        const int code = (st[i] & 2) ? UVH_E_NAN_K : (st[i] & 1) ? UVH_E_NAN_R : UVH_OK;
+        if (member_status) member_status[i] = code;
+        if (code != UVH_OK && first == UVH_OK) {
+            first = code;
+            b->err = "member " + std::to_string(i) + (code == UVH_E_NAN_K ? ":  NaN in K " : ": uvhMatrixAssembly:NaNinR");
+        }
+    }
+    return first;
 }
 
 }  // extern "C"

@@ -127,6 +127,25 @@ struct FieldPtrs {
     const double *dadh;
 };
 
+// One mesh of a batched assembly, as the batched kernels read it (device memory). The
+// mesh tables and work buffers are fixed when the batch is built; the fields and the
+// outputs are set per call.
+struct BatchMember {
+    const int32_t *conn;
+    const double *x, *y;
+    const ShapeTables *tab;
+    const ShapeTables32 *tab32;
+    const int32_t *inc_ptr, *inc_pos;
+    const uint8_t *pos_inc, *diag;
+    const int64_t *rowptr_n;
+    double *b;
+    void *EB, *RE;
+    int64_t nnzN;
+    int32_t nele, nn, nod, nip, epb, pad;
+    FieldPtrs f;
+    double *R, *Tint, *Fext, *Kval;
+};
+
 // Per-assembly argument block of the fused kernel (device memory; see MeshArgs).
 struct CallArgs {             // per assembly
     FieldPtrs f;

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <map>
+#include <mutex>
 
 #include "uvh_internal.h"
 
@@ -444,14 +446,13 @@ __device__ __forceinline__ int row_table_doubles(int tstride) {
 // the physics and the element rows (double, or float for the precision form); the rows are
 // stored in T at their position in the node-incidence order.
 template <int NOD, typename T, bool RONLY>
-__global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, const int32_t *__restrict__ conn,
-                                                 const double *__restrict__ xn, const double *__restrict__ yn,
-                                                 FieldPtrs f, const double *__restrict__ bnod,
-                                                 const ShapeTables *__restrict__ tab,
-                                                 const typename RowTab<T>::type *__restrict__ rtab, Params p,
-                                                 const int32_t *__restrict__ inc_pos,
-                                                 T *__restrict__ EB, T *__restrict__ RE) {
-    extern __shared__ double lds[];
+__device__ __forceinline__ void element_block(double *lds, int nele, int nip, int epb, int lb,
+                                              const int32_t *__restrict__ conn, const double *__restrict__ xn,
+                                              const double *__restrict__ yn, const FieldPtrs &f,
+                                              const double *__restrict__ bnod, const ShapeTables *__restrict__ tab,
+                                              const typename RowTab<T>::type *__restrict__ rtab, const Params &p,
+                                              const int32_t *__restrict__ inc_pos, T *__restrict__ EB,
+                                              T *__restrict__ RE) {
     const int tstride = nip * NOD;
     double *sN = lds;                       // [nip][NOD]
     double *sXi = sN + tstride;
@@ -476,9 +477,6 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
         }
     }
     if (tid < nip) sW[tid] = tab->w[tid];
-    const int nblocks = (nele + epb - 1) / epb;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    if (lb >= nblocks) return;  // whole block: no barrier has been reached
     const int64_t e0 = (int64_t)lb * epb;
 
     // stage 1: gathers, one (element, node) per thread
@@ -533,6 +531,21 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     }
 }
 
+template <int NOD, typename T, bool RONLY>
+__global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, const int32_t *__restrict__ conn,
+                                                 const double *__restrict__ xn, const double *__restrict__ yn,
+                                                 FieldPtrs f, const double *__restrict__ bnod,
+                                                 const ShapeTables *__restrict__ tab,
+                                                 const typename RowTab<T>::type *__restrict__ rtab, Params p,
+                                                 const int32_t *__restrict__ inc_pos,
+                                                 T *__restrict__ EB, T *__restrict__ RE) {
+    extern __shared__ double lds[];
+    const int nblocks = (nele + epb - 1) / epb;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    if (lb >= nblocks) return;  // whole block
+    element_block<NOD, T, RONLY>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB, RE);
+}
+
 // Node rows: kRowLanes lanes per node, 64 / kRowLanes nodes per wave. The incident element
 // rows (stored contiguously per node, in incidence order, by k_element) are added, in
 // ascending element order, into an LDS buffer that holds the node's three CSR rows
@@ -545,19 +558,17 @@ constexpr int kRowNodesPerWave = 64 / kRowLanes;
 static_assert(kRowNodesPerBlock == 4 * kRowNodesPerWave, "row kernel: 4 waves of lane groups per block");
 
 template <int NOD, typename TE>
-__global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict__ inc_ptr,
-                                              const uint8_t *__restrict__ pos,
-                                              const int64_t *__restrict__ rowptr_n, const uint8_t *__restrict__ diag,
-                                              int64_t nnzN, const TE *__restrict__ EB,
-                                              const TE *__restrict__ RE, double *__restrict__ R,
-                                              double *__restrict__ Tint, double *__restrict__ Fext,
-                                              double *__restrict__ Kval, int check_nan, int *__restrict__ status) {
-    extern __shared__ double lds[];
+__device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const int32_t *__restrict__ inc_ptr,
+                                           const uint8_t *__restrict__ pos, const int64_t *__restrict__ rowptr_n,
+                                           const uint8_t *__restrict__ diag, int64_t nnzN,
+                                           const TE *__restrict__ EB, const TE *__restrict__ RE,
+                                           double *__restrict__ R, double *__restrict__ Tint,
+                                           double *__restrict__ Fext, double *__restrict__ Kval, int check_nan,
+                                           int *__restrict__ status) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / kRowLanes, gl = lane - sub * kRowLanes;
     const int slot = wave * kRowNodesPerWave + sub;
-    const int k0 = xcd_block(blockIdx.x, gridDim.x) * kRowNodesPerBlock;  // the block's nodes are consecutive
-    const int k = k0 + slot;
+    const int k = k0 + slot;   // the block's nodes are consecutive
     if (k >= nn) return;  // whole lane group; no barriers or cross-group shuffles below
     const int64_t rp = rowptr_n[k];
     // the block's row buffers are packed: node k's starts at 9 * (rowptr_n[k] - rowptr_n[k0]);
@@ -626,6 +637,20 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
     if (bad) atomicOr(status, bad);
 }
 
+template <int NOD, typename TE>
+__global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict__ inc_ptr,
+                                              const uint8_t *__restrict__ pos,
+                                              const int64_t *__restrict__ rowptr_n, const uint8_t *__restrict__ diag,
+                                              int64_t nnzN, const TE *__restrict__ EB,
+                                              const TE *__restrict__ RE, double *__restrict__ R,
+                                              double *__restrict__ Tint, double *__restrict__ Fext,
+                                              double *__restrict__ Kval, int check_nan, int *__restrict__ status) {
+    extern __shared__ double lds[];
+    const int k0 = xcd_block(blockIdx.x, gridDim.x) * kRowNodesPerBlock;
+    rows_block<NOD, TE>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan,
+                        status);
+}
+
 // Residual only: one thread per node sums its incident element residual rows (contiguous).
 template <typename TE>
 __global__ __launch_bounds__(256) void k_rows_r(int nn, const int32_t *__restrict__ inc_ptr,
@@ -645,6 +670,128 @@ __global__ __launch_bounds__(256) void k_rows_r(int nn, const int32_t *__restric
     if (Tint) { Tint[k] = T0; Tint[nn + k] = T1; Tint[2 * (int64_t)nn + k] = T2; }
     if (Fext) { Fext[k] = F0; Fext[nn + k] = F1; Fext[2 * (int64_t)nn + k] = F2; }
     if (check_nan && (isnan(r0) || isnan(r1) || isnan(r2))) atomicOr(status, 1);
+}
+
+// ------------------------------------------------------------------ batched assembly
+// Many independent meshes (ensemble members, mixed element types) in one launch per stage.
+// The work is a flattened list of (member, block) items; persistent workgroups take items
+// from an atomic counter until the list is exhausted, so large and small members balance
+// across the CUs. Every item writes only its own element rows or node rows, so the order in
+// which workgroups take items does not change any result.
+
+__global__ void k_clear_counters(int *counters, int n, int *status, int nstatus) {
+    const int t = threadIdx.x;
+    if (t < n) counters[t] = 0;
+    for (int k = t; k < nstatus; k += blockDim.x) status[k] = 0;
+}
+
+// next item of a persistent workgroup: thread 0 takes it, the block reads it from LDS
+__device__ __forceinline__ int next_item(int *counter, int *s_item) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = *s_item;
+    __syncthreads();  // all threads have read it before thread 0 takes the next one
+    return it;
+}
+
+// items: (member, node block of 256)
+__global__ __launch_bounds__(256) void k_nodal_b_batch(const BatchMember *__restrict__ mem,
+                                                       const int2 *__restrict__ items, Params p) {
+    const int2 w = items[blockIdx.x];
+    const BatchMember &M = mem[w.x];
+    const int k = w.y * 256 + threadIdx.x;
+    if (k >= M.nn) return;
+    const FieldPtrs &f = M.f;
+    double h = p.zero_fields ? f.h0[k] : f.h[k];
+    if (p.reset_thickness && h < p.thick_min) h = p.thick_min;
+    const double S = f.S[k], B = f.B[k], rho = f.rho[k];
+    const double hf = p.rhow * (S - B) / rho;
+    const double GF = 0.5 + 0.5 * tanh(p.kH * ((h - hf) - p.Hh0));
+    const double bfloat = S - rho * h / p.rhow;
+    double bb = GF * B + (1.0 - GF) * bfloat;
+    if (p.enforce_b_above && bb < B) bb = B;
+    M.b[k] = bb;
+}
+
+// items: (member, element block of the member's epb elements)
+template <typename T, bool RONLY>
+__global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__restrict__ mem,
+                                                       const int2 *__restrict__ items, int nitems,
+                                                       int *__restrict__ counter, Params p) {
+    extern __shared__ double lds[];
+    __shared__ int s_item;
+    for (;;) {
+        const int it = next_item(counter, &s_item);
+        if (it >= nitems) break;
+        const int2 w = items[it];
+        const BatchMember &M = mem[w.x];
+        const typename RowTab<T>::type *rtab;
+        if constexpr (sizeof(T) == sizeof(double)) rtab = M.tab;
+        else rtab = M.tab32;
+        T *EB = static_cast<T *>(M.EB), *RE = static_cast<T *>(M.RE);
+        if (M.nod == 3)
+            element_block<3, T, RONLY>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
+                                       M.inc_pos, EB, RE);
+        else if (M.nod == 6)
+            element_block<6, T, RONLY>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
+                                       M.inc_pos, EB, RE);
+        else
+            element_block<10, T, RONLY>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
+                                        M.inc_pos, EB, RE);
+        __syncthreads();  // the LDS of this item is dead before the next one overwrites it
+    }
+}
+
+// items: (member, node block of kRowNodesPerBlock nodes); status has one word per member
+template <typename TE>
+__global__ __launch_bounds__(256) void k_rows_batch(const BatchMember *__restrict__ mem,
+                                                    const int2 *__restrict__ items, int nitems,
+                                                    int *__restrict__ counter, int check_nan,
+                                                    int *__restrict__ status) {
+    extern __shared__ double lds[];
+    __shared__ int s_item;
+    for (;;) {
+        const int it = next_item(counter, &s_item);
+        if (it >= nitems) break;
+        const int2 w = items[it];
+        const BatchMember &M = mem[w.x];
+        const TE *EB = static_cast<const TE *>(M.EB), *RE = static_cast<const TE *>(M.RE);
+        const int k0 = w.y * kRowNodesPerBlock;
+        if (M.nod == 3)
+            rows_block<3, TE>(lds, M.nn, k0, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN, EB, RE, M.R, M.Tint,
+                              M.Fext, M.Kval, check_nan, status + w.x);
+        else if (M.nod == 6)
+            rows_block<6, TE>(lds, M.nn, k0, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN, EB, RE, M.R, M.Tint,
+                              M.Fext, M.Kval, check_nan, status + w.x);
+        else
+            rows_block<10, TE>(lds, M.nn, k0, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN, EB, RE, M.R, M.Tint,
+                               M.Fext, M.Kval, check_nan, status + w.x);
+        __syncthreads();
+    }
+}
+
+// R only: items (member, node block of 256), one thread per node
+template <typename TE>
+__global__ __launch_bounds__(256) void k_rows_r_batch(const BatchMember *__restrict__ mem,
+                                                      const int2 *__restrict__ items, int check_nan,
+                                                      int *__restrict__ status) {
+    const int2 w = items[blockIdx.x];
+    const BatchMember &M = mem[w.x];
+    const int k = w.y * 256 + threadIdx.x;
+    if (k >= M.nn) return;
+    const TE *RE = static_cast<const TE *>(M.RE);
+    double T0 = 0, T1 = 0, T2 = 0, F0 = 0, F1 = 0, F2 = 0;
+    for (int t = M.inc_ptr[k]; t < M.inc_ptr[k + 1]; ++t) {
+        const TE *re = RE + (int64_t)t * 6;
+        T0 += double(re[0]); F0 += double(re[1]); T1 += double(re[2]);
+        F1 += double(re[3]); T2 += double(re[4]); F2 += double(re[5]);
+    }
+    const int nn = M.nn;
+    const double r0 = T0 - F0, r1 = T1 - F1, r2 = T2 - F2;
+    M.R[k] = r0; M.R[nn + k] = r1; M.R[2 * (int64_t)nn + k] = r2;
+    if (M.Tint) { M.Tint[k] = T0; M.Tint[nn + k] = T1; M.Tint[2 * (int64_t)nn + k] = T2; }
+    if (M.Fext) { M.Fext[k] = F0; M.Fext[nn + k] = F1; M.Fext[2 * (int64_t)nn + k] = F2; }
+    if (check_nan && (isnan(r0) || isnan(r1) || isnan(r2))) atomicOr(status + w.x, 1);
 }
 
 // Fused assembly: one 512-thread workgroup per patch. Per batch of `epb` elements: zero the
@@ -797,6 +944,19 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restr
 
 namespace {
 constexpr size_t kLdsLimit = 64 * 1024;
+
+// Raises a kernel's dynamic-LDS limit once (not at every launch, so that launches can be
+// captured into a HIP graph without other runtime calls in between).
+void lds_attr(const void *kernel, size_t lds) {
+    static std::mutex mu;
+    static std::map<const void *, size_t> done;
+    if (lds <= kLdsLimit) return;
+    std::lock_guard<std::mutex> g(mu);
+    size_t &cur = done[kernel];
+    if (cur >= lds) return;
+    hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    cur = lds;
+}
 // Default LDS cap of the element kernel: two 256-thread workgroups per CU (160 KiB per CU).
 constexpr size_t kElementLds = 80 * 1024;
 
@@ -825,8 +985,7 @@ hipError_t launch_element_t(int nele, int nip, int epb, const int32_t *conn, con
     auto kr = k_element<NOD, T, true>;
     auto kk = k_element<NOD, T, false>;
     auto kern = ronly ? kr : kk;
-    if (lds > kLdsLimit)
-        hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    lds_attr(reinterpret_cast<const void *>(kern), lds);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
                        static_cast<T *>(EB), static_cast<T *>(RE));
     return hipGetLastError();
@@ -898,9 +1057,7 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
 #define UVH_ROWS(NODV, TE)                                                                                     \
     do {                                                                                                       \
         auto kern = k_rows<NODV, TE>;                                                                          \
-        if (lds > kLdsLimit)                                                                                   \
-            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                int(lds));                                                                     \
+        lds_attr(reinterpret_cast<const void *>(kern), lds);                                                   \
         hipLaunchKernelGGL(kern, grid, block, lds, s, nn, inc_ptr, pos, rowptr_n, diag, nnzN,                   \
                            static_cast<const TE *>(EB), static_cast<const TE *>(RE), R, Tint, Fext, Kval,      \
                            check_nan, status);                                                                 \
@@ -912,6 +1069,46 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
         default: return hipErrorInvalidValue;
     }
 #undef UVH_ROWS
+    return hipGetLastError();
+}
+
+size_t element_lds_bytes(int nod, int nip, int epb, bool fp32) {
+    return element_table_bytes(nod, nip, fp32) + element_per_el_bytes(nod, nip, fp32) * size_t(epb);
+}
+
+// Batched launches. `ctr` holds 2 work counters; `status` one word per member.
+hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
+                        const int2 *elem_items, int n_elem, const int2 *row_items, int n_row, size_t elem_lds,
+                        size_t row_lds, int persistent_blocks, bool fp32, bool ronly, int check_nan, int *ctr,
+                        int *status, const Params &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, ctr, 2, status, nmem);
+    hipLaunchKernelGGL(k_nodal_b_batch, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, p);
+    if (elem_lds > 160 * 1024 || row_lds > 160 * 1024) return hipErrorInvalidConfiguration;
+#define UVH_EB(T, RO)                                                                                      \
+    do {                                                                                                   \
+        auto kern = k_element_batch<T, RO>;                                                                \
+        lds_attr(reinterpret_cast<const void *>(kern), elem_lds);                                          \
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks), dim3(256), elem_lds, s, mem, elem_items, n_elem, ctr, p); \
+    } while (0)
+    if (fp32) { if (ronly) UVH_EB(float, true); else UVH_EB(float, false); }
+    else { if (ronly) UVH_EB(double, true); else UVH_EB(double, false); }
+#undef UVH_EB
+    if (ronly) {
+        if (fp32)
+            hipLaunchKernelGGL(k_rows_r_batch<float>, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, check_nan, status);
+        else
+            hipLaunchKernelGGL(k_rows_r_batch<double>, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, check_nan, status);
+        return hipGetLastError();
+    }
+#define UVH_RB(TE)                                                                                         \
+    do {                                                                                                   \
+        auto kern = k_rows_batch<TE>;                                                                      \
+        lds_attr(reinterpret_cast<const void *>(kern), row_lds);                                           \
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks), dim3(256), row_lds, s, mem, row_items, n_row, ctr + 1, \
+                           check_nan, status);                                                             \
+    } while (0)
+    if (fp32) UVH_RB(float); else UVH_RB(double);
+#undef UVH_RB
     return hipGetLastError();
 }
 

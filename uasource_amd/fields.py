@@ -141,3 +141,20 @@ def newton_problem(MUA: MUA, w: Workload):
                              vbFixedValue=np.zeros(len(np.union1d(inflow, side))), hFixedNode=inflow,
                              hFixedValue=F1.h[inflow])
     return F0, F1, BCs, fixed
+
+
+def batch_workloads(n: int = 64, seed: int = 4, lo: int = 5000, hi: int = 80000, types=((3, 2), (6, 5)),
+                    weights=None, name: str = "batch") -> list:
+    """Ensemble of ``n`` independent meshes: element counts log-uniform in [lo, hi], element
+    types drawn from ``types`` ((nodes, quadrature degree) pairs, with ``weights``), each member
+    with its own seeded lognormal C (sigma 0.3) and AGlen (sigma 0.2) perturbations, on
+    2:1 rectangles of 100 km x 50 km."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nele = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        t = types[rng.choice(len(types), p=weights)]
+        cells_y = max(2, int(round(np.sqrt(nele / 4.0))))
+        out.append(Workload(f"{name}{k}", 100e3, 50e3, 2 * cells_y + 1, cells_y + 1, t[0], t[1],
+                            seed=10_000 + seed * 1000 + k, C_sigma=0.3, A_sigma=0.2))
+    return out

@@ -22,10 +22,10 @@ at ``gamma = 0`` is taken from the R+K assembly instead of being assembled again
 solve is on the host (SciPy's SuperLU), as in Úa, which calls ``solveKApe`` on the CPU.
 
 Precision form: with ``fp32=True`` the Jacobian is always assembled with FP32 element
-arithmetic, and so is the residual while ``rForce`` is above ``fp64_switch``; below it the
-residual (and with it the step's right-hand side, the line search and the convergence test)
-is FP64, so the iteration converges to the FP64 solution and the final convergence check is
-FP64.
+arithmetic, and so is the residual while ``rForce`` is above ``fp64_switch`` and the FP32 line
+search still reduces it; after that the residual (and with it the step's right-hand side, the
+line search and the convergence test) is FP64, so the iteration converges to the FP64 solution
+and the final convergence check is FP64.
 """
 from __future__ import annotations
 
@@ -223,7 +223,7 @@ def eliminate(linear_solver, fixed: np.ndarray | None):
     return solve_free
 
 
-def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, fp64_switch: float = 1e-10,
+def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, fp64_switch: float = 1e-8,
           fixed: np.ndarray | None = None, verbose: bool = False) -> NewtonResult:
     """Newton-Raphson uvh iteration (SSTREAM_TransientImplicit.m:224-418). ``fixed``: degrees of
     freedom held at their current values (their residual rows are left out of the cost)."""
@@ -246,10 +246,21 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             res.reason = "NRitmax"
             break
         if not backtrack_ok or gamma == 0:
+            if fp32 and not residual64:
+                # the FP32 residual has reached its rounding floor: continue with the FP64 one
+                residual64, backtrack_ok, gamma = True, True, 1.0
+                r, r0, ratios[:] = float("inf"), float("inf"), 0.0
+                rForce, rWork = float("inf"), float("nan")   # no exit test on an FP32 residual
+                continue
             res.reason = "backtracking stagnated"
             break
         ratios[iteration % 3] = r / r0 if np.isfinite(r0) and r0 > 0 else 0.0
         if np.all(ratios > 0.98):
+            if fp32 and not residual64:
+                residual64, gamma = True, 1.0
+                r, r0, ratios[:] = float("inf"), float("inf"), 0.0
+                rForce, rWork = float("inf"), float("nan")
+                continue
             res.converged = rw < 1e-14
             res.reason = "stagnated"
             break
