@@ -7,7 +7,11 @@ The line printed reports elements assembled per second, the HBM-roofline
 fraction of the assembly, per-kernel times from HIP events on the launch
 stream, and the oracle's C restatement timed on this host's cores beside it.
 
-    python bench.py [--steps K] [--warmup W] [--workload plain_p2] [--ronly]
+    python bench.py [--steps K] [--warmup W] [--workload plain_p2] [--ronly] [--fp32]
+
+Other forms of the path, one JSON line each: ``--batch`` (64 meshes in one persistent
+launch per stage), ``--serve`` (500 requests, latency percentiles over a load sweep),
+``--newton`` (a full Newton solve).
 """
 from __future__ import annotations
 
@@ -71,6 +75,9 @@ def main(argv=None):
                     help="precision form: FP32 element arithmetic, FP64 accumulation (reports its error vs FP64)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", action="store_true", help="64-mesh batched assembly")
+    ap.add_argument("--serve", action="store_true", help="request serving, latency over a load sweep")
+    ap.add_argument("--newton", action="store_true", help="full Newton solve (newton_p2 workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -90,6 +97,18 @@ def main(argv=None):
               f"(--gpus {args.gpus} is not supported; independent meshes shard across GPUs with no exchange).",
               file=sys.stderr)
         return 2
+
+    if args.batch or args.serve or args.newton:
+        from uasource_amd import bench_modes
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        if args.batch:
+            line = bench_modes.bench_batch(args.steps, args.warmup, log=log)
+        elif args.serve:
+            line = bench_modes.bench_serve(log=log)
+        else:
+            line = bench_modes.bench_newton(log=log)
+        print(json.dumps(line), flush=True)
+        return 0
 
     import numpy as np
     import torch

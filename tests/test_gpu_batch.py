@@ -21,7 +21,8 @@ _members = None
 def members():
     global _members
     if _members is None:
-        ws = batch_workloads(n=7, seed=9, lo=300, hi=6000, types=((3, 2), (6, 5), (10, 8)), weights=(0.4, 0.4, 0.2))
+        ws = batch_workloads(n=7, seed=9, lo=300, hi=6000, types=((3, 2), (6, 5), (10, 8)), weights=(0.4, 0.4, 0.2),
+                             C_sigma=0.2)
         _members = []
         for w in ws:
             M = build_mesh(w)

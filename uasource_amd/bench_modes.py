@@ -1,0 +1,226 @@
+"""Measurements of the batched, serving and Newton forms (``bench.py --batch / --serve /
+--newton``). Each returns the dict that bench.py prints as one JSON line."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import uvh
+from .ctrlvar import default_ctrlvar
+from .fields import WORKLOADS, batch_workloads, build_mesh, make_fields, newton_problem
+
+
+def _device_fields(torch, dev, F0, F1):
+    keep = []
+
+    class _F:
+        pass
+    f1, f0 = _F(), _F()
+    for k in uvh.F1_KEYS:
+        t = torch.from_numpy(np.ascontiguousarray(getattr(F1, k))).to(dev)
+        keep.append(t)
+        setattr(f1, k, t)
+    for k in uvh.F0_KEYS:
+        t = torch.from_numpy(np.ascontiguousarray(getattr(F0, k))).to(dev)
+        keep.append(t)
+        setattr(f0, k, t)
+    return uvh.make_fields(f0, f1), keep
+
+
+def _time(torch, fn, steps, stream):
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def bench_batch(steps: int, warmup: int, n_members: int = 64, log=print) -> dict:
+    """64 independent meshes (log-uniform 5k-80k elements, P1/P2 mixed, perturbed C and AGlen):
+    batched assembly (one persistent launch per stage), FP32 element arithmetic with FP64
+    accumulation and FP64, autotuned tiles, eager and HIP-graph replay; against the same work
+    as 64 single-mesh assemblies."""
+    import torch
+    from .batch import UvhBatch, autotune
+    dev = torch.device("cuda:0")
+    cv = default_ctrlvar()
+    ws = batch_workloads(n=n_members)
+    t0 = time.perf_counter()
+    members = []
+    for w in ws:
+        M = build_mesh(w)
+        F0, F1, _ = make_fields(M, w)
+        members.append((M, F0, F1))
+    b = UvhBatch(0)
+    for M, _, _ in members:
+        b.add_mesh(M)
+    log(f"batch: {len(members)} meshes, {sum(m[0].Nele for m in members)} elements, set-up {time.perf_counter() - t0:.1f} s")
+    fields, keep, R, K = [], [], [], []
+    for (M, F0, F1), mv in zip(members, b.members):
+        f, k = _device_fields(torch, dev, F0, F1)
+        fields.append(f)
+        keep += k
+        R.append(torch.zeros(3 * M.Nnodes, dtype=torch.float64, device=dev))
+        K.append(torch.zeros(mv.nnz, dtype=torch.float64, device=dev))
+    ctrl = uvh.make_ctrl(cv, members[0][2])
+    s = torch.cuda.Stream(dev)
+    sp = s.cuda_stream
+    nele = sum(m[0].Nele for m in members)
+    out = {}
+    with torch.cuda.stream(s):
+        for fp32 in (True, False):
+            flags = uvh.UVH_FP32 if fp32 else 0
+            tiles = autotune(b, ctrl, fields, R, K, flags, sp, torch, reps=3)
+            for _ in range(warmup):
+                b.assemble_device(ctrl, fields, flags, R, K, stream=sp)
+            b.sync_status()
+            ms = _time(torch, lambda: b.assemble_device(ctrl, fields, flags, R, K, stream=sp), steps, s)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=s):
+                b.assemble_device(ctrl, fields, flags, R, K, stream=sp)
+            ms_graph = _time(torch, g.replay, steps, s)
+            key = "fp32" if fp32 else "fp64"
+            out[key] = {"ms": ms, "ms_graph": ms_graph, "elements_per_s": nele / (min(ms, ms_graph) * 1e-3),
+                        "tiles": {f"nod{k[0]}": v[0] for k, v in tiles.items()}}
+            out[key + "_R"] = [r.clone() for r in R]
+            out[key + "_K"] = [k.clone() for k in K]
+            log(f"batch {key}: {ms:.3f} ms eager, {ms_graph:.3f} ms graph, tiles {out[key]['tiles']}")
+        # the same work as separate single-mesh assemblies (one context per member)
+        ctxs = [mv for mv in b.members]
+
+        def singles():
+            for (M, F0, F1), mv, f, r, k in zip(members, ctxs, fields, R, K):
+                mv.assemble_device(ctrl, f, 0, r, k, None, None, stream=sp)
+        for _ in range(warmup):
+            singles()
+        ms_single = _time(torch, singles, steps, s)
+    err = {"R": 0.0, "K": 0.0}
+    for i, (M, F0, F1) in enumerate(members):
+        r32, r64 = out["fp32_R"][i], out["fp64_R"][i]
+        k32, k64 = out["fp32_K"][i], out["fp64_K"][i]
+        err["R"] = max(err["R"], (torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)).item())
+        err["K"] = max(err["K"], (torch.linalg.norm(k32 - k64) / torch.linalg.norm(k64)).item())
+    b.close()
+    single_rate = nele / (ms_single * 1e-3)
+    return {
+        "metric": "uvh elements assembled/s (R+K), batched ensemble",
+        "value": out["fp32"]["elements_per_s"], "unit": "elements/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": min(out["fp32"]["ms"], out["fp32"]["ms_graph"]), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 element arithmetic, f64 accumulation",
+        "data": "synthetic (64 seeded jittered-Delaunay meshes, perturbed C and AGlen)",
+        "config": {"workload": "batch64", "members": len(members), "Nele_total": nele,
+                   "types": sorted({f"P{1 if m[0].nod == 3 else 2 if m[0].nod == 6 else 3}" for m in members}),
+                   "parallelism": "1 GPU"},
+        "batched": {k: {kk: vv for kk, vv in v.items()} for k, v in out.items() if k in ("fp32", "fp64")},
+        "single_mesh_fp64_elements_per_s": single_rate,
+        "speedup_fp32_batched_vs_single_fp64": out["fp32"]["elements_per_s"] / single_rate,
+        "fp32_max_rel_err_vs_fp64": err,
+    }
+
+
+def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=print) -> dict:
+    """500 R/K requests (Newton step : line search = 1:3) over meshes of 2k-300k elements
+    (lognormal, P1/P2/P3 = 50/35/15 %), Poisson arrivals at fractions of the saturated rate."""
+    import torch
+    from .serve import UvhServer, make_requests
+    rng = np.random.default_rng(17)
+    sizes = np.clip(np.exp(rng.normal(np.log(20000), 1.0, n_meshes)), 2000, 300000)
+    types = [((3, 2), (6, 5), (10, 8))[rng.choice(3, p=(0.5, 0.35, 0.15))] for _ in range(n_meshes)]
+    meshes = []
+    t0 = time.perf_counter()
+    for k, (nele, t) in enumerate(zip(sizes, types)):
+        w = batch_workloads(n=1, seed=100 + k, lo=nele, hi=nele * 1.0001, types=(t,), name=f"srv{k}",
+                            C_sigma=0.2, A_sigma=0.0)[0]
+        M = build_mesh(w)
+        F0, F1, _ = make_fields(M, w)
+        meshes.append((M, F0, F1))
+    cv = default_ctrlvar()
+    srv = UvhServer(cv, meshes, slots=slots, hbm_budget=32e9, keep_outputs=False)
+    log(f"serve: {n_meshes} meshes, {sum(m[0].Nele for m in meshes)} elements, set-up {time.perf_counter() - t0:.1f} s")
+    # service time of each (mesh, kind) alone
+    alone = {}
+    for mi in range(n_meshes):
+        for newton in (False, True):
+            from .serve import Request
+            r = Request(0, mi, newton, 0.0, meshes[mi][2].ub, meshes[mi][2].vb, meshes[mi][2].h)
+            srv._launch(r, 0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(3):
+                srv._launch(r, 0)
+            torch.cuda.synchronize()
+            alone[(mi, newton)] = (time.perf_counter() - t) / 3
+    # saturation: everything at t = 0
+    reqs = make_requests(meshes, n_requests, float("inf"), seed=11)
+    t = time.perf_counter()
+    srv.run(reqs)
+    makespan = max(r.done for r in reqs)
+    sat_rate = n_requests / makespan
+    elements = sum(meshes[r.mesh][0].Nele for r in reqs)
+    log(f"serve: saturated {sat_rate:.0f} requests/s, {elements / makespan / 1e6:.1f} M elements/s")
+    sweep = []
+    for frac in (0.3, 0.5, 0.7, 0.9):
+        reqs = make_requests(meshes, n_requests, frac * sat_rate, seed=12)
+        srv.run(reqs)
+        lat = np.array([r.latency for r in reqs])
+        slow = np.array([r.latency / alone[(r.mesh, r.newton)] for r in reqs])
+        span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+        el = sum(meshes[r.mesh][0].Nele for r in reqs)
+        row = {"load": frac, "rate_per_s": frac * sat_rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+               "p99_ms": float(np.percentile(lat, 99) * 1e3), "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+               "slowdown_p50": float(np.percentile(slow, 50)), "slowdown_p99": float(np.percentile(slow, 99)),
+               "elements_per_s": el / span, "peak_inflight": srv.peak_inflight}
+        sweep.append(row)
+        log(f"serve load {frac}: p50 {row['p50_ms']:.3f} ms p99 {row['p99_ms']:.3f} ms, slowdown p50 "
+            f"{row['slowdown_p50']:.2f} p99 {row['slowdown_p99']:.2f}")
+    srv.close()
+    at70 = next(r for r in sweep if r["load"] == 0.7)
+    return {
+        "metric": "uvh request latency p50/p99 at 70% of saturated load",
+        "value": at70["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1, "steps": n_requests, "warmup": 0,
+        "ms_per_step": at70["p50_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded meshes and Poisson arrivals)",
+        "config": {"workload": "serve500", "meshes": n_meshes, "slots": slots,
+                   "Nele_range": [int(sizes.min()), int(sizes.max())], "parallelism": "1 GPU"},
+        "saturated_requests_per_s": sat_rate, "saturated_elements_per_s": elements / makespan, "sweep": sweep,
+    }
+
+
+def bench_newton(workload: str = "newton_p2", log=print) -> dict:
+    """Full Newton uvh solve (Úa's default tolerance rForce < 1e-15), GPU assembly and a host
+    direct solve, FP64 and the precision form."""
+    from . import newton
+    w = WORKLOADS[workload]
+    M = build_mesh(w)
+    F0, F1, BCs, fixed = newton_problem(M, w)
+    cv = default_ctrlvar()
+    out = {}
+    for fp32 in (False, True):
+        t = time.perf_counter()
+        asm = newton.GpuAssembler(cv, M, F0, F1)
+        res = newton.solve(asm, cv, fixed=fixed, fp32=fp32)
+        total = time.perf_counter() - t
+        out["fp32" if fp32 else "fp64"] = {
+            "converged": res.converged, "iterations": res.iterations, "rForce": res.rForce, "seconds": total,
+            "assembly_s": res.t_assembly, "solve_s": res.t_solve, "linesearch_s": res.t_linesearch,
+            "rk_assemblies": res.n_rk, "r_assemblies": res.n_r, "state": (res.ub, res.vb, res.h)}
+        asm.ctx.close()
+        log(f"newton {'fp32' if fp32 else 'fp64'}: {res.iterations} iterations, {res.reason}, {total:.1f} s "
+            f"(assembly+K download {res.t_assembly:.2f} s, solve {res.t_solve:.2f} s, line search {res.t_linesearch:.2f} s)")
+    u64, v64, h64 = out["fp64"].pop("state")
+    u32, v32, h32 = out["fp32"].pop("state")
+    speed = np.linalg.norm(np.concatenate([u64, v64]))
+    err = {"u": float(np.linalg.norm(u32 - u64) / speed), "v": float(np.linalg.norm(v32 - v64) / speed),
+           "h": float(np.linalg.norm(h32 - h64) / np.linalg.norm(h64))}
+    return {
+        "metric": "uvh Newton solve to rForce < 1e-15", "value": out["fp64"]["seconds"], "unit": "s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": out["fp64"]["seconds"] * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, inflow conditions)",
+        "config": {"workload": workload, "Nele": M.Nele, "Nnodes": M.Nnodes, "linear_solver": "SciPy SuperLU (host)",
+                   "parallelism": "1 GPU"},
+        "fp64": out["fp64"], "fp32_form": out["fp32"], "fp32_form_rel_err_vs_fp64": err,
+    }

@@ -84,6 +84,8 @@ WORKLOADS = {
     "medium_p3": Workload("medium_p3", 100e3, 50e3, 41, 21, 10, 8, seed=33, bump=True),
     "plain_p2": Workload("plain_p2", 400e3, 100e3, 633, 159, 6, 5, seed=2),
     "precision_p3": Workload("precision_p3", 400e3, 100e3, 1001, 251, 10, 8, seed=3, bump=True),
+    # Newton solve with a host direct solver: the plain_p2 recipe on a 161 x 41 grid
+    "newton_p2": Workload("newton_p2", 400e3, 100e3, 161, 41, 6, 5, seed=2),
 }
 
 
@@ -144,11 +146,12 @@ def newton_problem(MUA: MUA, w: Workload):
 
 
 def batch_workloads(n: int = 64, seed: int = 4, lo: int = 5000, hi: int = 80000, types=((3, 2), (6, 5)),
-                    weights=None, name: str = "batch") -> list:
+                    weights=None, name: str = "batch", C_sigma: float = 0.3, A_sigma: float = 0.2) -> list:
     """Ensemble of ``n`` independent meshes: element counts log-uniform in [lo, hi], element
     types drawn from ``types`` ((nodes, quadrature degree) pairs, with ``weights``), each member
-    with its own seeded lognormal C (sigma 0.3) and AGlen (sigma 0.2) perturbations, on
-    2:1 rectangles of 100 km x 50 km."""
+    with its own seeded lognormal C (``C_sigma``) and AGlen (``A_sigma``) perturbations, on
+    2:1 rectangles of 100 km x 50 km. (On 10-node elements a C perturbation of sigma 0.3 can
+    make the interpolated C + Czero negative at a point, which Úa reports as a NaN in R.)"""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n):
@@ -156,5 +159,5 @@ def batch_workloads(n: int = 64, seed: int = 4, lo: int = 5000, hi: int = 80000,
         t = types[rng.choice(len(types), p=weights)]
         cells_y = max(2, int(round(np.sqrt(nele / 4.0))))
         out.append(Workload(f"{name}{k}", 100e3, 50e3, 2 * cells_y + 1, cells_y + 1, t[0], t[1],
-                            seed=10_000 + seed * 1000 + k, C_sigma=0.3, A_sigma=0.2))
+                            seed=10_000 + seed * 1000 + k, C_sigma=C_sigma, A_sigma=A_sigma))
     return out
