@@ -27,8 +27,8 @@ int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
 size_t element_lds_bytes(int nod, int nip, int epb, bool fp32);
 hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
                         const int2 *elem_items, int n_elem, const int2 *row_items, int n_row, size_t elem_lds,
-                        size_t row_lds, int persistent_blocks, bool fp32, bool ronly, int check_nan, int *ctr,
-                        int *status, const Params &p, hipStream_t s);
+                        size_t row_lds, bool fp32, bool ronly, int check_nan, int *ctr, int *status,
+                        const Params &p, hipStream_t s);
 int element_epb_default(int nod, int nip, bool fp32);
 size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
@@ -537,7 +537,6 @@ struct uvh_batch {
     int2 *d_items[2][3] = {};   // nodal, element, row items
     int n_items[2][3] = {};
     size_t elem_lds[2] = {0, 0}, row_lds = 0;
-    int persistent_blocks[2] = {0, 0};
     uvh::BatchMember *d_mem = nullptr, *h_mem = nullptr;  // device array and its pinned host copy
     std::vector<uvh::BatchMember> last;
     hipEvent_t upload_done = nullptr;
@@ -581,11 +580,6 @@ int batch_build_lists(uvh_batch *b, int t) {
     }
     b->elem_lds[t] = elds;
     b->row_lds = rlds;
-    // persistent grid: every CU holds as many element workgroups as LDS and registers allow
-    int cus = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-    const int by_lds = std::max<int>(1, int((160 * 1024) / std::max<size_t>(elds, 1)));
-    b->persistent_blocks[t] = std::max(1, cus) * std::min(by_lds, 4);
     b->built[t] = true;
     return UVH_OK;
 }
@@ -734,8 +728,8 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
     const uvh::Params p = make_params(*ctrl, flags);
     const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
     e = uvh::launch_batch(b->d_mem, nm, b->d_items[t][0], b->n_items[t][0], b->d_items[t][1], b->n_items[t][1],
-                          b->d_items[t][2], b->n_items[t][2], b->elem_lds[t], b->row_lds, b->persistent_blocks[t],
-                          fp32, ronly, check, b->d_ctr, b->d_status, p, s);
+                          b->d_items[t][2], b->n_items[t][2], b->elem_lds[t], b->row_lds, fp32, ronly, check,
+                          b->d_ctr, b->d_status, p, s);
     if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_assemble: launch: ") + hipGetErrorString(e));
     return UVH_OK;
 }

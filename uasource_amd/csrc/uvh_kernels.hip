@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "uvh_internal.h"
 
@@ -1077,10 +1078,28 @@ size_t element_lds_bytes(int nod, int nip, int epb, bool fp32) {
 }
 
 // Batched launches. `ctr` holds 2 work counters; `status` one word per member.
+// Workgroups of a persistent launch: every CU filled to the kernel's own occupancy at this LDS size.
+// Computed once per (device, kernel, LDS size), so that captured launches make no queries.
+int persistent_grid(const void *kernel, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void *, size_t>, int> cache;
+    int dev = 0;
+    hipGetDevice(&dev);
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find({dev, kernel, lds});
+    if (it != cache.end()) return it->second;
+    int cus = 0, per_cu = 0;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    const int grid = std::max(1, cus) * per_cu;
+    cache[{dev, kernel, lds}] = grid;
+    return grid;
+}
+
 hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
                         const int2 *elem_items, int n_elem, const int2 *row_items, int n_row, size_t elem_lds,
-                        size_t row_lds, int persistent_blocks, bool fp32, bool ronly, int check_nan, int *ctr,
-                        int *status, const Params &p, hipStream_t s) {
+                        size_t row_lds, bool fp32, bool ronly, int check_nan, int *ctr, int *status,
+                        const Params &p, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, ctr, 2, status, nmem);
     hipLaunchKernelGGL(k_nodal_b_batch, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, p);
     if (elem_lds > 160 * 1024 || row_lds > 160 * 1024) return hipErrorInvalidConfiguration;
@@ -1088,7 +1107,8 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
     do {                                                                                                   \
         auto kern = k_element_batch<T, RO>;                                                                \
         lds_attr(reinterpret_cast<const void *>(kern), elem_lds);                                          \
-        hipLaunchKernelGGL(kern, dim3(persistent_blocks), dim3(256), elem_lds, s, mem, elem_items, n_elem, ctr, p); \
+        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), elem_lds);                  \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), elem_lds, s, mem, elem_items, n_elem, ctr, p);     \
     } while (0)
     if (fp32) { if (ronly) UVH_EB(float, true); else UVH_EB(float, false); }
     else { if (ronly) UVH_EB(double, true); else UVH_EB(double, false); }
@@ -1104,7 +1124,8 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
     do {                                                                                                   \
         auto kern = k_rows_batch<TE>;                                                                      \
         lds_attr(reinterpret_cast<const void *>(kern), row_lds);                                           \
-        hipLaunchKernelGGL(kern, dim3(persistent_blocks), dim3(256), row_lds, s, mem, row_items, n_row, ctr + 1, \
+        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), row_lds);                   \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), row_lds, s, mem, row_items, n_row, ctr + 1,        \
                            check_nan, status);                                                             \
     } while (0)
     if (fp32) UVH_RB(float); else UVH_RB(double);
