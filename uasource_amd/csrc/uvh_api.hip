@@ -15,10 +15,10 @@
 namespace uvh {
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s);
 hipError_t launch_clear_status(int *status, hipStream_t s);
-hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, const int32_t *conn, const double *x,
-                          const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
-                          const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly, void *EB,
-                          void *RE, hipStream_t s);
+hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
+                          const double *x, const double *y, const FieldPtrs &f, const double *b,
+                          const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
+                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s);
 hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
@@ -91,6 +91,10 @@ struct uvh_ctx {
     // work
     double *d_b = nullptr, *d_RE = nullptr, *d_EB = nullptr;
     int epb[2] = {0, 0};    // elements per block of the element kernel: [FP64, FP32]
+    // one-pass element rows (all nine blocks per thread) for 3- and 6-node elements: faster with
+    // FP32 arithmetic; with FP64 the three-pass rows keep the registers for two waves per SIMD.
+    // UVH_ROWS=full or =split forces one form for both.
+    int rows_mode = 0;      // 0 automatic, 1 one-pass, 2 three-pass
     int *d_status = nullptr;
     // host-path staging
     double *d_fields = nullptr;  // 18 * nn
@@ -203,6 +207,8 @@ int uvh_create(uvh_ctx **out, int device) {
     // two-pass path (element rows through HBM, then one wave per node).
     const char *am = std::getenv("UVH_ASSEMBLY");
     ctx->two_pass = !(am && std::strcmp(am, "fused") == 0);
+    const char *rows = std::getenv("UVH_ROWS");
+    ctx->rows_mode = !rows ? 0 : std::strcmp(rows, "full") == 0 ? 1 : std::strcmp(rows, "split") == 0 ? 2 : 0;
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
     if (e == hipSuccess)
@@ -430,7 +436,9 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     }
     if (e == hipSuccess)
         e = ctx->timed(fp32 ? (ronly ? "element32_r" : "element32_rk") : (ronly ? "element_r" : "element_rk"), s, [&] {
-            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[fp32 ? 1 : 0], fp32, ctx->d_conn, ctx->d_x,
+            const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
+            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[fp32 ? 1 : 0], fp32, full, ctx->d_conn,
+                                       ctx->d_x,
                                        ctx->d_y, f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly,
                                        ctx->d_EB,
                                        ctx->d_RE, s);

@@ -434,6 +434,70 @@ __device__ __forceinline__ void element_row_b(int i, int nip, const T *__restric
     Tr = Ts; Fr = Fs;
 }
 
+// Row i of the element in one pass: T and F of the three block rows and all nine Jacobian
+// blocks, with the column derivatives Nx_j, Ny_j formed once per (j, point) for all blocks
+// (the same regrouped terms as element_row_b). Used where the 9 x NOD accumulators fit the
+// register budget (3- and 6-node elements).
+template <int NOD, typename T, bool RONLY>
+__device__ __forceinline__ void element_row_full(int i, int nip, const T *__restrict__ cr, const T *__restrict__ sN,
+                                                 const T *__restrict__ sXi, const T *__restrict__ sEta,
+                                                 const typename RowTab<T>::type *__restrict__ g, T (&TF)[6],
+                                                 T (&K)[RONLY ? 1 : 9][NOD]) {
+    T Tx = 0, Fx = 0, Ty = 0, Fy = 0, Th = 0, Fh = 0;
+    if (!RONLY) {
+#pragma unroll
+        for (int b = 0; b < 9; ++b)
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) K[b][j] = T(0);
+    }
+    for (int q = 0; q < nip; ++q) {
+        const T *r = cr + q * kNC;
+        const T *qN = g->N[q], *qXi = g->dxi[q], *qEta = g->deta[q];
+        const T I11 = r[C_I11], I12 = r[C_I12], I21 = r[C_I21], I22 = r[C_I22];
+        const T Ni = sN[q * NOD + i];
+        const T Nxi = I11 * sXi[q * NOD + i] + I12 * sEta[q * NOD + i];
+        const T Nyi = I21 * sXi[q * NOD + i] + I22 * sEta[q * NOD + i];
+        const T S1 = r[C_S1], S2 = r[C_S2], EXY = r[C_EXY];
+        const T wheta = r[C_WHETA], whE2 = r[C_WHE2];
+        const T ai = S1 * Nxi + EXY * Nyi;
+        const T bi = S2 * Nyi + EXY * Nxi;
+        const T psi = Ni + r[C_TU] * Nxi + r[C_TV] * Nyi;
+        Tx += T(2) * wheta * ai + r[C_WTX] * Ni;
+        Fx += r[C_WF1X] * Ni + r[C_WF2] * Nxi;
+        Ty += T(2) * wheta * bi + r[C_WTY] * Ni;
+        Fy += r[C_WF1Y] * Ni + r[C_WF2] * Nyi;
+        Th += r[C_TH] * psi;
+        Fh += r[C_FH] * psi;
+        if (!RONLY) {
+            const T wa = whE2 * ai, wb = whE2 * bi;
+            const T xxx = T(4) * wheta * Nxi + wa * S1, xmix = wheta * Nyi + wa * EXY, xyy = T(2) * wheta * Nxi + wa * S2;
+            const T yxx = T(2) * wheta * Nyi + wb * S1, ymix = wheta * Nxi + wb * EXY, yyy = T(4) * wheta * Nyi + wb * S2;
+            const T dxu = r[C_WDTXDU] * Ni, dxv = r[C_WDTXDV] * Ni, dyv = r[C_WDTYDV] * Ni;
+            const T xh = r[C_PX] * Nxi + r[C_PXY] * Nyi + r[C_PCX] * Ni;
+            const T yh = r[C_PY] * Nyi + r[C_PXY] * Nxi + r[C_PCY] * Ni;
+            const T Wpsi = r[C_W] * psi;
+            const T hAX = Wpsi * r[C_AX], hAY = Wpsi * r[C_AY], hRH = Wpsi * r[C_RH];
+            const T hG = Wpsi * r[C_GAM], hRU = Wpsi * r[C_RU], hRV = Wpsi * r[C_RV];
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) {
+                const T Nj = qN[j];
+                const T Nxj = I11 * qXi[j] + I12 * qEta[j];
+                const T Nyj = I21 * qXi[j] + I22 * qEta[j];
+                K[0][j] += xxx * Nxj + xmix * Nyj + dxu * Nj;   // Kxu
+                K[1][j] += xyy * Nyj + xmix * Nxj + dxv * Nj;   // Kxv
+                K[2][j] += xh * Nj;                             // Kxh
+                K[3][j] += yxx * Nxj + ymix * Nyj + dxv * Nj;   // Kyu
+                K[4][j] += yyy * Nyj + ymix * Nxj + dyv * Nj;   // Kyv
+                K[5][j] += yh * Nj;                             // Kyh
+                K[6][j] += hAX * Nj + hRH * Nxj;                // Khu
+                K[7][j] += hAY * Nj + hRH * Nyj;                // Khv
+                K[8][j] += hG * Nj + hRU * Nxj + hRV * Nyj;     // Khh
+            }
+        }
+    }
+    TF[0] = Tx; TF[1] = Fx; TF[2] = Ty; TF[3] = Fy; TF[4] = Th; TF[5] = Fh;
+}
+
 // Shape tables in LDS for the row stage, in type T: the FP64 tables themselves for the plain
 // form, FP32 copies for the precision form.
 template <typename T>
@@ -446,7 +510,7 @@ __device__ __forceinline__ int row_table_doubles(int tstride) {
 // rows. The caller guarantees epb*NOD <= 256 and epb*nip <= 256. T is the arithmetic type of
 // the physics and the element rows (double, or float for the precision form); the rows are
 // stored in T at their position in the node-incidence order.
-template <int NOD, typename T, bool RONLY>
+template <int NOD, typename T, bool RONLY, bool FULL>
 __device__ __forceinline__ void element_block(double *lds, int nele, int nip, int epb, int lb,
                                               const int32_t *__restrict__ conn, const double *__restrict__ xn,
                                               const double *__restrict__ yn, const FieldPtrs &f,
@@ -504,9 +568,30 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     }
     __syncthreads();
 
+    const int nrow = epb * NOD;
+    if (FULL) {
+        // stage 3, one pass: (element, row) items, all nine blocks per item
+        for (int it = tid; it < nrow; it += 256) {
+            const int el = it / NOD, i = it - el * NOD;
+            const int64_t e = e0 + el;
+            if (e >= nele) continue;
+            const int64_t row = inc_pos[e * NOD + i];
+            T TF[6], Kf[RONLY ? 1 : 9][NOD];
+            element_row_full<NOD, T, RONLY>(i, nip, rec + el * nip * kNC, rN, rXi, rEta, rtab, TF, Kf);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) RE[row * 6 + k] = TF[k];
+            if (!RONLY) {
+                T *eb = EB + row * eb_stride<NOD, T>();
+#pragma unroll
+                for (int b = 0; b < (RONLY ? 1 : 9); ++b)
+#pragma unroll
+                    for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = Kf[b][j];
+            }
+        }
+        return;
+    }
     // stage 3: element rows, as (block row, element, row) items dealt round-robin to the
     // threads; each item accumulates 3 x NOD Jacobian terms and its T and F over the points
-    const int nrow = epb * NOD;
     for (int it = tid; it < 3 * nrow; it += 256) {
         const int bi = it / nrow;
         const int rr = it - bi * nrow;
@@ -532,7 +617,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     }
 }
 
-template <int NOD, typename T, bool RONLY>
+template <int NOD, typename T, bool RONLY, bool FULL>
 __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, const int32_t *__restrict__ conn,
                                                  const double *__restrict__ xn, const double *__restrict__ yn,
                                                  FieldPtrs f, const double *__restrict__ bnod,
@@ -544,7 +629,7 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     const int nblocks = (nele + epb - 1) / epb;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     if (lb >= nblocks) return;  // whole block
-    element_block<NOD, T, RONLY>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB, RE);
+    element_block<NOD, T, RONLY, FULL>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB, RE);
 }
 
 // Node rows: kRowLanes lanes per node, 64 / kRowLanes nodes per wave. The incident element
@@ -731,13 +816,13 @@ __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__rest
         else rtab = M.tab32;
         T *EB = static_cast<T *>(M.EB), *RE = static_cast<T *>(M.RE);
         if (M.nod == 3)
-            element_block<3, T, RONLY>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
+            element_block<3, T, RONLY, sizeof(T) == sizeof(float)>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
                                        M.inc_pos, EB, RE);
         else if (M.nod == 6)
-            element_block<6, T, RONLY>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
+            element_block<6, T, RONLY, sizeof(T) == sizeof(float)>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
                                        M.inc_pos, EB, RE);
         else
-            element_block<10, T, RONLY>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
+            element_block<10, T, RONLY, false>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
                                         M.inc_pos, EB, RE);
         __syncthreads();  // the LDS of this item is dead before the next one overwrites it
     }
@@ -973,9 +1058,10 @@ size_t element_per_el_bytes(int nod, int nip, bool fp32) {
 }
 
 template <int NOD, typename T>
-hipError_t launch_element_t(int nele, int nip, int epb, const int32_t *conn, const double *x, const double *y,
-                            const FieldPtrs &f, const double *b, const ShapeTables *tab, const ShapeTables32 *tab32,
-                            const Params &p, const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
+hipError_t launch_element_t(int nele, int nip, int epb, bool full, const int32_t *conn, const double *x,
+                            const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
+                            const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly,
+                            void *EB, void *RE, hipStream_t s) {
     const typename RowTab<T>::type *rtab;
     if constexpr (sizeof(T) == sizeof(double)) rtab = tab;
     else rtab = tab32;
@@ -983,9 +1069,9 @@ hipError_t launch_element_t(int nele, int nip, int epb, const int32_t *conn, con
     const int grid = ((nblocks + kXcds - 1) / kXcds) * kXcds;  // whole rounds over the XCDs
     const bool fp32 = sizeof(T) == sizeof(float);
     const size_t lds = element_table_bytes(NOD, nip, fp32) + element_per_el_bytes(NOD, nip, fp32) * epb;
-    auto kr = k_element<NOD, T, true>;
-    auto kk = k_element<NOD, T, false>;
-    auto kern = ronly ? kr : kk;
+    constexpr bool can_full = NOD <= 6;
+    auto kern = ronly ? (full && can_full ? k_element<NOD, T, true, can_full> : k_element<NOD, T, true, false>)
+                      : (full && can_full ? k_element<NOD, T, false, can_full> : k_element<NOD, T, false, false>);
     lds_attr(reinterpret_cast<const void *>(kern), lds);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
                        static_cast<T *>(EB), static_cast<T *>(RE));
@@ -1016,14 +1102,14 @@ hipError_t launch_clear_status(int *status, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, const int32_t *conn, const double *x,
-                          const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
-                          const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly, void *EB,
-                          void *RE, hipStream_t s) {
+hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
+                          const double *x, const double *y, const FieldPtrs &f, const double *b,
+                          const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
+                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
     if (epb < 1 || epb * nip > 256 || epb * nod > 256) return hipErrorInvalidConfiguration;
 #define UVH_EL(NODV)                                                                                           \
-    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s) \
-                : launch_element_t<NODV, double>(nele, nip, epb, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s)
+    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s) \
+                : launch_element_t<NODV, double>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s)
     switch (nod) {
         case 3: UVH_EL(3);
         case 6: UVH_EL(6);
