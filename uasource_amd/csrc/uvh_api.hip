@@ -26,9 +26,9 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
 int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
 size_t element_lds_bytes(int nod, int nip, int epb, bool fp32);
 hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
-                        const int2 *elem_items, int n_elem, const int2 *row_items, int n_row, size_t elem_lds,
-                        size_t row_lds, bool fp32, bool ronly, int check_nan, int *ctr, int *status,
-                        const Params &p, hipStream_t s);
+                        const int2 *elem_items, const int *eb, const int2 *row_items, const int *rb,
+                        const size_t *elem_lds, const size_t *row_lds, bool fp32, bool ronly, int check_nan,
+                        int *ctr, int *status, const Params &p, hipStream_t s);
 int element_epb_default(int nod, int nip, bool fp32);
 size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
@@ -400,7 +400,7 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL is the HIP null stream
     const uvh::MeshHost &m = ctx->mesh;
     if (ctx->two_pass && !ronly && !ctx->d_EB) {
-        // FP64 rows padded to 128 bytes; the FP32 rows of the precision form fit in the same buffer
+        // FP64 rows; the FP32 rows of the precision form fit in the same buffer
         const size_t stride = uvh::element_row_stride(m.nod, false);
         if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * stride) != hipSuccess)
             return ctx->fail(UVH_E_NOMEM, "uvh_assemble: cannot allocate element rows");
@@ -542,9 +542,10 @@ struct uvh_batch {
     std::vector<uvh_ctx *> members;
     // work lists for the FP64 [0] and FP32 [1] element tiles
     bool built[2] = {false, false};
-    int2 *d_items[2][3] = {};   // nodal, element, row items
+    int2 *d_items[2][3] = {};   // nodal, element, row items (element and row items grouped by type)
     int n_items[2][3] = {};
-    size_t elem_lds[2] = {0, 0}, row_lds = 0;
+    int ebeg[2][4] = {}, rbeg[2][4] = {};  // per element type (3, 6, 10 nodes): item ranges
+    size_t elem_lds[2][3] = {}, row_lds[2][3] = {};
     uvh::BatchMember *d_mem = nullptr, *h_mem = nullptr;  // device array and its pinned host copy
     std::vector<uvh::BatchMember> last;
     hipEvent_t upload_done = nullptr;
@@ -561,33 +562,44 @@ struct uvh_batch {
 
 namespace {
 
-// Work lists of one element tile choice: (member, block) pairs in member order.
+// Work lists of one element tile choice: (member, block) pairs; element and row items grouped
+// by element type (3, 6, 10 nodes), members in order within a type.
 int batch_build_lists(uvh_batch *b, int t) {
     const bool fp32 = t == 1;
     std::vector<int2> items[3];
-    size_t elds = 0, rlds = 0;
+    const int nods[3] = {3, 6, 10};
+    for (int k = 0; k < 3; ++k) { b->elem_lds[t][k] = 0; b->row_lds[t][k] = 0; }
     for (int mi = 0; mi < int(b->members.size()); ++mi) {
-        const uvh_ctx *c = b->members[mi];
-        const uvh::MeshHost &m = c->mesh;
-        const int epb = c->epb[t];
+        const uvh::MeshHost &m = b->members[mi]->mesh;
         for (int k = 0; k < (m.nn + 255) / 256; ++k) items[0].push_back(make_int2(mi, k));
-        for (int k = 0; k < (m.nele + epb - 1) / epb; ++k) items[1].push_back(make_int2(mi, k));
-        for (int k = 0; k < (m.nn + uvh::kRowNodesPerBlock - 1) / uvh::kRowNodesPerBlock; ++k)
-            items[2].push_back(make_int2(mi, k));
-        elds = std::max(elds, uvh::element_lds_bytes(m.nod, m.nip, epb, fp32));
-        rlds = std::max(rlds, size_t(9) * size_t(m.max_block_deg) * sizeof(double));
     }
+    for (int ty = 0; ty < 3; ++ty) {
+        b->ebeg[t][ty] = int(items[1].size());
+        b->rbeg[t][ty] = int(items[2].size());
+        for (int mi = 0; mi < int(b->members.size()); ++mi) {
+            const uvh_ctx *c = b->members[mi];
+            const uvh::MeshHost &m = c->mesh;
+            if (m.nod != nods[ty]) continue;
+            const int epb = c->epb[t];
+            for (int k = 0; k < (m.nele + epb - 1) / epb; ++k) items[1].push_back(make_int2(mi, k));
+            for (int k = 0; k < (m.nn + uvh::kRowNodesPerBlock - 1) / uvh::kRowNodesPerBlock; ++k)
+                items[2].push_back(make_int2(mi, k));
+            b->elem_lds[t][ty] = std::max(b->elem_lds[t][ty], uvh::element_lds_bytes(m.nod, m.nip, epb, fp32));
+            b->row_lds[t][ty] = std::max(b->row_lds[t][ty], size_t(9) * size_t(m.max_block_deg) * sizeof(double));
+        }
+    }
+    b->ebeg[t][3] = int(items[1].size());
+    b->rbeg[t][3] = int(items[2].size());
     for (int k = 0; k < 3; ++k) {
         hipFree(b->d_items[t][k]);
         b->d_items[t][k] = nullptr;
         b->n_items[t][k] = int(items[k].size());
+        if (items[k].empty()) continue;
         if (dalloc(&b->d_items[t][k], items[k].size()) != hipSuccess ||
             hipMemcpy(b->d_items[t][k], items[k].data(), items[k].size() * sizeof(int2), hipMemcpyHostToDevice) !=
                 hipSuccess)
             return b->fail(UVH_E_NOMEM, "uvhb: cannot upload work lists");
     }
-    b->elem_lds[t] = elds;
-    b->row_lds = rlds;
     b->built[t] = true;
     return UVH_OK;
 }
@@ -604,7 +616,7 @@ int uvhb_create(uvh_batch **out, int device) {
     hipSetDevice(device);
     auto *b = new uvh_batch();
     b->device = device;
-    if (hipMalloc(reinterpret_cast<void **>(&b->d_ctr), 2 * sizeof(int)) != hipSuccess ||
+    if (hipMalloc(reinterpret_cast<void **>(&b->d_ctr), 6 * sizeof(int)) != hipSuccess ||
         hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming) != hipSuccess) {
         delete b;
         return UVH_E_HIP;
@@ -735,8 +747,8 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
     if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_assemble: upload: ") + hipGetErrorString(e));
     const uvh::Params p = make_params(*ctrl, flags);
     const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
-    e = uvh::launch_batch(b->d_mem, nm, b->d_items[t][0], b->n_items[t][0], b->d_items[t][1], b->n_items[t][1],
-                          b->d_items[t][2], b->n_items[t][2], b->elem_lds[t], b->row_lds, fp32, ronly, check,
+    e = uvh::launch_batch(b->d_mem, nm, b->d_items[t][0], b->n_items[t][0], b->d_items[t][1], b->ebeg[t],
+                          b->d_items[t][2], b->rbeg[t], b->elem_lds[t], b->row_lds[t], fp32, ronly, check,
                           b->d_ctr, b->d_status, p, s);
     if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_assemble: launch: ") + hipGetErrorString(e));
     return UVH_OK;

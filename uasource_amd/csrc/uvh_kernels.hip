@@ -47,12 +47,10 @@ enum : int {
 };
 constexpr int kNC = C_COUNT + (C_COUNT & 1);  // even, for 16-byte LDS reads
 
-// Element rows in HBM (two-pass path) are padded to whole 128-byte lines.
+// Element rows in HBM (two-pass path): 9 x NOD values each, back to back. The rows of one
+// node are contiguous (incidence order), so the row scatter reads one contiguous range per node.
 template <int NOD, typename T>
-constexpr int eb_stride() {
-    constexpr int per_line = 128 / int(sizeof(T));
-    return ((9 * NOD + per_line - 1) / per_line) * per_line;
-}
+constexpr int eb_stride() { return 9 * NOD; }
 
 __global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -715,9 +713,11 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
             }
         }
     }
-    for (int v = gl; v < 9 * deg; v += kRowLanes) {
-        const int bi = v / (3 * deg);
-        Kval[(int64_t)bi * 3 * nnzN + 3 * rp + (v - bi * 3 * deg)] = buf[v];
+#pragma unroll
+    for (int bi = 0; bi < 3; ++bi) {
+        double *dst = Kval + (int64_t)bi * 3 * nnzN + 3 * rp;
+        const double *src = buf + bi * 3 * deg;
+        for (int v = gl; v < 3 * deg; v += kRowLanes) dst[v] = src[v];
     }
     if (check_nan && gl < 3 && isnan(buf[gl * 3 * deg + gl * deg + diag[k]])) bad |= 2;
     if (bad) atomicOr(status, bad);
@@ -775,7 +775,9 @@ __global__ void k_clear_counters(int *counters, int n, int *status, int nstatus)
 __device__ __forceinline__ int next_item(int *counter, int *s_item) {
     if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int it = *s_item;
+    // the same value in every lane: make it a scalar, so the member's table entries and
+    // pointers are read with scalar loads and kept in scalar registers
+    const int it = __builtin_amdgcn_readfirstlane(*s_item);
     __syncthreads();  // all threads have read it before thread 0 takes the next one
     return it;
 }
@@ -799,8 +801,8 @@ __global__ __launch_bounds__(256) void k_nodal_b_batch(const BatchMember *__rest
     M.b[k] = bb;
 }
 
-// items: (member, element block of the member's epb elements)
-template <typename T, bool RONLY>
+// items: (member, element block of the member's epb elements), all members with NOD nodes
+template <int NOD, typename T, bool RONLY>
 __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__restrict__ mem,
                                                        const int2 *__restrict__ items, int nitems,
                                                        int *__restrict__ counter, Params p) {
@@ -814,22 +816,16 @@ __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__rest
         const typename RowTab<T>::type *rtab;
         if constexpr (sizeof(T) == sizeof(double)) rtab = M.tab;
         else rtab = M.tab32;
-        T *EB = static_cast<T *>(M.EB), *RE = static_cast<T *>(M.RE);
-        if (M.nod == 3)
-            element_block<3, T, RONLY, sizeof(T) == sizeof(float)>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
-                                       M.inc_pos, EB, RE);
-        else if (M.nod == 6)
-            element_block<6, T, RONLY, sizeof(T) == sizeof(float)>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
-                                       M.inc_pos, EB, RE);
-        else
-            element_block<10, T, RONLY, false>(lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p,
-                                        M.inc_pos, EB, RE);
+        element_block<NOD, T, RONLY, (NOD <= 6 && sizeof(T) == sizeof(float))>(
+            lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p, M.inc_pos,
+            static_cast<T *>(M.EB), static_cast<T *>(M.RE));
         __syncthreads();  // the LDS of this item is dead before the next one overwrites it
     }
 }
 
-// items: (member, node block of kRowNodesPerBlock nodes); status has one word per member
-template <typename TE>
+// items: (member, node block of kRowNodesPerBlock nodes), all members with NOD nodes;
+// status has one word per member
+template <int NOD, typename TE>
 __global__ __launch_bounds__(256) void k_rows_batch(const BatchMember *__restrict__ mem,
                                                     const int2 *__restrict__ items, int nitems,
                                                     int *__restrict__ counter, int check_nan,
@@ -841,17 +837,9 @@ __global__ __launch_bounds__(256) void k_rows_batch(const BatchMember *__restric
         if (it >= nitems) break;
         const int2 w = items[it];
         const BatchMember &M = mem[w.x];
-        const TE *EB = static_cast<const TE *>(M.EB), *RE = static_cast<const TE *>(M.RE);
-        const int k0 = w.y * kRowNodesPerBlock;
-        if (M.nod == 3)
-            rows_block<3, TE>(lds, M.nn, k0, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN, EB, RE, M.R, M.Tint,
-                              M.Fext, M.Kval, check_nan, status + w.x);
-        else if (M.nod == 6)
-            rows_block<6, TE>(lds, M.nn, k0, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN, EB, RE, M.R, M.Tint,
-                              M.Fext, M.Kval, check_nan, status + w.x);
-        else
-            rows_block<10, TE>(lds, M.nn, k0, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN, EB, RE, M.R, M.Tint,
-                               M.Fext, M.Kval, check_nan, status + w.x);
+        rows_block<NOD, TE>(lds, M.nn, w.y * kRowNodesPerBlock, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN,
+                            static_cast<const TE *>(M.EB), static_cast<const TE *>(M.RE), M.R, M.Tint, M.Fext, M.Kval,
+                            check_nan, status + w.x);
         __syncthreads();
     }
 }
@@ -1119,10 +1107,7 @@ hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool f
 #undef UVH_EL
 }
 
-size_t element_row_stride(int nod, bool fp32) {
-    const int per_line = fp32 ? 32 : 16;
-    return size_t((9 * nod + per_line - 1) / per_line) * per_line;
-}
+size_t element_row_stride(int nod, bool) { return size_t(9 * nod); }
 
 hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
@@ -1182,22 +1167,36 @@ int persistent_grid(const void *kernel, size_t lds) {
     return grid;
 }
 
+// Batched launches, one persistent element launch and one persistent row launch per element
+// type present (each instantiation keeps its own register budget). Items are grouped by type:
+// type k (3, 6, 10 nodes) owns elem_items[eb[k] .. eb[k+1]) and row_items[rb[k] .. rb[k+1]).
+// `ctr` holds 6 work counters; `status` one word per member.
 hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
-                        const int2 *elem_items, int n_elem, const int2 *row_items, int n_row, size_t elem_lds,
-                        size_t row_lds, bool fp32, bool ronly, int check_nan, int *ctr, int *status,
-                        const Params &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, ctr, 2, status, nmem);
+                        const int2 *elem_items, const int *eb, const int2 *row_items, const int *rb,
+                        const size_t *elem_lds, const size_t *row_lds, bool fp32, bool ronly, int check_nan,
+                        int *ctr, int *status, const Params &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, ctr, 6, status, nmem);
     hipLaunchKernelGGL(k_nodal_b_batch, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, p);
-    if (elem_lds > 160 * 1024 || row_lds > 160 * 1024) return hipErrorInvalidConfiguration;
-#define UVH_EB(T, RO)                                                                                      \
+    for (int k = 0; k < 3; ++k)
+        if (elem_lds[k] > 156 * 1024 || row_lds[k] > 156 * 1024) return hipErrorInvalidConfiguration;
+#define UVH_EB(NODV, T, RO, K)                                                                             \
     do {                                                                                                   \
-        auto kern = k_element_batch<T, RO>;                                                                \
-        lds_attr(reinterpret_cast<const void *>(kern), elem_lds);                                          \
-        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), elem_lds);                  \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), elem_lds, s, mem, elem_items, n_elem, ctr, p);     \
+        auto kern = k_element_batch<NODV, T, RO>;                                                          \
+        lds_attr(reinterpret_cast<const void *>(kern), elem_lds[K]);                                       \
+        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), elem_lds[K]);               \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), elem_lds[K], s, mem, elem_items + eb[K],           \
+                           eb[K + 1] - eb[K], ctr + K, p);                                                 \
     } while (0)
-    if (fp32) { if (ronly) UVH_EB(float, true); else UVH_EB(float, false); }
-    else { if (ronly) UVH_EB(double, true); else UVH_EB(double, false); }
+#define UVH_EB_T(NODV, K)                                                                                  \
+    do {                                                                                                   \
+        if (eb[K + 1] == eb[K]) break;                                                                     \
+        if (fp32) { if (ronly) UVH_EB(NODV, float, true, K); else UVH_EB(NODV, float, false, K); }        \
+        else { if (ronly) UVH_EB(NODV, double, true, K); else UVH_EB(NODV, double, false, K); }           \
+    } while (0)
+    UVH_EB_T(3, 0);
+    UVH_EB_T(6, 1);
+    UVH_EB_T(10, 2);
+#undef UVH_EB_T
 #undef UVH_EB
     if (ronly) {
         if (fp32)
@@ -1206,15 +1205,23 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
             hipLaunchKernelGGL(k_rows_r_batch<double>, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, check_nan, status);
         return hipGetLastError();
     }
-#define UVH_RB(TE)                                                                                         \
+#define UVH_RB(NODV, TE, K)                                                                                \
     do {                                                                                                   \
-        auto kern = k_rows_batch<TE>;                                                                      \
-        lds_attr(reinterpret_cast<const void *>(kern), row_lds);                                           \
-        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), row_lds);                   \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), row_lds, s, mem, row_items, n_row, ctr + 1,        \
-                           check_nan, status);                                                             \
+        auto kern = k_rows_batch<NODV, TE>;                                                                \
+        lds_attr(reinterpret_cast<const void *>(kern), row_lds[K]);                                        \
+        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), row_lds[K]);                \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), row_lds[K], s, mem, row_items + rb[K],             \
+                           rb[K + 1] - rb[K], ctr + 3 + K, check_nan, status);                             \
     } while (0)
-    if (fp32) UVH_RB(float); else UVH_RB(double);
+#define UVH_RB_T(NODV, K)                                                                                  \
+    do {                                                                                                   \
+        if (rb[K + 1] == rb[K]) break;                                                                     \
+        if (fp32) UVH_RB(NODV, float, K); else UVH_RB(NODV, double, K);                                    \
+    } while (0)
+    UVH_RB_T(3, 0);
+    UVH_RB_T(6, 1);
+    UVH_RB_T(10, 2);
+#undef UVH_RB_T
 #undef UVH_RB
     return hipGetLastError();
 }
