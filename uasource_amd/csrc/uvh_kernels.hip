@@ -765,6 +765,47 @@ __global__ __launch_bounds__(256) void k_rows_r(int nn, const int32_t *__restric
 // across the CUs. Every item writes only its own element rows or node rows, so the order in
 // which workgroups take items does not change any result.
 
+// The member table as the batched kernels read it: the same layout as BatchMember, with every
+// pointer qualified as global memory (address space 1). A pointer loaded from memory is
+// otherwise generic, and generic (flat) accesses need 64-bit vector addresses and lose the
+// scalar loads of wave-uniform data; through this view the compiler emits global and scalar
+// loads, as for kernel arguments.
+#define UVH_G __attribute__((address_space(1)))
+struct FieldPtrsG {
+    const UVH_G double *h, *u, *v, *S, *B, *rho, *C, *m, *n, *A, *as1, *ab1;
+    const UVH_G double *h0, *u0, *v0, *as0, *ab0;
+    const UVH_G double *dadh;
+};
+struct BatchMemberG {
+    const UVH_G int32_t *conn;
+    const UVH_G double *x, *y;
+    const UVH_G ShapeTables *tab;
+    const UVH_G ShapeTables32 *tab32;
+    const UVH_G int32_t *inc_ptr, *inc_pos;
+    const UVH_G uint8_t *pos_inc, *diag;
+    const UVH_G int64_t *rowptr_n;
+    UVH_G double *b;
+    UVH_G void *EB, *RE;
+    int64_t nnzN;
+    int32_t nele, nn, nod, nip, epb, pad;
+    FieldPtrsG f;
+    UVH_G double *R, *Tint, *Fext, *Kval;
+};
+static_assert(sizeof(BatchMemberG) == sizeof(BatchMember), "member table views differ");
+static_assert(sizeof(FieldPtrsG) == sizeof(FieldPtrs), "field table views differ");
+
+__device__ __forceinline__ const UVH_G BatchMemberG *member(const BatchMember *mem, int i) {
+    return (const UVH_G BatchMemberG *)mem + i;
+}
+
+__device__ __forceinline__ FieldPtrs generic_fields(const UVH_G FieldPtrsG *g) {
+    return FieldPtrs{(const double *)g->h, (const double *)g->u, (const double *)g->v, (const double *)g->S,
+                     (const double *)g->B, (const double *)g->rho, (const double *)g->C, (const double *)g->m,
+                     (const double *)g->n, (const double *)g->A, (const double *)g->as1, (const double *)g->ab1,
+                     (const double *)g->h0, (const double *)g->u0, (const double *)g->v0, (const double *)g->as0,
+                     (const double *)g->ab0, (const double *)g->dadh};
+}
+
 __global__ void k_clear_counters(int *counters, int n, int *status, int nstatus) {
     const int t = threadIdx.x;
     if (t < n) counters[t] = 0;
@@ -812,13 +853,19 @@ __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__rest
         const int it = next_item(counter, &s_item);
         if (it >= nitems) break;
         const int2 w = items[it];
-        const BatchMember &M = mem[w.x];
+        const UVH_G BatchMemberG *M = member(mem, w.x);
+        // the shape tables are not written while any kernel runs: read them as constant memory
+        // (address space 4), so their wave-uniform entries come through scalar loads
         const typename RowTab<T>::type *rtab;
-        if constexpr (sizeof(T) == sizeof(double)) rtab = M.tab;
-        else rtab = M.tab32;
+        if constexpr (sizeof(T) == sizeof(double))
+            rtab = (const ShapeTables *)(const __attribute__((address_space(4))) ShapeTables *)M->tab;
+        else
+            rtab = (const ShapeTables32 *)(const __attribute__((address_space(4))) ShapeTables32 *)M->tab32;
+        const FieldPtrs f = generic_fields(&M->f);
         element_block<NOD, T, RONLY, (NOD <= 6 && sizeof(T) == sizeof(float))>(
-            lds, M.nele, M.nip, M.epb, w.y, M.conn, M.x, M.y, M.f, M.b, M.tab, rtab, p, M.inc_pos,
-            static_cast<T *>(M.EB), static_cast<T *>(M.RE));
+            lds, M->nele, M->nip, M->epb, w.y, (const int32_t *)M->conn, (const double *)M->x,
+            (const double *)M->y, f, (const double *)M->b, (const ShapeTables *)M->tab, rtab, p,
+            (const int32_t *)M->inc_pos, (T *)M->EB, (T *)M->RE);
         __syncthreads();  // the LDS of this item is dead before the next one overwrites it
     }
 }
@@ -836,10 +883,11 @@ __global__ __launch_bounds__(256) void k_rows_batch(const BatchMember *__restric
         const int it = next_item(counter, &s_item);
         if (it >= nitems) break;
         const int2 w = items[it];
-        const BatchMember &M = mem[w.x];
-        rows_block<NOD, TE>(lds, M.nn, w.y * kRowNodesPerBlock, M.inc_ptr, M.pos_inc, M.rowptr_n, M.diag, M.nnzN,
-                            static_cast<const TE *>(M.EB), static_cast<const TE *>(M.RE), M.R, M.Tint, M.Fext, M.Kval,
-                            check_nan, status + w.x);
+        const UVH_G BatchMemberG *M = member(mem, w.x);
+        rows_block<NOD, TE>(lds, M->nn, w.y * kRowNodesPerBlock, (const int32_t *)M->inc_ptr,
+                            (const uint8_t *)M->pos_inc, (const int64_t *)M->rowptr_n, (const uint8_t *)M->diag,
+                            M->nnzN, (const TE *)M->EB, (const TE *)M->RE, (double *)M->R, (double *)M->Tint,
+                            (double *)M->Fext, (double *)M->Kval, check_nan, status + w.x);
         __syncthreads();
     }
 }
