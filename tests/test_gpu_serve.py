@@ -32,7 +32,7 @@ def test_requests_in_the_mix_equal_requests_alone(rate):
     print(f"[gpu] serve 60 requests rate={rate}", flush=True)
     srv = UvhServer(default_ctrlvar(), meshes(), slots=6, hbm_budget=8e9)
     try:
-        reqs = make_requests(meshes(), 60, rate, seed=3)
+        reqs = make_requests(meshes(), 60, rate, seed=3, pinned=not np.isfinite(rate))
         srv.run(reqs)
         assert all(np.isfinite(r.done) and r.done >= r.start >= r.arrival for r in reqs)
         for r in reqs:

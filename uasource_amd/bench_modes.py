@@ -146,7 +146,9 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
     for mi in range(n_meshes):
         for newton in (False, True):
             from .serve import Request
-            r = Request(0, mi, newton, 0.0, meshes[mi][2].ub, meshes[mi][2].vb, meshes[mi][2].h)
+            st = [torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+                  for a in (meshes[mi][2].ub, meshes[mi][2].vb, meshes[mi][2].h)]
+            r = Request(0, mi, newton, 0.0, *st)
             srv._launch(r, 0)
             torch.cuda.synchronize()
             t = time.perf_counter()
@@ -155,7 +157,7 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
             torch.cuda.synchronize()
             alone[(mi, newton)] = (time.perf_counter() - t) / 3
     # saturation: everything at t = 0
-    reqs = make_requests(meshes, n_requests, float("inf"), seed=11)
+    reqs = make_requests(meshes, n_requests, float("inf"), seed=11, pinned=True)
     t = time.perf_counter()
     srv.run(reqs)
     makespan = max(r.done for r in reqs)
@@ -164,7 +166,7 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
     log(f"serve: saturated {sat_rate:.0f} requests/s, {elements / makespan / 1e6:.1f} M elements/s")
     sweep = []
     for frac in (0.3, 0.5, 0.7, 0.9):
-        reqs = make_requests(meshes, n_requests, frac * sat_rate, seed=12)
+        reqs = make_requests(meshes, n_requests, frac * sat_rate, seed=12, pinned=True)
         srv.run(reqs)
         lat = np.array([r.latency for r in reqs])
         slow = np.array([r.latency / alone[(r.mesh, r.newton)] for r in reqs])

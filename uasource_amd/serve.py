@@ -96,8 +96,12 @@ class UvhServer:
         s = self.streams[slot]
         with torch.cuda.stream(s):
             for name, val in (("ub", req.ub), ("vb", req.vb), ("h", req.h)):
-                if val is not None:
+                if val is None:
+                    continue
+                if isinstance(val, np.ndarray):
                     m.f1[name].copy_(torch.from_numpy(val), non_blocking=False)
+                else:   # a pinned host tensor: asynchronous on the request's stream
+                    m.f1[name].copy_(val, non_blocking=True)
             R = torch.empty(3 * m.nn, dtype=torch.float64, device=self.dev)
             K = torch.empty(m.ctx.nnz, dtype=torch.float64, device=self.dev) if req.newton else None
             flags = self.flags | (0 if req.newton else uvh.UVH_RONLY) | uvh.UVH_NO_NAN_CHECK
@@ -176,6 +180,7 @@ class UvhServer:
 
     def serve_alone(self, req: Request) -> Request:
         """The same request with nothing else on the device (for the mix-vs-alone check)."""
+        self.torch.cuda.synchronize()
         r = Request(req.rid, req.mesh, req.newton, 0.0, req.ub, req.vb, req.h)
         item = self._launch(r, 0)
         self.torch.cuda.synchronize()
@@ -190,10 +195,11 @@ class UvhServer:
 
 
 def make_requests(meshes, n: int, rate: float, seed: int = 5, newton_fraction: float = 0.25,
-                  perturb: float = 1.0) -> list[Request]:
+                  perturb: float = 1.0, pinned: bool = False) -> list[Request]:
     """``n`` requests with Poisson arrivals at ``rate`` per second (rate = inf: all at once),
     meshes drawn uniformly, Newton steps : line-search residuals = 1 : 3, each with its own
-    seeded state (the mesh's F1 with u, v perturbed by up to ``perturb`` m/a)."""
+    seeded state (the mesh's F1 with u, v perturbed by up to ``perturb`` m/a). ``pinned``: the
+    states are page-locked host tensors, uploaded asynchronously."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -205,7 +211,11 @@ def make_requests(meshes, n: int, rate: float, seed: int = 5, newton_fraction: f
         newton = bool(rng.random() < newton_fraction)
         ub = F1.ub + perturb * rng.standard_normal(M.Nnodes)
         vb = F1.vb + perturb * rng.standard_normal(M.Nnodes)
-        out.append(Request(k, mi, newton, t if np.isfinite(rate) else 0.0, ub, vb, F1.h.copy()))
+        h = F1.h.copy()
+        if pinned:
+            import torch
+            ub, vb, h = (torch.from_numpy(a).pin_memory() for a in (ub, vb, h))
+        out.append(Request(k, mi, newton, t if np.isfinite(rate) else 0.0, ub, vb, h))
     return out
 
 
