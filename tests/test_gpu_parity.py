@@ -235,3 +235,27 @@ def test_fused_path_matches_oracle(o2lib, monkeypatch, name):
         assert np.array_equal(r_only["R"], out2["R"])
     finally:
         c2.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "medium_p2", "medium_p3"])
+def test_gather_scatter_matches_lds_scatter(monkeypatch, name):
+    """The gather form of the row scatter (UVH_SCATTER=gather: one thread per node-graph
+    entry, no LDS) sums in the same order as the default LDS scatter: bit-identical R and K."""
+    M, F0, F1, BCs = mesh_fields(name)
+    cv = default_ctrlvar()
+    ref = uvh.UvhContext(0)
+    monkeypatch.setenv("UVH_SCATTER", "gather")
+    c2 = uvh.UvhContext(0)
+    monkeypatch.delenv("UVH_SCATTER")
+    try:
+        say(f"[gpu] gather scatter {name}")
+        ref.set_mesh(M)
+        c2.set_mesh(M)
+        for fp32 in (False, True):
+            a = ref.assemble(cv, F0, F1, want_tf=True, fp32=fp32)
+            b = c2.assemble(cv, F0, F1, want_tf=True, fp32=fp32)
+            for k in ("R", "Kval", "Tint", "Fext"):
+                assert np.array_equal(a[k], b[k]), (k, fp32)
+    finally:
+        ref.close()
+        c2.close()

@@ -146,7 +146,7 @@ def tile_candidates(nod: int, nip: int, fp32: bool) -> list[int]:
     """Elements per workgroup to try: the largest tile under LDS caps of 40, 64, 80 and
     156 KiB (4, 2, 2 and 1 workgroups per CU), and the sizes that fill the 256 threads of the
     row stage in whole rounds."""
-    per_el = max(21 * nod * 8, nip * 36 * (4 if fp32 else 8))
+    per_el = max((21 * nod + 1) * 8, (nip * 36 + 1) * (4 if fp32 else 8))
     tables = (3 * nip * nod + nip + 1) * 8 + (3 * nip * nod * 4 + 16 if fp32 else 0)
     cap = min(256 // nip, 256 // nod)
     out = set()

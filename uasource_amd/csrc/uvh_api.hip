@@ -24,6 +24,10 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
                        hipStream_t s);
 int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
+hipError_t launch_rows_gather(int nod, int nn, int64_t nnzN, bool fp32, const int32_t *nzrow, const int64_t *rowptr_n,
+                              const int32_t *inc_ptr, const int64_t *imap_ptr, const uint8_t *imap,
+                              const uint8_t *diag, const void *EB, const void *RE, double *R, double *Tint,
+                              double *Fext, double *Kval, int check_nan, int *status, hipStream_t s);
 size_t element_lds_bytes(int nod, int nip, int epb, bool fp32);
 hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
                         const int2 *elem_items, const int *eb, const int2 *row_items, const int *rb,
@@ -72,6 +76,10 @@ struct uvh_ctx {
     int32_t *d_inc_ptr = nullptr, *d_inc = nullptr;
     uint8_t *d_pos = nullptr, *d_diag = nullptr, *d_pos_inc = nullptr;
     int32_t *d_inc_pos = nullptr;
+    int32_t *d_nzrow = nullptr;            // gather form of the row scatter
+    int64_t *d_imap_ptr = nullptr;
+    uint8_t *d_imap = nullptr;
+    bool gather_rows = false;              // UVH_SCATTER=gather
     // fused-assembly schedule
     bool two_pass = true;   // false: fused LDS-window kernel (UVH_ASSEMBLY=fused)
     uvh::Schedule sched;
@@ -120,6 +128,8 @@ struct uvh_ctx {
         hipFree(d_diag); hipFree(d_pbp); hipFree(d_belem); hipFree(d_easlot); hipFree(d_zptr); hipFree(d_zslot);
         hipFree(d_zlen); hipFree(d_fptr); hipFree(d_fnode); hipFree(d_fslot); hipFree(d_earank); hipFree(d_bmax);
         hipFree(d_margs); hipFree(d_pos_inc); hipFree(d_inc_pos);
+        hipFree(d_nzrow); hipFree(d_imap_ptr); hipFree(d_imap);
+        d_nzrow = nullptr; d_imap_ptr = nullptr; d_imap = nullptr;
         d_margs = nullptr; d_pos_inc = nullptr; d_inc_pos = nullptr;
         last_slot = -1;
         d_diag = d_earank = d_bmax = nullptr;
@@ -209,6 +219,8 @@ int uvh_create(uvh_ctx **out, int device) {
     ctx->two_pass = !(am && std::strcmp(am, "fused") == 0);
     const char *rows = std::getenv("UVH_ROWS");
     ctx->rows_mode = !rows ? 0 : std::strcmp(rows, "full") == 0 ? 1 : std::strcmp(rows, "split") == 0 ? 2 : 0;
+    const char *sc = std::getenv("UVH_SCATTER");
+    ctx->gather_rows = sc && std::strcmp(sc, "gather") == 0;
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
     if (e == hipSuccess)
@@ -297,6 +309,11 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
         chk(dalloc(&ctx->d_RE, ne * nod * 6));
         chk(dalloc(&ctx->d_inc_pos, ne * nod));
         chk(dalloc(&ctx->d_pos_inc, ne * nod * nod));
+        if (ctx->gather_rows) {
+            chk(dalloc(&ctx->d_nzrow, m.nzrow.size()));
+            chk(dalloc(&ctx->d_imap_ptr, m.imap_ptr.size()));
+            chk(dalloc(&ctx->d_imap, m.imap.size()));
+        }
     }
     if (e != hipSuccess) {
         ctx->free_mesh();
@@ -324,6 +341,12 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     if (ctx->two_pass) {
         chk(hipMemcpy(ctx->d_inc_pos, m.inc_pos.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
         chk(hipMemcpy(ctx->d_pos_inc, m.pos_inc.data(), ne * nod * nod, hipMemcpyHostToDevice));
+        if (ctx->gather_rows) {
+            chk(hipMemcpy(ctx->d_nzrow, m.nzrow.data(), m.nzrow.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+            chk(hipMemcpy(ctx->d_imap_ptr, m.imap_ptr.data(), m.imap_ptr.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+            chk(hipMemcpy(ctx->d_imap, m.imap.data(), m.imap.size(), hipMemcpyHostToDevice));
+        }
     }
     auto up = [&](auto *dst, const auto &v) {
         if (!v.empty()) chk(hipMemcpy(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
@@ -445,6 +468,10 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
         });
     if (e == hipSuccess)
         e = ctx->timed(fp32 ? (ronly ? "rows32_r" : "rows32_rk") : (ronly ? "rows_r" : "rows_rk"), s, [&] {
+            if (ctx->gather_rows && !ronly)
+                return uvh::launch_rows_gather(m.nod, m.nn, m.nnzN(), fp32, ctx->d_nzrow, ctx->d_rowptr_n,
+                                               ctx->d_inc_ptr, ctx->d_imap_ptr, ctx->d_imap, ctx->d_diag, ctx->d_EB,
+                                               ctx->d_RE, dR, dT, dF, dK, check, ctx->d_status, s);
             return uvh::launch_rows(m.nod, m.nn, m.max_block_deg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
                                     ctx->d_diag, m.nnzN(), ctx->d_EB, ctx->d_RE, dR, dT, dF, ronly ? nullptr : dK,
                                     check, ctx->d_status, s);

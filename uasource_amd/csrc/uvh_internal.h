@@ -44,6 +44,12 @@ struct MeshHost {
     std::vector<uint8_t> diag;        // nn: index of node k in adj(k)
     std::vector<int32_t> inc_pos;     // nele*nod: position of (e, a) in the incidence order (inverse of inc)
     std::vector<uint8_t> pos_inc;     // nele*nod*nod: pos rows in the incidence order
+    // gather form of the row scatter: nzrow[x] is the node of node-graph entry x; for node n,
+    // incidence t (t0 <= t < t1) and column c < deg(n), imap[imap_ptr[n] + (t - t0)*deg + c]
+    // is the local index in element(t) of node adj(n)[c], or 0xFF if that element lacks it
+    std::vector<int32_t> nzrow;       // nnzN
+    std::vector<int64_t> imap_ptr;    // nn+1
+    std::vector<uint8_t> imap;
     int32_t maxdeg = 0;               // largest node degree
     int64_t max_block_deg = 0;        // largest sum of degrees over kRowNodesPerBlock consecutive nodes
     ShapeTables tab{};

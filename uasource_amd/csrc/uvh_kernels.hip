@@ -496,6 +496,14 @@ __device__ __forceinline__ void element_row_full(int i, int nip, const T *__rest
     TF[0] = Tx; TF[1] = Fx; TF[2] = Ty; TF[3] = Fy; TF[4] = Th; TF[5] = Fh;
 }
 
+// Per-element strides of the LDS staging (doubles) and of the coefficient records (values of
+// type T): one value of padding each, so that the records of the ~11 elements a wave reads at
+// once fall on different LDS banks (an unpadded stride of 7 x 36 values repeats every 4th
+// element on gfx950's 32 four-byte banks).
+template <int NOD>
+__host__ __device__ constexpr int nv_stride() { return NV_COUNT * NOD + 1; }
+__host__ __device__ constexpr int rec_stride(int nip) { return nip * kNC + 1; }
+
 // Shape tables in LDS for the row stage, in type T: the FP64 tables themselves for the plain
 // form, FP32 copies for the precision form.
 template <typename T>
@@ -545,7 +553,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     // stage 1: gathers, one (element, node) per thread
     if (tid < epb * NOD) {
         const int el = tid / NOD, a = tid - el * NOD;
-        if (e0 + el < nele) gather_node<NOD>(conn, xn, yn, f, bnod, p, e0 + el, a, work + el * (NV_COUNT * NOD));
+        if (e0 + el < nele) gather_node<NOD>(conn, xn, yn, f, bnod, p, e0 + el, a, work + el * nv_stride<NOD>());
     }
     __syncthreads();
 
@@ -553,12 +561,12 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     T c[kNC];
     const int pel = tid / nip, pq = tid - pel * nip;
     const bool phys = tid < epb * nip && e0 + pel < nele;
-    if (phys) ip_physics<NOD, T>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
+    if (phys) ip_physics<NOD, T>(work + pel * nv_stride<NOD>(), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
                                  sW[pq], p, c);
     __syncthreads();  // the nodal staging is dead from here; the coefficient records reuse it
     T *rec = reinterpret_cast<T *>(work);
     if (phys) {
-        T *dst = rec + (pel * nip + pq) * kNC;
+        T *dst = rec + pel * rec_stride(nip) + pq * kNC;
 #pragma unroll
         for (int k = C_COUNT; k < kNC; ++k) c[k] = T(0);
 #pragma unroll
@@ -575,7 +583,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
             if (e >= nele) continue;
             const int64_t row = inc_pos[e * NOD + i];
             T TF[6], Kf[RONLY ? 1 : 9][NOD];
-            element_row_full<NOD, T, RONLY>(i, nip, rec + el * nip * kNC, rN, rXi, rEta, rtab, TF, Kf);
+            element_row_full<NOD, T, RONLY>(i, nip, rec + el * rec_stride(nip), rN, rXi, rEta, rtab, TF, Kf);
 #pragma unroll
             for (int k = 0; k < 6; ++k) RE[row * 6 + k] = TF[k];
             if (!RONLY) {
@@ -598,7 +606,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
         if (e >= nele) continue;
         // the row is stored at its position in the node-incidence order
         const int64_t row = inc_pos[e * NOD + i];
-        const T *crr = rec + el * nip * kNC;
+        const T *crr = rec + el * rec_stride(nip);
         T Tr, Fr, Kb[3][NOD];
         if (bi == 0) element_row_b<NOD, 0, T>(i, nip, crr, rN, rXi, rEta, rtab, Tr, Fr, Kb);
         else if (bi == 1) element_row_b<NOD, 1, T>(i, nip, crr, rN, rXi, rEta, rtab, Tr, Fr, Kb);
@@ -735,6 +743,64 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
     const int k0 = xcd_block(blockIdx.x, gridDim.x) * kRowNodesPerBlock;
     rows_block<NOD, TE>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan,
                         status);
+}
+
+// Node rows, gather form: one thread per node-graph entry (node n, column c). It sums the
+// nine K values of that entry over n's incident element rows in ascending element order (the
+// inverse column map says which local node of each element is adj(n)[c]), so every sum has
+// the same fixed order as in k_rows and the results agree bit for bit; no LDS, no atomics.
+// The thread of column 0 also sums the node's residual rows.
+template <int NOD, typename TE>
+__global__ __launch_bounds__(256) void k_rows_gather(int nn, int64_t nnzN, const int32_t *__restrict__ nzrow,
+                                                     const int64_t *__restrict__ rowptr_n,
+                                                     const int32_t *__restrict__ inc_ptr,
+                                                     const int64_t *__restrict__ imap_ptr,
+                                                     const uint8_t *__restrict__ imap,
+                                                     const uint8_t *__restrict__ diag,
+                                                     const TE *__restrict__ EB, const TE *__restrict__ RE,
+                                                     double *__restrict__ R, double *__restrict__ Tint,
+                                                     double *__restrict__ Fext, double *__restrict__ Kval,
+                                                     int check_nan, int *__restrict__ status) {
+    const int64_t x = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (x >= nnzN) return;
+    const int n = nzrow[x];
+    const int64_t rp = rowptr_n[n];
+    const int deg = (int)(rowptr_n[n + 1] - rp);
+    const int c = (int)(x - rp);
+    const int t0 = inc_ptr[n], t1 = inc_ptr[n + 1];
+    const uint8_t *im = imap + imap_ptr[n] + c;
+    double acc[9];
+#pragma unroll
+    for (int b = 0; b < 9; ++b) acc[b] = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const int j = im[(int64_t)(t - t0) * deg];
+        if (j != 0xFF) {
+            const TE *row = EB + (int64_t)t * (9 * NOD) + j;
+#pragma unroll
+            for (int b = 0; b < 9; ++b) acc[b] += double(row[b * NOD]);
+        }
+    }
+    int bad = 0;
+#pragma unroll
+    for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 3; ++bj) Kval[(int64_t)bi * 3 * nnzN + 3 * rp + bj * deg + c] = acc[bi * 3 + bj];
+    if (check_nan && c == diag[n] && (isnan(acc[0]) || isnan(acc[4]) || isnan(acc[8]))) bad |= 2;
+    if (c == 0) {
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        for (int t = t0; t < t1; ++t)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] += double(RE[(int64_t)t * 6 + k]);
+#pragma unroll
+        for (int bi = 0; bi < 3; ++bi) {
+            const double r = v[2 * bi] - v[2 * bi + 1];
+            R[(int64_t)bi * nn + n] = r;
+            if (Tint) Tint[(int64_t)bi * nn + n] = v[2 * bi];
+            if (Fext) Fext[(int64_t)bi * nn + n] = v[2 * bi + 1];
+            if (check_nan && isnan(r)) bad |= 1;
+        }
+    }
+    if (bad) atomicOr(status, bad);
 }
 
 // Residual only: one thread per node sums its incident element residual rows (contiguous).
@@ -1090,7 +1156,8 @@ size_t element_table_bytes(int nod, int nip, bool fp32) {
 }
 
 size_t element_per_el_bytes(int nod, int nip, bool fp32) {
-    return std::max(size_t(NV_COUNT) * nod * sizeof(double), size_t(nip) * kNC * (fp32 ? sizeof(float) : sizeof(double)));
+    return std::max(size_t(NV_COUNT * nod + 1) * sizeof(double),
+                    size_t(rec_stride(nip)) * (fp32 ? sizeof(float) : sizeof(double)));
 }
 
 template <int NOD, typename T>
@@ -1213,6 +1280,26 @@ int persistent_grid(const void *kernel, size_t lds) {
     const int grid = std::max(1, cus) * per_cu;
     cache[{dev, kernel, lds}] = grid;
     return grid;
+}
+
+hipError_t launch_rows_gather(int nod, int nn, int64_t nnzN, bool fp32, const int32_t *nzrow, const int64_t *rowptr_n,
+                              const int32_t *inc_ptr, const int64_t *imap_ptr, const uint8_t *imap,
+                              const uint8_t *diag, const void *EB, const void *RE, double *R, double *Tint,
+                              double *Fext, double *Kval, int check_nan, int *status, hipStream_t s) {
+    const int64_t nblocks = (nnzN + 255) / 256;
+    const dim3 grid(unsigned(((nblocks + kXcds - 1) / kXcds) * kXcds)), block(256);
+#define UVH_RG(NODV, TE)                                                                                   \
+    hipLaunchKernelGGL((k_rows_gather<NODV, TE>), grid, block, 0, s, nn, nnzN, nzrow, rowptr_n, inc_ptr, imap_ptr, \
+                       imap, diag, static_cast<const TE *>(EB), static_cast<const TE *>(RE), R, Tint, Fext, Kval,  \
+                       check_nan, status)
+    switch (nod) {
+        case 3: if (fp32) UVH_RG(3, float); else UVH_RG(3, double); break;
+        case 6: if (fp32) UVH_RG(6, float); else UVH_RG(6, double); break;
+        case 10: if (fp32) UVH_RG(10, float); else UVH_RG(10, double); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef UVH_RG
+    return hipGetLastError();
 }
 
 // Batched launches, one persistent element launch and one persistent row launch per element

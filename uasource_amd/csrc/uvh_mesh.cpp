@@ -164,6 +164,32 @@ std::string build_mesh_host(MeshHost &m) {
         std::copy(m.pos.begin() + size_t(ea) * nod, m.pos.begin() + size_t(ea + 1) * nod,
                   m.pos_inc.begin() + size_t(t) * nod);
     }
+    // gather tables of the row scatter
+    m.nzrow.assign(size_t(m.nnzN()), 0);
+    m.imap_ptr.assign(size_t(nn) + 1, 0);
+    for (int32_t k = 0; k < nn; ++k) {
+        for (int64_t x = m.rowptr_n[k]; x < m.rowptr_n[size_t(k) + 1]; ++x) m.nzrow[x] = k;
+        m.imap_ptr[size_t(k) + 1] = m.imap_ptr[k] + int64_t(m.inc_ptr[k + 1] - m.inc_ptr[k]) *
+                                                        (m.rowptr_n[size_t(k) + 1] - m.rowptr_n[k]);
+    }
+    m.imap.assign(size_t(m.imap_ptr[nn]), 0xFF);
+    {
+        std::vector<std::thread> th;
+        for (unsigned w = 0; w < nthreads; ++w)
+            th.emplace_back([&, w] {
+                for (int64_t k = w; k < nn; k += nthreads) {
+                    const int64_t deg = m.rowptr_n[k + 1] - m.rowptr_n[k];
+                    for (int32_t t = m.inc_ptr[k]; t < m.inc_ptr[k + 1]; ++t) {
+                        const int32_t ea = m.inc[t];
+                        const int32_t e = ea / nod;
+                        uint8_t *row = m.imap.data() + m.imap_ptr[k] + int64_t(t - m.inc_ptr[k]) * deg;
+                        // pos[(e*nod + a)*nod + b] is the column of node conn[e][b] in adj(k)
+                        for (int b = 0; b < nod; ++b) row[m.pos[size_t(ea) * nod + b]] = uint8_t(b);
+                    }
+                }
+            });
+        for (auto &x : th) x.join();
+    }
     return "";
 }
 
