@@ -259,3 +259,29 @@ def test_gather_scatter_matches_lds_scatter(monkeypatch, name):
     finally:
         ref.close()
         c2.close()
+
+
+@pytest.mark.parametrize("name,chunks", [("medium_p2", 5), ("medium_p3", 3), ("plain_p2", 8)])
+def test_pipelined_assembly_matches_single_pass(monkeypatch, name, chunks):
+    """The pipelined assembly (UVH_CHUNKS: element tiles in chunks on the caller's stream, the
+    node rows of each finished chunk on a second stream) runs the same kernels on the same
+    data: bit-identical R and K, FP64 and FP32."""
+    M, F0, F1, BCs = mesh_fields(name)
+    cv = default_ctrlvar()
+    monkeypatch.setenv("UVH_CHUNKS", "1")
+    ref = uvh.UvhContext(0)
+    monkeypatch.setenv("UVH_CHUNKS", str(chunks))
+    c2 = uvh.UvhContext(0)
+    monkeypatch.delenv("UVH_CHUNKS")
+    try:
+        say(f"[gpu] pipelined {name} chunks={chunks}")
+        ref.set_mesh(M)
+        c2.set_mesh(M)
+        for fp32 in (False, True):
+            a = ref.assemble(cv, F0, F1, want_tf=True, fp32=fp32)
+            b = c2.assemble(cv, F0, F1, want_tf=True, fp32=fp32)
+            for k in ("R", "Kval", "Tint", "Fext"):
+                assert np.array_equal(a[k], b[k]), (k, fp32)
+    finally:
+        ref.close()
+        c2.close()

@@ -18,11 +18,11 @@ hipError_t launch_clear_status(int *status, hipStream_t s);
 hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
-                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s);
+                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, hipStream_t s);
 hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
-                       hipStream_t s);
+                       const int32_t *blocks, int nblk, hipStream_t s);
 int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
 hipError_t launch_rows_gather(int nod, int nn, int64_t nnzN, bool fp32, const int32_t *nzrow, const int64_t *rowptr_n,
                               const int32_t *inc_ptr, const int64_t *imap_ptr, const uint8_t *imap,
@@ -40,6 +40,9 @@ hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, cons
 }  // namespace uvh
 
 namespace {
+
+// Chunks of the pipelined R+K assembly (UVH_CHUNKS overrides; 1 = element kernel, then rows).
+constexpr int kDefaultChunks = 1;
 
 struct KernelTimer {
     const char *name;
@@ -80,6 +83,17 @@ struct uvh_ctx {
     int64_t *d_imap_ptr = nullptr;
     uint8_t *d_imap = nullptr;
     bool gather_rows = false;              // UVH_SCATTER=gather
+    // pipelined R+K assembly: element tiles in `nchunks` chunks on the caller's stream, the node
+    // rows of each chunk (the node blocks whose incident elements all lie in chunks <= c) on
+    // `aux_stream` as soon as their chunk is done; [FP64, FP32] plans (the tiles differ)
+    int nchunks = 1;                       // UVH_CHUNKS
+    hipStream_t aux_stream = nullptr;
+    std::vector<hipEvent_t> chunk_ev;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool plan_ok[2] = {false, false};
+    std::vector<int> plan_tiles[2];        // nchunks+1 tile boundaries
+    std::vector<int> plan_boff[2];         // nchunks+1 offsets into the block list
+    int32_t *d_plan_blocks[2] = {nullptr, nullptr};
     // fused-assembly schedule
     bool two_pass = true;   // false: fused LDS-window kernel (UVH_ASSEMBLY=fused)
     uvh::Schedule sched;
@@ -130,6 +144,11 @@ struct uvh_ctx {
         hipFree(d_margs); hipFree(d_pos_inc); hipFree(d_inc_pos);
         hipFree(d_nzrow); hipFree(d_imap_ptr); hipFree(d_imap);
         d_nzrow = nullptr; d_imap_ptr = nullptr; d_imap = nullptr;
+        for (int t = 0; t < 2; ++t) {
+            hipFree(d_plan_blocks[t]);
+            d_plan_blocks[t] = nullptr;
+            plan_ok[t] = false;
+        }
         d_margs = nullptr; d_pos_inc = nullptr; d_inc_pos = nullptr;
         last_slot = -1;
         d_diag = d_earank = d_bmax = nullptr;
@@ -201,6 +220,46 @@ uvh::FieldPtrs to_ptrs(const uvh_fields &f) {
                           f.h0, f.ub0, f.vb0, f.as0, f.ab0, f.dadh};
 }
 
+// Chunk plan of the pipelined assembly for element tiles of `epb` elements: tile boundaries
+// of nchunks equal chunks, and for each chunk the node blocks (kRowNodesPerBlock consecutive
+// nodes) whose last incident element lies in it. The incidence lists are in element order, so
+// a node's last incident element is the last entry of its list.
+int build_plan(uvh_ctx *ctx, int t) {
+    const uvh::MeshHost &m = ctx->mesh;
+    const int epb = ctx->epb[t];
+    const int ntiles = (m.nele + epb - 1) / epb;
+    const int C = std::max(1, std::min(ctx->nchunks, ntiles));
+    std::vector<int> tiles(size_t(C) + 1);
+    for (int c = 0; c <= C; ++c) tiles[c] = int((int64_t(ntiles) * c) / C);
+    const int nblk = (m.nn + uvh::kRowNodesPerBlock - 1) / uvh::kRowNodesPerBlock;
+    std::vector<std::vector<int32_t>> per(static_cast<size_t>(C));
+    for (int b = 0; b < nblk; ++b) {
+        int last = 0;
+        for (int k = b * uvh::kRowNodesPerBlock; k < std::min(m.nn, (b + 1) * uvh::kRowNodesPerBlock); ++k)
+            if (m.inc_ptr[k + 1] > m.inc_ptr[k]) last = std::max(last, m.inc[m.inc_ptr[k + 1] - 1] / m.nod);
+        const int tile = last / epb;
+        const int c = int(std::upper_bound(tiles.begin(), tiles.end(), tile) - tiles.begin()) - 1;
+        per[size_t(std::max(0, std::min(C - 1, c)))].push_back(b);
+    }
+    std::vector<int32_t> flat;
+    std::vector<int> off(size_t(C) + 1, 0);
+    for (int c = 0; c < C; ++c) {
+        off[c] = int(flat.size());
+        flat.insert(flat.end(), per[c].begin(), per[c].end());
+    }
+    off[C] = int(flat.size());
+    hipFree(ctx->d_plan_blocks[t]);
+    ctx->d_plan_blocks[t] = nullptr;
+    if (dalloc(&ctx->d_plan_blocks[t], flat.size()) != hipSuccess ||
+        hipMemcpy(ctx->d_plan_blocks[t], flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+            hipSuccess)
+        return ctx->fail(UVH_E_NOMEM, "uvh_assemble: cannot upload the chunk plan");
+    ctx->plan_tiles[t] = tiles;
+    ctx->plan_boff[t] = off;
+    ctx->plan_ok[t] = true;
+    return UVH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -221,6 +280,8 @@ int uvh_create(uvh_ctx **out, int device) {
     ctx->rows_mode = !rows ? 0 : std::strcmp(rows, "full") == 0 ? 1 : std::strcmp(rows, "split") == 0 ? 2 : 0;
     const char *sc = std::getenv("UVH_SCATTER");
     ctx->gather_rows = sc && std::strcmp(sc, "gather") == 0;
+    const char *ch = std::getenv("UVH_CHUNKS");
+    ctx->nchunks = ch ? std::max(1, std::min(64, std::atoi(ch))) : kDefaultChunks;
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
     if (e == hipSuccess)
@@ -228,6 +289,14 @@ int uvh_create(uvh_ctx **out, int device) {
     if (e == hipSuccess)
         e = hipMalloc(reinterpret_cast<void **>(&ctx->d_cargs), sizeof(uvh::CallArgs) * uvh_ctx::kRing);
     for (int k = 0; e == hipSuccess && k < uvh_ctx::kRing; ++k) e = hipEventCreateWithFlags(&ctx->ring_ev[k], hipEventDisableTiming);
+    if (e == hipSuccess && ctx->nchunks > 1) {
+        e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);  // default priority
+        ctx->chunk_ev.assign(size_t(ctx->nchunks), nullptr);
+        for (int k = 0; e == hipSuccess && k < ctx->nchunks; ++k)
+            e = hipEventCreateWithFlags(&ctx->chunk_ev[k], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         delete ctx;
         return UVH_E_HIP;
@@ -247,6 +316,11 @@ void uvh_destroy(uvh_ctx *ctx) {
         if (ctx->ring_ev[k]) hipEventDestroy(ctx->ring_ev[k]);
     hipHostFree(ctx->h_cargs);
     hipFree(ctx->d_cargs);
+    for (auto ev : ctx->chunk_ev)
+        if (ev) hipEventDestroy(ev);
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) hipEventDestroy(ctx->join_ev);
+    if (ctx->aux_stream) hipStreamDestroy(ctx->aux_stream);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -457,25 +531,50 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
         if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: kernel launch");
         return UVH_OK;
     }
-    if (e == hipSuccess)
-        e = ctx->timed(fp32 ? (ronly ? "element32_r" : "element32_rk") : (ronly ? "element_r" : "element_rk"), s, [&] {
-            const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
-            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[fp32 ? 1 : 0], fp32, full, ctx->d_conn,
-                                       ctx->d_x,
-                                       ctx->d_y, f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly,
-                                       ctx->d_EB,
-                                       ctx->d_RE, s);
+    const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
+    const int t = fp32 ? 1 : 0;
+    const char *el_name = fp32 ? (ronly ? "element32_r" : "element32_rk") : (ronly ? "element_r" : "element_rk");
+    const char *row_name = fp32 ? (ronly ? "rows32_r" : "rows32_rk") : (ronly ? "rows_r" : "rows_rk");
+    auto elements = [&](int lb0, int lb1, hipStream_t st) {
+        return ctx->timed(el_name, st, [&] {
+            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], fp32, full, ctx->d_conn, ctx->d_x, ctx->d_y,
+                                       f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, ctx->d_EB,
+                                       ctx->d_RE, lb0, lb1, st);
         });
-    if (e == hipSuccess)
-        e = ctx->timed(fp32 ? (ronly ? "rows32_r" : "rows32_rk") : (ronly ? "rows_r" : "rows_rk"), s, [&] {
+    };
+    auto rows = [&](const int32_t *blocks, int nblk, hipStream_t st) {
+        return ctx->timed(row_name, st, [&] {
             if (ctx->gather_rows && !ronly)
                 return uvh::launch_rows_gather(m.nod, m.nn, m.nnzN(), fp32, ctx->d_nzrow, ctx->d_rowptr_n,
                                                ctx->d_inc_ptr, ctx->d_imap_ptr, ctx->d_imap, ctx->d_diag, ctx->d_EB,
-                                               ctx->d_RE, dR, dT, dF, dK, check, ctx->d_status, s);
+                                               ctx->d_RE, dR, dT, dF, dK, check, ctx->d_status, st);
             return uvh::launch_rows(m.nod, m.nn, m.max_block_deg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
                                     ctx->d_diag, m.nnzN(), ctx->d_EB, ctx->d_RE, dR, dT, dF, ronly ? nullptr : dK,
-                                    check, ctx->d_status, s);
+                                    check, ctx->d_status, blocks, nblk, st);
         });
+    };
+    if (ronly || ctx->nchunks <= 1 || ctx->gather_rows) {
+        if (e == hipSuccess) e = elements(0, -1, s);
+        if (e == hipSuccess) e = rows(nullptr, 0, s);
+    } else {
+        if (!ctx->plan_ok[t]) {
+            const int st = build_plan(ctx, t);
+            if (st != UVH_OK) return st;
+        }
+        const std::vector<int> &tiles = ctx->plan_tiles[t], &off = ctx->plan_boff[t];
+        const int C = int(tiles.size()) - 1;
+        hipStream_t a = ctx->aux_stream;
+        if (e == hipSuccess) e = hipEventRecord(ctx->fork_ev, s);   // status cleared and b computed
+        if (e == hipSuccess) e = hipStreamWaitEvent(a, ctx->fork_ev, 0);
+        for (int c = 0; c < C && e == hipSuccess; ++c) {
+            e = elements(tiles[c], tiles[c + 1], s);
+            if (e == hipSuccess) e = hipEventRecord(ctx->chunk_ev[c], s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(a, ctx->chunk_ev[c], 0);
+            if (e == hipSuccess && off[c + 1] > off[c]) e = rows(ctx->d_plan_blocks[t] + off[c], off[c + 1] - off[c], a);
+        }
+        if (e == hipSuccess) e = hipEventRecord(ctx->join_ev, a);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join_ev, 0);   // the caller's stream sees the end
+    }
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: kernel launch");
     return UVH_OK;
 }

@@ -630,11 +630,10 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
                                                  const ShapeTables *__restrict__ tab,
                                                  const typename RowTab<T>::type *__restrict__ rtab, Params p,
                                                  const int32_t *__restrict__ inc_pos,
-                                                 T *__restrict__ EB, T *__restrict__ RE) {
+                                                 T *__restrict__ EB, T *__restrict__ RE, int lb0, int lb1) {
     extern __shared__ double lds[];
-    const int nblocks = (nele + epb - 1) / epb;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    if (lb >= nblocks) return;  // whole block
+    const int lb = lb0 + xcd_block(blockIdx.x, gridDim.x);   // tiles lb0 .. lb1-1 of this launch
+    if (lb >= lb1) return;  // whole block
     element_block<NOD, T, RONLY, FULL>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB, RE);
 }
 
@@ -738,9 +737,13 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
                                               int64_t nnzN, const TE *__restrict__ EB,
                                               const TE *__restrict__ RE, double *__restrict__ R,
                                               double *__restrict__ Tint, double *__restrict__ Fext,
-                                              double *__restrict__ Kval, int check_nan, int *__restrict__ status) {
+                                              double *__restrict__ Kval, int check_nan, int *__restrict__ status,
+                                              const int32_t *__restrict__ blocks, int nblk) {
     extern __shared__ double lds[];
-    const int k0 = xcd_block(blockIdx.x, gridDim.x) * kRowNodesPerBlock;
+    // node blocks of this launch: blocks[0 .. nblk-1], or 0 .. nblk-1 when blocks is null
+    const int i = xcd_block(blockIdx.x, gridDim.x);
+    if (i >= nblk) return;
+    const int k0 = (blocks ? blocks[i] : i) * kRowNodesPerBlock;
     rows_block<NOD, TE>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan,
                         status);
 }
@@ -1164,11 +1167,12 @@ template <int NOD, typename T>
 hipError_t launch_element_t(int nele, int nip, int epb, bool full, const int32_t *conn, const double *x,
                             const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
                             const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly,
-                            void *EB, void *RE, hipStream_t s) {
+                            void *EB, void *RE, int lb0, int lb1, hipStream_t s) {
     const typename RowTab<T>::type *rtab;
     if constexpr (sizeof(T) == sizeof(double)) rtab = tab;
     else rtab = tab32;
-    const int nblocks = (nele + epb - 1) / epb;
+    const int nblocks = lb1 - lb0;
+    if (nblocks <= 0) return hipSuccess;
     const int grid = ((nblocks + kXcds - 1) / kXcds) * kXcds;  // whole rounds over the XCDs
     const bool fp32 = sizeof(T) == sizeof(float);
     const size_t lds = element_table_bytes(NOD, nip, fp32) + element_per_el_bytes(NOD, nip, fp32) * epb;
@@ -1177,7 +1181,7 @@ hipError_t launch_element_t(int nele, int nip, int epb, bool full, const int32_t
                       : (full && can_full ? k_element<NOD, T, false, can_full> : k_element<NOD, T, false, false>);
     lds_attr(reinterpret_cast<const void *>(kern), lds);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
-                       static_cast<T *>(EB), static_cast<T *>(RE));
+                       static_cast<T *>(EB), static_cast<T *>(RE), lb0, lb1);
     return hipGetLastError();
 }
 }  // namespace
@@ -1205,14 +1209,18 @@ hipError_t launch_clear_status(int *status, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Element tiles lb0 .. lb1-1 (tiles of epb elements); lb1 < 0 means all tiles.
 hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
-                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, hipStream_t s) {
+                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, hipStream_t s) {
     if (epb < 1 || epb * nip > 256 || epb * nod > 256) return hipErrorInvalidConfiguration;
+    const int ntiles = (nele + epb - 1) / epb;
+    if (lb1 < 0 || lb1 > ntiles) lb1 = ntiles;
+    if (lb0 < 0) lb0 = 0;
 #define UVH_EL(NODV)                                                                                           \
-    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s) \
-                : launch_element_t<NODV, double>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, s)
+    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s) \
+                : launch_element_t<NODV, double>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s)
     switch (nod) {
         case 3: UVH_EL(3);
         case 6: UVH_EL(6);
@@ -1224,10 +1232,11 @@ hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool f
 
 size_t element_row_stride(int nod, bool) { return size_t(9 * nod); }
 
+// Node rows of the node blocks `blocks[0 .. nblk-1]` (all blocks when blocks is null).
 hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
-                       hipStream_t s) {
+                       const int32_t *blocks, int nblk, hipStream_t s) {
     if (!Kval) {
         if (fp32)
             hipLaunchKernelGGL(k_rows_r<float>, dim3((nn + 255) / 256), dim3(256), 0, s, nn, inc_ptr,
@@ -1237,7 +1246,8 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
                                static_cast<const double *>(RE), R, Tint, Fext, check_nan, status);
         return hipGetLastError();
     }
-    const int nblocks = (nn + kRowNodesPerBlock - 1) / kRowNodesPerBlock;
+    const int nblocks = blocks ? nblk : (nn + kRowNodesPerBlock - 1) / kRowNodesPerBlock;
+    if (nblocks <= 0) return hipSuccess;
     const dim3 grid(((nblocks + kXcds - 1) / kXcds) * kXcds), block(256);
     const size_t lds = size_t(9) * size_t(block_vals) * sizeof(double);
     if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
@@ -1247,7 +1257,7 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
         lds_attr(reinterpret_cast<const void *>(kern), lds);                                                   \
         hipLaunchKernelGGL(kern, grid, block, lds, s, nn, inc_ptr, pos, rowptr_n, diag, nnzN,                   \
                            static_cast<const TE *>(EB), static_cast<const TE *>(RE), R, Tint, Fext, Kval,      \
-                           check_nan, status);                                                                 \
+                           check_nan, status, blocks, nblocks);                                                \
     } while (0)
     switch (nod) {
         case 3: if (fp32) UVH_ROWS(3, float); else UVH_ROWS(3, double); break;
