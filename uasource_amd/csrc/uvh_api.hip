@@ -13,8 +13,7 @@
 #include "uvh_internal.h"
 
 namespace uvh {
-hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s);
-hipError_t launch_clear_status(int *status, hipStream_t s);
+hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, int *status, hipStream_t s);
 hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
@@ -505,9 +504,7 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     const uvh::Params p = make_params(*ctrl, flags);
     const uvh::FieldPtrs f = to_ptrs(*df);
     const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
-    hipError_t e = ctx->timed("clear_status", s, [&] { return uvh::launch_clear_status(ctx->d_status, s); });
-    if (e == hipSuccess)
-        e = ctx->timed("nodal_b", s, [&] { return uvh::launch_nodal_b(m.nn, f, p, ctx->d_b, s); });
+    hipError_t e = ctx->timed("nodal_b", s, [&] { return uvh::launch_nodal_b(m.nn, f, p, ctx->d_b, ctx->d_status, s); });
     if (e == hipSuccess && !ctx->two_pass) {
         uvh::CallArgs ca{};
         ca.f = f; ca.p = p; ca.R = dR; ca.Tint = dT; ca.Fext = dF; ca.Kval = ronly ? nullptr : dK;

@@ -1,18 +1,24 @@
 // HIP kernels of the uvh assembly for gfx950 (CDNA4, wave64).
 //
 // Three kernels per assembly:
-//   k_nodal_b    per node: lower surface b from floatation (UaMain/Calc_bs_From_hBS.m:71-79,109)
-//   k_element    per element: gather of nodal fields through the connectivity into LDS,
+//   k_nodal_b    per node: lower surface b from floatation (UaMain/Calc_bs_From_hBS.m:71-79,109);
+//                also clears the assembly's NaN flags
+//   k_element    per tile of elements: gather of nodal fields through the connectivity into LDS,
 //                Gauss-point physics (UaMain/uvhAssemblyIntPointImplicitSUPG.m:62-395) and the
-//                element residual rows and Jacobian rows (UaMain/uvhNodalLoopSSTREAM.m:39-228)
+//                element residual rows and Jacobian rows (UaMain/uvhNodalLoopSSTREAM.m:39-228),
+//                stored in the node-incidence order
 //   k_rows       per node: deterministic sum of the element rows into R (Tint - Fext) and into
 //                the CSR rows of K (UaMain/uvhMatrixAssembly.m:416-509); incident elements are
 //                summed in ascending element order, so results are bitwise reproducible.
+// Alternatives of the same contract: k_rows_gather (one thread per K entry), the batched
+// persistent kernels (many meshes per launch) and k_fused (owner-computes patches with an LDS
+// window of node rows, no element rows in HBM).
 //
 // Thread mapping of k_element: a 256-thread block owns `epb` elements. Threads first gather
 // (element, node) pairs into LDS, then compute the physics of (element, integration point)
-// pairs and publish a record of per-point coefficients in LDS, then accumulate (element, row)
-// pairs: 6 residual terms and 9 x nod Jacobian terms per row, over all points, in registers.
+// pairs and publish a record of per-point coefficients in LDS, then accumulate element rows
+// ((block row, element, row) items, or (element, row) items for all nine blocks at once)
+// over all points in registers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,8 +58,9 @@ constexpr int kNC = C_COUNT + (C_COUNT & 1);  // even, for 16-byte LDS reads
 template <int NOD, typename T>
 constexpr int eb_stride() { return 9 * NOD; }
 
-__global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b) {
+__global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b, int *__restrict__ status) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0 && status) *status = 0;   // the assembly's NaN flags start clear
     if (k >= nn) return;
     double h = p.zero_fields ? f.h0[k] : f.h[k];
     if (p.reset_thickness && h < p.thick_min) h = p.thick_min;
@@ -64,10 +71,6 @@ __global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b)
     double bb = GF * B + (1.0 - GF) * bfloat;
     if (p.enforce_b_above && bb < B) bb = B;
     b[k] = bb;
-}
-
-__global__ void k_clear_status(int *status) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *status = 0;
 }
 
 template <int NOD>
@@ -1199,13 +1202,8 @@ int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap) {
 
 int element_epb_default(int nod, int nip, bool fp32) { return element_epb_max(nod, nip, fp32, kElementLds); }
 
-hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, hipStream_t s) {
-    hipLaunchKernelGGL(k_nodal_b, dim3((nn + 255) / 256), dim3(256), 0, s, nn, f, p, b);
-    return hipGetLastError();
-}
-
-hipError_t launch_clear_status(int *status, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, s, status);
+hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, int *status, hipStream_t s) {
+    hipLaunchKernelGGL(k_nodal_b, dim3((nn + 255) / 256), dim3(256), 0, s, nn, f, p, b, status);
     return hipGetLastError();
 }
 

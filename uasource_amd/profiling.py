@@ -34,7 +34,7 @@ PROFILES = os.path.join(REPO, "profiles")
 # kernel base name -> the name bench.py's per-kernel HIP-event timers use (without the _rk/_r suffix)
 MODE_KERNELS = {"k_element": "element", "k_element_f32": "element", "k_rows": "rows", "k_rows_r": "rows",
                 "k_fused": "fused"}
-PLAIN_KERNELS = {"k_nodal_b": "nodal_b", "k_clear_status": "clear_status"}
+PLAIN_KERNELS = {"k_nodal_b": "nodal_b"}
 COPY_BYTES = 1 << 30
 
 
