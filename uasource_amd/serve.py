@@ -7,7 +7,8 @@ every registered mesh resident in HBM (connectivity, CSR pattern, element-row bu
 fields that do not change between iterations), so a request uploads only its state.
 
 Scheduling: a request is admitted when
-  * a slot (a HIP stream) is free,
+  * a slot (a HIP stream with output buffers of the largest mesh; nothing is allocated while
+    serving) is free,
   * its outputs (R, and K for a Newton step) fit in what is left of the HBM budget for
     in-flight outputs, and
   * no other request on the same mesh is in flight (they share the mesh's work buffers);
@@ -82,6 +83,12 @@ class UvhServer:
         self.dev = torch.device(f"cuda:{device}")
         self.meshes = [_Mesh(torch, self.dev, CtrlVar, *m, device) for m in meshes]
         self.streams = [torch.cuda.Stream(self.dev) for _ in range(slots)]   # default priority
+        # each slot owns output buffers of the largest registered mesh, so that serving
+        # allocates nothing; a request's outputs are views of its slot's buffers
+        n3 = max(3 * m.nn for m in self.meshes)
+        nnz = max(m.ctx.nnz for m in self.meshes)
+        self.slot_R = [torch.empty(n3, dtype=torch.float64, device=self.dev) for _ in range(slots)]
+        self.slot_K = [torch.empty(nnz, dtype=torch.float64, device=self.dev) for _ in range(slots)]
         self.budget = hbm_budget
         self.flags = uvh.UVH_FP32 if fp32 else 0
         self.max_bypass = max_bypass
@@ -102,8 +109,8 @@ class UvhServer:
                     m.f1[name].copy_(torch.from_numpy(val), non_blocking=False)
                 else:   # a pinned host tensor: asynchronous on the request's stream
                     m.f1[name].copy_(val, non_blocking=True)
-            R = torch.empty(3 * m.nn, dtype=torch.float64, device=self.dev)
-            K = torch.empty(m.ctx.nnz, dtype=torch.float64, device=self.dev) if req.newton else None
+            R = self.slot_R[slot][:3 * m.nn]
+            K = self.slot_K[slot][:m.ctx.nnz] if req.newton else None
             flags = self.flags | (0 if req.newton else uvh.UVH_RONLY) | uvh.UVH_NO_NAN_CHECK
             m.ctx.assemble_device(m.ctrl, m.fields, flags, R, K, None, None, stream=s.cuda_stream)
             ev = torch.cuda.Event()
