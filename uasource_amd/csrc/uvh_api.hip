@@ -220,7 +220,7 @@ uvh::FieldPtrs to_ptrs(const uvh_fields &f) {
 }
 
 // Chunk plan of the pipelined assembly for element tiles of `epb` elements: tile boundaries
-// of nchunks equal chunks, and for each chunk the node blocks (kRowNodesPerBlock consecutive
+// of nchunks equal chunks, and for each chunk the node blocks (row_nodes_per_block consecutive
 // nodes) whose last incident element lies in it. The incidence lists are in element order, so
 // a node's last incident element is the last entry of its list.
 int build_plan(uvh_ctx *ctx, int t) {
@@ -230,11 +230,12 @@ int build_plan(uvh_ctx *ctx, int t) {
     const int C = std::max(1, std::min(ctx->nchunks, ntiles));
     std::vector<int> tiles(size_t(C) + 1);
     for (int c = 0; c <= C; ++c) tiles[c] = int((int64_t(ntiles) * c) / C);
-    const int nblk = (m.nn + uvh::kRowNodesPerBlock - 1) / uvh::kRowNodesPerBlock;
+    const int nbn = uvh::row_nodes_per_block(m.nod);
+    const int nblk = (m.nn + nbn - 1) / nbn;
     std::vector<std::vector<int32_t>> per(static_cast<size_t>(C));
     for (int b = 0; b < nblk; ++b) {
         int last = 0;
-        for (int k = b * uvh::kRowNodesPerBlock; k < std::min(m.nn, (b + 1) * uvh::kRowNodesPerBlock); ++k)
+        for (int k = b * nbn; k < std::min(m.nn, (b + 1) * nbn); ++k)
             if (m.inc_ptr[k + 1] > m.inc_ptr[k]) last = std::max(last, m.inc[m.inc_ptr[k + 1] - 1] / m.nod);
         const int tile = last / epb;
         const int c = int(std::upper_bound(tiles.begin(), tiles.end(), tile) - tiles.begin()) - 1;
@@ -705,7 +706,7 @@ int batch_build_lists(uvh_batch *b, int t) {
             if (m.nod != nods[ty]) continue;
             const int epb = c->epb[t];
             for (int k = 0; k < (m.nele + epb - 1) / epb; ++k) items[1].push_back(make_int2(mi, k));
-            for (int k = 0; k < (m.nn + uvh::kRowNodesPerBlock - 1) / uvh::kRowNodesPerBlock; ++k)
+            for (int k = 0; k < (m.nn + uvh::row_nodes_per_block(m.nod) - 1) / uvh::row_nodes_per_block(m.nod); ++k)
                 items[2].push_back(make_int2(mi, k));
             b->elem_lds[t][ty] = std::max(b->elem_lds[t][ty], uvh::element_lds_bytes(m.nod, m.nip, epb, fp32));
             b->row_lds[t][ty] = std::max(b->row_lds[t][ty], size_t(9) * size_t(m.max_block_deg) * sizeof(double));

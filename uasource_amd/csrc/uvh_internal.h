@@ -9,8 +9,10 @@ namespace uvh {
 
 constexpr int kMaxNod = 10;
 constexpr int kMaxNip = 32;
-// Nodes per workgroup of the row-scatter kernel (consecutive node numbers).
-constexpr int kRowNodesPerBlock = 16;
+// Row scatter: lanes per node (16 for 3- and 6-node elements, 32 for 10-node elements, whose
+// element rows are 90 values long) and nodes per 256-thread workgroup (consecutive numbers).
+constexpr int row_lanes(int nod) { return nod >= 10 ? 32 : 16; }
+constexpr int row_nodes_per_block(int nod) { return 4 * (64 / row_lanes(nod)); }
 
 // Shape-function values and local derivatives at the integration points, plus the
 // weights, for one element type and quadrature rule. Filled on the host from the
@@ -51,7 +53,7 @@ struct MeshHost {
     std::vector<int64_t> imap_ptr;    // nn+1
     std::vector<uint8_t> imap;
     int32_t maxdeg = 0;               // largest node degree
-    int64_t max_block_deg = 0;        // largest sum of degrees over kRowNodesPerBlock consecutive nodes
+    int64_t max_block_deg = 0;        // largest sum of degrees over row_nodes_per_block(nod) consecutive nodes
     ShapeTables tab{};
     int64_t nnzN() const { return rowptr_n.empty() ? 0 : rowptr_n.back(); }
 };

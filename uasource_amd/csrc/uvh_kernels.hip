@@ -640,16 +640,13 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     element_block<NOD, T, RONLY, FULL>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB, RE);
 }
 
-// Node rows: kRowLanes lanes per node, 64 / kRowLanes nodes per wave. The incident element
+// Node rows: row_lanes(NOD) lanes per node, 64 / row_lanes nodes per wave. The incident element
 // rows (stored contiguously per node, in incidence order, by k_element) are added, in
 // ascending element order, into an LDS buffer that holds the node's three CSR rows
 // (9*deg values); within one element row no two values share a column, and the adds of one
 // lane group are issued in program order, so every sum has a fixed order and the result is
 // bitwise reproducible. The buffer is then written to K, contiguous along the rows. Element
 // rows of type TE (float in the precision form) are accumulated in FP64.
-constexpr int kRowLanes = 16;
-constexpr int kRowNodesPerWave = 64 / kRowLanes;
-static_assert(kRowNodesPerBlock == 4 * kRowNodesPerWave, "row kernel: 4 waves of lane groups per block");
 
 template <int NOD, typename TE>
 __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const int32_t *__restrict__ inc_ptr,
@@ -659,6 +656,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                                            double *__restrict__ R, double *__restrict__ Tint,
                                            double *__restrict__ Fext, double *__restrict__ Kval, int check_nan,
                                            int *__restrict__ status) {
+    constexpr int kRowLanes = row_lanes(NOD), kRowNodesPerWave = 64 / kRowLanes;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / kRowLanes, gl = lane - sub * kRowLanes;
     const int slot = wave * kRowNodesPerWave + sub;
@@ -746,7 +744,7 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
     // node blocks of this launch: blocks[0 .. nblk-1], or 0 .. nblk-1 when blocks is null
     const int i = xcd_block(blockIdx.x, gridDim.x);
     if (i >= nblk) return;
-    const int k0 = (blocks ? blocks[i] : i) * kRowNodesPerBlock;
+    const int k0 = (blocks ? blocks[i] : i) * row_nodes_per_block(NOD);
     rows_block<NOD, TE>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan,
                         status);
 }
@@ -942,7 +940,7 @@ __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__rest
     }
 }
 
-// items: (member, node block of kRowNodesPerBlock nodes), all members with NOD nodes;
+// items: (member, node block of row_nodes_per_block(NOD) nodes), all members with NOD nodes;
 // status has one word per member
 template <int NOD, typename TE>
 __global__ __launch_bounds__(256) void k_rows_batch(const BatchMember *__restrict__ mem,
@@ -956,7 +954,7 @@ __global__ __launch_bounds__(256) void k_rows_batch(const BatchMember *__restric
         if (it >= nitems) break;
         const int2 w = items[it];
         const UVH_G BatchMemberG *M = member(mem, w.x);
-        rows_block<NOD, TE>(lds, M->nn, w.y * kRowNodesPerBlock, (const int32_t *)M->inc_ptr,
+        rows_block<NOD, TE>(lds, M->nn, w.y * row_nodes_per_block(NOD), (const int32_t *)M->inc_ptr,
                             (const uint8_t *)M->pos_inc, (const int64_t *)M->rowptr_n, (const uint8_t *)M->diag,
                             M->nnzN, (const TE *)M->EB, (const TE *)M->RE, (double *)M->R, (double *)M->Tint,
                             (double *)M->Fext, (double *)M->Kval, check_nan, status + w.x);
@@ -1244,7 +1242,7 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
                                static_cast<const double *>(RE), R, Tint, Fext, check_nan, status);
         return hipGetLastError();
     }
-    const int nblocks = blocks ? nblk : (nn + kRowNodesPerBlock - 1) / kRowNodesPerBlock;
+    const int nblocks = blocks ? nblk : (nn + row_nodes_per_block(nod) - 1) / row_nodes_per_block(nod);
     if (nblocks <= 0) return hipSuccess;
     const dim3 grid(((nblocks + kXcds - 1) / kXcds) * kXcds), block(256);
     const size_t lds = size_t(9) * size_t(block_vals) * sizeof(double);

@@ -121,9 +121,9 @@ std::string build_mesh_host(MeshHost &m) {
         m.rowptr_n[size_t(k) + 1] = m.rowptr_n[k] + deg[size_t(k) + 1];
         m.maxdeg = std::max<int32_t>(m.maxdeg, int32_t(deg[size_t(k) + 1]));
     }
-    for (int64_t k0 = 0; k0 < nn; k0 += kRowNodesPerBlock)
-        m.max_block_deg = std::max<int64_t>(m.max_block_deg,
-                                            m.rowptr_n[std::min<int64_t>(k0 + kRowNodesPerBlock, nn)] - m.rowptr_n[k0]);
+    const int64_t nb = row_nodes_per_block(nod);
+    for (int64_t k0 = 0; k0 < nn; k0 += nb)
+        m.max_block_deg = std::max<int64_t>(m.max_block_deg, m.rowptr_n[std::min<int64_t>(k0 + nb, nn)] - m.rowptr_n[k0]);
     m.colind_n.assign(size_t(m.rowptr_n[nn]), 0);
     parallel([&](int32_t k, std::vector<int32_t> &buf) {
         adjacency_of(k, buf);
