@@ -130,8 +130,10 @@ int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t
 
 /* ---------------------------------------------------------------- batched assembly
  * Many independent meshes (ensemble members with perturbed C and AGlen, mixed 3-, 6- and
- * 10-node elements) assembled together: one persistent element launch and one persistent
- * row launch over a flattened (member, block) work list. Replaces N separate uvhAssembly
+ * 10-node elements) assembled together: per element type, one element launch and one row
+ * launch over a flattened (member, block) work list (one workgroup per item; with the
+ * environment variable UVH_BATCH=persistent read at uvhb_create, persistent workgroups that
+ * take items from an atomic counter instead). Replaces N separate uvhAssembly
  * calls (UaMain/uvhAssembly.m:1), one per member; Úa itself spreads such work over MATLAB
  * workers (uvhMatrixAssemblySSTREAM_SPMD.m), which here share one GPU instead.
  * Each member's results are those of uvh_assemble on that member alone to rounding (the

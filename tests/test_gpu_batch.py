@@ -1,7 +1,8 @@
-"""Batched assembly (uvhb_*): one persistent launch per stage over many meshes of mixed
-element types gives each member the R and K of a single-mesh assembly of that member to
-rounding (1e-14 normwise; the persistent kernel is compiled separately, so the compiler may
+"""Batched assembly (uvhb_*): one launch per stage and element type over many meshes of
+mixed element types gives each member the R and K of a single-mesh assembly of that member
+to rounding (1e-14 normwise; the batch kernels are compiled separately, so the compiler may
 fuse a different product of an a*b + c*d into an FMA) and the oracle's to the 1e-12 bar;
+the persistent-workgroup schedule gives the results of the default one to the same bounds;
 repeated batched assemblies agree bit for bit; NaN status is reported per member; a captured
 HIP graph of the batched assembly replays to the eager result; the autotuner's tile choice
 changes no result."""
@@ -180,3 +181,38 @@ def test_autotuner_choice_changes_no_result(batch):
     batch.sync_status()
     for a, b in zip(before, K):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_persistent_schedule_matches_default(batch, fp32, monkeypatch):
+    import torch
+    dev = torch.device("cuda:0")
+    print(f"[gpu] batch persistent schedule fp32={fp32}", flush=True)
+    cv = default_ctrlvar()
+    fields, R, K, keep = device_setup(batch, torch, dev)
+    ctrl = uvh.make_ctrl(cv, members()[0][2])
+    flags = uvh.UVH_FP32 if fp32 else 0
+    stream = torch.cuda.current_stream().cuda_stream
+    batch.assemble_device(ctrl, fields, flags, R, K, stream=stream)
+    batch.sync_status()
+    ref = [(r.clone(), k.clone()) for r, k in zip(R, K)]
+    monkeypatch.setenv("UVH_BATCH", "persistent")
+    pb = UvhBatch(0)
+    try:
+        for M, _, _ in members():
+            pb.add_mesh(M)
+        for nod in {m.nod for m in batch.members}:
+            pb.set_tile(nod, fp32, batch.get_tile(nod, fp32))
+        for rep in range(2):
+            for r, k in zip(R, K):
+                r.fill_(-1.0)
+                k.fill_(-1.0)
+            pb.assemble_device(ctrl, fields, flags, R, K, stream=stream)
+            assert pb.sync_status() == [0] * len(pb.members)
+            for i, (r0, k0) in enumerate(ref):
+                errR = ((R[i] - r0).norm() / r0.norm()).item()
+                errK = ((K[i] - k0).norm() / k0.norm()).item()
+                tol = 1e-6 if fp32 else 1e-14   # FP32: roundings of separately compiled FP32 arithmetic
+                assert errR <= tol and errK <= tol, f"member {i} rep {rep}: R {errR:.2e} K {errK:.2e}"
+    finally:
+        pb.close()

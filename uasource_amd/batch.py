@@ -2,8 +2,9 @@
 and the tile autotuner.
 
 A batch holds ensemble members (each its own mesh, fields and outputs) and assembles all of
-them with one persistent element launch and one persistent row launch over a flattened
-(member, block) work list. Each member's R and K are those of a single-mesh ``uvhAssembly``
+them with, per element type, one element launch and one row launch over a flattened
+(member, block) work list: one workgroup per item, or persistent workgroups that take items
+from an atomic counter when ``UVH_BATCH=persistent`` is set at creation. Each member's R and K are those of a single-mesh ``uvhAssembly``
 (``UaMain/uvhAssembly.m:1``) on that member to rounding, and bit for bit the same from run to
 run.
 """

@@ -41,7 +41,7 @@ def _time(torch, fn, steps, stream):
 
 def bench_batch(steps: int, warmup: int, n_members: int = 64, log=print) -> dict:
     """64 independent meshes (log-uniform 5k-80k elements, P1/P2 mixed, perturbed C and AGlen):
-    batched assembly (one persistent launch per stage), FP32 element arithmetic with FP64
+    batched assembly (one launch per stage and element type), FP32 element arithmetic with FP64
     accumulation and FP64, autotuned tiles, eager and HIP-graph replay; against the same work
     as 64 single-mesh assemblies."""
     import torch

@@ -876,12 +876,6 @@ __device__ __forceinline__ FieldPtrs generic_fields(const UVH_G FieldPtrsG *g) {
                      (const double *)g->ab0, (const double *)g->dadh};
 }
 
-__global__ void k_clear_counters(int *counters, int n, int *status, int nstatus) {
-    const int t = threadIdx.x;
-    if (t < n) counters[t] = 0;
-    for (int k = t; k < nstatus; k += blockDim.x) status[k] = 0;
-}
-
 // next item of a persistent workgroup: thread 0 takes it, the block reads it from LDS
 __device__ __forceinline__ int next_item(int *counter, int *s_item) {
     if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1);
@@ -894,8 +888,15 @@ __device__ __forceinline__ int next_item(int *counter, int *s_item) {
 }
 
 // items: (member, node block of 256)
+// Block 0 also zeroes the work counters and the per-member status words, which only the
+// later launches of the same assembly read.
 __global__ __launch_bounds__(256) void k_nodal_b_batch(const BatchMember *__restrict__ mem,
-                                                       const int2 *__restrict__ items, Params p) {
+                                                       const int2 *__restrict__ items, Params p, int *counters,
+                                                       int ncounters, int *status, int nstatus) {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < ncounters) counters[threadIdx.x] = 0;
+        for (int k = threadIdx.x; k < nstatus; k += blockDim.x) status[k] = 0;
+    }
     const int2 w = items[blockIdx.x];
     const BatchMember &M = mem[w.x];
     const int k = w.y * 256 + threadIdx.x;
@@ -938,6 +939,44 @@ __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__rest
             (const int32_t *)M->inc_pos, (T *)M->EB, (T *)M->RE);
         __syncthreads();  // the LDS of this item is dead before the next one overwrites it
     }
+}
+
+// The same work list as k_element_batch, one workgroup per item instead of persistent
+// workgroups: item blockIdx (XCD-aware), no work counter, the register budget of the
+// single-mesh kernel.
+template <int NOD, typename T, bool RONLY>
+__global__ __launch_bounds__(256) void k_element_batch_flat(const BatchMember *__restrict__ mem,
+                                                            const int2 *__restrict__ items, int nitems, Params p) {
+    extern __shared__ double lds[];
+    const int it = xcd_block(blockIdx.x, gridDim.x);
+    if (it >= nitems) return;
+    const int2 w = items[it];
+    const UVH_G BatchMemberG *M = member(mem, w.x);
+    const typename RowTab<T>::type *rtab;
+    if constexpr (sizeof(T) == sizeof(double))
+        rtab = (const ShapeTables *)(const __attribute__((address_space(4))) ShapeTables *)M->tab;
+    else
+        rtab = (const ShapeTables32 *)(const __attribute__((address_space(4))) ShapeTables32 *)M->tab32;
+    const FieldPtrs f = generic_fields(&M->f);
+    element_block<NOD, T, RONLY, (NOD <= 6 && sizeof(T) == sizeof(float))>(
+        lds, M->nele, M->nip, M->epb, w.y, (const int32_t *)M->conn, (const double *)M->x, (const double *)M->y, f,
+        (const double *)M->b, (const ShapeTables *)M->tab, rtab, p, (const int32_t *)M->inc_pos, (T *)M->EB,
+        (T *)M->RE);
+}
+
+template <int NOD, typename TE>
+__global__ __launch_bounds__(256) void k_rows_batch_flat(const BatchMember *__restrict__ mem,
+                                                         const int2 *__restrict__ items, int nitems, int check_nan,
+                                                         int *__restrict__ status) {
+    extern __shared__ double lds[];
+    const int it = xcd_block(blockIdx.x, gridDim.x);
+    if (it >= nitems) return;
+    const int2 w = items[it];
+    const UVH_G BatchMemberG *M = member(mem, w.x);
+    rows_block<NOD, TE>(lds, M->nn, w.y * row_nodes_per_block(NOD), (const int32_t *)M->inc_ptr,
+                        (const uint8_t *)M->pos_inc, (const int64_t *)M->rowptr_n, (const uint8_t *)M->diag, M->nnzN,
+                        (const TE *)M->EB, (const TE *)M->RE, (double *)M->R, (double *)M->Tint, (double *)M->Fext,
+                        (double *)M->Kval, check_nan, status + w.x);
 }
 
 // items: (member, node block of row_nodes_per_block(NOD) nodes), all members with NOD nodes;
@@ -1315,18 +1354,25 @@ hipError_t launch_rows_gather(int nod, int nn, int64_t nnzN, bool fp32, const in
 hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_items, int n_nodal,
                         const int2 *elem_items, const int *eb, const int2 *row_items, const int *rb,
                         const size_t *elem_lds, const size_t *row_lds, bool fp32, bool ronly, int check_nan,
-                        int *ctr, int *status, const Params &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, ctr, 6, status, nmem);
-    hipLaunchKernelGGL(k_nodal_b_batch, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, p);
+                        int *ctr, int *status, const Params &p, bool flat, hipStream_t s) {
+    hipLaunchKernelGGL(k_nodal_b_batch, dim3(n_nodal), dim3(256), 0, s, mem, nodal_items, p, ctr, 6, status, nmem);
     for (int k = 0; k < 3; ++k)
         if (elem_lds[k] > 156 * 1024 || row_lds[k] > 156 * 1024) return hipErrorInvalidConfiguration;
 #define UVH_EB(NODV, T, RO, K)                                                                             \
     do {                                                                                                   \
-        auto kern = k_element_batch<NODV, T, RO>;                                                          \
-        lds_attr(reinterpret_cast<const void *>(kern), elem_lds[K]);                                       \
-        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), elem_lds[K]);               \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), elem_lds[K], s, mem, elem_items + eb[K],           \
-                           eb[K + 1] - eb[K], ctr + K, p);                                                 \
+        const int n_it = eb[K + 1] - eb[K];                                                                \
+        if (flat) {                                                                                        \
+            auto kern = k_element_batch_flat<NODV, T, RO>;                                                 \
+            lds_attr(reinterpret_cast<const void *>(kern), elem_lds[K]);                                   \
+            const int grid = ((n_it + kXcds - 1) / kXcds) * kXcds;                                         \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), elem_lds[K], s, mem, elem_items + eb[K], n_it, p); \
+        } else {                                                                                           \
+            auto kern = k_element_batch<NODV, T, RO>;                                                      \
+            lds_attr(reinterpret_cast<const void *>(kern), elem_lds[K]);                                   \
+            const int grid = persistent_grid(reinterpret_cast<const void *>(kern), elem_lds[K]);           \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), elem_lds[K], s, mem, elem_items + eb[K], n_it, \
+                               ctr + K, p);                                                                \
+        }                                                                                                  \
     } while (0)
 #define UVH_EB_T(NODV, K)                                                                                  \
     do {                                                                                                   \
@@ -1348,11 +1394,20 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
     }
 #define UVH_RB(NODV, TE, K)                                                                                \
     do {                                                                                                   \
-        auto kern = k_rows_batch<NODV, TE>;                                                                \
-        lds_attr(reinterpret_cast<const void *>(kern), row_lds[K]);                                        \
-        const int grid = persistent_grid(reinterpret_cast<const void *>(kern), row_lds[K]);                \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), row_lds[K], s, mem, row_items + rb[K],             \
-                           rb[K + 1] - rb[K], ctr + 3 + K, check_nan, status);                             \
+        const int n_it = rb[K + 1] - rb[K];                                                                \
+        if (flat) {                                                                                        \
+            auto kern = k_rows_batch_flat<NODV, TE>;                                                       \
+            lds_attr(reinterpret_cast<const void *>(kern), row_lds[K]);                                    \
+            const int grid = ((n_it + kXcds - 1) / kXcds) * kXcds;                                         \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), row_lds[K], s, mem, row_items + rb[K], n_it,   \
+                               check_nan, status);                                                         \
+        } else {                                                                                           \
+            auto kern = k_rows_batch<NODV, TE>;                                                            \
+            lds_attr(reinterpret_cast<const void *>(kern), row_lds[K]);                                    \
+            const int grid = persistent_grid(reinterpret_cast<const void *>(kern), row_lds[K]);            \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), row_lds[K], s, mem, row_items + rb[K], n_it,   \
+                               ctr + 3 + K, check_nan, status);                                            \
+        }                                                                                                  \
     } while (0)
 #define UVH_RB_T(NODV, K)                                                                                  \
     do {                                                                                                   \
