@@ -77,7 +77,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", action="store_true", help="64-mesh batched assembly")
     ap.add_argument("--serve", action="store_true", help="request serving, latency over a load sweep")
-    ap.add_argument("--newton", action="store_true", help="full Newton solve (newton_p2 workload)")
+    ap.add_argument("--newton", action="store_true",
+                    help="full Newton solve (newton_p2 with the host solver; --workload with --device-solver)")
+    ap.add_argument("--device-solver", action="store_true",
+                    help="with --newton: linear solves on the GPU (block ILU(0) BiCGStab)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -106,7 +109,8 @@ def main(argv=None):
         elif args.serve:
             line = bench_modes.bench_serve(log=log)
         else:
-            line = bench_modes.bench_newton(log=log)
+            line = bench_modes.bench_newton("newton_p2" if args.workload == "plain_p2" and not args.device_solver
+                                            else args.workload, device_solver=args.device_solver, log=log)
         print(json.dumps(line), flush=True)
         return 0
 

@@ -42,7 +42,9 @@ enum uvh_status {
     UVH_E_NAN_K = 4,        /* NaN on the diagonal of K */
     UVH_E_UNSUPPORTED = 5,  /* option outside this library: caller falls back to uvhMatrixAssembly */
     UVH_E_STATE = 6,        /* call out of order (e.g. assemble before set_mesh) */
-    UVH_E_NOMEM = 7         /* device allocation failed */
+    UVH_E_NOMEM = 7,        /* device allocation failed */
+    UVH_E_NOCONV = 8        /* linear solve: zero pivot, breakdown or tolerance not reached
+                               (solveKApe's gmres flag != 0) */
 };
 
 enum uvh_flags {
@@ -127,6 +129,35 @@ int uvh_sync_status(uvh_ctx *ctx);
  * (name, total ms, launches) accumulated since the last reset, and returns the count. */
 int uvh_profile(uvh_ctx *ctx, int enable);
 int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t *launches, int cap);
+
+/* ---------------------------------------------------------------- Newton-step linear solve
+ * K d = rhs on the device with Dirichlet-fixed degrees of freedom eliminated: the branch
+ * CtrlVar.AsymmSolver = 'EliminateBCsSolveSystemIterativly' of solveKApe
+ * (UaMain/solveKApe.m:117-151: constrained rows and columns removed, a no-fill incomplete LU,
+ * a preconditioned Krylov solve to a relative residual tolerance, CtrlVar.LinSolveTol =
+ * 1e-8 by default). K is taken in 3 x 3 node blocks (block ILU(0) of the node graph) and the
+ * Krylov method is BiCGStab. A fixed degree of freedom gets an identity row and column and a
+ * zero right-hand side, so the other unknowns are those of the eliminated system and the
+ * fixed ones are zero (the Newton-step form of x(iConstrainedDOF) = g).
+ *
+ * uvh_lin_create: for the mesh of `ctx` (set_mesh first; a later set_mesh invalidates the
+ *   solver); `fixed` is a host mask of 3*Nn bytes in [ub; vb; h] order (nonzero = fixed), or
+ *   NULL. Analyses the pattern once.
+ * uvh_lin_factor: block ILU(0) of the device K values `dKval` (the context's CSR order, as
+ *   uvh_assemble_device writes them). UVH_E_NOCONV on a zero pivot.
+ * uvh_lin_solve: dx = K^-1 drhs (device vectors of 3*Nn in [ub; vb; h] order) to
+ *   ||rhs - K dx|| <= rtol ||rhs|| within maxit iterations; *iterations and *relres (the true
+ *   relative residual of dx; either may be NULL) are set on every return. UVH_E_NOCONV if the
+ *   tolerance is not reached or BiCGStab breaks down; dx then holds the last iterate. Blocks
+ *   the host once per reduction (a few times per iteration). */
+typedef struct uvh_lin uvh_lin;
+
+int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **lin);
+void uvh_lin_destroy(uvh_lin *lin);
+const char *uvh_lin_last_error(const uvh_lin *lin);
+int uvh_lin_factor(uvh_lin *lin, const double *dKval, void *stream);
+int uvh_lin_solve(uvh_lin *lin, const double *drhs, double *dx, double rtol, int32_t maxit, int32_t *iterations,
+                  double *relres, void *stream);
 
 /* ---------------------------------------------------------------- batched assembly
  * Many independent meshes (ensemble members with perturbed C and AGlen, mixed 3-, 6- and

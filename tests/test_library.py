@@ -19,7 +19,8 @@ def declared_entry_points():
 
 def test_header_and_binding_list_agree():
     from uasource_amd.batch import EXPORTED_BATCH
-    assert declared_entry_points() == sorted(uvh.EXPORTED + EXPORTED_BATCH)
+    from uasource_amd.linsolve import EXPORTED_LIN
+    assert declared_entry_points() == sorted(uvh.EXPORTED + EXPORTED_BATCH + EXPORTED_LIN)
 
 
 def test_library_exports_every_declared_symbol():

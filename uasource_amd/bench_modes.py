@@ -192,10 +192,12 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
     }
 
 
-def bench_newton(workload: str = "newton_p2", log=print) -> dict:
-    """Full Newton uvh solve (Úa's default tolerance rForce < 1e-15), GPU assembly and a host
-    direct solve, FP64 and the precision form."""
+def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=print) -> dict:
+    """Full Newton uvh solve (Úa's default tolerance rForce < 1e-15) with the GPU assembly,
+    FP64 and the precision form; the linear solve on the host (SciPy SuperLU) or, with
+    ``device_solver``, on the GPU (block ILU(0) BiCGStab to LinSolveTol = 1e-8)."""
     from . import newton
+    from .linsolve import GpuLinearSolver
     w = WORKLOADS[workload]
     M = build_mesh(w)
     F0, F1, BCs, fixed = newton_problem(M, w)
@@ -204,15 +206,20 @@ def bench_newton(workload: str = "newton_p2", log=print) -> dict:
     for fp32 in (False, True):
         t = time.perf_counter()
         asm = newton.GpuAssembler(cv, M, F0, F1)
-        res = newton.solve(asm, cv, fixed=fixed, fp32=fp32)
+        solver = GpuLinearSolver(asm.ctx, fixed, rtol=cv.LinSolveTol) if device_solver else newton.superlu_solve
+        res = newton.solve(asm, cv, linear_solver=solver, fixed=fixed, fp32=fp32)
         total = time.perf_counter() - t
         out["fp32" if fp32 else "fp64"] = {
             "converged": res.converged, "iterations": res.iterations, "rForce": res.rForce, "seconds": total,
             "assembly_s": res.t_assembly, "solve_s": res.t_solve, "linesearch_s": res.t_linesearch,
             "rk_assemblies": res.n_rk, "r_assemblies": res.n_r, "state": (res.ub, res.vb, res.h)}
+        if device_solver:
+            out["fp32" if fp32 else "fp64"]["krylov_iterations"] = res.linear_iterations
+            solver.close()
         asm.ctx.close()
         log(f"newton {'fp32' if fp32 else 'fp64'}: {res.iterations} iterations, {res.reason}, {total:.1f} s "
-            f"(assembly+K download {res.t_assembly:.2f} s, solve {res.t_solve:.2f} s, line search {res.t_linesearch:.2f} s)")
+            f"(assembly {res.t_assembly:.2f} s, solve {res.t_solve:.2f} s, line search {res.t_linesearch:.2f} s)"
+            + (f", Krylov iterations {res.linear_iterations}" if device_solver else ""))
     u64, v64, h64 = out["fp64"].pop("state")
     u32, v32, h32 = out["fp32"].pop("state")
     speed = np.linalg.norm(np.concatenate([u64, v64]))
@@ -222,7 +229,8 @@ def bench_newton(workload: str = "newton_p2", log=print) -> dict:
         "metric": "uvh Newton solve to rForce < 1e-15", "value": out["fp64"]["seconds"], "unit": "s",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": out["fp64"]["seconds"] * 1e3, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, inflow conditions)",
-        "config": {"workload": workload, "Nele": M.Nele, "Nnodes": M.Nnodes, "linear_solver": "SciPy SuperLU (host)",
-                   "parallelism": "1 GPU"},
+        "config": {"workload": workload, "Nele": M.Nele, "Nnodes": M.Nnodes,
+                   "linear_solver": "block ILU(0) BiCGStab (GPU), LinSolveTol 1e-8" if device_solver
+                   else "SciPy SuperLU (host)", "parallelism": "1 GPU"},
         "fp64": out["fp64"], "fp32_form": out["fp32"], "fp32_form_rel_err_vs_fp64": err,
     }

@@ -15,7 +15,7 @@ REPO = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libuvh.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["uvh_kernels.hip", "uvh_api.hip"]
+HIP_SOURCES = ["uvh_kernels.hip", "uvh_api.hip", "uvh_linsolve.hip"]
 CXX_SOURCES = ["uvh_mesh.cpp", "uvh_schedule.cpp", "uvh_host_api.cpp"]
 HEADERS = ["uvh_internal.h", os.path.join("..", "..", "include", "uvh.h")]
 
@@ -61,7 +61,10 @@ def build(verbose: bool = False) -> str:
         _run(cmd)
         objs.append(obj)
     tmp = OUT + ".tmp"
-    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"]
+    # rocSPARSE (block ILU(0) and triangular solves of the Newton-step linear solve)
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, f"-L{rocm_lib}", f"-Wl,-rpath,{rocm_lib}",
+           "-lrocsparse", "-lpthread"]
     if verbose:
         print(" ".join(cmd))
     _run(cmd)

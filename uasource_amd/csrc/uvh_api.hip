@@ -905,3 +905,8 @@ int uvhb_sync_status(uvh_batch *b, int32_t *member_status) {
 }
 
 }  // extern "C"
+
+namespace uvh {
+const MeshHost *ctx_mesh(const uvh_ctx *ctx) { return ctx && ctx->has_mesh ? &ctx->mesh : nullptr; }
+int ctx_device(const uvh_ctx *ctx) { return ctx ? ctx->device : 0; }
+}  // namespace uvh

@@ -5,6 +5,8 @@
 #include <string>
 #include <vector>
 
+struct uvh_ctx;
+
 namespace uvh {
 
 constexpr int kMaxNod = 10;
@@ -160,5 +162,10 @@ struct CallArgs {             // per assembly
     Params p;
     double *R, *Tint, *Fext, *Kval;
 };
+
+// Read access to a context's mesh (null before uvh_set_mesh) and device, for the parts of
+// the library outside uvh_api.hip.
+const MeshHost *ctx_mesh(const uvh_ctx *ctx);
+int ctx_device(const uvh_ctx *ctx);
 
 }  // namespace uvh

@@ -84,4 +84,6 @@ def default_ctrlvar() -> Struct:
         uvhDesiredWorkOrForceTolerances=[float("inf"), 1e-15],    # :498
         uvhExitBackTrackingStepLength=1e-3,       # :502
         uvhAcceptableWorkAndForceTolerances=[float("inf"), 1e-6], # :504
+        AsymmSolver="Auto",                       # :656 (the device solver is the 'EliminateBCsSolveSystemIterativly' branch)
+        LinSolveTol=1e-8,                         # :659, relative residual of iterative linear solves
     )

@@ -1,0 +1,111 @@
+"""Newton-step linear solve on the GPU (``uvh_lin_*`` in include/uvh.h).
+
+The device form of ``CtrlVar.AsymmSolver = 'EliminateBCsSolveSystemIterativly'`` in
+``UaMain/solveKApe.m:117-151``: constrained degrees of freedom eliminated, a no-fill
+incomplete LU (here the block ILU(0) of the 3 x 3 node blocks), and a preconditioned Krylov
+solve (here BiCGStab) to ``CtrlVar.LinSolveTol`` (1e-8 by default,
+``UaMain/Ua2D_DefaultParameters.m:659``). K and the right-hand side never leave the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import uvh
+
+EXPORTED_LIN = ("uvh_lin_create", "uvh_lin_destroy", "uvh_lin_last_error", "uvh_lin_factor", "uvh_lin_solve")
+
+UVH_E_NOCONV = 8
+
+_bound = False
+
+
+def _bind(lib):
+    global _bound
+    if _bound:
+        return lib
+    vp = C.c_void_p
+    lib.uvh_lin_create.argtypes = [vp, vp, C.POINTER(vp)]
+    lib.uvh_lin_create.restype = C.c_int
+    lib.uvh_lin_destroy.argtypes = [vp]
+    lib.uvh_lin_destroy.restype = None
+    lib.uvh_lin_last_error.argtypes = [vp]
+    lib.uvh_lin_last_error.restype = C.c_char_p
+    lib.uvh_lin_factor.argtypes = [vp, vp, vp]
+    lib.uvh_lin_factor.restype = C.c_int
+    lib.uvh_lin_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), vp]
+    lib.uvh_lin_solve.restype = C.c_int
+    _bound = True
+    return lib
+
+
+class LinearSolveError(RuntimeError):
+    """The solve did not reach its tolerance (solveKApe's Krylov flag != 0) or hit a zero pivot."""
+
+
+class GpuLinearSolver:
+    """Block-ILU(0) BiCGStab for ``K d = rhs`` of one context's mesh, with the degrees of
+    freedom in ``fixed`` (indices into [ub; vb; h]) held at zero.
+
+    ``factor(K)`` takes the device K values of ``UvhContext.assemble_device``; ``solve(rhs, x)``
+    takes and fills device vectors of 3*Nn values. ``on_device`` tells ``newton.solve`` to hand
+    over the device K and residual instead of host copies."""
+
+    on_device = True
+
+    def __init__(self, ctx: uvh.UvhContext, fixed=None, rtol: float = 1e-8, maxit: int = 5000,
+                 raise_on_noconv: bool = False):
+        self.lib = _bind(uvh.load_library())
+        self.ctx = ctx
+        self.nn = ctx.nn
+        self.fixed = None if fixed is None or len(fixed) == 0 else np.unique(np.asarray(fixed, dtype=np.int64))
+        mask = None
+        if self.fixed is not None:
+            mask = np.zeros(3 * self.nn, dtype=np.uint8)
+            mask[self.fixed] = 1
+        h = C.c_void_p()
+        st = self.lib.uvh_lin_create(ctx.h, None if mask is None else mask.ctypes.data, C.byref(h))
+        if st != uvh.UVH_OK:
+            raise uvh.UvhLibraryError(f"uvh_lin_create failed with status {st}")
+        self.h = h
+        self.rtol, self.maxit = float(rtol), int(maxit)
+        self.raise_on_noconv = raise_on_noconv
+        self.last_iterations = 0
+        self.last_relres = float("nan")
+        self.total_iterations = 0
+
+    def close(self):
+        if self.h:
+            self.lib.uvh_lin_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, st, what):
+        msg = self.lib.uvh_lin_last_error(self.h).decode()
+        if st == UVH_E_NOCONV:
+            return LinearSolveError(f"{what}: {msg}")
+        return uvh.UvhLibraryError(f"{what}: status {st}: {msg}")
+
+    def factor(self, K, stream=None):
+        st = self.lib.uvh_lin_factor(self.h, uvh._ptr(K), stream)
+        if st != uvh.UVH_OK:
+            raise self._err(st, "uvh_lin_factor")
+
+    def solve(self, rhs, x, stream=None, rtol: float | None = None, maxit: int | None = None):
+        """x = K^-1 rhs; returns (iterations, true relative residual). Raises LinearSolveError
+        when the tolerance is missed only if ``raise_on_noconv`` (x then holds the last iterate,
+        as the reference's gmres returns its last iterate with flag != 0)."""
+        its, rel = C.c_int32(), C.c_double()
+        st = self.lib.uvh_lin_solve(self.h, uvh._ptr(rhs), uvh._ptr(x), self.rtol if rtol is None else rtol,
+                                    self.maxit if maxit is None else maxit, C.byref(its), C.byref(rel), stream)
+        self.last_iterations, self.last_relres = its.value, rel.value
+        self.total_iterations += its.value
+        if st != uvh.UVH_OK and (st != UVH_E_NOCONV or self.raise_on_noconv):
+            raise self._err(st, "uvh_lin_solve")
+        return its.value, rel.value

@@ -19,7 +19,9 @@ without boundary-condition constraints (``L`` empty):
 The residual cost is device-resident (the R-only assemblies, the axpy to the trial state
 and the dot products stay in HBM; three scalars come back per evaluation) and the residual
 at ``gamma = 0`` is taken from the R+K assembly instead of being assembled again. The linear
-solve is on the host (SciPy's SuperLU), as in Úa, which calls ``solveKApe`` on the CPU.
+solve is either on the host (SciPy's SuperLU, the default, as Úa's direct ``solveKApe``
+branches) or, with a ``linsolve.GpuLinearSolver``, on the device (``solveKApe``'s iterative
+branch: block ILU(0) and BiCGStab), in which case K and R never leave HBM.
 
 Precision form: with ``fp32=True`` the Jacobian is always assembled with FP32 element
 arithmetic, and so is the residual while ``rForce`` is above ``fp64_switch`` and the FP32 line
@@ -54,6 +56,7 @@ class NewtonResult:
     t_linesearch: float = 0.0
     n_rk: int = 0
     n_r: int = 0
+    linear_iterations: list = field(default_factory=list)   # Krylov iterations per Newton step (device solver)
     ub: np.ndarray | None = None
     vb: np.ndarray | None = None
     h: np.ndarray | None = None
@@ -124,9 +127,10 @@ class GpuAssembler:
         self.n_r += 1
         return float(self.torch.dot(self.R, self.R).item()) + 1000 * EPS
 
-    def assemble_rk(self, fp32: bool, fp32_residual: bool):
-        """K (host CSR) at the current state; R(gamma = 0) is kept on the device. With an FP32
-        Jacobian and an FP64 residual, the residual comes from an extra R-only assembly."""
+    def assemble_rk_device(self, fp32: bool, fp32_residual: bool):
+        """K (device, ``self.K``) and R(gamma = 0) (device, ``self.R0``) at the current state.
+        With an FP32 Jacobian and an FP64 residual, the residual comes from an extra R-only
+        assembly."""
         self.xt.copy_(self.x)
         self._assemble(uvh.UVH_FP32 if fp32 else 0, self.K)
         self.n_rk += 1
@@ -134,7 +138,17 @@ class GpuAssembler:
             self._assemble(uvh.UVH_RONLY)
             self.n_r += 1
         self.R0.copy_(self.R)
+
+    def assemble_rk(self, fp32: bool, fp32_residual: bool):
+        """As ``assemble_rk_device``; returns K (host CSR) and -R(gamma = 0) (host)."""
+        self.assemble_rk_device(fp32, fp32_residual)
         return self.ctx.K_matrix(self.K.cpu().numpy()), (-self.R0).cpu().numpy()
+
+    def step_on_device(self, solver):
+        """d = K^-1 (-R0) with a device solver (``linsolve.GpuLinearSolver``), into ``self.d``."""
+        rhs = self.torch.neg(self.R0)
+        solver.factor(self.K, self.stream)
+        solver.solve(rhs, self.d, self.stream)
 
     def set_step(self, d: np.ndarray):
         self.d.copy_(self.torch.from_numpy(d))
@@ -227,7 +241,16 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
           fixed: np.ndarray | None = None, verbose: bool = False) -> NewtonResult:
     """Newton-Raphson uvh iteration (SSTREAM_TransientImplicit.m:224-418). ``fixed``: degrees of
     freedom held at their current values (their residual rows are left out of the cost)."""
-    linear_solver = eliminate(linear_solver, fixed)
+    on_device = getattr(linear_solver, "on_device", False)
+    if on_device:
+        # the device solver holds its own fixed set (identity rows and columns)
+        same = (linear_solver.fixed is None and (fixed is None or len(fixed) == 0)) or (
+            linear_solver.fixed is not None and fixed is not None
+            and np.array_equal(linear_solver.fixed, np.unique(np.asarray(fixed))))
+        if not same:
+            raise ValueError("the device linear solver was built for a different set of fixed degrees of freedom")
+    else:
+        linear_solver = eliminate(linear_solver, fixed)
     assembler.set_fixed(fixed)
     N = assembler.normalisation()
     res = NewtonResult(False, 0, "", float("inf"), float("nan"))
@@ -269,11 +292,18 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             residual64 = True   # from here on the residual, the step and the exit test are FP64
         fp32_res = fp32 and not residual64
         t0 = time.perf_counter()
-        K, rhs = assembler.assemble_rk(fp32, fp32_res)
-        t1 = time.perf_counter()
-        d = linear_solver(K, rhs)
-        t2 = time.perf_counter()
-        assembler.set_step(d)
+        if on_device:
+            assembler.assemble_rk_device(fp32, fp32_res)
+            t1 = time.perf_counter()
+            assembler.step_on_device(linear_solver)
+            t2 = time.perf_counter()
+            res.linear_iterations.append(linear_solver.last_iterations)
+        else:
+            K, rhs = assembler.assemble_rk(fp32, fp32_res)
+            t1 = time.perf_counter()
+            d = linear_solver(K, rhs)
+            t2 = time.perf_counter()
+            assembler.set_step(d)
         cost = lambda g: assembler.cost(g, fp32_res, N)
         c0 = cost(0.0)
         c1 = cost(1.0)
