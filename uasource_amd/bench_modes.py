@@ -207,7 +207,8 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
         t = time.perf_counter()
         asm = newton.GpuAssembler(cv, M, F0, F1)
         solver = GpuLinearSolver(asm.ctx, fixed, rtol=cv.LinSolveTol) if device_solver else newton.superlu_solve
-        res = newton.solve(asm, cv, linear_solver=solver, fixed=fixed, fp32=fp32)
+        log(f"newton {'fp32' if fp32 else 'fp64'} on {workload} ({M.Nele} elements, {3 * M.Nnodes} unknowns)")
+        res = newton.solve(asm, cv, linear_solver=solver, fixed=fixed, fp32=fp32, verbose=True, log=log)
         total = time.perf_counter() - t
         out["fp32" if fp32 else "fp64"] = {
             "converged": res.converged, "iterations": res.iterations, "rForce": res.rForce, "seconds": total,

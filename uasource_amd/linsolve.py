@@ -73,6 +73,7 @@ class GpuLinearSolver:
         self.raise_on_noconv = raise_on_noconv
         self.last_iterations = 0
         self.last_relres = float("nan")
+        self.last_message = ""          # empty when the last solve reached its tolerance
         self.total_iterations = 0
 
     def close(self):
@@ -105,6 +106,7 @@ class GpuLinearSolver:
         st = self.lib.uvh_lin_solve(self.h, uvh._ptr(rhs), uvh._ptr(x), self.rtol if rtol is None else rtol,
                                     self.maxit if maxit is None else maxit, C.byref(its), C.byref(rel), stream)
         self.last_iterations, self.last_relres = its.value, rel.value
+        self.last_message = "" if st == uvh.UVH_OK else self.lib.uvh_lin_last_error(self.h).decode()
         self.total_iterations += its.value
         if st != uvh.UVH_OK and (st != UVH_E_NOCONV or self.raise_on_noconv):
             raise self._err(st, "uvh_lin_solve")

@@ -238,7 +238,7 @@ def eliminate(linear_solver, fixed: np.ndarray | None):
 
 
 def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, fp64_switch: float = 1e-8,
-          fixed: np.ndarray | None = None, verbose: bool = False) -> NewtonResult:
+          fixed: np.ndarray | None = None, verbose: bool = False, log=print) -> NewtonResult:
     """Newton-Raphson uvh iteration (SSTREAM_TransientImplicit.m:224-418). ``fixed``: degrees of
     freedom held at their current values (their residual rows are left out of the cost)."""
     on_device = getattr(linear_solver, "on_device", False)
@@ -320,8 +320,11 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
         res.t_linesearch += t3 - t2
         res.history.append((iteration, gamma, rForce, rWork, not fp32_res))
         if verbose:
-            print(f"  it {iteration}: gamma {gamma:.3g} rForce {rForce:.3e} rWork {rWork:.3e}"
-                  f"{'' if not fp32 else (' (fp64 R)' if not fp32_res else ' (fp32 R)')}", flush=True)
+            lin = (f" Krylov {res.linear_iterations[-1]} relres {linear_solver.last_relres:.1e}"
+                   + (f" [{linear_solver.last_message}]" if linear_solver.last_message else "")) if on_device else ""
+            log(f"  it {iteration}: gamma {gamma:.3g} rForce {rForce:.3e} rWork {rWork:.3e} "
+                f"(assembly {t1 - t0:.3f} s, solve {t2 - t1:.3f} s{lin})"
+                f"{'' if not fp32 else (' (fp64 R)' if not fp32_res else ' (fp32 R)')}")
     res.iterations, res.rForce, res.rWork = iteration, rForce, rWork
     res.n_rk, res.n_r = assembler.n_rk, assembler.n_r
     res.ub, res.vb, res.h = assembler.state()
