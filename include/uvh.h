@@ -147,12 +147,20 @@ int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t
  *   uvh_assemble_device writes them). UVH_E_NOCONV on a zero pivot.
  * uvh_lin_solve: dx = K^-1 drhs (device vectors of 3*Nn in [ub; vb; h] order) to
  *   ||rhs - K dx|| <= rtol ||rhs|| within maxit iterations; *iterations and *relres (the true
- *   relative residual of dx; either may be NULL) are set on every return. UVH_E_NOCONV if the
- *   tolerance is not reached or BiCGStab breaks down; dx then holds the last iterate. Blocks
- *   the host once per reduction (a few times per iteration). */
+ *   relative residual of dx; either may be NULL) are set on every return; iterations counts
+ *   BiCGStab iterations plus GMRES steps. UVH_E_NOCONV if the tolerance is not reached; dx then
+ *   holds the best iterate seen. Blocks the host once per reduction (a few times per
+ *   iteration). */
 typedef struct uvh_lin uvh_lin;
 
+/* Krylov methods (uvh_lin_set_krylov): BiCGStab; GMRES(restart), the reference branch's
+ * method (gmres with restarts, solveKApe.m:143-144); AUTO (the default): BiCGStab, and when it
+ * breaks down or misses the tolerance, GMRES(restart) from its best iterate. */
+enum { UVH_LIN_BICGSTAB = 0, UVH_LIN_GMRES = 1, UVH_LIN_AUTO = 2 };
+
 int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **lin);
+/* method: UVH_LIN_*; restart: GMRES cycle length, 2..64 (default 30). */
+int uvh_lin_set_krylov(uvh_lin *lin, int32_t method, int32_t restart);
 void uvh_lin_destroy(uvh_lin *lin);
 const char *uvh_lin_last_error(const uvh_lin *lin);
 int uvh_lin_factor(uvh_lin *lin, const double *dKval, void *stream);

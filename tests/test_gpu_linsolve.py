@@ -6,7 +6,8 @@ degrees of freedom removed, as solveKApe.m:126-129 does): at rtol 1e-12 the solu
 to 1e-8 (the solve's residual bound times the conditioning of these systems) and the true
 residual, recomputed on the host, is below 2e-12; fixed degrees of freedom are exactly zero;
 a solve is bit for bit reproducible; a zero right-hand side gives zero in no iterations; the
-iteration limit and the call order are enforced. The Newton iteration with the device solver
+iteration limit and the call order are enforced. Each Krylov method (BiCGStab, GMRES(m) as in
+the reference branch, and the default BiCGStab with a GMRES fall-back) meets the same bars. The Newton iteration with the device solver
 (LinSolveTol = 1e-8, Úa's default) converges like the oracle's Newton with a direct solve:
 iteration counts within one, rForce < 1e-15, the same u, v, h."""
 import numpy as np
@@ -73,6 +74,44 @@ def test_solution_matches_direct_solve(name):
         assert abs(true_res - relres) <= 0.02 * true_res + 1e-14
         assert err < 1e-8
         assert np.all(xg[fixed] == 0.0)
+    finally:
+        solver.close()
+
+
+@pytest.mark.parametrize("method", ["bicgstab", "gmres"])
+@pytest.mark.parametrize("name", ["tiny_p3", "medium_p2"])
+def test_each_krylov_method_matches_direct_solve(name, method):
+    import torch
+    print(f"[gpu] linear solve {name} {method}", flush=True)
+    M, fixed, ctx, K, rhs, dK, drhs = system(name)
+    solver = GpuLinearSolver(ctx, fixed, rtol=1e-12, maxit=20000, method=method, restart=40)
+    try:
+        x = torch.empty_like(drhs)
+        solver.factor(dK)
+        its, relres = solver.solve(drhs, x)
+        xg = x.cpu().numpy()
+        xd, free = direct(K, rhs, fixed)
+        err = np.linalg.norm(xg - xd) / np.linalg.norm(xd)
+        print(f"   {its} iterations, relres {relres:.2e}, err {err:.2e} {solver.last_message}", flush=True)
+        assert solver.last_message == "" and relres < 2e-12 and err < 1e-8
+        y = torch.empty_like(drhs)
+        solver.solve(drhs, y)
+        assert torch.equal(x, y)
+    finally:
+        solver.close()
+
+
+def test_krylov_settings_are_checked():
+    print("[gpu] linear solve settings", flush=True)
+    M, fixed, ctx, K, rhs, dK, drhs = system("tiny_p2")
+    solver = GpuLinearSolver(ctx, fixed)
+    try:
+        with pytest.raises(uvh.UvhLibraryError):
+            solver.set_krylov("gmres", 1)
+        with pytest.raises(uvh.UvhLibraryError):
+            solver.set_krylov("gmres", 65)
+        with pytest.raises(KeyError):
+            solver.set_krylov("cg", 30)
     finally:
         solver.close()
 

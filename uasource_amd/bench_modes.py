@@ -211,7 +211,8 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
         res = newton.solve(asm, cv, linear_solver=solver, fixed=fixed, fp32=fp32, verbose=True, log=log)
         total = time.perf_counter() - t
         out["fp32" if fp32 else "fp64"] = {
-            "converged": res.converged, "iterations": res.iterations, "rForce": res.rForce, "seconds": total,
+            "converged": bool(res.converged), "iterations": int(res.iterations), "rForce": float(res.rForce),
+            "seconds": total, "reason": res.reason,
             "assembly_s": res.t_assembly, "solve_s": res.t_solve, "linesearch_s": res.t_linesearch,
             "rk_assemblies": res.n_rk, "r_assemblies": res.n_r, "state": (res.ub, res.vb, res.h)}
         if device_solver:

@@ -61,10 +61,7 @@ def build(verbose: bool = False) -> str:
         _run(cmd)
         objs.append(obj)
     tmp = OUT + ".tmp"
-    # rocSPARSE (block ILU(0) and triangular solves of the Newton-step linear solve)
-    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
-    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, f"-L{rocm_lib}", f"-Wl,-rpath,{rocm_lib}",
-           "-lrocsparse", "-lpthread"]
+    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"]
     if verbose:
         print(" ".join(cmd))
     _run(cmd)

@@ -18,13 +18,13 @@
 //   * the Krylov method is BiCGStab with right preconditioning (the reference's GMRES(10)
 //     took 3-6x more iterations than BiCGStab on these systems when both were tried with the
 //     same ILU(0));
-//   * the factorization and the triangular solves are rocSPARSE's bsrilu0 / bsrsv; the
-//     CSR-to-block gather, the block product, the vector updates and the reductions are the
-//     kernels below. Reductions have a fixed shape (fixed grid, fixed tree), so a solve is
-//     bit for bit reproducible.
+//   * in that order the rows of one colour share no entries, so the factorization and both
+//     triangular solves run colour by colour, one thread per block row, all rows of a colour
+//     at once (k_ilu_color, k_lsolve_color, k_usolve_color); the diagonal blocks of U are kept
+//     inverted. Reductions have a fixed shape (fixed grid, fixed tree): a solve is bit for bit
+//     reproducible.
 
 #include <hip/hip_runtime.h>
-#include <rocsparse/rocsparse.h>
 
 #include <algorithm>
 #include <cmath>
@@ -60,19 +60,47 @@ __global__ void k_to_blocks(int64_t total, int nn, int64_t nnzN, const int64_t *
     A[int64_t(slot[k]) * 9 + bc] = v;
 }
 
-// y = A x for 3 x 3 row-major blocks: one thread per (node, component).
+// Row kernels below give kRowLanes lanes to one block row: lane l takes the row's blocks
+// l, l + kRowLanes, ..., and the lane sums are combined by a fixed butterfly, so the result
+// does not depend on timing.
+constexpr int kRowLanes = 8;
+
+__device__ __forceinline__ void row_sum(double &t0, double &t1, double &t2) {
+#pragma unroll
+    for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+        t0 += __shfl_xor(t0, off, kRowLanes);
+        t1 += __shfl_xor(t1, off, kRowLanes);
+        t2 += __shfl_xor(t2, off, kRowLanes);
+    }
+}
+
+// t += A_p v_ci[p] over the row's blocks p0 .. p1-1 taken by this lane
+__device__ __forceinline__ void row_dot(int p0, int p1, int lane, const int32_t *__restrict__ ci,
+                                        const double *__restrict__ A, const double *__restrict__ v, double &t0,
+                                        double &t1, double &t2) {
+    for (int p = p0 + lane; p < p1; p += kRowLanes) {
+        const double *a = A + int64_t(p) * 9;
+        const int64_t m = 3 * int64_t(ci[p]);
+        const double v0 = v[m], v1 = v[m + 1], v2 = v[m + 2];
+        t0 += a[0] * v0 + a[1] * v1 + a[2] * v2;
+        t1 += a[3] * v0 + a[4] * v1 + a[5] * v2;
+        t2 += a[6] * v0 + a[7] * v1 + a[8] * v2;
+    }
+}
+
+// y = A x for 3 x 3 row-major blocks
 __global__ void k_block_mv(int nn, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                            const double *__restrict__ A, const double *__restrict__ x, double *__restrict__ y) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * nn) return;
-    const int n = t / 3, b = t - 3 * n;
-    double s = 0.0;
-    for (int k = rp[n]; k < rp[n + 1]; ++k) {
-        const int m = ci[k];
-        const double *a = A + int64_t(k) * 9 + 3 * b;
-        s += a[0] * x[3 * m] + a[1] * x[3 * m + 1] + a[2] * x[3 * m + 2];
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = int(g / kRowLanes), lane = int(g % kRowLanes);
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (i < nn) row_dot(rp[i], rp[i + 1], lane, ci, A, x, t0, t1, t2);
+    row_sum(t0, t1, t2);
+    if (i < nn && lane == 0) {
+        y[3 * int64_t(i)] = t0;
+        y[3 * int64_t(i) + 1] = t1;
+        y[3 * int64_t(i) + 2] = t2;
     }
-    y[t] = s;
 }
 
 // [ub; vb; h] order -> renumbered, node-interleaved; zero at fixed degrees of freedom
@@ -204,21 +232,141 @@ __global__ void k_copy(int64_t n, const double *__restrict__ a, double *__restri
         b[i] = a[i];
 }
 
-const char *sp_name(rocsparse_status s) {
-    switch (s) {
-        case rocsparse_status_success: return "success";
-        case rocsparse_status_invalid_handle: return "invalid handle";
-        case rocsparse_status_not_implemented: return "not implemented";
-        case rocsparse_status_invalid_pointer: return "invalid pointer";
-        case rocsparse_status_invalid_size: return "invalid size";
-        case rocsparse_status_memory_error: return "memory error";
-        case rocsparse_status_internal_error: return "internal error";
-        case rocsparse_status_invalid_value: return "invalid value";
-        case rocsparse_status_arch_mismatch: return "arch mismatch";
-        case rocsparse_status_zero_pivot: return "zero pivot";
-        default: return "error";
+// 3 x 3 row-major block helpers
+__device__ __forceinline__ void mm3(const double *a, const double *b, double *c) {   // c = a b
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) c[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+}
+
+// Block ILU(0), the rows of one colour (new numbers r0 .. r1-1): for each lower block (i, k)
+// in ascending k, L_ik = A_ik D_k^-1 and A_ij -= L_ik U_kj for every j > k in both rows;
+// then D_i^-1 replaces the diagonal block. Rows k < r0 are final (earlier colours); rows of
+// this colour do not read each other.
+__global__ void k_ilu_color(int r0, int r1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                            const int32_t *__restrict__ dslot, double *__restrict__ LU, int *__restrict__ status) {
+    const int i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r1) return;
+    const int e1 = rp[i + 1], di = dslot[i];
+    for (int p = rp[i]; p < di; ++p) {
+        const int k = ci[p];
+        double a[9], l[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a[q] = LU[int64_t(p) * 9 + q];
+        mm3(a, LU + int64_t(dslot[k]) * 9, l);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) LU[int64_t(p) * 9 + q] = l[q];
+        // merge the rest of row i with the upper part of row k (both sorted by column)
+        int q = p + 1, r = dslot[k] + 1;
+        const int k1 = rp[k + 1];
+        while (q < e1 && r < k1) {
+            const int jq = ci[q], jr = ci[r];
+            if (jq == jr) {
+                double u[9], lu[9];
+#pragma unroll
+                for (int t = 0; t < 9; ++t) u[t] = LU[int64_t(r) * 9 + t];
+                mm3(l, u, lu);
+#pragma unroll
+                for (int t = 0; t < 9; ++t) LU[int64_t(q) * 9 + t] -= lu[t];
+                ++q;
+                ++r;
+            } else if (jq < jr) {
+                ++q;
+            } else {
+                ++r;
+            }
+        }
+    }
+    // invert the pivot block (cofactors over the determinant)
+    double d[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) d[q] = LU[int64_t(di) * 9 + q];
+    const double c0 = d[4] * d[8] - d[5] * d[7], c1 = d[5] * d[6] - d[3] * d[8], c2 = d[3] * d[7] - d[4] * d[6];
+    const double det = d[0] * c0 + d[1] * c1 + d[2] * c2;
+    if (!(det != 0.0) || !isfinite(det)) {
+        atomicMin(status, i);   // first zero pivot (smallest row) reported
+        return;
+    }
+    const double r = 1.0 / det;
+    double v[9];
+    v[0] = c0 * r; v[1] = (d[2] * d[7] - d[1] * d[8]) * r; v[2] = (d[1] * d[5] - d[2] * d[4]) * r;
+    v[3] = c1 * r; v[4] = (d[0] * d[8] - d[2] * d[6]) * r; v[5] = (d[2] * d[3] - d[0] * d[5]) * r;
+    v[6] = c2 * r; v[7] = (d[1] * d[6] - d[0] * d[7]) * r; v[8] = (d[0] * d[4] - d[1] * d[3]) * r;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) LU[int64_t(di) * 9 + q] = v[q];
+}
+
+// y_i = x_i - sum over lower blocks L_ik y_k, rows of one colour (L has identity diagonal blocks)
+__global__ void k_lsolve_color(int r0, int r1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const int32_t *__restrict__ dslot, const double *__restrict__ LU,
+                               const double *__restrict__ x, double *__restrict__ y) {
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = r0 + int(g / kRowLanes), lane = int(g % kRowLanes);
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (i < r1) row_dot(rp[i], dslot[i], lane, ci, LU, y, t0, t1, t2);
+    row_sum(t0, t1, t2);
+    if (i < r1 && lane == 0) {
+        y[3 * int64_t(i)] = x[3 * int64_t(i)] - t0;
+        y[3 * int64_t(i) + 1] = x[3 * int64_t(i) + 1] - t1;
+        y[3 * int64_t(i) + 2] = x[3 * int64_t(i) + 2] - t2;
     }
 }
+
+// x_i = D_i^-1 (y_i - sum over upper blocks U_ij x_j), rows of one colour
+__global__ void k_usolve_color(int r0, int r1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const int32_t *__restrict__ dslot, const double *__restrict__ LU,
+                               const double *__restrict__ y, double *__restrict__ x) {
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = r0 + int(g / kRowLanes), lane = int(g % kRowLanes);
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (i < r1) row_dot(dslot[i] + 1, rp[i + 1], lane, ci, LU, x, t0, t1, t2);
+    row_sum(t0, t1, t2);
+    if (i < r1 && lane == 0) {
+        t0 = y[3 * int64_t(i)] - t0;
+        t1 = y[3 * int64_t(i) + 1] - t1;
+        t2 = y[3 * int64_t(i) + 2] - t2;
+        const double *d = LU + int64_t(dslot[i]) * 9;
+        x[3 * int64_t(i)] = d[0] * t0 + d[1] * t1 + d[2] * t2;
+        x[3 * int64_t(i) + 1] = d[3] * t0 + d[4] * t1 + d[5] * t2;
+        x[3 * int64_t(i) + 2] = d[6] * t0 + d[7] * t1 + d[8] * t2;
+    }
+}
+
+// GMRES helpers over a basis V of k vectors of length n, stored one after another
+constexpr int kMaxKrylov = 64;   // longest GMRES cycle
+
+// part[c * gridDim.x + blockIdx.x] = partial of (V_c, w), c = blockIdx.y
+__global__ __launch_bounds__(kThreads) void k_multi_dot(int64_t n, const double *__restrict__ V,
+                                                        const double *__restrict__ w, double *__restrict__ part) {
+    const double *v = V + int64_t(blockIdx.y) * n;
+    double acc[1] = {0.0};
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+        acc[0] += v[i] * w[i];
+    block_partials<1>(acc, part + int64_t(blockIdx.y) * gridDim.x);
+}
+
+// w += sign * sum_c coef[c] V_c (coefficients read on the device)
+__global__ void k_multi_axpy(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ coef,
+                             double sign, double *__restrict__ w) {
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+        double t = 0.0;
+        for (int c = 0; c < k; ++c) t += coef[c] * V[int64_t(c) * n + i];
+        w[i] += sign * t;
+    }
+}
+
+__global__ void k_scale(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+        y[i] = a * x[i];
+}
+
+__global__ void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+        y[i] += a * x[i];
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
 
 }  // namespace
 
@@ -238,12 +386,16 @@ struct uvh_lin {
     // BiCGStab vectors (node-interleaved) and reduction scratch
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_rhat = nullptr, *d_p = nullptr, *d_v = nullptr;
     double *d_s = nullptr, *d_t = nullptr, *d_y = nullptr, *d_z = nullptr, *d_w = nullptr;
-    double *d_part = nullptr, *d_out = nullptr;
-    double *h_out = nullptr;   // pinned, 2 values
-    rocsparse_handle sp = nullptr;
-    rocsparse_mat_descr dM = nullptr, dL = nullptr, dU = nullptr;
-    rocsparse_mat_info info = nullptr;
-    void *d_buf = nullptr;
+    double *d_part = nullptr, *d_out = nullptr;       // reduction partials and results (kMaxKrylov + 2 values)
+    double *h_out = nullptr;                          // pinned: 3 (kMaxKrylov + 2) values
+    double *d_xbest = nullptr;                        // BiCGStab: iterate with the smallest residual so far
+    // Krylov method (uvh_lin_set_krylov) and the GMRES basis (allocated on first use)
+    int method = UVH_LIN_AUTO, restart = 30;
+    double *d_V = nullptr, *d_coef = nullptr;
+    int32_t *d_dslot = nullptr;                         // slot of each row's diagonal block
+    std::vector<int32_t> cstart;                        // rows of colour c: cstart[c] .. cstart[c+1]-1
+    int *d_status = nullptr;                            // smallest row with a singular pivot block
+    int *h_status = nullptr;                            // pinned
     bool factored = false;
     int fail(int code, const std::string &m) {
         err = m;
@@ -271,40 +423,206 @@ int finish(uvh_lin *L, int np, hipStream_t s, double *out) {
     return UVH_OK;
 }
 
-// y = U^-1 L^-1 x
-int precondition(uvh_lin *L, const double *x, double *y) {
-    const double one = 1.0;
-    rocsparse_status st = rocsparse_dbsrsv_solve(L->sp, rocsparse_direction_row, rocsparse_operation_none, L->nn,
-                                                 rocsparse_int(L->nnzN), &one, L->dL, L->d_LU, L->d_rp, L->d_ci, 3,
-                                                 L->info, x, L->d_w, rocsparse_solve_policy_auto, L->d_buf);
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrsv_solve(L->sp, rocsparse_direction_row, rocsparse_operation_none, L->nn,
-                                    rocsparse_int(L->nnzN), &one, L->dU, L->d_LU, L->d_rp, L->d_ci, 3, L->info, L->d_w,
-                                    y, rocsparse_solve_policy_auto, L->d_buf);
-    if (st != rocsparse_status_success)
-        return L->fail(UVH_E_HIP, std::string("uvh_lin_solve: triangular solve: ") + sp_name(st));
-    return UVH_OK;
+// y = U^-1 L^-1 x: colours ascending for L, descending for U; w holds L^-1 x (x and y must
+// not be w)
+void precondition(uvh_lin *L, const double *x, double *y, hipStream_t s) {
+    const int nc = int(L->cstart.size()) - 1;
+    for (int c = 0; c < nc; ++c) {
+        const int r0 = L->cstart[c], r1 = L->cstart[c + 1];
+        hipLaunchKernelGGL(k_lsolve_color, dim3(grid_of(int64_t(r1 - r0) * kRowLanes)), dim3(kThreads), 0, s, r0, r1,
+                           L->d_rp, L->d_ci,
+                           L->d_dslot, L->d_LU, x, L->d_w);
+    }
+    for (int c = nc - 1; c >= 0; --c) {
+        const int r0 = L->cstart[c], r1 = L->cstart[c + 1];
+        hipLaunchKernelGGL(k_usolve_color, dim3(grid_of(int64_t(r1 - r0) * kRowLanes)), dim3(kThreads), 0, s, r0, r1,
+                           L->d_rp, L->d_ci,
+                           L->d_dslot, L->d_LU, L->d_w, y);
+    }
 }
 
 void block_mv(uvh_lin *L, const double *x, double *y, hipStream_t s) {
-    hipLaunchKernelGGL(k_block_mv, dim3(grid_of(L->n3)), dim3(kThreads), 0, s, L->nn, L->d_rp, L->d_ci, L->d_A, x, y);
+    hipLaunchKernelGGL(k_block_mv, dim3(grid_of(int64_t(L->nn) * kRowLanes)), dim3(kThreads), 0, s, L->nn, L->d_rp,
+                       L->d_ci, L->d_A, x, y);
+}
+
+// BiCGStab from x = 0 (d_b the right-hand side). Near-breakdown (rhat nearly orthogonal to
+// r or v, or omega = 0) restarts with the current residual as the new shadow vector, at most
+// kMaxRestarts times. The iterate with the smallest residual is kept and returned when the
+// tolerance is not reached, so a diverging run cannot hand back a worse vector than it saw.
+int bicgstab(uvh_lin *L, double rtol, int maxit, double bnorm, hipStream_t s, int &it, bool &converged,
+             std::string &why) {
+    const int64_t n = L->n3;
+    const dim3 G(kRedBlocks), B(kThreads);
+    constexpr double kBreak = 1e-12;
+    constexpr int kMaxRestarts = 20;
+    double red[2];
+    int st;
+    hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_r);
+    hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_rhat);
+    hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_p);
+    hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_v);
+    hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_xbest);
+    double rho = bnorm * bnorm, rho_old = 1.0, alpha = 1.0, omega = 1.0;   // rhat = r = b
+    double rhat2 = rho, rr = rho, best = bnorm;
+    int restarts = 0;
+    auto restart = [&]() {
+        hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_r, L->d_rhat);
+        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_p);
+        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_v);
+        rho = rhat2 = rr;
+        rho_old = alpha = omega = 1.0;
+        ++restarts;
+    };
+    auto keep_if_best = [&](double norm) {
+        if (norm < best) {
+            best = norm;
+            hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_x, L->d_xbest);
+        }
+    };
+    converged = false;
+    for (it = 1; it <= maxit; ++it) {
+        if (std::fabs(rho) <= kBreak * std::sqrt(rhat2 * rr)) {
+            if (restarts == kMaxRestarts) { why = "BiCGStab breakdown: (rhat, r) = 0"; break; }
+            restart();
+        }
+        const double beta = (rho / rho_old) * (alpha / omega);
+        hipLaunchKernelGGL(k_update_p, G, B, 0, s, n, L->d_r, L->d_v, beta, omega, L->d_p);
+        precondition(L, L->d_p, L->d_y, s);
+        block_mv(L, L->d_y, L->d_v, s);
+        hipLaunchKernelGGL(k_dots<2>, G, B, 0, s, n, L->d_rhat, L->d_v, L->d_v, L->d_v, L->d_part);
+        if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
+        if (std::fabs(red[0]) <= kBreak * std::sqrt(rhat2 * red[1])) {
+            if (restarts == kMaxRestarts) { why = "BiCGStab breakdown: (rhat, v) = 0"; break; }
+            restart();   // p and v are discarded; this iteration starts again
+            continue;
+        }
+        alpha = rho / red[0];
+        hipLaunchKernelGGL(k_update_s, G, B, 0, s, n, L->d_r, L->d_v, L->d_y, alpha, L->d_s, L->d_x, L->d_part);
+        if ((st = finish(L, 1, s, red)) != UVH_OK) return st;
+        if (std::sqrt(red[0]) <= rtol * bnorm) { converged = true; break; }
+        keep_if_best(std::sqrt(red[0]));
+        precondition(L, L->d_s, L->d_z, s);
+        block_mv(L, L->d_z, L->d_t, s);
+        hipLaunchKernelGGL(k_dots<2>, G, B, 0, s, n, L->d_t, L->d_s, L->d_t, L->d_t, L->d_part);
+        if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
+        omega = red[1] > 0.0 ? red[0] / red[1] : 0.0;
+        hipLaunchKernelGGL(k_update_xr, G, B, 0, s, n, L->d_s, L->d_t, L->d_z, L->d_rhat, omega, L->d_x, L->d_r,
+                           L->d_part);
+        if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
+        rr = red[1];
+        if (std::sqrt(rr) <= rtol * bnorm) { converged = true; break; }
+        keep_if_best(std::sqrt(rr));
+        rho_old = rho;
+        rho = red[0];
+        if (omega == 0.0) {
+            if (restarts == kMaxRestarts) { why = "BiCGStab breakdown: omega = 0"; break; }
+            restart();
+        }
+    }
+    if (it > maxit) it = maxit;
+    if (!converged) {
+        if (why.empty()) why = "BiCGStab iteration limit";
+        hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_xbest, L->d_x);
+    }
+    return UVH_OK;
+}
+
+// GMRES(m) with right preconditioning from the current d_x, m = L->restart: Arnoldi with
+// classical Gram-Schmidt and one re-orthogonalisation (one host synchronisation per step),
+// Givens rotations on the host. `it` counts Arnoldi steps (one preconditioner application
+// and one product each).
+int gmres(uvh_lin *L, double rtol, int maxit, double bnorm, hipStream_t s, int &it, bool &converged,
+          std::string &why) {
+    const int64_t n = L->n3;
+    const int m = L->restart;
+    const dim3 G(kRedBlocks), B(kThreads);
+    double red[2];
+    int st;
+    std::vector<double> H(size_t(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    double *h1 = L->h_out, *h2 = L->h_out + (kMaxKrylov + 2), *hw = L->h_out + 2 * (kMaxKrylov + 2);
+    double prev = HUGE_VAL;
+    it = 0;
+    converged = false;
+    while (it < maxit) {
+        // r = b - A x into V_0
+        block_mv(L, L->d_x, L->d_w, s);
+        hipLaunchKernelGGL(k_residual, G, B, 0, s, n, L->d_b, L->d_w, L->d_part);
+        if ((st = finish(L, 1, s, red)) != UVH_OK) return st;
+        const double beta = std::sqrt(red[0]);
+        if (beta <= rtol * bnorm) { converged = true; break; }
+        if (!(beta < prev * (1.0 - 1e-12))) { why = "GMRES stagnated"; break; }
+        prev = beta;
+        hipLaunchKernelGGL(k_scale, G, B, 0, s, n, 1.0 / beta, L->d_w, L->d_V);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;   // basis vectors used for the update
+        for (int j = 0; j < m && it < maxit; ++j) {
+            ++it;
+            double *vj = L->d_V + int64_t(j) * n, *vn = L->d_V + int64_t(j + 1) * n;
+            precondition(L, vj, L->d_z, s);
+            block_mv(L, L->d_z, vn, s);
+            // two passes of classical Gram-Schmidt against V_0 .. V_j
+            for (int pass = 0; pass < 2; ++pass) {
+                hipLaunchKernelGGL(k_multi_dot, dim3(kRedBlocks, j + 1), B, 0, s, n, L->d_V, vn, L->d_part);
+                hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, s, kRedBlocks, j + 1, L->d_part, L->d_out);
+                hipLaunchKernelGGL(k_multi_axpy, G, B, 0, s, n, j + 1, L->d_V, L->d_out, -1.0, vn);
+                hipMemcpyAsync(pass ? h2 : h1, L->d_out, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s);
+            }
+            hipLaunchKernelGGL(k_dots<1>, G, B, 0, s, n, vn, vn, nullptr, nullptr, L->d_part);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, s, kRedBlocks, 1, L->d_part, L->d_out);
+            hipMemcpyAsync(hw, L->d_out, sizeof(double), hipMemcpyDeviceToHost, s);
+            if (hipStreamSynchronize(s) != hipSuccess) return L->fail(UVH_E_HIP, "uvh_lin_solve: GMRES step");
+            double *col = &H[size_t(j) * (m + 1)];   // column j, m + 1 rows
+            for (int c = 0; c <= j; ++c) col[c] = h1[c] + h2[c];
+            col[j + 1] = std::sqrt(hw[0]);
+            const double hn = col[j + 1];
+            if (hn > 0.0) hipLaunchKernelGGL(k_scale, G, B, 0, s, n, 1.0 / hn, vn, vn);
+            for (int c = 0; c < j; ++c) {   // earlier rotations
+                const double a = col[c], b = col[c + 1];
+                col[c] = cs[c] * a + sn[c] * b;
+                col[c + 1] = -sn[c] * a + cs[c] * b;
+            }
+            const double r = std::hypot(col[j], col[j + 1]);
+            cs[j] = r > 0.0 ? col[j] / r : 1.0;
+            sn[j] = r > 0.0 ? col[j + 1] / r : 0.0;
+            col[j] = r;
+            col[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            k = j + 1;
+            if (std::fabs(g[j + 1]) <= rtol * bnorm || hn == 0.0) break;
+        }
+        // y = H(0:k, 0:k)^-1 g, x += M^-1 V y
+        for (int r = k - 1; r >= 0; --r) {
+            double t = g[r];
+            for (int c = r + 1; c < k; ++c) t -= H[size_t(c) * (m + 1) + r] * y[c];
+            const double d = H[size_t(r) * (m + 1) + r];
+            y[r] = d != 0.0 ? t / d : 0.0;
+        }
+        std::copy(y.begin(), y.begin() + k, h1);
+        hipMemcpyAsync(L->d_coef, h1, k * sizeof(double), hipMemcpyHostToDevice, s);
+        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_y);
+        hipLaunchKernelGGL(k_multi_axpy, G, B, 0, s, n, k, L->d_V, L->d_coef, 1.0, L->d_y);
+        precondition(L, L->d_y, L->d_z, s);
+        hipLaunchKernelGGL(k_axpy, G, B, 0, s, n, 1.0, L->d_z, L->d_x);
+    }
+    if (!converged && why.empty()) why = "GMRES iteration limit";
+    return UVH_OK;
 }
 
 void destroy(uvh_lin *L) {
     if (!L) return;
     hipSetDevice(L->device);
-    if (L->info) rocsparse_destroy_mat_info(L->info);
-    if (L->dM) rocsparse_destroy_mat_descr(L->dM);
-    if (L->dL) rocsparse_destroy_mat_descr(L->dL);
-    if (L->dU) rocsparse_destroy_mat_descr(L->dU);
-    if (L->sp) rocsparse_destroy_handle(L->sp);
     for (void *p : {(void *)L->d_rowptr64, (void *)L->d_colind, (void *)L->d_newnode, (void *)L->d_slot,
                     (void *)L->d_rp, (void *)L->d_ci, (void *)L->d_node_of, (void *)L->d_fixed,
                     (void *)L->d_A, (void *)L->d_LU, (void *)L->d_b, (void *)L->d_x, (void *)L->d_r, (void *)L->d_rhat,
                     (void *)L->d_p, (void *)L->d_v, (void *)L->d_s, (void *)L->d_t, (void *)L->d_y, (void *)L->d_z,
-                    (void *)L->d_w, (void *)L->d_part, (void *)L->d_out, L->d_buf})
+                    (void *)L->d_w, (void *)L->d_part, (void *)L->d_out, (void *)L->d_dslot, (void *)L->d_status,
+                    (void *)L->d_xbest, (void *)L->d_V, (void *)L->d_coef})
         if (p) hipFree(p);
     if (L->h_out) hipHostFree(L->h_out);
+    if (L->h_status) hipHostFree(L->h_status);
     delete L;
 }
 
@@ -317,7 +635,7 @@ int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **out) {
     *out = nullptr;
     const uvh::MeshHost *M = uvh::ctx_mesh(ctx);
     if (!M) return UVH_E_STATE;
-    if (M->nnzN() >= (int64_t(1) << 31) / 9) return UVH_E_ARG;   // rocSPARSE indices are 32-bit
+    if (M->nnzN() >= (int64_t(1) << 31) - 1) return UVH_E_ARG;   // 32-bit block indices
     auto *L = new uvh_lin();
     L->device = uvh::ctx_device(ctx);
     hipSetDevice(L->device);
@@ -344,12 +662,13 @@ int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **out) {
     std::vector<int32_t> cstart(ncol + 1, 0), newnode(nn), oldnode(nn);
     for (int n = 0; n < nn; ++n) ++cstart[color[n] + 1];
     for (int c = 0; c < ncol; ++c) cstart[c + 1] += cstart[c];
+    L->cstart = cstart;
     for (int n = 0; n < nn; ++n) {
         newnode[n] = cstart[color[n]]++;
         oldnode[newnode[n]] = n;
     }
     // renumbered block pattern (columns sorted) and the slot of every original entry
-    std::vector<int32_t> rp32(nn + 1, 0), ci32(L->nnzN), slot(L->nnzN), node_of(L->nnzN);
+    std::vector<int32_t> rp32(nn + 1, 0), ci32(L->nnzN), slot(L->nnzN), node_of(L->nnzN), dslot(nn, -1);
     std::vector<std::pair<int32_t, int32_t>> row;
     for (int r = 0; r < nn; ++r) {
         const int n = oldnode[r];
@@ -363,17 +682,26 @@ int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **out) {
         for (size_t q = 0; q < row.size(); ++q) {
             ci32[rp32[r] + q] = row[q].first;
             slot[row[q].second] = rp32[r] + int32_t(q);
+            if (row[q].first == r) dslot[r] = rp32[r] + int32_t(q);
+        }
+        if (dslot[r] < 0) {   // the node graph holds every node's own entry
+            delete L;
+            return UVH_E_ARG;
         }
     }
     bool ok = lalloc(&L->d_rowptr64, nn + 1) == hipSuccess && lalloc(&L->d_rp, nn + 1) == hipSuccess &&
               lalloc(&L->d_ci, L->nnzN) == hipSuccess && lalloc(&L->d_node_of, L->nnzN) == hipSuccess &&
               lalloc(&L->d_colind, L->nnzN) == hipSuccess && lalloc(&L->d_slot, L->nnzN) == hipSuccess &&
-              lalloc(&L->d_newnode, nn) == hipSuccess &&
+              lalloc(&L->d_newnode, nn) == hipSuccess && lalloc(&L->d_dslot, nn) == hipSuccess &&
+              lalloc(&L->d_status, 1) == hipSuccess &&
+              hipHostMalloc(reinterpret_cast<void **>(&L->h_status), sizeof(int), 0) == hipSuccess &&
               lalloc(&L->d_A, 9 * L->nnzN) == hipSuccess && lalloc(&L->d_LU, 9 * L->nnzN) == hipSuccess &&
-              lalloc(&L->d_part, 2 * kRedBlocks) == hipSuccess && lalloc(&L->d_out, 2) == hipSuccess &&
-              hipHostMalloc(reinterpret_cast<void **>(&L->h_out), 2 * sizeof(double), 0) == hipSuccess;
+              lalloc(&L->d_part, (kMaxKrylov + 2) * kRedBlocks) == hipSuccess &&
+              lalloc(&L->d_out, kMaxKrylov + 2) == hipSuccess && lalloc(&L->d_coef, kMaxKrylov + 2) == hipSuccess &&
+              hipHostMalloc(reinterpret_cast<void **>(&L->h_out), 3 * (kMaxKrylov + 2) * sizeof(double), 0) ==
+                  hipSuccess;
     for (double **v : {&L->d_b, &L->d_x, &L->d_r, &L->d_rhat, &L->d_p, &L->d_v, &L->d_s, &L->d_t, &L->d_y, &L->d_z,
-                       &L->d_w})
+                       &L->d_w, &L->d_xbest})
         ok = ok && lalloc(v, L->n3) == hipSuccess;
     if (ok && fixed) ok = lalloc(&L->d_fixed, L->n3) == hipSuccess;
     if (!ok) {
@@ -386,55 +714,11 @@ int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **out) {
     if (e == hipSuccess) e = hipMemcpy(L->d_colind, colind.data(), L->nnzN * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(L->d_slot, slot.data(), L->nnzN * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(L->d_newnode, newnode.data(), nn * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(L->d_dslot, dslot.data(), nn * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(L->d_node_of, node_of.data(), L->nnzN * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && fixed) e = hipMemcpy(L->d_fixed, fixed, L->n3, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        destroy(L);
-        return UVH_E_HIP;
-    }
-    // rocSPARSE: block ILU(0) and the two triangular solves share one analysis of the pattern
-    rocsparse_status st = rocsparse_create_handle(&L->sp);
-    if (st == rocsparse_status_success) st = rocsparse_create_mat_descr(&L->dM);
-    if (st == rocsparse_status_success) st = rocsparse_create_mat_descr(&L->dL);
-    if (st == rocsparse_status_success) st = rocsparse_create_mat_descr(&L->dU);
-    if (st == rocsparse_status_success) st = rocsparse_create_mat_info(&L->info);
-    if (st == rocsparse_status_success) st = rocsparse_set_mat_fill_mode(L->dL, rocsparse_fill_mode_lower);
-    if (st == rocsparse_status_success) st = rocsparse_set_mat_diag_type(L->dL, rocsparse_diag_type_unit);
-    if (st == rocsparse_status_success) st = rocsparse_set_mat_fill_mode(L->dU, rocsparse_fill_mode_upper);
-    if (st == rocsparse_status_success) st = rocsparse_set_mat_diag_type(L->dU, rocsparse_diag_type_non_unit);
-    size_t b0 = 0, b1 = 0, b2 = 0;
-    const rocsparse_int mb = L->nn, nnzb = rocsparse_int(L->nnzN);
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrilu0_buffer_size(L->sp, rocsparse_direction_row, mb, nnzb, L->dM, L->d_LU, L->d_rp, L->d_ci,
-                                            3, L->info, &b0);
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrsv_buffer_size(L->sp, rocsparse_direction_row, rocsparse_operation_none, mb, nnzb, L->dL,
-                                          L->d_LU, L->d_rp, L->d_ci, 3, L->info, &b1);
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrsv_buffer_size(L->sp, rocsparse_direction_row, rocsparse_operation_none, mb, nnzb, L->dU,
-                                          L->d_LU, L->d_rp, L->d_ci, 3, L->info, &b2);
-    if (st == rocsparse_status_success) {
-        const size_t nb = std::max(b0, std::max(b1, b2));
-        if (hipMalloc(&L->d_buf, nb ? nb : 1) != hipSuccess) {
-            destroy(L);
-            return UVH_E_NOMEM;
-        }
-    }
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrilu0_analysis(L->sp, rocsparse_direction_row, mb, nnzb, L->dM, L->d_LU, L->d_rp, L->d_ci, 3,
-                                         L->info, rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto,
-                                         L->d_buf);
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrsv_analysis(L->sp, rocsparse_direction_row, rocsparse_operation_none, mb, nnzb, L->dL,
-                                       L->d_LU, L->d_rp, L->d_ci, 3, L->info, rocsparse_analysis_policy_reuse,
-                                       rocsparse_solve_policy_auto, L->d_buf);
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrsv_analysis(L->sp, rocsparse_direction_row, rocsparse_operation_none, mb, nnzb, L->dU,
-                                       L->d_LU, L->d_rp, L->d_ci, 3, L->info, rocsparse_analysis_policy_reuse,
-                                       rocsparse_solve_policy_auto, L->d_buf);
-    if (st == rocsparse_status_success && hipDeviceSynchronize() != hipSuccess) st = rocsparse_status_internal_error;
-    if (st != rocsparse_status_success) {
         destroy(L);
         return UVH_E_HIP;
     }
@@ -455,23 +739,38 @@ int uvh_lin_factor(uvh_lin *L, const double *dK, void *stream) {
     hipLaunchKernelGGL(k_to_blocks, dim3(grid_of(total)), dim3(kThreads), 0, s, total, L->nn, L->nnzN, L->d_rowptr64,
                        L->d_colind, L->d_node_of, L->d_slot, L->d_fixed, dK, L->d_A);
     hipLaunchKernelGGL(k_copy, dim3(kRedBlocks * 4), dim3(kThreads), 0, s, total, L->d_A, L->d_LU);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return L->fail(UVH_E_HIP, std::string("uvh_lin_factor: ") + hipGetErrorString(e));
-    rocsparse_status st = rocsparse_set_stream(L->sp, s);
-    if (st == rocsparse_status_success)
-        st = rocsparse_dbsrilu0(L->sp, rocsparse_direction_row, L->nn, rocsparse_int(L->nnzN), L->dM, L->d_LU, L->d_rp,
-                                L->d_ci, 3, L->info, rocsparse_solve_policy_auto, L->d_buf);
-    if (st != rocsparse_status_success)
-        return L->fail(UVH_E_HIP, std::string("uvh_lin_factor: bsrilu0: ") + sp_name(st));
-    rocsparse_int pivot = -1;
-    st = rocsparse_bsrilu0_zero_pivot(L->sp, L->info, &pivot);   // waits for the factorization
-    if (st == rocsparse_status_zero_pivot) {
-        L->factored = false;
-        return L->fail(UVH_E_NOCONV, "uvh_lin_factor: zero pivot in block row " + std::to_string(pivot));
+    // block ILU(0), colour by colour
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, L->d_status, L->nn);
+    const int nc = int(L->cstart.size()) - 1;
+    for (int c = 0; c < nc; ++c) {
+        const int r0 = L->cstart[c], r1 = L->cstart[c + 1];
+        hipLaunchKernelGGL(k_ilu_color, dim3(grid_of(r1 - r0)), dim3(kThreads), 0, s, r0, r1, L->d_rp, L->d_ci,
+                           L->d_dslot, L->d_LU, L->d_status);
     }
-    if (st != rocsparse_status_success)
-        return L->fail(UVH_E_HIP, std::string("uvh_lin_factor: zero-pivot check: ") + sp_name(st));
+    hipError_t e = hipMemcpyAsync(L->h_status, L->d_status, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return L->fail(UVH_E_HIP, std::string("uvh_lin_factor: ") + hipGetErrorString(e));
+    if (*L->h_status < L->nn) {
+        L->factored = false;
+        return L->fail(UVH_E_NOCONV, "uvh_lin_factor: singular pivot block in (renumbered) block row " +
+                                         std::to_string(*L->h_status));
+    }
     L->factored = true;
+    return UVH_OK;
+}
+
+int uvh_lin_set_krylov(uvh_lin *L, int32_t method, int32_t restart) {
+    if (!L) return UVH_E_ARG;
+    if (method < UVH_LIN_BICGSTAB || method > UVH_LIN_AUTO)
+        return L->fail(UVH_E_ARG, "uvh_lin_set_krylov: unknown method " + std::to_string(method));
+    if (restart < 2 || restart > kMaxKrylov)
+        return L->fail(UVH_E_ARG, "uvh_lin_set_krylov: restart must be in [2, " + std::to_string(kMaxKrylov) + "]");
+    if (L->d_V && restart > L->restart) {   // the basis grows
+        hipFree(L->d_V);
+        L->d_V = nullptr;
+    }
+    L->method = method;
+    L->restart = restart;
     return UVH_OK;
 }
 
@@ -483,94 +782,46 @@ int uvh_lin_solve(uvh_lin *L, const double *drhs, double *dx, double rtol, int32
     if (!(rtol > 0.0) || maxit < 1) return L->fail(UVH_E_ARG, "uvh_lin_solve: rtol must be > 0 and maxit >= 1");
     hipSetDevice(L->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (rocsparse_set_stream(L->sp, s) != rocsparse_status_success)
-        return L->fail(UVH_E_HIP, "uvh_lin_solve: rocsparse_set_stream");
     const int64_t n = L->n3;
     const dim3 G(kRedBlocks), B(kThreads);
+    if (L->method != UVH_LIN_BICGSTAB && !L->d_V && lalloc(&L->d_V, size_t(L->restart + 1) * n) != hipSuccess)
+        return L->fail(UVH_E_NOMEM, "uvh_lin_solve: cannot allocate the GMRES basis");
     double red[2];
     int st;
     hipLaunchKernelGGL(k_gather_rhs, dim3(grid_of(n)), B, 0, s, L->nn, L->d_newnode, L->d_fixed, drhs, L->d_b);
-    hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_r);
-    hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_rhat);
-    for (double *v : {L->d_x, L->d_p, L->d_v}) hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, v);
+    hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_x);
     hipLaunchKernelGGL(k_dots<1>, G, B, 0, s, n, L->d_b, L->d_b, nullptr, nullptr, L->d_part);
     if ((st = finish(L, 1, s, red)) != UVH_OK) return st;
     const double bnorm = std::sqrt(red[0]);
     int it = 0;
     double res = 0.0;
     bool converged = bnorm == 0.0;
-    const char *breakdown = nullptr;
+    std::string why;
     if (!converged) {
-        // near-breakdown (rhat nearly orthogonal to r or v, or omega = 0): restart with the
-        // current residual as the new shadow vector, at most kMaxRestarts times
-        constexpr double kBreak = 1e-12;
-        constexpr int kMaxRestarts = 20;
-        double rho = red[0], rho_old = 1.0, alpha = 1.0, omega = 1.0;   // rhat = r = b
-        double rhat2 = red[0], rr = red[0];
-        int restarts = 0;
-        auto restart = [&]() {
-            hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_r, L->d_rhat);
-            hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_p);
-            hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_v);
-            rho = rhat2 = rr;
-            rho_old = alpha = omega = 1.0;
-            ++restarts;
-        };
-        for (it = 1; it <= maxit; ++it) {
-            if (std::fabs(rho) <= kBreak * std::sqrt(rhat2 * rr)) {
-                if (restarts == kMaxRestarts) { breakdown = "(rhat, r) = 0"; break; }
-                restart();
-            }
-            const double beta = (rho / rho_old) * (alpha / omega);
-            hipLaunchKernelGGL(k_update_p, G, B, 0, s, n, L->d_r, L->d_v, beta, omega, L->d_p);
-            if ((st = precondition(L, L->d_p, L->d_y)) != UVH_OK) return st;
-            block_mv(L, L->d_y, L->d_v, s);
-            hipLaunchKernelGGL(k_dots<2>, G, B, 0, s, n, L->d_rhat, L->d_v, L->d_v, L->d_v, L->d_part);
-            if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
-            if (std::fabs(red[0]) <= kBreak * std::sqrt(rhat2 * red[1])) {
-                if (restarts == kMaxRestarts) { breakdown = "(rhat, v) = 0"; break; }
-                restart();   // p and v are discarded; this iteration starts again
-                continue;
-            }
-            alpha = rho / red[0];
-            hipLaunchKernelGGL(k_update_s, G, B, 0, s, n, L->d_r, L->d_v, L->d_y, alpha, L->d_s, L->d_x, L->d_part);
-            if ((st = finish(L, 1, s, red)) != UVH_OK) return st;
-            if (std::sqrt(red[0]) <= rtol * bnorm) { converged = true; break; }
-            if ((st = precondition(L, L->d_s, L->d_z)) != UVH_OK) return st;
-            block_mv(L, L->d_z, L->d_t, s);
-            hipLaunchKernelGGL(k_dots<2>, G, B, 0, s, n, L->d_t, L->d_s, L->d_t, L->d_t, L->d_part);
-            if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
-            omega = red[1] > 0.0 ? red[0] / red[1] : 0.0;
-            hipLaunchKernelGGL(k_update_xr, G, B, 0, s, n, L->d_s, L->d_t, L->d_z, L->d_rhat, omega, L->d_x, L->d_r,
-                               L->d_part);
-            if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
-            rr = red[1];
-            if (std::sqrt(rr) <= rtol * bnorm) { converged = true; break; }
-            rho_old = rho;
-            rho = red[0];
-            if (omega == 0.0) {
-                if (restarts == kMaxRestarts) { breakdown = "omega = 0"; break; }
-                restart();
-            }
+        if (L->method != UVH_LIN_GMRES) {
+            st = bicgstab(L, rtol, maxit, bnorm, s, it, converged, why);
+            if (st != UVH_OK) return st;
         }
-        if (it > maxit) it = maxit;
+        if (!converged && L->method != UVH_LIN_BICGSTAB && it < maxit) {
+            int git = 0;
+            st = gmres(L, rtol, maxit - it, bnorm, s, git, converged, why);
+            if (st != UVH_OK) return st;
+            it += git;
+        }
         // the true residual of the returned solution
         block_mv(L, L->d_x, L->d_w, s);
         hipLaunchKernelGGL(k_residual, G, B, 0, s, n, L->d_b, L->d_w, L->d_part);
         if ((st = finish(L, 1, s, red)) != UVH_OK) return st;
         res = std::sqrt(red[0]) / bnorm;
-    } else {
-        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_x);
     }
     hipLaunchKernelGGL(k_scatter_sol, dim3(grid_of(n)), B, 0, s, L->nn, L->d_newnode, L->d_x, dx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return L->fail(UVH_E_HIP, std::string("uvh_lin_solve: ") + hipGetErrorString(e));
     if (iterations) *iterations = it;
     if (relres) *relres = res;
-    if (breakdown) return L->fail(UVH_E_NOCONV, std::string("uvh_lin_solve: BiCGStab breakdown: ") + breakdown);
     if (!converged)
         return L->fail(UVH_E_NOCONV, "uvh_lin_solve: relative residual " + std::to_string(res) + " after " +
-                                         std::to_string(it) + " iterations");
+                                         std::to_string(it) + " iterations" + (why.empty() ? "" : " (" + why + ")"));
     return UVH_OK;
 }
 

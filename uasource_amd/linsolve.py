@@ -14,9 +14,11 @@ import numpy as np
 
 from . import uvh
 
-EXPORTED_LIN = ("uvh_lin_create", "uvh_lin_destroy", "uvh_lin_last_error", "uvh_lin_factor", "uvh_lin_solve")
+EXPORTED_LIN = ("uvh_lin_create", "uvh_lin_destroy", "uvh_lin_last_error", "uvh_lin_set_krylov", "uvh_lin_factor",
+                "uvh_lin_solve")
 
 UVH_E_NOCONV = 8
+METHODS = {"bicgstab": 0, "gmres": 1, "auto": 2}
 
 _bound = False
 
@@ -32,6 +34,8 @@ def _bind(lib):
     lib.uvh_lin_destroy.restype = None
     lib.uvh_lin_last_error.argtypes = [vp]
     lib.uvh_lin_last_error.restype = C.c_char_p
+    lib.uvh_lin_set_krylov.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.uvh_lin_set_krylov.restype = C.c_int
     lib.uvh_lin_factor.argtypes = [vp, vp, vp]
     lib.uvh_lin_factor.restype = C.c_int
     lib.uvh_lin_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), vp]
@@ -45,8 +49,9 @@ class LinearSolveError(RuntimeError):
 
 
 class GpuLinearSolver:
-    """Block-ILU(0) BiCGStab for ``K d = rhs`` of one context's mesh, with the degrees of
-    freedom in ``fixed`` (indices into [ub; vb; h]) held at zero.
+    """Block-ILU(0) preconditioned Krylov solve (BiCGStab, GMRES or BiCGStab falling back to
+    GMRES) for ``K d = rhs`` of one context's mesh, with the degrees of freedom in ``fixed``
+    (indices into [ub; vb; h]) held at zero.
 
     ``factor(K)`` takes the device K values of ``UvhContext.assemble_device``; ``solve(rhs, x)``
     takes and fills device vectors of 3*Nn values. ``on_device`` tells ``newton.solve`` to hand
@@ -55,7 +60,7 @@ class GpuLinearSolver:
     on_device = True
 
     def __init__(self, ctx: uvh.UvhContext, fixed=None, rtol: float = 1e-8, maxit: int = 5000,
-                 raise_on_noconv: bool = False):
+                 raise_on_noconv: bool = False, method: str = "auto", restart: int = 30):
         self.lib = _bind(uvh.load_library())
         self.ctx = ctx
         self.nn = ctx.nn
@@ -69,6 +74,7 @@ class GpuLinearSolver:
         if st != uvh.UVH_OK:
             raise uvh.UvhLibraryError(f"uvh_lin_create failed with status {st}")
         self.h = h
+        self.set_krylov(method, restart)
         self.rtol, self.maxit = float(rtol), int(maxit)
         self.raise_on_noconv = raise_on_noconv
         self.last_iterations = 0
@@ -92,6 +98,14 @@ class GpuLinearSolver:
         if st == UVH_E_NOCONV:
             return LinearSolveError(f"{what}: {msg}")
         return uvh.UvhLibraryError(f"{what}: status {st}: {msg}")
+
+    def set_krylov(self, method: str = "auto", restart: int = 30):
+        """'bicgstab', 'gmres' (GMRES(restart), the reference branch's method) or 'auto'
+        (BiCGStab, then GMRES from its best iterate if it fails)."""
+        st = self.lib.uvh_lin_set_krylov(self.h, METHODS[method], int(restart))
+        if st != uvh.UVH_OK:
+            raise self._err(st, "uvh_lin_set_krylov")
+        self.method, self.restart = method, int(restart)
 
     def factor(self, K, stream=None):
         st = self.lib.uvh_lin_factor(self.h, uvh._ptr(K), stream)
