@@ -130,6 +130,41 @@ int uvh_sync_status(uvh_ctx *ctx);
 int uvh_profile(uvh_ctx *ctx, int enable);
 int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t *launches, int cap);
 
+/* ---------------------------------------------------------------- uv (diagnostic) assembly
+ * The velocity-only SSTREAM system on the same context and mesh:
+ *   [Ruv,Kuv,Tint,Fext]=uvMatrixAssemblySSTREAM(CtrlVar,MUA,F,BCs)
+ * (UaMain/uvMatrixAssemblySSTREAM.m:6, reached through uvMatrixAssembly.m), 2025 form.
+ * R is 2Nn ([x; y], R = Tint - Fext); K is on the pattern of uvh_uv_get_pattern: 2Nn rows,
+ * row (b, n) = [adj(n); adj(n) + Nn], 4*nnzN values, blocks [Kxu Kxv; Kyu Kyv]. The reference
+ * symmetrises K as (K + K')/2 (:545-549); the uv Jacobian is a Hessian, and the library's K
+ * is symmetric to rounding (within 1e-15 of the symmetrised matrix) without that step.
+ * s is the upper surface (b = s - h at the points, :219); C and AGlen are clipped at Cmin and
+ * AGlenmin (:185, :211); nrviscosity scales the viscosity-derivative terms (Dvisk, :72-76:
+ * CtrlVar.NRviscosity, or 0 with CtrlVar.Picard). Flags: UVH_RONLY, UVH_FP32,
+ * UVH_NO_NAN_CHECK; UVH_ZEROFIELDS is refused (UVH_E_UNSUPPORTED): the reference's uv
+ * ZeroFields sets u, v to zero except at the fixed nodes (:24-28), which the caller passes
+ * as ub, vb directly. */
+typedef struct {
+    const double *h, *s, *ub, *vb, *S, *B, *rho, *C, *m, *n, *AGlen;
+} uvh_uv_fields;
+
+typedef struct {
+    double cmin;         /* CtrlVar.Cmin */
+    double aglenmin;     /* CtrlVar.AGlenmin */
+    double nrviscosity;  /* CtrlVar.NRviscosity, 0 when CtrlVar.Picard */
+} uvh_uv_ctrl;
+
+/* Pattern of the uv K: neq = 2*nnodes rows, nnz = 4*nnzN values; rowptr neq+1, colind nnz. */
+int uvh_uv_pattern_info(const uvh_ctx *ctx, int64_t *neq, int64_t *nnz);
+int uvh_uv_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind);
+/* Host buffers (R 2Nn required; Kval, Tint, Fext may be NULL); blocks until done. */
+int uvh_uv_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *fields,
+                    uint32_t flags, double *R, double *Kval, double *Tint, double *Fext);
+/* Device buffers, enqueued on `stream`; NaN checks deferred to uvh_sync_status. */
+int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc,
+                           const uvh_uv_fields *dfields, uint32_t flags, double *dR, double *dKval, double *dTint,
+                           double *dFext, void *stream);
+
 /* ---------------------------------------------------------------- Newton-step linear solve
  * K d = rhs on the device with Dirichlet-fixed degrees of freedom eliminated: the branch
  * CtrlVar.AsymmSolver = 'EliminateBCsSolveSystemIterativly' of solveKApe

@@ -51,6 +51,11 @@ def lib():
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        _lib.o2_uv_assemble.restype = C.c_int
+        _lib.o2_uv_assemble.argtypes = [C.POINTER(O2Params), C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_int]
         _lib.o2_shape.restype = None
         _lib.o2_shape.argtypes = [C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
     return _lib
@@ -157,4 +162,51 @@ def assemble(CtrlVar, MUA, F0, F1, mesh: O2Mesh | None = None, nthreads: int = 0
     if Kval is not None:
         rowptr, colind = mesh.csr_pattern()
         K = sp.csr_matrix((Kval, colind, rowptr), shape=(3 * nn, 3 * nn))
+    return dict(R=R, Tint=Tint, Fext=Fext, Kval=Kval, K=K, status=st, mesh=mesh)
+
+
+UV_FIELDS = ("h", "s", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen")
+
+
+def uv_csr_pattern(mesh: O2Mesh):
+    """CSR pattern of the uv K (2 x 2 node blocks): row (bi, node) = [adj(node); adj(node) + nn]."""
+    nn = mesh.MUA.Nnodes
+    rowptr_n, colind_n = mesh.rowptr_n, mesh.colind_n
+    deg = np.diff(rowptr_n)
+    rowptr = np.zeros(2 * nn + 1, dtype=np.int64)
+    rowptr[1:] = np.concatenate([np.cumsum(2 * deg), 2 * mesh.nnzN + np.cumsum(2 * deg)])
+    colind = np.empty(4 * mesh.nnzN, dtype=np.int64)
+    node_of_entry = np.repeat(np.arange(nn), deg)
+    start2 = 2 * rowptr_n[:-1]
+    for bi in range(2):
+        for bj in range(2):
+            pos = bi * 2 * mesh.nnzN + start2[node_of_entry] + bj * deg[node_of_entry] + \
+                (np.arange(mesh.nnzN) - rowptr_n[node_of_entry])
+            colind[pos] = colind_n + bj * nn
+    return rowptr, colind
+
+
+def assemble_uv(CtrlVar, MUA, F, mesh: O2Mesh | None = None, ronly: bool = False, nthreads: int = 0,
+                want_K: bool = True):
+    """uv (diagnostic) assembly, UaMain/uvMatrixAssemblySSTREAM.m. ``F`` carries the surface
+    ``s`` besides the uvh fields. Returns dict(R, Tint, Fext (2nn), Kval, K (CSR or None),
+    status, mesh)."""
+    if mesh is None:
+        mesh = O2Mesh(MUA)
+    P = params_from(CtrlVar, F, ronly=ronly, zero_fields=False)
+    P.reset_thickness = 0
+    dvisk = 0.0 if CtrlVar.Picard else float(CtrlVar.NRviscosity)
+    nn = MUA.Nnodes
+    arrs = [np.ascontiguousarray(getattr(F, k), dtype=np.float64) for k in UV_FIELDS]
+    fp = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    R, Tint, Fext = np.zeros(2 * nn), np.zeros(2 * nn), np.zeros(2 * nn)
+    Kval = np.zeros(4 * mesh.nnzN) if (want_K and not ronly) else None
+    st = lib().o2_uv_assemble(C.byref(P), CtrlVar.Cmin, CtrlVar.AGlenmin, dvisk, MUA.Nele, MUA.nod, nn,
+                              _p(mesh.conn), _p(mesh.x), _p(mesh.y), MUA.nip, _p(mesh.pts), _p(mesh.wts),
+                              C.cast(fp, C.c_void_p), _p(mesh.rowptr_n), _p(mesh.colind_n), _p(R), _p(Tint),
+                              _p(Fext), _p(Kval), int(nthreads))
+    K = None
+    if Kval is not None:
+        rowptr, colind = uv_csr_pattern(mesh)
+        K = sp.csr_matrix((Kval, colind, rowptr), shape=(2 * nn, 2 * nn))
     return dict(R=R, Tint=Tint, Fext=Fext, Kval=Kval, K=K, status=st, mesh=mesh)

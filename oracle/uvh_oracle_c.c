@@ -450,3 +450,188 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
     if (bad & 1) return 1;
     return 0;
 }
+
+/*
+ * uv (diagnostic SSTREAM) assembly: UaMain/uvMatrixAssemblySSTREAM.m (2025 form of :224-240,
+ * uvGroupAssembly = false, no melange physics). Fields f[]: h s ub vb S B rho C m n AGlen.
+ * Differences from the uvh element: s is an input and b = s - h at the points (:219), C and
+ * AGlen are clipped at Cmin and AGlenmin (:185, :211), the viscosity derivative enters with
+ * Dvisk = NRviscosity (:72-76, :418-421), there is no thickness equation and no SUPG.
+ * R = Tint - Fext (2nn, [x; y]); K (4*nnzN values) on the CSR rows r = blk*nn + node with
+ * row = [adj(node); adj + nn], symmetrised as (K + K')/2 (:545-549).
+ */
+enum { UV_H, UV_S, UV_U, UV_V, UV_SS, UV_B, UV_RHO, UV_C, UV_M, UV_N, UV_A, NUV };
+
+static void uv_element(const o2_params *P, double cmin, double aglenmin, double dvisk, int nod, int nip,
+                       const double *pts, const double *wts, const double *xn, const double *yn,
+                       const double nv[][MAXNOD], double *Rel, double *Kel)
+{
+    const double ca = cos(P->alpha), sa = sin(P->alpha), g = P->g, rhow = P->rhow;
+    double Tx[MAXNOD] = {0}, Fx[MAXNOD] = {0}, Ty[MAXNOD] = {0}, Fy[MAXNOD] = {0};
+    if (Kel) memset(Kel, 0, sizeof(double) * 4 * nod * nod);
+    for (int q = 0; q < nip; ++q) {
+        double N[MAXNOD], Nxi[MAXNOD], Neta[MAXNOD], Nx[MAXNOD], Ny[MAXNOD];
+        shape_barycentric(nod, pts[2 * q], pts[2 * q + 1], N, Nxi, Neta);
+        double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+        for (int i = 0; i < nod; ++i) {
+            J11 += Nxi[i] * xn[i]; J12 += Nxi[i] * yn[i];
+            J21 += Neta[i] * xn[i]; J22 += Neta[i] * yn[i];
+        }
+        const double det = J11 * J22 - J12 * J21;
+        for (int i = 0; i < nod; ++i) {
+            Nx[i] = (J22 * Nxi[i] - J12 * Neta[i]) / det;
+            Ny[i] = (-J21 * Nxi[i] + J11 * Neta[i]) / det;
+        }
+        double val[NUV];
+        for (int f = 0; f < NUV; ++f) {
+            double s = 0;
+            for (int i = 0; i < nod; ++i) s += N[i] * nv[f][i];
+            val[f] = s;
+        }
+        const double h = val[UV_H], u = val[UV_U], v = val[UV_V], rho = val[UV_RHO];
+        double Cq = val[UV_C];
+        if (Cq < cmin) Cq = cmin;
+        double AG = val[UV_A];
+        if (AG < aglenmin) AG = aglenmin;
+        const double m = val[UV_M], n = val[UV_N];
+        const double Hth = val[UV_SS] - val[UV_B];
+        double dsdx = 0, dsdy = 0, dhdx = 0, dhdy = 0, exx = 0, eyy = 0, exy = 0;
+        for (int i = 0; i < nod; ++i) {
+            dsdx += Nx[i] * nv[UV_S][i]; dsdy += Ny[i] * nv[UV_S][i];
+            dhdx += Nx[i] * nv[UV_H][i]; dhdy += Ny[i] * nv[UV_H][i];
+            exx += Nx[i] * nv[UV_U][i]; eyy += Ny[i] * nv[UV_V][i];
+            exy += 0.5 * (Nx[i] * nv[UV_V][i] + Ny[i] * nv[UV_U][i]);
+        }
+        const double dbdx = dsdx - dhdx, dbdy = dsdy - dhdy;
+        /* floatation at the point, 2025 form */
+        const double hf = rhow * Hth / rho;
+        const double He = soft_step(P->kH, h - hf, P->Hh0);
+        const double HE = soft_step(P->kH, hf - h, P->Hh0);
+        const double Hpos = soft_step(P->kH, Hth, P->Hh0) * Hth;
+        const double d = HE * rho * h / rhow + He * Hpos;
+        /* Weertman drag (BasalDrag.m) and Glen's law (EffectiveViscositySSTREAM.m) */
+        const double uz = P->speed_zero, Cz = Cq + P->czero;
+        const double Hec = He < 0 ? 0 : He;
+        const double U2 = u * u + v * v + uz * uz;
+        const double beta2 = pow(Cz, -1.0 / m) * pow(sqrt(U2), 1.0 / m - 1.0);
+        const double Dbeta2 = (1.0 / m - 1.0) * pow(Cz, -1.0 / m) * pow(U2, (1.0 - 3.0 * m) / (2.0 * m));
+        const double taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
+        const double dtxdu = Hec * (beta2 + Dbeta2 * u * u), dtydv = Hec * (beta2 + Dbeta2 * v * v);
+        const double dtxdv = Hec * Dbeta2 * u * v, dtydu = dtxdv;
+        const double e = sqrt(P->eps_zero * P->eps_zero + exx * exx + eyy * eyy + exx * eyy + exy * exy);
+        const double eta = 0.5 * pow(AG, -1.0 / n) * pow(e, (1.0 - n) / n) + P->eta_zero;
+        const double E = (1.0 - n) / (4.0 * n) * pow(AG, -1.0 / n) * pow(e, (1.0 - 3.0 * n) / n);
+        const double W = det * wts[q];
+        for (int i = 0; i < nod; ++i) {
+            if (Kel) {
+                for (int j = 0; j < nod; ++j) {
+                    const double NN2 = N[i] * N[j];
+                    const double Deu = E * ((2 * exx + eyy) * Nx[j] + exy * Ny[j]);
+                    const double Dev = E * ((2 * eyy + exx) * Ny[j] + exy * Nx[j]);
+                    const double E11 = h * (4 * exx + 2 * eyy) * Deu * Nx[i] + 2 * h * exy * Deu * Ny[i];
+                    const double E12 = h * (4 * exx + 2 * eyy) * Dev * Nx[i] + 2 * h * exy * Dev * Ny[i];
+                    const double E22 = h * (4 * eyy + 2 * exx) * Dev * Ny[i] + 2 * h * exy * Dev * Nx[i];
+                    const double E21 = h * (4 * eyy + 2 * exx) * Deu * Ny[i] + 2 * h * exy * Deu * Nx[i];
+                    const double d11 = 4 * h * eta * Nx[i] * Nx[j] + h * eta * Ny[i] * Ny[j] + dtxdu * NN2 + dvisk * E11;
+                    const double d22 = 4 * h * eta * Ny[i] * Ny[j] + h * eta * Nx[i] * Nx[j] + dtydv * NN2 + dvisk * E22;
+                    const double d12 = eta * h * (2 * Nx[i] * Ny[j] + Ny[i] * Nx[j]) + dtxdv * NN2 + dvisk * E12;
+                    const double d21 = eta * h * (2 * Ny[i] * Nx[j] + Nx[i] * Ny[j]) + dtydu * NN2 + dvisk * E21;
+                    Kel[(0 * nod + i) * nod + j] += d11 * W;   /* Kxu */
+                    Kel[(1 * nod + i) * nod + j] += d12 * W;   /* Kxv */
+                    Kel[(2 * nod + i) * nod + j] += d21 * W;   /* Kyu */
+                    Kel[(3 * nod + i) * nod + j] += d22 * W;   /* Kyv */
+                }
+            }
+            const double t1x = -g * (rho * h - rhow * d) * dbdx * N[i] * ca + rho * g * h * sa * N[i];
+            const double t2x = 0.5 * g * ca * (rho * h * h - rhow * d * d) * Nx[i];
+            const double t3x = h * eta * (4 * exx + 2 * eyy) * Nx[i];
+            const double t4x = h * eta * 2 * exy * Ny[i];
+            Tx[i] += (t3x + t4x + taux * N[i]) * W;
+            Fx[i] += (t1x + t2x) * W;
+            const double t1y = -g * (rho * h - rhow * d) * dbdy * N[i] * ca;
+            const double t2y = 0.5 * ca * g * (rho * h * h - rhow * d * d) * Ny[i];
+            const double t3y = h * eta * (4 * eyy + 2 * exx) * Ny[i];
+            const double t4y = h * eta * 2 * exy * Nx[i];
+            Ty[i] += (t3y + t4y + tauy * N[i]) * W;
+            Fy[i] += (t1y + t2y) * W;
+        }
+    }
+    for (int i = 0; i < nod; ++i) {
+        Rel[4 * i + 0] = Tx[i]; Rel[4 * i + 1] = Fx[i];
+        Rel[4 * i + 2] = Ty[i]; Rel[4 * i + 3] = Fy[i];
+    }
+}
+
+int o2_uv_assemble(const o2_params *P, double cmin, double aglenmin, double dvisk, int nele, int nod, int64_t nn,
+                   const int32_t *conn, const double *x, const double *y, int nip, const double *pts,
+                   const double *wts, const double *const *f, const int64_t *rowptr_n, const int32_t *colind_n,
+                   double *R, double *Tint, double *Fext, double *Kval, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    const int want_k = !P->ronly && Kval;
+    double *Rel = malloc(sizeof(double) * (size_t)nele * nod * 4);
+    double *Kel = want_k ? malloc(sizeof(double) * (size_t)nele * 4 * nod * nod) : NULL;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t e = 0; e < nele; ++e) {
+        double xn[MAXNOD], yn[MAXNOD], nv[NUV][MAXNOD];
+        for (int a = 0; a < nod; ++a) {
+            const int32_t k = conn[e * nod + a];
+            xn[a] = x[k]; yn[a] = y[k];
+            for (int fi = 0; fi < NUV; ++fi) nv[fi][a] = f[fi][k];
+        }
+        uv_element(P, cmin, aglenmin, dvisk, nod, nip, pts, wts, xn, yn, (const double (*)[MAXNOD])nv,
+                   Rel + (size_t)e * nod * 4, want_k ? Kel + (size_t)e * 4 * nod * nod : NULL);
+    }
+    /* R: element contributions summed per node in element order */
+    double *T = calloc((size_t)nn * 2, sizeof(double)), *F = calloc((size_t)nn * 2, sizeof(double));
+    for (int64_t e = 0; e < nele; ++e)
+        for (int a = 0; a < nod; ++a) {
+            const int32_t k = conn[e * nod + a];
+            const double *re = Rel + ((size_t)e * nod + a) * 4;
+            T[k] += re[0]; F[k] += re[1]; T[nn + k] += re[2]; F[nn + k] += re[3];
+        }
+    int bad = 0;
+    for (int64_t r = 0; r < 2 * nn; ++r) {
+        R[r] = T[r] - F[r];
+        if (Tint) Tint[r] = T[r];
+        if (Fext) Fext[r] = F[r];
+        if (isnan(R[r])) bad = 1;
+    }
+    if (want_k) {
+        const int64_t nnzN = rowptr_n[nn];
+        double *Kraw = calloc((size_t)(4 * nnzN), sizeof(double));
+        for (int64_t e = 0; e < nele; ++e)
+            for (int a = 0; a < nod; ++a) {
+                const int32_t k = conn[e * nod + a];
+                const int64_t deg = rowptr_n[k + 1] - rowptr_n[k];
+                const int32_t *cols = colind_n + rowptr_n[k];
+                for (int b = 0; b < nod; ++b) {
+                    const int64_t pos = find_col(cols, deg, conn[e * nod + b]);
+                    for (int bi = 0; bi < 2; ++bi)
+                        for (int bj = 0; bj < 2; ++bj)
+                            Kraw[(int64_t)bi * 2 * nnzN + 2 * rowptr_n[k] + bj * deg + pos] +=
+                                Kel[((size_t)e * 4 + bi * 2 + bj) * nod * nod + a * nod + b];
+                }
+            }
+        /* (K + K')/2: entry (bi, k; bj, c) pairs with (bj, c; bi, k) */
+#pragma omp parallel for schedule(dynamic, 512)
+        for (int64_t k = 0; k < nn; ++k) {
+            const int64_t deg = rowptr_n[k + 1] - rowptr_n[k];
+            for (int bi = 0; bi < 2; ++bi)
+                for (int bj = 0; bj < 2; ++bj)
+                    for (int64_t t = 0; t < deg; ++t) {
+                        const int32_t c = colind_n[rowptr_n[k] + t];
+                        const int64_t degc = rowptr_n[c + 1] - rowptr_n[c];
+                        const int64_t tp = find_col(colind_n + rowptr_n[c], degc, (int32_t)k);
+                        const double a = Kraw[(int64_t)bi * 2 * nnzN + 2 * rowptr_n[k] + bj * deg + t];
+                        const double b = Kraw[(int64_t)bj * 2 * nnzN + 2 * rowptr_n[c] + bi * degc + tp];
+                        Kval[(int64_t)bi * 2 * nnzN + 2 * rowptr_n[k] + bj * deg + t] = 0.5 * (a + b);
+                    }
+        }
+        free(Kraw);
+    }
+    free(Rel); free(Kel); free(T); free(F);
+    return bad;
+}

@@ -1,0 +1,100 @@
+"""The uv (diagnostic SSTREAM) oracle, o2.assemble_uv (UaMain/uvMatrixAssemblySSTREAM.m),
+pinned three ways that do not retype its formulas:
+
+* against the uvh oracle (itself pinned by tests/test_oracle_invariants.py and the golden
+  vectors): with the surface s = b + h from the uvh floatation pre-pass, the uv momentum
+  equations are the uvh momentum equations, so R_uv equals the first two block rows of R_uvh
+  and K_uv the (u, v) blocks of K_uvh (the two restatements follow different reference files:
+  uvMatrixAssemblySSTREAM.m and uvhNodalLoopSSTREAM.m);
+* K is the Jacobian of R: central differences along random directions in (u, v);
+* K is exactly symmetric (the reference's (K + K')/2), and the pattern is the uvh pattern's
+  (u, v) part.
+Plus the uv-only options: C and AGlen are clipped at Cmin and AGlenmin, and NRviscosity = 0
+(Picard) drops the viscosity-derivative terms, leaving R unchanged."""
+import numpy as np
+import pytest
+
+from uasource_amd.ctrlvar import default_ctrlvar
+from uasource_amd.fields import WORKLOADS, build_mesh, make_fields
+
+
+def _setup(o2lib, name):
+    w = WORKLOADS[name]
+    M = build_mesh(w)
+    F0, F1, _ = make_fields(M, w)
+    cv = default_ctrlvar()
+    ref = o2lib.assemble(cv, M, F0, F1)          # leaves the nodal b of the floatation pre-pass in mesh.b
+    F = F1.copy()
+    F.s = ref["mesh"].b + F1.h
+    return cv, M, F, ref
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3", "medium_p3"])
+def test_uv_equals_the_uvh_momentum_part(o2lib, name):
+    cv, M, F, ref = _setup(o2lib, name)
+    uv = o2lib.assemble_uv(cv, M, F, mesh=ref["mesh"])
+    n2 = 2 * M.Nnodes
+    scale = max(np.linalg.norm(uv["Tint"]), np.linalg.norm(uv["Fext"]))
+    assert np.linalg.norm(uv["R"] - ref["R"][:n2]) <= 1e-13 * scale
+    assert np.linalg.norm(uv["Tint"] - ref["Tint"][:n2]) <= 1e-13 * scale
+    Kuvh = ref["K"][:n2, :n2]
+    Kuv = uv["K"]
+    assert abs(Kuv - Kuvh).max() <= 1e-13 * abs(Kuvh).max()
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3"])
+def test_uv_K_is_jacobian_of_R(o2lib, name):
+    cv, M, F, ref = _setup(o2lib, name)
+    K = o2lib.assemble_uv(cv, M, F, mesh=ref["mesh"])["K"]
+    Nn = M.Nnodes
+
+    def R(x):
+        G = F.copy()
+        G.ub, G.vb = x[:Nn], x[Nn:]
+        return o2lib.assemble_uv(cv, M, G, mesh=ref["mesh"], ronly=True)["R"]
+    x = np.concatenate([F.ub, F.vb])
+    rng = np.random.default_rng(4)
+    for _ in range(2):
+        d = 10.0 * rng.standard_normal(2 * Nn)
+        eps = 1e-4
+        fd = (R(x + eps * d) - R(x - eps * d)) / (2 * eps)
+        assert np.linalg.norm(fd - K @ d) / np.linalg.norm(K @ d) < 1e-8
+
+
+@pytest.mark.parametrize("name", ["tiny_p2", "cpu_reference"])
+def test_uv_K_is_symmetric_on_the_uvh_pattern(o2lib, name):
+    cv, M, F, ref = _setup(o2lib, name)
+    K = o2lib.assemble_uv(cv, M, F, mesh=ref["mesh"])["K"]
+    assert (K != K.T).nnz == 0
+    n2 = 2 * M.Nnodes
+    P = ref["K"][:n2, :n2].copy()
+    P.data[:] = 1.0
+    Q = K.copy()
+    Q.data[:] = 1.0
+    assert (P != Q).nnz == 0
+    rowptr, colind = o2lib.uv_csr_pattern(ref["mesh"])
+    assert np.array_equal(rowptr, K.indptr) and np.array_equal(colind, K.indices)
+
+
+def test_uv_clips_and_picard(o2lib):
+    cv, M, F, ref = _setup(o2lib, "tiny_p2")
+    base = o2lib.assemble_uv(cv, M, F, mesh=ref["mesh"])
+    # C and AGlen below the floors act as the floors
+    G = F.copy()
+    G.C = np.full(M.Nnodes, -1.0)
+    G.AGlen = np.full(M.Nnodes, 0.0)
+    c = cv.copy()
+    c.Cmin, c.AGlenmin = 2e-2, 3e-9
+    a = o2lib.assemble_uv(c, M, G, mesh=ref["mesh"])
+    H = F.copy()
+    H.C = np.full(M.Nnodes, 2e-2)
+    H.AGlen = np.full(M.Nnodes, 3e-9)
+    b = o2lib.assemble_uv(c, M, H, mesh=ref["mesh"])
+    assert np.array_equal(a["R"], b["R"]) and np.array_equal(a["Kval"], b["Kval"])
+    # Picard: same R, K without the viscosity-derivative terms (smaller in norm, still symmetric)
+    p = cv.copy()
+    p.Picard = True
+    pk = o2lib.assemble_uv(p, M, F, mesh=ref["mesh"])
+    assert np.array_equal(pk["R"], base["R"])
+    assert not np.array_equal(pk["Kval"], base["Kval"])
+    assert (pk["K"] != pk["K"].T).nnz == 0

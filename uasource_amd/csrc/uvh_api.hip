@@ -14,6 +14,7 @@
 
 namespace uvh {
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, int *status, hipStream_t s);
+hipError_t launch_nodal_s(int nn, const double *sfc, const double *h, double *b, int *status, hipStream_t s);
 hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
@@ -21,7 +22,7 @@ hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool f
 hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
-                       const int32_t *blocks, int nblk, hipStream_t s);
+                       const int32_t *blocks, int nblk, hipStream_t s, int nbk = 3);
 int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
 hipError_t launch_rows_gather(int nod, int nn, int64_t nnzN, bool fp32, const int32_t *nzrow, const int64_t *rowptr_n,
                               const int32_t *inc_ptr, const int64_t *imap_ptr, const uint8_t *imap,
@@ -111,6 +112,7 @@ struct uvh_ctx {
     uvh::CallArgs last_args{};
     // work
     double *d_b = nullptr, *d_RE = nullptr, *d_EB = nullptr;
+    double *d_zero = nullptr;   // uv assembly: zero nodal field (lazily allocated)
     int epb[2] = {0, 0};    // elements per block of the element kernel: [FP64, FP32]
     // one-pass element rows (all nine blocks per thread) for 3- and 6-node elements: faster with
     // FP32 arithmetic; with FP64 the three-pass rows keep the registers for two waves per SIMD.
@@ -141,8 +143,8 @@ struct uvh_ctx {
         hipFree(d_diag); hipFree(d_pbp); hipFree(d_belem); hipFree(d_easlot); hipFree(d_zptr); hipFree(d_zslot);
         hipFree(d_zlen); hipFree(d_fptr); hipFree(d_fnode); hipFree(d_fslot); hipFree(d_earank); hipFree(d_bmax);
         hipFree(d_margs); hipFree(d_pos_inc); hipFree(d_inc_pos);
-        hipFree(d_nzrow); hipFree(d_imap_ptr); hipFree(d_imap);
-        d_nzrow = nullptr; d_imap_ptr = nullptr; d_imap = nullptr;
+        hipFree(d_nzrow); hipFree(d_imap_ptr); hipFree(d_imap); hipFree(d_zero);
+        d_nzrow = nullptr; d_imap_ptr = nullptr; d_imap = nullptr; d_zero = nullptr;
         for (int t = 0; t < 2; ++t) {
             hipFree(d_plan_blocks[t]);
             d_plan_blocks[t] = nullptr;
@@ -203,6 +205,7 @@ uvh::Params make_params(const uvh_ctrl &c, uint32_t flags) {
     p.zero_fields = (flags & UVH_ZEROFIELDS) ? 1 : 0;
     p.reset_thickness = c.reset_thickness;
     p.enforce_b_above = c.enforce_b_above_B;
+    p.cmin = 0.0; p.aglenmin = 0.0; p.dvisk = 1.0; p.uv = 0;
     return p;
 }
 
@@ -474,6 +477,31 @@ int uvh_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind) {
     return UVH_OK;
 }
 
+int uvh_uv_pattern_info(const uvh_ctx *ctx, int64_t *neq, int64_t *nnz) {
+    if (!ctx || !ctx->has_mesh) return UVH_E_STATE;
+    if (neq) *neq = 2 * int64_t(ctx->mesh.nn);
+    if (nnz) *nnz = 4 * ctx->mesh.nnzN();
+    return UVH_OK;
+}
+
+int uvh_uv_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind) {
+    if (!ctx || !ctx->has_mesh) return UVH_E_STATE;
+    if (!rowptr || !colind) return UVH_E_ARG;
+    const uvh::MeshHost &m = ctx->mesh;
+    const int64_t nn = m.nn, nnzN = m.nnzN();
+    for (int bi = 0; bi < 2; ++bi)
+        for (int64_t k = 0; k < nn; ++k) {
+            const int64_t start = int64_t(bi) * 2 * nnzN + 2 * m.rowptr_n[k];
+            const int64_t deg = m.rowptr_n[k + 1] - m.rowptr_n[k];
+            rowptr[bi * nn + k] = start;
+            for (int bj = 0; bj < 2; ++bj)
+                for (int64_t t = 0; t < deg; ++t)
+                    colind[start + bj * deg + t] = int32_t(m.colind_n[m.rowptr_n[k] + t] + bj * nn);
+        }
+    rowptr[2 * nn] = 4 * nnzN;
+    return UVH_OK;
+}
+
 int uvh_get_csc_map(const uvh_ctx *ctx, int64_t *map) {
     if (!ctx || !ctx->has_mesh) return UVH_E_STATE;
     if (!map) return UVH_E_ARG;
@@ -629,6 +657,103 @@ int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *hf, uint3
     if (dK) chk(hipMemcpyAsync(Kval, dK, size_t(9) * m.nnzN() * sizeof(double), hipMemcpyDeviceToHost, s));
     chk(hipStreamSynchronize(s));
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: download");
+    if (flags & UVH_NO_NAN_CHECK) return UVH_OK;
+    return uvh_sync_status(ctx);
+}
+
+int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *df,
+                           uint32_t flags, double *dR, double *dK, double *dT, double *dF, void *stream) {
+    if (!ctx) return UVH_E_ARG;
+    if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_uv_assemble: no mesh (call uvh_set_mesh first)");
+    if (!ctrl || !uvc || !df || !dR) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: null ctrl, fields or R");
+    for (const double *q : {df->h, df->s, df->ub, df->vb, df->S, df->B, df->rho, df->C, df->m, df->n, df->AGlen})
+        if (!q) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: a required field pointer is null");
+    if (flags & UVH_ZEROFIELDS)
+        return ctx->fail(UVH_E_UNSUPPORTED, "uvh_uv_assemble: ZeroFields (pass the zeroed velocities instead)");
+    if (!ctx->two_pass) return ctx->fail(UVH_E_UNSUPPORTED, "uvh_uv_assemble: needs the two-pass assembly");
+    const bool fp32 = (flags & UVH_FP32) != 0, ronly = (flags & UVH_RONLY) != 0;
+    if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: K output required unless UVH_RONLY");
+    hipSetDevice(ctx->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uvh::MeshHost &m = ctx->mesh;
+    if (!ronly && !ctx->d_EB &&
+        dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * uvh::element_row_stride(m.nod, false)) != hipSuccess)
+        return ctx->fail(UVH_E_NOMEM, "uvh_uv_assemble: cannot allocate element rows");
+    if (!ctx->d_zero) {   // stands in for the thickness-equation fields the element gather reads
+        if (dalloc(&ctx->d_zero, size_t(m.nn)) != hipSuccess ||
+            hipMemset(ctx->d_zero, 0, size_t(m.nn) * sizeof(double)) != hipSuccess)
+            return ctx->fail(UVH_E_NOMEM, "uvh_uv_assemble: cannot allocate the zero field");
+    }
+    uvh::Params p = make_params(*ctrl, flags & ~uint32_t(UVH_ZEROFIELDS));
+    p.uv = 1;
+    p.reset_thickness = 0;
+    p.enforce_b_above = 0;
+    p.cmin = uvc->cmin;
+    p.aglenmin = uvc->aglenmin;
+    p.dvisk = uvc->nrviscosity;
+    const double *z = ctx->d_zero;
+    const uvh::FieldPtrs f{df->h, df->ub, df->vb, df->S, df->B, df->rho, df->C, df->m, df->n, df->AGlen, z, z,
+                           df->h, df->ub, df->vb, z, z, nullptr};
+    const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
+    const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
+    const int t = fp32 ? 1 : 0;
+    hipError_t e = ctx->timed("nodal_s", s, [&] { return uvh::launch_nodal_s(m.nn, df->s, df->h, ctx->d_b, ctx->d_status, s); });
+    if (e == hipSuccess)
+        e = ctx->timed(fp32 ? (ronly ? "uv_element32_r" : "uv_element32_rk") : (ronly ? "uv_element_r" : "uv_element_rk"),
+                       s, [&] {
+                           return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], fp32, full, ctx->d_conn,
+                                                      ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p,
+                                                      ctx->d_inc_pos, ronly, ctx->d_EB, ctx->d_RE, 0, -1, s);
+                       });
+    if (e == hipSuccess)
+        e = ctx->timed(fp32 ? (ronly ? "uv_rows32_r" : "uv_rows32_rk") : (ronly ? "uv_rows_r" : "uv_rows_rk"), s, [&] {
+            return uvh::launch_rows(m.nod, m.nn, m.max_block_deg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc,
+                                    ctx->d_rowptr_n, ctx->d_diag, m.nnzN(), ctx->d_EB, ctx->d_RE, dR, dT, dF,
+                                    ronly ? nullptr : dK, check, ctx->d_status, nullptr, 0, s, 2);
+        });
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_uv_assemble: kernel launch");
+    return UVH_OK;
+}
+
+int uvh_uv_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *hf, uint32_t flags,
+                    double *R, double *Kval, double *Tint, double *Fext) {
+    if (!ctx) return UVH_E_ARG;
+    if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_uv_assemble: no mesh (call uvh_set_mesh first)");
+    if (!ctrl || !uvc || !hf || !R) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: null ctrl, fields or R");
+    const double *src[11] = {hf->h, hf->s, hf->ub, hf->vb, hf->S, hf->B, hf->rho, hf->C, hf->m, hf->n, hf->AGlen};
+    for (auto q : src)
+        if (!q) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: a required field pointer is null");
+    const bool ronly = (flags & UVH_RONLY) != 0;
+    hipSetDevice(ctx->device);
+    const uvh::MeshHost &m = ctx->mesh;
+    const size_t nn = size_t(m.nn);
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    if (!ctx->d_fields) chk(dalloc(&ctx->d_fields, 18 * nn));
+    if (!ctx->d_R) chk(dalloc(&ctx->d_R, 3 * nn));
+    if (!ctx->d_T) chk(dalloc(&ctx->d_T, 3 * nn));
+    if (!ctx->d_F) chk(dalloc(&ctx->d_F, 3 * nn));
+    if (!ronly && Kval && !ctx->d_K) chk(dalloc(&ctx->d_K, size_t(9) * m.nnzN()));
+    if (e != hipSuccess) return ctx->fail(UVH_E_NOMEM, "uvh_uv_assemble: device allocation failed");
+    hipStream_t s = ctx->own_stream;
+    const double *dst[11];
+    for (int k = 0; k < 11; ++k) {
+        double *d = ctx->d_fields + k * nn;
+        dst[k] = d;
+        chk(hipMemcpyAsync(d, src[k], nn * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_uv_assemble: field upload");
+    const uvh_uv_fields df{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8], dst[9], dst[10]};
+    double *dK = (!ronly && Kval) ? ctx->d_K : nullptr;
+    if (!ronly && !Kval) flags |= UVH_RONLY;
+    int st = uvh_uv_assemble_device(ctx, ctrl, uvc, &df, flags, ctx->d_R, dK, ctx->d_T, ctx->d_F, s);
+    if (st != UVH_OK) return st;
+    chk(hipMemcpyAsync(R, ctx->d_R, 2 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (Tint) chk(hipMemcpyAsync(Tint, ctx->d_T, 2 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (Fext) chk(hipMemcpyAsync(Fext, ctx->d_F, 2 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (dK) chk(hipMemcpyAsync(Kval, dK, size_t(4) * m.nnzN() * sizeof(double), hipMemcpyDeviceToHost, s));
+    chk(hipStreamSynchronize(s));
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_uv_assemble: download");
     if (flags & UVH_NO_NAN_CHECK) return UVH_OK;
     return uvh_sync_status(ctx);
 }

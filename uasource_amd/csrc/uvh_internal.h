@@ -128,6 +128,10 @@ FusedLayout fused_layout(int nod, int nip);
 struct Params {
     double dt, theta, beta0, tau_mult, uz, eps0, eta0, czero, kH, Hh0, g, rhow, ca, sa, thick_min;
     int32_t tau_opt, zero_fields, reset_thickness, enforce_b_above;
+    // uv (diagnostic) assembly: uv != 0 clips C and AGlen at cmin and aglenmin and skips the
+    // thickness rows; dvisk scales the viscosity-derivative terms (NRviscosity; 1 for uvh)
+    double cmin, aglenmin, dvisk;
+    int32_t uv, pad;
 };
 
 // Device pointers of the nodal fields, in the order of uvh_fields (dadh may be null).

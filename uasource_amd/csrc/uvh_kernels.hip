@@ -73,6 +73,17 @@ __global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b,
     b[k] = bb;
 }
 
+// uv assembly: the lower surface at the nodes from the given upper surface, b = s - h (the
+// points interpolate s and h, uvMatrixAssemblySSTREAM.m:219); clears the NaN flags like
+// k_nodal_b.
+__global__ void k_nodal_s(int nn, const double *__restrict__ s, const double *__restrict__ h,
+                          double *__restrict__ b, int *__restrict__ status) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0 && status) *status = 0;
+    if (k >= nn) return;
+    b[k] = s[k] - h[k];
+}
+
 template <int NOD>
 __device__ __forceinline__ void corner_ids(int &c0, int &c1, int &c2) {
     if (NOD == 3) { c0 = 0; c1 = 1; c2 = 2; }
@@ -217,6 +228,10 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     }
     const T h = T(hd), h0 = T(h0d), rho = T(rhod);
     const T exy = T(0.5) * exy2;
+    if (p.uv) {   // uvMatrixAssemblySSTREAM.m:185,211
+        if (C < T(p.cmin)) C = T(p.cmin);
+        if (A < T(p.aglenmin)) A = T(p.aglenmin);
+    }
     const T a1 = as1 + ab1, a0 = as0 + ab0;
 
     // floatation (uvhAssemblyIntPointImplicitSUPG.m:209-245), FP64, and the combinations of
@@ -289,7 +304,7 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
 
     c[C_I11] = i11; c[C_I12] = i12; c[C_I21] = i21; c[C_I22] = i22;
     c[C_TU] = T(0.5) * tau0 * u0; c[C_TV] = T(0.5) * tau0 * v0;
-    c[C_WHETA] = W * h * eta; c[C_WHE2] = T(2) * W * h * E;
+    c[C_WHETA] = W * h * eta; c[C_WHE2] = T(2) * W * h * E * T(p.dvisk);
     c[C_S1] = T(2) * exx + eyy; c[C_S2] = T(2) * eyy + exx; c[C_EXY] = exy;
     c[C_WDTXDU] = W * dtxdu; c[C_WDTYDV] = W * dtydv; c[C_WDTXDV] = W * dtxdv;
     const T c2 = ca * g * hyd4;
@@ -606,7 +621,8 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     }
     // stage 3: element rows, as (block row, element, row) items dealt round-robin to the
     // threads; each item accumulates 3 x NOD Jacobian terms and its T and F over the points
-    for (int it = tid; it < 3 * nrow; it += 256) {
+    const int nbr = p.uv ? 2 : 3;   // the uv assembly has no thickness rows
+    for (int it = tid; it < nbr * nrow; it += 256) {
         const int bi = it / nrow;
         const int rr = it - bi * nrow;
         const int el = rr / NOD, i = rr - el * NOD;
@@ -653,7 +669,9 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
 // bitwise reproducible. The buffer is then written to K, contiguous along the rows. Element
 // rows of type TE (float in the precision form) are accumulated in FP64.
 
-template <int NOD, typename TE>
+// NBK = 3: the uvh system (u, v, h rows and columns); NBK = 2: the uv system, which reads
+// the (u, v) part of the same element rows and writes R (2 Nn) and the 2 x 2-block K.
+template <int NOD, typename TE, int NBK = 3>
 __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const int32_t *__restrict__ inc_ptr,
                                            const uint8_t *__restrict__ pos, const int64_t *__restrict__ rowptr_n,
                                            const uint8_t *__restrict__ diag, int64_t nnzN,
@@ -668,13 +686,13 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     const int k = k0 + slot;   // the block's nodes are consecutive
     if (k >= nn) return;  // whole lane group; no barriers or cross-group shuffles below
     const int64_t rp = rowptr_n[k];
-    // the block's row buffers are packed: node k's starts at 9 * (rowptr_n[k] - rowptr_n[k0]);
+    // the block's row buffers are packed: node k's starts at NBK^2 (rowptr_n[k] - rowptr_n[k0]);
     // the launch provides the largest such total over the blocks
-    double *buf = lds + 9 * (rp - rowptr_n[k0]);
+    double *buf = lds + NBK * NBK * (rp - rowptr_n[k0]);
     const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
     const int deg = (int)(rowptr_n[k + 1] - rp);
     int bad = 0;
-    if (gl < 6) {
+    if (gl < 2 * NBK) {
         // lane 2*bi: T of block row bi, lane 2*bi+1: F
         double v = 0;
         for (int t = t0; t < t1; ++t) v += double(RE[(int64_t)t * 6 + gl]);
@@ -691,10 +709,12 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
             if (check_nan && isnan(r)) bad |= 1;
         }
     }
-    for (int v = gl; v < 9 * deg; v += kRowLanes) buf[v] = 0.0;
+    for (int v = gl; v < NBK * NBK * deg; v += kRowLanes) buf[v] = 0.0;
     // the node's element rows t0..t1-1 are contiguous (incidence order); up to 4 at a time:
-    // all loads first, then the adds in ascending element order
-    constexpr int NV = 9 * NOD;                           // values per element row
+    // all loads first, then the adds in ascending element order. Value u of the NBK^2 x NOD
+    // used ones is block (bi, bj) = (u / (NBK NOD), u / NOD mod NBK), column j = u mod NOD,
+    // at (3 bi + bj) NOD + j of the element row
+    constexpr int NV = NBK * NBK * NOD;                   // values used per element row
     constexpr int VPL = (NV + kRowLanes - 1) / kRowLanes; // values per lane
     constexpr int CH = 4;
     for (int c0 = t0; c0 < t1; c0 += CH) {
@@ -708,7 +728,8 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                 const int v = gl + kRowLanes * u;
                 if (c < cnt && v < NV) {
                     const int bk = v / NOD, j = v - bk * NOD;
-                    vals[c][u] = EB[(int64_t)(c0 + c) * eb_stride<NOD, TE>() + v];
+                    const int src = NBK == 3 ? v : ((bk / NBK) * 3 + bk % NBK) * NOD + j;
+                    vals[c][u] = EB[(int64_t)(c0 + c) * eb_stride<NOD, TE>() + src];
                     cols[c][u] = pos[(int64_t)(c0 + c) * NOD + j];
                 }
             }
@@ -720,23 +741,23 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                 const int v = gl + kRowLanes * u;
                 if (c < cnt && v < NV) {
                     const int bk = v / NOD;
-                    const int bi = bk / 3, bj = bk - bi * 3;
-                    lds_add(buf + bi * 3 * deg + bj * deg + cols[c][u], double(vals[c][u]));
+                    const int bi = bk / NBK, bj = bk - bi * NBK;
+                    lds_add(buf + bi * NBK * deg + bj * deg + cols[c][u], double(vals[c][u]));
                 }
             }
         }
     }
 #pragma unroll
-    for (int bi = 0; bi < 3; ++bi) {
-        double *dst = Kval + (int64_t)bi * 3 * nnzN + 3 * rp;
-        const double *src = buf + bi * 3 * deg;
-        for (int v = gl; v < 3 * deg; v += kRowLanes) dst[v] = src[v];
+    for (int bi = 0; bi < NBK; ++bi) {
+        double *dst = Kval + (int64_t)bi * NBK * nnzN + NBK * rp;
+        const double *src = buf + bi * NBK * deg;
+        for (int v = gl; v < NBK * deg; v += kRowLanes) dst[v] = src[v];
     }
-    if (check_nan && gl < 3 && isnan(buf[gl * 3 * deg + gl * deg + diag[k]])) bad |= 2;
+    if (check_nan && gl < NBK && isnan(buf[gl * NBK * deg + gl * deg + diag[k]])) bad |= 2;
     if (bad) atomicOr(status, bad);
 }
 
-template <int NOD, typename TE>
+template <int NOD, typename TE, int NBK = 3>
 __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict__ inc_ptr,
                                               const uint8_t *__restrict__ pos,
                                               const int64_t *__restrict__ rowptr_n, const uint8_t *__restrict__ diag,
@@ -750,8 +771,8 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
     const int i = xcd_block(blockIdx.x, gridDim.x);
     if (i >= nblk) return;
     const int k0 = (blocks ? blocks[i] : i) * row_nodes_per_block(NOD);
-    rows_block<NOD, TE>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval, check_nan,
-                        status);
+    rows_block<NOD, TE, NBK>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval,
+                             check_nan, status);
 }
 
 // Node rows, gather form: one thread per node-graph entry (node n, column c). It sums the
@@ -814,23 +835,34 @@ __global__ __launch_bounds__(256) void k_rows_gather(int nn, int64_t nnzN, const
 
 // Residual only: one thread per node sums its incident element residual rows (contiguous).
 template <typename TE>
-__global__ __launch_bounds__(256) void k_rows_r(int nn, const int32_t *__restrict__ inc_ptr,
+__global__ __launch_bounds__(256) void k_rows_r(int nn, int nbk, const int32_t *__restrict__ inc_ptr,
                                                 const TE *__restrict__ RE, double *__restrict__ R,
                                                 double *__restrict__ Tint, double *__restrict__ Fext, int check_nan,
                                                 int *__restrict__ status) {
+    // nbk block rows: 3 (uvh) or 2 (uv, whose element rows carry no thickness terms)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nn) return;
-    double T0 = 0, T1 = 0, T2 = 0, F0 = 0, F1 = 0, F2 = 0;
+    double T[3] = {0, 0, 0}, F[3] = {0, 0, 0};
     for (int t = inc_ptr[k]; t < inc_ptr[k + 1]; ++t) {
         const TE *re = RE + (int64_t)t * 6;
-        T0 += double(re[0]); F0 += double(re[1]); T1 += double(re[2]);
-        F1 += double(re[3]); T2 += double(re[4]); F2 += double(re[5]);
+#pragma unroll
+        for (int bi = 0; bi < 3; ++bi)
+            if (bi < nbk) {
+                T[bi] += double(re[2 * bi]);
+                F[bi] += double(re[2 * bi + 1]);
+            }
     }
-    const double r0 = T0 - F0, r1 = T1 - F1, r2 = T2 - F2;
-    R[k] = r0; R[nn + k] = r1; R[2 * (int64_t)nn + k] = r2;
-    if (Tint) { Tint[k] = T0; Tint[nn + k] = T1; Tint[2 * (int64_t)nn + k] = T2; }
-    if (Fext) { Fext[k] = F0; Fext[nn + k] = F1; Fext[2 * (int64_t)nn + k] = F2; }
-    if (check_nan && (isnan(r0) || isnan(r1) || isnan(r2))) atomicOr(status, 1);
+    bool bad = false;
+#pragma unroll
+    for (int bi = 0; bi < 3; ++bi)
+        if (bi < nbk) {
+            const double r = T[bi] - F[bi];
+            R[(int64_t)bi * nn + k] = r;
+            if (Tint) Tint[(int64_t)bi * nn + k] = T[bi];
+            if (Fext) Fext[(int64_t)bi * nn + k] = F[bi];
+            bad = bad || isnan(r);
+        }
+    if (check_nan && bad) atomicOr(status, 1);
 }
 
 // ------------------------------------------------------------------ batched assembly
@@ -1255,6 +1287,11 @@ hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b
     return hipGetLastError();
 }
 
+hipError_t launch_nodal_s(int nn, const double *sfc, const double *h, double *b, int *status, hipStream_t s) {
+    hipLaunchKernelGGL(k_nodal_s, dim3((nn + 255) / 256), dim3(256), 0, s, nn, sfc, h, b, status);
+    return hipGetLastError();
+}
+
 // Element tiles lb0 .. lb1-1 (tiles of epb elements); lb1 < 0 means all tiles.
 hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
@@ -1282,24 +1319,25 @@ size_t element_row_stride(int nod, bool) { return size_t(9 * nod); }
 hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
-                       const int32_t *blocks, int nblk, hipStream_t s) {
+                       const int32_t *blocks, int nblk, hipStream_t s, int nbk) {
+    if (nbk != 2 && nbk != 3) return hipErrorInvalidValue;
     if (!Kval) {
         if (fp32)
-            hipLaunchKernelGGL(k_rows_r<float>, dim3((nn + 255) / 256), dim3(256), 0, s, nn, inc_ptr,
+            hipLaunchKernelGGL(k_rows_r<float>, dim3((nn + 255) / 256), dim3(256), 0, s, nn, nbk, inc_ptr,
                                static_cast<const float *>(RE), R, Tint, Fext, check_nan, status);
         else
-            hipLaunchKernelGGL(k_rows_r<double>, dim3((nn + 255) / 256), dim3(256), 0, s, nn, inc_ptr,
+            hipLaunchKernelGGL(k_rows_r<double>, dim3((nn + 255) / 256), dim3(256), 0, s, nn, nbk, inc_ptr,
                                static_cast<const double *>(RE), R, Tint, Fext, check_nan, status);
         return hipGetLastError();
     }
     const int nblocks = blocks ? nblk : (nn + row_nodes_per_block(nod) - 1) / row_nodes_per_block(nod);
     if (nblocks <= 0) return hipSuccess;
     const dim3 grid(((nblocks + kXcds - 1) / kXcds) * kXcds), block(256);
-    const size_t lds = size_t(9) * size_t(block_vals) * sizeof(double);
+    const size_t lds = size_t(nbk * nbk) * size_t(block_vals) * sizeof(double);
     if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
 #define UVH_ROWS(NODV, TE)                                                                                     \
     do {                                                                                                       \
-        auto kern = k_rows<NODV, TE>;                                                                          \
+        auto kern = nbk == 3 ? k_rows<NODV, TE, 3> : k_rows<NODV, TE, 2>;                                      \
         lds_attr(reinterpret_cast<const void *>(kern), lds);                                                   \
         hipLaunchKernelGGL(kern, grid, block, lds, s, nn, inc_ptr, pos, rowptr_n, diag, nnzN,                   \
                            static_cast<const TE *>(EB), static_cast<const TE *>(RE), R, Tint, Fext, Kval,      \

@@ -84,6 +84,13 @@ def default_ctrlvar() -> Struct:
         uvhDesiredWorkOrForceTolerances=[float("inf"), 1e-15],    # :498
         uvhExitBackTrackingStepLength=1e-3,       # :502
         uvhAcceptableWorkAndForceTolerances=[float("inf"), 1e-6], # :504
+        # uv (diagnostic) assembly, uvMatrixAssemblySSTREAM.m
+        Cmin=1e-20,                               # :458 (FlowApproximation "SSTREAM")
+        AGlenmin=1e-20,                           # :463
+        NRviscosity=1,                            # :594
+        Picard=False,                             # :593
+        uvMatrixAssembly=S(Ronly=False),
+        uvAssembly=S(ZeroFields=False),
         AsymmSolver="Auto",                       # :656 (the device solver is the 'EliminateBCsSolveSystemIterativly' branch)
         LinSolveTol=1e-8,                         # :659, relative residual of iterative linear solves
     )

@@ -35,7 +35,8 @@ TAU_OPTIONS = {"taus": 0, "taut": 1, "tau1": 2, "tau2": 3, "tau0": 4}
 # Every entry point declared in include/uvh.h.
 EXPORTED = ("uvh_create", "uvh_destroy", "uvh_last_error", "uvh_set_mesh", "uvh_pattern_info",
             "uvh_get_pattern", "uvh_get_csc_map", "uvh_assemble", "uvh_assemble_device", "uvh_sync_status",
-            "uvh_profile", "uvh_kernel_times")
+            "uvh_profile", "uvh_kernel_times", "uvh_uv_pattern_info", "uvh_uv_get_pattern", "uvh_uv_assemble",
+            "uvh_uv_assemble_device")
 
 
 class UaError(RuntimeError):
@@ -64,6 +65,37 @@ class UvhCtrl(C.Structure):
 class UvhFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("h", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "as_", "ab",
                                           "h0", "ub0", "vb0", "as0", "ab0", "dadh")]
+
+
+class UvhUvFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("h", "s", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen")]
+
+
+class UvhUvCtrl(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("cmin", "aglenmin", "nrviscosity")]
+
+
+UV_KEYS = ("h", "s", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen")
+
+
+def make_uv_ctrl(CtrlVar) -> UvhUvCtrl:
+    """CtrlVar.Cmin, AGlenmin and NRviscosity (Dvisk: 0 with CtrlVar.Picard,
+    uvMatrixAssemblySSTREAM.m:72-76)."""
+    return UvhUvCtrl(float(CtrlVar.Cmin), float(CtrlVar.AGlenmin),
+                     0.0 if CtrlVar.Picard else float(CtrlVar.NRviscosity))
+
+
+def make_uv_fields(F, keep=None) -> UvhUvFields:
+    """Field pointer block of the uv assembly (host arrays or device tensors)."""
+    f = UvhUvFields()
+    for k in UV_KEYS:
+        a = getattr(F, k)
+        if isinstance(a, np.ndarray):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+        if keep is not None:
+            keep.append(a)
+        setattr(f, k, _ptr(a))
+    return f
 
 
 _lib = None
@@ -104,6 +136,16 @@ def load_library(path: str = LIB_PATH):
     lib.uvh_assemble.restype = C.c_int
     lib.uvh_assemble_device.argtypes = [vp, C.POINTER(UvhCtrl), C.POINTER(UvhFields), C.c_uint32, dp, dp, dp, dp, vp]
     lib.uvh_assemble_device.restype = C.c_int
+    lib.uvh_uv_pattern_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+    lib.uvh_uv_pattern_info.restype = C.c_int
+    lib.uvh_uv_get_pattern.argtypes = [vp, vp, vp]
+    lib.uvh_uv_get_pattern.restype = C.c_int
+    lib.uvh_uv_assemble.argtypes = [vp, C.POINTER(UvhCtrl), C.POINTER(UvhUvCtrl), C.POINTER(UvhUvFields), C.c_uint32,
+                                    dp, dp, dp, dp]
+    lib.uvh_uv_assemble.restype = C.c_int
+    lib.uvh_uv_assemble_device.argtypes = [vp, C.POINTER(UvhCtrl), C.POINTER(UvhUvCtrl), C.POINTER(UvhUvFields),
+                                           C.c_uint32, dp, dp, dp, dp, vp]
+    lib.uvh_uv_assemble_device.restype = C.c_int
     lib.uvh_sync_status.argtypes = [vp]
     lib.uvh_sync_status.restype = C.c_int
     lib.uvh_profile.argtypes = [vp, C.c_int]
@@ -297,6 +339,44 @@ class UvhContext:
 
     def sync_status(self):
         self._check(self.lib.uvh_sync_status(self.h), "uvh_sync_status")
+
+    # ---------------------------------------------------------------- uv assembly
+    def uv_pattern(self):
+        """(rowptr int64[2Nn+1], colind int32[4 nnzN]) of the uv K."""
+        neq, nnz = C.c_int64(), C.c_int64()
+        self._check(self.lib.uvh_uv_pattern_info(self.h, C.byref(neq), C.byref(nnz)), "uvh_uv_pattern_info")
+        rowptr = np.zeros(neq.value + 1, dtype=np.int64)
+        colind = np.zeros(nnz.value, dtype=np.int32)
+        self._check(self.lib.uvh_uv_get_pattern(self.h, _ptr(rowptr), _ptr(colind)), "uvh_uv_get_pattern")
+        return rowptr, colind
+
+    def assemble_uv(self, CtrlVar, F, ronly=False, want_tf=False, nan_check=True, fp32=False):
+        """uv (diagnostic) assembly from host arrays (``F`` with the upper surface ``s``);
+        returns dict(R (2Nn), Kval (4 nnzN) or None, Tint, Fext)."""
+        ctrl = make_ctrl(CtrlVar, F)
+        uvc = make_uv_ctrl(CtrlVar)
+        keep = []
+        fields = make_uv_fields(F, keep)
+        flags = (UVH_RONLY if ronly else 0) | (0 if nan_check else UVH_NO_NAN_CHECK) | (UVH_FP32 if fp32 else 0)
+        nnzN = self.nnz // 9
+        R = np.zeros(2 * self.nn)
+        Kval = None if ronly else np.zeros(4 * nnzN)
+        Tint = np.zeros(2 * self.nn) if want_tf else None
+        Fext = np.zeros(2 * self.nn) if want_tf else None
+        st = self.lib.uvh_uv_assemble(self.h, C.byref(ctrl), C.byref(uvc), C.byref(fields), flags, _ptr(R),
+                                      _ptr(Kval), _ptr(Tint), _ptr(Fext))
+        self._check(st, "uvh_uv_assemble")
+        return dict(R=R, Kval=Kval, Tint=Tint, Fext=Fext)
+
+    def assemble_uv_device(self, ctrl: UvhCtrl, uvc: UvhUvCtrl, fields: UvhUvFields, flags: int, R, K=None,
+                           Tint=None, Fext=None, stream=None):
+        st = self.lib.uvh_uv_assemble_device(self.h, C.byref(ctrl), C.byref(uvc), C.byref(fields), int(flags),
+                                             _ptr(R), _ptr(K), _ptr(Tint), _ptr(Fext), stream)
+        self._check(st, "uvh_uv_assemble_device")
+
+    def uv_K_matrix(self, Kval):
+        rowptr, colind = self.uv_pattern()
+        return sp.csr_matrix((Kval, colind, rowptr), shape=(2 * self.nn, 2 * self.nn))
 
     def profile(self, enable: bool):
         self._check(self.lib.uvh_profile(self.h, int(enable)), "uvh_profile")
