@@ -11,7 +11,7 @@ stream, and the oracle's C restatement timed on this host's cores beside it.
 
 Other forms of the path, one JSON line each: ``--batch`` (64 meshes in one persistent
 launch per stage), ``--serve`` (500 requests, latency percentiles over a load sweep),
-``--newton`` (a full Newton solve).
+``--newton`` (a full Newton solve), ``--uv`` (the uv diagnostic assembly).
 """
 from __future__ import annotations
 
@@ -79,6 +79,7 @@ def main(argv=None):
     ap.add_argument("--serve", action="store_true", help="request serving, latency over a load sweep")
     ap.add_argument("--newton", action="store_true",
                     help="full Newton solve (newton_p2 with the host solver; --workload with --device-solver)")
+    ap.add_argument("--uv", action="store_true", help="uv (diagnostic) assembly of the config-2 mesh")
     ap.add_argument("--device-solver", action="store_true",
                     help="with --newton: linear solves on the GPU (block ILU(0) BiCGStab)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -101,13 +102,15 @@ def main(argv=None):
               file=sys.stderr)
         return 2
 
-    if args.batch or args.serve or args.newton:
+    if args.batch or args.serve or args.newton or args.uv:
         from uasource_amd import bench_modes
         log = lambda m: print(m, file=sys.stderr, flush=True)
         if args.batch:
             line = bench_modes.bench_batch(args.steps, args.warmup, log=log)
         elif args.serve:
             line = bench_modes.bench_serve(log=log)
+        elif args.uv:
+            line = bench_modes.bench_uv(args.steps, args.warmup, cpu_seconds=args.cpu_seconds, log=log)
         else:
             line = bench_modes.bench_newton("newton_p2" if args.workload == "plain_p2" and not args.device_solver
                                             else args.workload, device_solver=args.device_solver, log=log)

@@ -236,3 +236,79 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
                    else "SciPy SuperLU (host)", "parallelism": "1 GPU"},
         "fp64": out["fp64"], "fp32_form": out["fp32"], "fp32_form_rel_err_vs_fp64": err,
     }
+
+
+def bench_uv(steps: int, warmup: int, workload: str = "plain_p2", cpu_seconds: float = 10.0, log=print) -> dict:
+    """uv (diagnostic SSTREAM) assembly, R and the 2 x 2-block K, of the config-2 mesh (same
+    fields as the uvh workload, s from the floatation of h): FP64 and the precision form, with
+    the oracle's C restatement of uvMatrixAssemblySSTREAM timed on the host beside it."""
+    import os
+    import torch
+    from .fields import surface_from_floatation
+    dev = torch.device("cuda:0")
+    w = WORKLOADS[workload]
+    M = build_mesh(w)
+    _, F1, _ = make_fields(M, w)
+    cv = default_ctrlvar()
+    F = F1.copy()
+    F.s = surface_from_floatation(F1, cv)
+    ctx = uvh.UvhContext(0)
+    ctx.set_mesh(M)
+    keep = []
+
+    class _F:
+        pass
+    fd = _F()
+    for k in uvh.UV_KEYS:
+        t = torch.from_numpy(np.ascontiguousarray(getattr(F, k), dtype=np.float64)).to(dev)
+        keep.append(t)
+        setattr(fd, k, t)
+    fields = uvh.make_uv_fields(fd)
+    ctrl, uvc = uvh.make_ctrl(cv, F), uvh.make_uv_ctrl(cv)
+    nnzN = ctx.nnz // 9
+    R = torch.zeros(2 * M.Nnodes, dtype=torch.float64, device=dev)
+    K = torch.zeros(4 * nnzN, dtype=torch.float64, device=dev)
+    s = torch.cuda.current_stream()
+    out = {}
+    for fp32 in (False, True):
+        flags = uvh.UVH_NO_NAN_CHECK | (uvh.UVH_FP32 if fp32 else 0)
+        run = lambda: ctx.assemble_uv_device(ctrl, uvc, fields, flags, R, K, stream=s.cuda_stream)  # noqa: E731
+        for _ in range(warmup):
+            run()
+        ms = _time(torch, run, steps, s)
+        ctx.profile(True)            # per-kernel times from a separate pass
+        _time(torch, run, steps, s)
+        kt = {k: 1e3 * v[0] / max(v[1], 1) for k, v in ctx.kernel_times().items()}
+        ctx.profile(False)
+        out["fp32" if fp32 else "fp64"] = {"ms": ms, "elements_per_s": M.Nele / (ms * 1e-3),
+                                           "kernels_us": {k: round(v, 1) for k, v in kt.items()}}
+        log(f"uv {'fp32' if fp32 else 'fp64'}: {ms:.3f} ms, {M.Nele / (ms * 1e-3) / 1e6:.1f} M el/s, kernels {kt}")
+    # algorithmic bytes: 11 nodal inputs and 2 outputs (FP64), connectivity, K written once
+    nbytes = M.Nnodes * (11 + 2) * 8 + M.Nele * M.nod * 4 + 4 * nnzN * 8
+    ms64 = out["fp64"]["ms"]
+    cpu = None
+    try:
+        from oracle import o2
+        threads = min(16, os.cpu_count() or 1)
+        o2.assemble_uv(cv, M, F, nthreads=threads)      # warm
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < cpu_seconds / 2 or n == 0:
+            o2.assemble_uv(cv, M, F, nthreads=threads)
+            n += 1
+        cpu = {"value": n * M.Nele / (time.perf_counter() - t0), "unit": "elements/s", "cores": threads,
+               "kind": "port", "sample": f"{n} full uv assemblies of {workload}"}
+    except Exception as exc:   # the oracle may be absent where only the library travels
+        log(f"uv CPU baseline skipped: {exc}")
+    ctx.close()
+    return {
+        "metric": "uv elements assembled/s (R+K), diagnostic SSTREAM", "value": M.Nele / (ms64 * 1e-3),
+        "unit": "elements/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": ms64,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered-Delaunay mesh, config-2 fields, s from floatation)",
+        "config": {"workload": f"uv_{workload}", "Nele": M.Nele, "Nnodes": M.Nnodes, "nod": M.nod, "nip": M.nip,
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": nbytes / (ms64 * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": nbytes / (ms64 * 1e-3) / 1e9 / 8000.0, "traffic": None},
+        "fp64": out["fp64"], "fp32_form": out["fp32"], "cpu_baseline": cpu,
+    }

@@ -161,3 +161,12 @@ def batch_workloads(n: int = 64, seed: int = 4, lo: int = 5000, hi: int = 80000,
         out.append(Workload(f"{name}{k}", 100e3, 50e3, 2 * cells_y + 1, cells_y + 1, t[0], t[1],
                             seed=10_000 + seed * 1000 + k, C_sigma=C_sigma, A_sigma=A_sigma))
     return out
+
+
+def surface_from_floatation(F, CtrlVar):
+    """Upper surface s = b + h with b from the floatation of h over the bed B and sea level S
+    (UaMain/Calc_bs_From_hBS.m:71-79): the input the uv assembly takes as ``F.s``."""
+    hf = F.rhow * (F.S - F.B) / F.rho
+    GF = 0.5 + 0.5 * np.tanh(CtrlVar.kH * ((F.h - hf) - CtrlVar.Hh0))
+    b = GF * F.B + (1.0 - GF) * (F.S - F.rho * F.h / F.rhow)
+    return b + F.h
