@@ -170,11 +170,15 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
         srv.run(reqs)
         lat = np.array([r.latency for r in reqs])
         slow = np.array([r.latency / alone[(r.mesh, r.newton)] for r in reqs])
+        svc = np.array([alone[(r.mesh, r.newton)] for r in reqs])   # the mix's own service times
         span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
         el = sum(meshes[r.mesh][0].Nele for r in reqs)
         row = {"load": frac, "rate_per_s": frac * sat_rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                "p99_ms": float(np.percentile(lat, 99) * 1e3), "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
                "slowdown_p50": float(np.percentile(slow, 50)), "slowdown_p99": float(np.percentile(slow, 99)),
+               "service_alone_p50_ms": float(np.percentile(svc, 50) * 1e3),
+               "service_alone_p99_ms": float(np.percentile(svc, 99) * 1e3),
+               "service_alone_p99_over_p50": float(np.percentile(svc, 99) / np.percentile(svc, 50)),
                "elements_per_s": el / span, "peak_inflight": srv.peak_inflight}
         sweep.append(row)
         log(f"serve load {frac}: p50 {row['p50_ms']:.3f} ms p99 {row['p99_ms']:.3f} ms, slowdown p50 "

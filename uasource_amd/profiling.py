@@ -38,14 +38,24 @@ PLAIN_KERNELS = {"k_nodal_b": "nodal_b"}
 COPY_BYTES = 1 << 30
 
 
+ANON = "(anonymous namespace)::"
+
+
+def signature_name(full: str) -> str:
+    """Kernel name without its argument list; the "(anonymous namespace)::" qualifier of
+    file-local kernels is kept."""
+    head = full.replace(ANON, "\0")
+    return head.split("(")[0].replace("\0", ANON).strip()
+
+
 def base_name(full: str) -> str:
-    return full.split("(")[0].split("<")[0].replace("void ", "").replace("uvh::", "").strip()
+    return signature_name(full).split("<")[0].replace("void ", "").replace("uvh::", "").replace(ANON, "").strip()
 
 
 def timer_name(full: str, ronly: bool) -> str | None:
     base = base_name(full)
     if base in MODE_KERNELS:
-        fp32 = "float" in full.split("(")[0] and base in ("k_element", "k_rows", "k_rows_r")
+        fp32 = "float" in signature_name(full) and base in ("k_element", "k_rows", "k_rows_r")
         return MODE_KERNELS[base] + ("32" if fp32 else "") + ("_r" if ronly else "_rk")
     return PLAIN_KERNELS.get(base)
 
@@ -59,7 +69,7 @@ def per_kernel(path: str, counter: str) -> dict:
     for r in rows:
         if r["Counter_Name"] != counter:
             continue
-        name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+        name = signature_name(r["Kernel_Name"]).replace("void ", "").strip()
         tot[name] += float(r["Counter_Value"])
         cnt[name].add(r["Dispatch_Id"])
     return {k: tot[k] / len(cnt[k]) for k in tot}
@@ -100,7 +110,7 @@ def stats_md(trace_dir: str, workload: str, command: str) -> str:
              "Command (repository root on the GPU host):", "", "```", command, "```", "",
              "| Kernel | Calls | Average (us) | Min (us) | Max (us) | Share |", "|---|---|---|---|---|---|"]
     for r in rows:
-        name = r["Name"].split("(")[0].strip()
+        name = signature_name(r["Name"])
         lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f}% |")
     return "\n".join(lines) + "\n"
