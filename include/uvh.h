@@ -177,7 +177,8 @@ int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl
  *
  * uvh_lin_create: for the mesh of `ctx` (set_mesh first; a later set_mesh invalidates the
  *   solver); `fixed` is a host mask of 3*Nn bytes in [ub; vb; h] order (nonzero = fixed), or
- *   NULL. Analyses the pattern once.
+ *   NULL. Analyses the pattern once. Every node must belong to an element (a node in none has
+ *   an empty row, K is singular there): UVH_E_ARG otherwise.
  * uvh_lin_factor: block ILU(0) of the device K values `dKval` (the context's CSR order, as
  *   uvh_assemble_device writes them). UVH_E_NOCONV on a zero pivot.
  * uvh_lin_solve: dx = K^-1 drhs (device vectors of 3*Nn in [ub; vb; h] order) to

@@ -442,7 +442,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
         for (int bi = 0; bi < 3; ++bi) {
             const double *row = Kval + (int64_t)bi * 3 * nnzN + 3 * rowptr_n[k];
             int64_t p = find_col(cols, deg, (int32_t)k);
-            if (isnan(row[bi * deg + p])) bad |= 2;
+            if (p >= 0 && isnan(row[bi * deg + p])) bad |= 2;   /* p < 0: a node no element references */
         }
     }
     free(ip); free(inc); free(fill); free(zf); free(zbuf);

@@ -285,3 +285,35 @@ def test_pipelined_assembly_matches_single_pass(monkeypatch, name, chunks):
     finally:
         ref.close()
         c2.close()
+
+
+@pytest.mark.parametrize("where", ["middle", "end"])
+def test_node_in_no_element(o2lib, where):
+    """A node that belongs to no element (mid-numbering, and last, i.e. last in its row block):
+    empty K rows, zero R rows, no NaN status, parity with the oracle elsewhere; the same for
+    the uv assembly."""
+    from test_oracle_invariants import mesh_with_isolated_node
+    print(f"[gpu] node in no element ({where})", flush=True)
+    w = WORKLOADS["tiny_p2"]
+    M = build_mesh(w)
+    F0, F1, _ = make_fields(M, w)
+    pos = M.Nnodes // 2 if where == "middle" else M.Nnodes
+    M2, G0, G1 = mesh_with_isolated_node(M, F0, F1, pos)
+    cv = default_ctrlvar()
+    ref = o2lib.assemble(cv, M2, G0, G1)
+    ctx = uvh.UvhContext(0)
+    try:
+        ctx.set_mesh(M2)
+        gpu = ctx.assemble(cv, G0, G1, want_tf=True)   # raises on a (false) NaN status
+        assert_parity(ctx, M2, gpu, ref)
+        n2 = M2.Nnodes
+        assert all(gpu["R"][b * n2 + pos] == 0.0 for b in range(3))
+        G = G1.copy()
+        G.s = ref["mesh"].b + G1.h
+        uv_ref = o2lib.assemble_uv(cv, M2, G, mesh=ref["mesh"])
+        uv = ctx.assemble_uv(cv, G)
+        scale = max(np.linalg.norm(uv_ref["Tint"]), np.linalg.norm(uv_ref["Fext"]))
+        assert np.linalg.norm(uv["R"] - uv_ref["R"]) <= 1e-12 * scale
+        assert np.linalg.norm(uv["Kval"] - uv_ref["Kval"]) <= 1e-12 * np.linalg.norm(uv_ref["Kval"])
+    finally:
+        ctx.close()

@@ -392,3 +392,42 @@ def test_o2_planted_mistake_is_caught(o2lib):
     b = o2lib.assemble(cv, M, F0, F1b)
     scale = max(np.linalg.norm(a["Tint"]), np.linalg.norm(a["Fext"]))
     assert np.linalg.norm(a["R"] - b["R"]) > 1e-12 * scale
+
+
+def mesh_with_isolated_node(M, F0, F1, pos):
+    """The mesh with one more node at index ``pos`` that no element references (and fields
+    extended with copies of a neighbour's values)."""
+    import dataclasses
+    coords = np.insert(M.coordinates, pos, M.coordinates[min(pos, M.Nnodes - 1)] + 1.0, axis=0)
+    conn = M.connectivity + (M.connectivity >= pos)
+    M2 = dataclasses.replace(M, coordinates=coords, connectivity=conn)
+    G0, G1 = F0.copy(), F1.copy()
+    for F, src in ((G0, F0), (G1, F1)):
+        for k, v in vars(src).items():
+            if isinstance(v, np.ndarray) and v.shape == (M.Nnodes,):
+                setattr(F, k, np.insert(v, pos, v[min(pos, M.Nnodes - 1)]))
+    return M2, G0, G1
+
+
+@pytest.mark.parametrize("where", ["middle", "end"])
+def test_node_in_no_element_gives_empty_rows(o2lib, where):
+    """A node that belongs to no element: its rows of R are zero and of K empty, the status
+    is clean (uvhMatrixAssembly's sparse K has an empty row there), and every other row is the
+    one of the mesh without the node."""
+    cv = default_ctrlvar()
+    M, F0, F1, _ = _mesh_fields("tiny_p2")
+    pos = M.Nnodes // 2 if where == "middle" else M.Nnodes
+    M2, G0, G1 = mesh_with_isolated_node(M, F0, F1, pos)
+    ref = o2lib.assemble(cv, M, F0, F1)
+    out = o2lib.assemble(cv, M2, G0, G1)
+    assert out["status"] == 0
+    n, n2 = M.Nnodes, M2.Nnodes
+    keep = np.delete(np.arange(n2), pos)
+    for b in range(3):
+        assert out["R"][b * n2 + pos] == 0.0
+        assert np.array_equal(out["R"][b * n2 + keep], ref["R"][b * n:(b + 1) * n])
+    K2 = out["K"]
+    rows = [b * n2 + pos for b in range(3)]
+    assert all(K2.indptr[r + 1] == K2.indptr[r] for r in rows)
+    idx = np.concatenate([b * n2 + keep for b in range(3)])
+    assert abs(K2[idx][:, idx] - ref["K"]).max() == 0.0

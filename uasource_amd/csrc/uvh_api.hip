@@ -350,6 +350,8 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     if (msg.empty()) msg = uvh::build_mesh_host(m);
     ctx->layout = uvh::fused_layout(nod, nip);
     ctx->sched = uvh::Schedule{};
+    if (msg.empty() && !ctx->two_pass && m.has_isolated)
+        msg = "the fused assembly (UVH_ASSEMBLY=fused) needs every node in an element";
     if (msg.empty() && !ctx->two_pass) {  // the fused path's patch schedule
         const char *pe = std::getenv("UVH_PATCHES");
         const int target = pe ? std::max(1, std::atoi(pe)) : 256;
@@ -570,7 +572,9 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     };
     auto rows = [&](const int32_t *blocks, int nblk, hipStream_t st) {
         return ctx->timed(row_name, st, [&] {
-            if (ctx->gather_rows && !ronly)
+            // the gather form has one thread per node-graph entry, so a node in no element
+            // (no entries) would keep a stale R: such meshes use the scatter form
+            if (ctx->gather_rows && !ronly && !m.has_isolated)
                 return uvh::launch_rows_gather(m.nod, m.nn, m.nnzN(), fp32, ctx->d_nzrow, ctx->d_rowptr_n,
                                                ctx->d_inc_ptr, ctx->d_imap_ptr, ctx->d_imap, ctx->d_diag, ctx->d_EB,
                                                ctx->d_RE, dR, dT, dF, dK, check, ctx->d_status, st);

@@ -55,6 +55,7 @@ struct MeshHost {
     std::vector<int64_t> imap_ptr;    // nn+1
     std::vector<uint8_t> imap;
     int32_t maxdeg = 0;               // largest node degree
+    bool has_isolated = false;        // some node belongs to no element (its rows are empty)
     int64_t max_block_deg = 0;        // largest sum of degrees over row_nodes_per_block(nod) consecutive nodes
     ShapeTables tab{};
     int64_t nnzN() const { return rowptr_n.empty() ? 0 : rowptr_n.back(); }

@@ -753,7 +753,8 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
         const double *src = buf + bi * NBK * deg;
         for (int v = gl; v < NBK * deg; v += kRowLanes) dst[v] = src[v];
     }
-    if (check_nan && gl < NBK && isnan(buf[gl * NBK * deg + gl * deg + diag[k]])) bad |= 2;
+    // a node no element references has an empty row (R = 0, no diagonal to check)
+    if (check_nan && deg > 0 && gl < NBK && isnan(buf[gl * NBK * deg + gl * deg + diag[k]])) bad |= 2;
     if (bad) atomicOr(status, bad);
 }
 

@@ -120,6 +120,7 @@ std::string build_mesh_host(MeshHost &m) {
         if (deg[size_t(k) + 1] > 255) return "node degree above 255 (per-element placement uses 8 bits)";
         m.rowptr_n[size_t(k) + 1] = m.rowptr_n[k] + deg[size_t(k) + 1];
         m.maxdeg = std::max<int32_t>(m.maxdeg, int32_t(deg[size_t(k) + 1]));
+        if (deg[size_t(k) + 1] == 0) m.has_isolated = true;
     }
     const int64_t nb = row_nodes_per_block(nod);
     for (int64_t k0 = 0; k0 < nn; k0 += nb)
