@@ -317,3 +317,27 @@ def test_node_in_no_element(o2lib, where):
         assert np.linalg.norm(uv["Kval"] - uv_ref["Kval"]) <= 1e-12 * np.linalg.norm(uv_ref["Kval"])
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("tri", [3, 6, 10])
+def test_single_element_mesh(o2lib, tri):
+    """One element: a single tile, a single row block and a dense pattern."""
+    from uasource_amd.mesh import CreateMUA
+    print(f"[gpu] single {tri}-node element", flush=True)
+    M = CreateMUA(np.array([[1000.0, 2000.0], [4000.0, 2500.0], [1500.0, 5000.0]]), np.array([[0, 1, 2]]),
+                  TriNodes=tri)
+    F0, F1, _ = make_fields(M, WORKLOADS["tiny_p2"])
+    cv = default_ctrlvar()
+    ref = o2lib.assemble(cv, M, F0, F1)
+    ctx = uvh.UvhContext(0)
+    try:
+        ctx.set_mesh(M)
+        for fp32 in (False, True):
+            gpu = ctx.assemble(cv, F0, F1, want_tf=True, fp32=fp32)
+            if not fp32:
+                assert_parity(ctx, M, gpu, ref)
+            else:
+                scale = max(np.linalg.norm(ref["Tint"]), np.linalg.norm(ref["Fext"]))
+                assert np.linalg.norm(gpu["R"] - ref["R"]) <= 1e-6 * scale
+    finally:
+        ctx.close()

@@ -12,13 +12,18 @@ Scheduling: a request is admitted when
   * its outputs (R, and K for a Newton step) fit in what is left of the HBM budget for
     in-flight outputs, and
   * no other request on the same mesh is in flight (they share the mesh's work buffers);
-requests that arrive while others run are admitted as soon as slots and memory free up, in
-arrival order but without letting a blocked request hold back later ones that fit (a waiting
-request is passed over at most ``max_bypass`` times). Admitted requests run concurrently on
-their streams, so their kernels overlap on the device.
+requests that arrive while others run are admitted as soon as slots and memory free up.
+Waiting requests are taken in arrival order (``policy="fifo"``, the default), later ones that
+fit passing a blocked one, or shortest expected service first (mesh elements, x 1.6 for a
+Newton step's R and K; ``policy="sjf"``); either way nothing passes a request that later
+arrivals have passed ``max_bypass`` times. (Measured at 70 % load on a 2k-128k-element mix,
+sjf did not lower the latency tail: the small requests' tail comes from sharing the device
+with large kernels, not from queueing.) Admitted requests run concurrently on their streams,
+so their kernels overlap on the device.
 """
 from __future__ import annotations
 
+import gc
 import time
 from dataclasses import dataclass, field
 
@@ -75,8 +80,10 @@ class _Mesh:
 
 
 class UvhServer:
+    kWindow = 32   # waiting requests considered per admission round, earliest arrivals first
+
     def __init__(self, CtrlVar, meshes, slots: int = 8, hbm_budget: float = 16e9, device: int = 0,
-                 fp32: bool = False, max_bypass: int = 16, keep_outputs: bool = True):
+                 fp32: bool = False, max_bypass: int = 16, keep_outputs: bool = True, policy: str = "fifo"):
         """``meshes``: list of (MUA, F0, F1). ``hbm_budget``: bytes for in-flight outputs."""
         import torch
         self.torch = torch
@@ -93,6 +100,9 @@ class UvhServer:
         self.flags = uvh.UVH_FP32 if fp32 else 0
         self.max_bypass = max_bypass
         self.keep_outputs = keep_outputs
+        if policy not in ("sjf", "fifo"):
+            raise ValueError(f"unknown admission policy {policy!r}")
+        self.policy = policy
         self.peak_used = 0.0
         self.peak_inflight = 0
 
@@ -119,7 +129,18 @@ class UvhServer:
 
     def run(self, requests: list[Request], clock=time.perf_counter) -> list[Request]:
         """Serves ``requests`` in real time (arrival times from the call); returns them completed.
-        A request whose outputs exceed the whole budget is refused (ValueError)."""
+        A request whose outputs exceed the whole budget is refused (ValueError). Python's cyclic
+        garbage collector is paused while serving (a collection over the requests' tensors can
+        hold the polling loop for milliseconds)."""
+        was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            return self._run(requests, clock)
+        finally:
+            if was_enabled:
+                gc.enable()
+
+    def _run(self, requests: list[Request], clock) -> list[Request]:
         for r in requests:
             if self.meshes[r.mesh].out_bytes(r.newton) > self.budget:
                 raise ValueError(f"request {r.rid}: outputs exceed the HBM budget of {self.budget:.3g} bytes")
@@ -152,9 +173,19 @@ class UvhServer:
                 else:
                     still.append(item)
             inflight = still
-            # admission: arrival order, skipping requests that do not fit yet (bounded)
+            # admission. fifo (default): arrival order, skipping requests that do not fit yet; a
+            # skipped request counts as overtaken when a later one is admitted, and nothing
+            # passes a request overtaken max_bypass times. sjf: the kWindow earliest waiting
+            # requests by expected service (requests overtaken max_bypass times first); the
+            # window bounds the host work per admission and holds every request a later
+            # arrival can overtake
             admitted, skipped = [], []
-            for req in waiting:
+            if self.policy == "sjf" and free_slots and waiting:
+                window = waiting[:self.kWindow]
+                order = sorted(window, key=lambda r: (r.bypassed < self.max_bypass, self._cost(r), r.arrival))
+            else:
+                window, order = None, waiting
+            for req in order:
                 if not free_slots:
                     break
                 m = self.meshes[req.mesh]
@@ -164,8 +195,14 @@ class UvhServer:
                         break   # no one overtakes a request that has been passed over enough
                     skipped.append(req)
                     continue
-                for w in skipped:   # each of these is overtaken once more
-                    w.bypassed += 1
+                if window is None:
+                    for w in skipped:   # each of these is overtaken once more
+                        w.bypassed += 1
+                else:
+                    taken = {id(a) for a in admitted}
+                    for w in window:    # every earlier arrival still waiting is overtaken once more
+                        if w.arrival < req.arrival and w is not req and id(w) not in taken:
+                            w.bypassed += 1
                 slot = free_slots.pop()
                 m.busy = True
                 used += need
@@ -184,6 +221,10 @@ class UvhServer:
                     time.sleep(dt - 1e-4)
         torch.cuda.synchronize()
         return requests
+
+    def _cost(self, req: Request) -> float:
+        """Expected service time, in element units (R and K cost ~1.6x R alone)."""
+        return self.meshes[req.mesh].nele * (1.6 if req.newton else 1.0)
 
     def serve_alone(self, req: Request) -> Request:
         """The same request with nothing else on the device (for the mix-vs-alone check)."""
