@@ -96,6 +96,9 @@ class UvhServer:
         nnz = max(m.ctx.nnz for m in self.meshes)
         self.slot_R = [torch.empty(n3, dtype=torch.float64, device=self.dev) for _ in range(slots)]
         self.slot_K = [torch.empty(nnz, dtype=torch.float64, device=self.dev) for _ in range(slots)]
+        # one completion event per slot, recorded again for each request the slot takes (its
+        # previous request has completed by then)
+        self.slot_ev = [torch.cuda.Event() for _ in range(slots)]
         self.budget = hbm_budget
         self.flags = uvh.UVH_FP32 if fp32 else 0
         self.max_bypass = max_bypass
@@ -105,6 +108,8 @@ class UvhServer:
         self.policy = policy
         self.peak_used = 0.0
         self.peak_inflight = 0
+        self.stall_threshold = 2e-3
+        self.stalls: list[float] = []   # host stalls of the last run, seconds
 
     # one request on one stream: state upload, assembly, result download
     def _launch(self, req: Request, slot: int):
@@ -123,7 +128,7 @@ class UvhServer:
             K = self.slot_K[slot][:m.ctx.nnz] if req.newton else None
             flags = self.flags | (0 if req.newton else uvh.UVH_RONLY) | uvh.UVH_NO_NAN_CHECK
             m.ctx.assemble_device(m.ctrl, m.fields, flags, R, K, None, None, stream=s.cuda_stream)
-            ev = torch.cuda.Event()
+            ev = self.slot_ev[slot]
             ev.record(s)
         return (req, slot, R, K, ev)
 
@@ -152,8 +157,17 @@ class UvhServer:
         used = 0.0
         t0 = clock()
         i = 0
+        self.stalls = []
+        prev = 0.0
         while i < len(pending) or waiting or inflight:
             now = clock() - t0
+            # a pass of this loop takes microseconds; one that took longer than stall_threshold
+            # was held up inside a call (a result download with keep_outputs, or, rarely, a
+            # state upload or kernel launch that held the host for 6-13 ms), which delays every
+            # request in the system
+            if now - prev > self.stall_threshold:
+                self.stalls.append(now - prev)
+            prev = now
             while i < len(pending) and pending[i].arrival <= now:
                 waiting.append(pending[i])
                 i += 1
@@ -219,6 +233,7 @@ class UvhServer:
                 dt = pending[i].arrival - (clock() - t0)
                 if dt > 2e-4:
                     time.sleep(dt - 1e-4)
+                    prev = clock() - t0
         torch.cuda.synchronize()
         return requests
 
