@@ -56,6 +56,18 @@ enum uvh_flags {
 
 enum uvh_supg_tau { UVH_TAU_S = 0, UVH_TAU_T = 1, UVH_TAU_1 = 2, UVH_TAU_2 = 3, UVH_TAU_0 = 4 };
 
+/* CtrlVar.SlidingLaw (UaMain/BasalDrag.m:182-257): "Weertman"/"W"; "Budd"/"W-N0" (needs q);
+ * "Coulomb"/"C" (needs muk); "Tsai"/"minCW-N0" (needs muk: the smaller in magnitude of the
+ * Weertman and Coulomb drag at each point). The effective pressure of the last three is N0
+ * (BasalDrag.m:500-521): rho g max(h - hf, 0) + CtrlVar.Nzero. Other laws return
+ * UVH_E_UNSUPPORTED. */
+enum uvh_sliding_law {
+    UVH_SLIDING_WEERTMAN = 0,
+    UVH_SLIDING_BUDD = 1,
+    UVH_SLIDING_COULOMB = 2,
+    UVH_SLIDING_TSAI = 3
+};
+
 /* Scalars the assembly reads from CtrlVar and F (defaults: UaMain/Ua2D_DefaultParameters.m). */
 typedef struct {
     double dt;                  /* CtrlVar.dt */
@@ -75,15 +87,18 @@ typedef struct {
     int32_t supg_tau;           /* CtrlVar.uvh.SUPG.tau as enum uvh_supg_tau */
     int32_t reset_thickness;    /* ResetThicknessToMinThickness && ResetThicknessInNonLinLoop */
     int32_t enforce_b_above_B;  /* CtrlVar.Enforce_bAboveB */
-    int32_t reserved;
+    int32_t sliding_law;        /* CtrlVar.SlidingLaw as enum uvh_sliding_law */
+    double nzero;               /* CtrlVar.Nzero (1e-20), lower bound of the effective pressure */
 } uvh_ctrl;
 
 /* Nodal fields of F1 (current) and F0 (previous time step); each nnodes long.
- * dadh may be NULL (zero: MassBalanceGeometryFeedback < 2). */
+ * dadh may be NULL (zero: MassBalanceGeometryFeedback < 2). q (F1.q, Budd) and muk (F1.muk,
+ * Coulomb and Tsai) are read only by the sliding laws that use them (NULL otherwise). */
 typedef struct {
     const double *h, *ub, *vb, *S, *B, *rho, *C, *m, *n, *AGlen, *as, *ab;
     const double *h0, *ub0, *vb0, *as0, *ab0;
     const double *dadh;
+    const double *q, *muk;
 } uvh_fields;
 
 /* Context on HIP device `device` (one per process and device; not thread-safe). */
@@ -146,6 +161,7 @@ int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t
  * as ub, vb directly. */
 typedef struct {
     const double *h, *s, *ub, *vb, *S, *B, *rho, *C, *m, *n, *AGlen;
+    const double *q, *muk;   /* as in uvh_fields */
 } uvh_uv_fields;
 
 typedef struct {

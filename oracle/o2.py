@@ -24,7 +24,18 @@ class O2Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dt", "theta", "beta0", "tau_multiplier", "speed_zero", "eps_zero",
                                           "eta_zero", "czero", "kH", "Hh0", "g", "rhow", "alpha")] + \
                [("tau_option", C.c_int), ("ronly", C.c_int), ("zero_fields", C.c_int),
-                ("reset_thickness", C.c_int), ("thick_min", C.c_double)]
+                ("reset_thickness", C.c_int), ("thick_min", C.c_double), ("law", C.c_int), ("nzero", C.c_double)]
+
+# CtrlVar.SlidingLaw (UaMain/BasalDrag.m:182-257) -> the restatement's law code
+SLIDING_LAWS = {"Weertman": 0, "W": 0, "Budd": 1, "W-N0": 1, "Coulomb": 2, "C": 2, "Tsai": 3, "minCW-N0": 3}
+
+
+def _optional(F, k):
+    """F.q / F.muk as a contiguous array, or None when F does not carry it."""
+    a = getattr(F, k, None)
+    if a is None or np.size(a) == 0:
+        return None
+    return np.ascontiguousarray(a, dtype=np.float64)
 
 
 def build(force: bool = False) -> str:
@@ -91,6 +102,8 @@ def params_from(CtrlVar, F1, ronly=None, zero_fields=None) -> O2Params:
     P.zero_fields = int(CtrlVar.uvhMatrixAssembly.ZeroFields if zero_fields is None else zero_fields)
     P.reset_thickness = int(bool(CtrlVar.ResetThicknessToMinThickness) and bool(CtrlVar.ResetThicknessInNonLinLoop))
     P.thick_min = CtrlVar.ThickMin
+    P.law = SLIDING_LAWS[str(CtrlVar.get("SlidingLaw", "Weertman"))]
+    P.nzero = float(CtrlVar.get("Nzero", 1e-20))
     return P
 
 
@@ -143,8 +156,9 @@ def assemble(CtrlVar, MUA, F0, F1, mesh: O2Mesh | None = None, nthreads: int = 0
     nele, nod, nn = MUA.Nele, MUA.nod, MUA.Nnodes
     f1 = [np.ascontiguousarray(getattr(F1, k), dtype=np.float64)
           for k in ("h", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "as_", "ab")]
+    f1 += [_optional(F1, "q"), _optional(F1, "muk")]
     f0 = [np.ascontiguousarray(getattr(F0, k), dtype=np.float64) for k in ("h", "ub", "vb", "as_", "ab")]
-    f1p = (C.c_void_p * 12)(*[a.ctypes.data for a in f1])
+    f1p = (C.c_void_p * 14)(*[None if a is None else a.ctypes.data for a in f1])
     f0p = (C.c_void_p * 5)(*[a.ctypes.data for a in f0])
     R = np.zeros(3 * nn)
     Tint = np.zeros(3 * nn)
@@ -198,7 +212,8 @@ def assemble_uv(CtrlVar, MUA, F, mesh: O2Mesh | None = None, ronly: bool = False
     dvisk = 0.0 if CtrlVar.Picard else float(CtrlVar.NRviscosity)
     nn = MUA.Nnodes
     arrs = [np.ascontiguousarray(getattr(F, k), dtype=np.float64) for k in UV_FIELDS]
-    fp = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    arrs += [_optional(F, "q"), _optional(F, "muk")]
+    fp = (C.c_void_p * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
     R, Tint, Fext = np.zeros(2 * nn), np.zeros(2 * nn), np.zeros(2 * nn)
     Kval = np.zeros(4 * mesh.nnzN) if (want_K and not ronly) else None
     st = lib().o2_uv_assemble(C.byref(P), CtrlVar.Cmin, CtrlVar.AGlenmin, dvisk, MUA.Nele, MUA.nod, nn,

@@ -41,10 +41,12 @@ typedef struct {
     int zero_fields;
     int reset_thickness; /* ResetThicknessToMinThickness && ResetThicknessInNonLinLoop */
     double thick_min;
+    int law;             /* CtrlVar.SlidingLaw: 0 Weertman, 1 Budd, 2 Coulomb, 3 Tsai */
+    double nzero;        /* CtrlVar.Nzero */
 } o2_params;
 
 /* field order in the `f1` array of pointers: h ub vb S B rho C m n AGlen as ab ; f0: h ub vb as ab */
-enum { F1_H, F1_U, F1_V, F1_S, F1_B, F1_RHO, F1_C, F1_M, F1_N, F1_A, F1_AS, F1_AB, NF1 };
+enum { F1_H, F1_U, F1_V, F1_S, F1_B, F1_RHO, F1_C, F1_M, F1_N, F1_A, F1_AS, F1_AB, F1_Q, F1_MUK, NF1 };
 enum { F0_H, F0_U, F0_V, F0_AS, F0_AB, NF0 };
 
 #define MAXNOD 10
@@ -137,13 +139,52 @@ static void nodal_b(const o2_params *P, int64_t nn, const double *h, const doubl
     }
 }
 
+/* Sliding laws other than Weertman, UaMain/BasalDrag.m:182-235: Budd (:448-498), Coulomb
+ * (:365-408) and Tsai (the Coulomb values where |tau_C|^2 < |tau_W|^2), with the effective
+ * pressure N0 (:500-521). Inputs at the point: h, H = S - B, rho, u, v, m, q, muk, and the
+ * Weertman quantities (U2, beta2, Dbeta2; He and delta clamped at zero). The outputs hold the
+ * Weertman values on entry and are replaced where the law says so. */
+static void other_drag(const o2_params *P, double h, double H, double rho, double u, double v, double m, double q,
+                       double muk, double U2, double beta2, double Dbeta2, double He, double delta,
+                       double *taux, double *tauy, double *dtxdu, double *dtydv, double *dtxdv, double *dtxdh,
+                       double *dtydh)
+{
+    double hf = P->rhow * H / rho;
+    if (hf < DBL_EPSILON) hf = 0.0;
+    double Dh = h - hf;
+    const int I = Dh < DBL_EPSILON;
+    if (I) Dh = 0.0;
+    const double N = rho * P->g * Dh + P->nzero;
+    const double dNdh = I ? 0.0 : rho * P->g;
+    if (P->law == 1) {
+        const double qm = q / m, Nqm = pow(N, qm), T = He * Nqm * beta2;
+        *taux = T * u; *tauy = T * v;
+        *dtxdu = He * Nqm * (beta2 + Dbeta2 * u * u);
+        *dtydv = He * Nqm * (beta2 + Dbeta2 * v * v);
+        *dtxdv = He * Nqm * Dbeta2 * u * v;
+        const double E = delta * T + He * qm * pow(N, qm - 1.0) * dNdh * beta2;
+        *dtxdh = E * u; *dtydh = E * v;
+        return;
+    }
+    const double speed = sqrt(U2), Tau = muk * N;
+    const double cx = Tau * u / speed, cy = Tau * v / speed;
+    if (P->law == 3 && !(cx * cx + cy * cy < *taux * *taux + *tauy * *tauy)) return;
+    const double temp = speed * speed * speed;
+    *taux = cx; *tauy = cy;
+    *dtxdu = Tau * (1.0 / speed - u * u / temp);
+    *dtydv = Tau * (1.0 / speed - v * v / temp);
+    *dtxdv = -Tau * u * v / temp;
+    const double E = I ? 0.0 : muk * dNdh / speed;
+    *dtxdh = E * u; *dtydh = E * v;
+}
+
 /* One element: R parts (Tx,Fx,Ty,Fy,Th,Fh) x nod and 9 K blocks in the order
    Kxu Kxv Kxh Kyu Kyv Kyh Khu Khv Khh, each nod x nod row-major (i = row node). */
 static void element(const o2_params *P, int nod, int nip, const double *pts, const double *wts,
                     const double *xn, const double *yn, const double nv[][MAXNOD], double area,
                     double *Rel, double *Kel)
 {
-    enum { H, U, V, S, B, RHO, C, M, NN, A, AS1, AB1, H0, U0, V0, AS0, AB0, DADH, BL };
+    enum { H, U, V, S, B, RHO, C, M, NN, A, AS1, AB1, H0, U0, V0, AS0, AB0, DADH, BL, QQ, MUK };
     const double ca = cos(P->alpha), sa = sin(P->alpha);
     const double g = P->g, rhow = P->rhow, theta = P->theta, dt = P->dt;
     double Tx[MAXNOD] = {0}, Fx[MAXNOD] = {0}, Ty[MAXNOD] = {0}, Fy[MAXNOD] = {0}, Th[MAXNOD] = {0}, Fh[MAXNOD] = {0};
@@ -161,8 +202,8 @@ static void element(const o2_params *P, int nod, int nip, const double *pts, con
             Nx[i] = (J22 * Nxi[i] - J12 * Neta[i]) / det;
             Ny[i] = (-J21 * Nxi[i] + J11 * Neta[i]) / det;
         }
-        double val[BL + 1];
-        for (int f = 0; f <= BL; ++f) {
+        double val[MUK + 1];
+        for (int f = 0; f <= MUK; ++f) {
             double s = 0;
             for (int i = 0; i < nod; ++i) s += N[i] * nv[f][i];
             val[f] = s;
@@ -207,6 +248,11 @@ static void element(const o2_params *P, int nod, int nip, const double *pts, con
         double dtxdu = Hec * (beta2 + Dbeta2 * u * u), dtydv = Hec * (beta2 + Dbeta2 * v * v);
         double dtxdv = Hec * Dbeta2 * u * v, dtydu = dtxdv;
         double dtxdh = dlc * beta2 * u, dtydh = dlc * beta2 * v;
+        if (P->law != 0) {
+            other_drag(P, h, Hth, rho, u, v, m, val[QQ], val[MUK], U2, beta2, Dbeta2, Hec, dlc, &taux, &tauy,
+                       &dtxdu, &dtydv, &dtxdv, &dtxdh, &dtydh);
+            dtydu = dtxdv;
+        }
         /* SUPG */
         double l = sqrt(2.0 * area);
         double speed0 = sqrt(u0 * u0 + v0 * v0 + uz * uz);
@@ -373,7 +419,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
     const int *cn = corners[nod == 3 ? 0 : (nod == 6 ? 1 : 2)];
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t e = 0; e < nele; ++e) {
-        double xn[MAXNOD], yn[MAXNOD], nv[19][MAXNOD];
+        double xn[MAXNOD], yn[MAXNOD], nv[21][MAXNOD];
         for (int a = 0; a < nod; ++a) {
             int32_t k = conn[e * nod + a];
             xn[a] = x[k]; yn[a] = y[k];
@@ -385,6 +431,8 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
             nv[15][a] = f0[F0_AS][k]; nv[16][a] = f0[F0_AB][k];
             nv[17][a] = 0.0;              /* dadh */
             nv[18][a] = bwork[k];
+            nv[19][a] = f1[F1_Q] ? f1[F1_Q][k] : 0.0;
+            nv[20][a] = f1[F1_MUK] ? f1[F1_MUK][k] : 0.0;
         }
         double x1 = xn[cn[0]], x2 = xn[cn[1]], x3 = xn[cn[2]];
         double y1 = yn[cn[0]], y2 = yn[cn[1]], y3 = yn[cn[2]];
@@ -460,7 +508,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
  * R = Tint - Fext (2nn, [x; y]); K (4*nnzN values) on the CSR rows r = blk*nn + node with
  * row = [adj(node); adj + nn], symmetrised as (K + K')/2 (:545-549).
  */
-enum { UV_H, UV_S, UV_U, UV_V, UV_SS, UV_B, UV_RHO, UV_C, UV_M, UV_N, UV_A, NUV };
+enum { UV_H, UV_S, UV_U, UV_V, UV_SS, UV_B, UV_RHO, UV_C, UV_M, UV_N, UV_A, UV_Q, UV_MUK, NUV };
 
 static void uv_element(const o2_params *P, double cmin, double aglenmin, double dvisk, int nod, int nip,
                        const double *pts, const double *wts, const double *xn, const double *yn,
@@ -515,9 +563,16 @@ static void uv_element(const o2_params *P, double cmin, double aglenmin, double 
         const double U2 = u * u + v * v + uz * uz;
         const double beta2 = pow(Cz, -1.0 / m) * pow(sqrt(U2), 1.0 / m - 1.0);
         const double Dbeta2 = (1.0 / m - 1.0) * pow(Cz, -1.0 / m) * pow(U2, (1.0 - 3.0 * m) / (2.0 * m));
-        const double taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
-        const double dtxdu = Hec * (beta2 + Dbeta2 * u * u), dtydv = Hec * (beta2 + Dbeta2 * v * v);
-        const double dtxdv = Hec * Dbeta2 * u * v, dtydu = dtxdv;
+        double taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
+        double dtxdu = Hec * (beta2 + Dbeta2 * u * u), dtydv = Hec * (beta2 + Dbeta2 * v * v);
+        double dtxdv = Hec * Dbeta2 * u * v, dtydu = dtxdv;
+        if (P->law != 0) {   /* the thickness derivatives are not used by the uv system */
+            double dxh = 0.0, dyh = 0.0;
+            const double dl = soft_delta(P->kH, h - hf, P->Hh0);
+            other_drag(P, h, Hth, rho, u, v, m, val[UV_Q], val[UV_MUK], U2, beta2, Dbeta2, Hec, dl < 0 ? 0 : dl,
+                       &taux, &tauy, &dtxdu, &dtydv, &dtxdv, &dxh, &dyh);
+            dtydu = dtxdv;
+        }
         const double e = sqrt(P->eps_zero * P->eps_zero + exx * exx + eyy * eyy + exx * eyy + exy * exy);
         const double eta = 0.5 * pow(AG, -1.0 / n) * pow(e, (1.0 - n) / n) + P->eta_zero;
         const double E = (1.0 - n) / (4.0 * n) * pow(AG, -1.0 / n) * pow(e, (1.0 - 3.0 * n) / n);
@@ -579,7 +634,7 @@ int o2_uv_assemble(const o2_params *P, double cmin, double aglenmin, double dvis
         for (int a = 0; a < nod; ++a) {
             const int32_t k = conn[e * nod + a];
             xn[a] = x[k]; yn[a] = y[k];
-            for (int fi = 0; fi < NUV; ++fi) nv[fi][a] = f[fi][k];
+            for (int fi = 0; fi < NUV; ++fi) nv[fi][a] = f[fi] ? f[fi][k] : 0.0;
         }
         uv_element(P, cmin, aglenmin, dvisk, nod, nip, pts, wts, xn, yn, (const double (*)[MAXNOD])nv,
                    Rel + (size_t)e * nod * 4, want_k ? Kel + (size_t)e * 4 * nod * nod : NULL);

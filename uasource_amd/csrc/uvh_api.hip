@@ -206,8 +206,28 @@ uvh::Params make_params(const uvh_ctrl &c, uint32_t flags) {
     p.reset_thickness = c.reset_thickness;
     p.enforce_b_above = c.enforce_b_above_B;
     p.cmin = 0.0; p.aglenmin = 0.0; p.dvisk = 1.0; p.uv = 0;
+    p.nzero = c.nzero; p.law = c.sliding_law;
     return p;
 }
+
+// UVH_OK, or why the sliding law cannot run: an unknown law, or a field it reads is missing
+int check_law(const uvh_ctrl &c, const double *q, const double *muk, std::string &why) {
+    switch (c.sliding_law) {
+        case UVH_SLIDING_WEERTMAN: return UVH_OK;
+        case UVH_SLIDING_BUDD:
+            if (q) return UVH_OK;
+            why = "sliding law Budd needs q";
+            return UVH_E_ARG;
+        case UVH_SLIDING_COULOMB:
+        case UVH_SLIDING_TSAI:
+            if (muk) return UVH_OK;
+            why = "sliding laws Coulomb and Tsai need muk";
+            return UVH_E_ARG;
+        default: why = "sliding law " + std::to_string(c.sliding_law) + " is not supported"; return UVH_E_UNSUPPORTED;
+    }
+}
+
+constexpr int kHostFields = 20;   // uvh_fields arrays the host entry points upload
 
 bool fields_complete(const uvh_fields &f) {
     const double *req[] = {f.h, f.ub, f.vb, f.S, f.B, f.rho, f.C, f.m, f.n, f.AGlen, f.as, f.ab,
@@ -219,7 +239,7 @@ bool fields_complete(const uvh_fields &f) {
 
 uvh::FieldPtrs to_ptrs(const uvh_fields &f) {
     return uvh::FieldPtrs{f.h, f.ub, f.vb, f.S, f.B, f.rho, f.C, f.m, f.n, f.AGlen, f.as, f.ab,
-                          f.h0, f.ub0, f.vb0, f.as0, f.ab0, f.dadh};
+                          f.h0, f.ub0, f.vb0, f.as0, f.ab0, f.dadh, f.q, f.muk};
 }
 
 // Chunk plan of the pipelined assembly for element tiles of `epb` elements: tile boundaries
@@ -517,6 +537,11 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_assemble: no mesh (call uvh_set_mesh first)");
     if (!ctrl || !df || !dR) return ctx->fail(UVH_E_ARG, "uvh_assemble: null ctrl, fields or R");
     if (!fields_complete(*df)) return ctx->fail(UVH_E_ARG, "uvh_assemble: a required field pointer is null");
+    {
+        std::string why;
+        const int st = check_law(*ctrl, df->q, df->muk, why);
+        if (st != UVH_OK) return ctx->fail(st, "uvh_assemble: " + why);
+    }
     const bool fp32 = (flags & UVH_FP32) != 0;
     if (fp32 && !ctx->two_pass)
         return ctx->fail(UVH_E_UNSUPPORTED, "uvh_assemble: UVH_FP32 needs the two-pass assembly");
@@ -633,24 +658,25 @@ int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *hf, uint3
     const size_t nn = size_t(m.nn);
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
-    if (!ctx->d_fields) chk(dalloc(&ctx->d_fields, 18 * nn));
+    if (!ctx->d_fields) chk(dalloc(&ctx->d_fields, kHostFields * nn));
     if (!ctx->d_R) chk(dalloc(&ctx->d_R, 3 * nn));
     if (!ctx->d_T) chk(dalloc(&ctx->d_T, 3 * nn));
     if (!ctx->d_F) chk(dalloc(&ctx->d_F, 3 * nn));
     if (!ronly && Kval && !ctx->d_K) chk(dalloc(&ctx->d_K, size_t(9) * m.nnzN()));
     if (e != hipSuccess) return ctx->fail(UVH_E_NOMEM, "uvh_assemble: device allocation failed");
-    const double *src[18] = {hf->h, hf->ub, hf->vb, hf->S, hf->B, hf->rho, hf->C, hf->m, hf->n, hf->AGlen,
-                             hf->as, hf->ab, hf->h0, hf->ub0, hf->vb0, hf->as0, hf->ab0, hf->dadh};
-    const double *dst[18];
+    const double *src[kHostFields] = {hf->h, hf->ub, hf->vb, hf->S, hf->B, hf->rho, hf->C, hf->m, hf->n, hf->AGlen,
+                                      hf->as, hf->ab, hf->h0, hf->ub0, hf->vb0, hf->as0, hf->ab0, hf->dadh,
+                                      hf->q, hf->muk};
+    const double *dst[kHostFields];
     hipStream_t s = ctx->own_stream;
-    for (int k = 0; k < 18; ++k) {
+    for (int k = 0; k < kHostFields; ++k) {
         double *d = ctx->d_fields + k * nn;
         dst[k] = src[k] ? d : nullptr;
         if (src[k]) chk(hipMemcpyAsync(d, src[k], nn * sizeof(double), hipMemcpyHostToDevice, s));
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: field upload");
-    uvh_fields df{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8], dst[9], dst[10],
-                  dst[11], dst[12], dst[13], dst[14], dst[15], dst[16], dst[17]};
+    uvh_fields df{dst[0],  dst[1],  dst[2],  dst[3],  dst[4],  dst[5],  dst[6],  dst[7],  dst[8],  dst[9],
+                  dst[10], dst[11], dst[12], dst[13], dst[14], dst[15], dst[16], dst[17], dst[18], dst[19]};
     double *dK = (!ronly && Kval) ? ctx->d_K : nullptr;
     if (!ronly && !Kval) flags |= UVH_RONLY;
     int st = uvh_assemble_device(ctx, ctrl, &df, flags, ctx->d_R, dK, ctx->d_T, ctx->d_F, s);
@@ -674,6 +700,11 @@ int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl
         if (!q) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: a required field pointer is null");
     if (flags & UVH_ZEROFIELDS)
         return ctx->fail(UVH_E_UNSUPPORTED, "uvh_uv_assemble: ZeroFields (pass the zeroed velocities instead)");
+    {
+        std::string why;
+        const int st = check_law(*ctrl, df->q, df->muk, why);
+        if (st != UVH_OK) return ctx->fail(st, "uvh_uv_assemble: " + why);
+    }
     if (!ctx->two_pass) return ctx->fail(UVH_E_UNSUPPORTED, "uvh_uv_assemble: needs the two-pass assembly");
     const bool fp32 = (flags & UVH_FP32) != 0, ronly = (flags & UVH_RONLY) != 0;
     if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: K output required unless UVH_RONLY");
@@ -697,7 +728,7 @@ int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl
     p.dvisk = uvc->nrviscosity;
     const double *z = ctx->d_zero;
     const uvh::FieldPtrs f{df->h, df->ub, df->vb, df->S, df->B, df->rho, df->C, df->m, df->n, df->AGlen, z, z,
-                           df->h, df->ub, df->vb, z, z, nullptr};
+                           df->h, df->ub, df->vb, z, z, nullptr, df->q, df->muk};
     const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
     const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
     const int t = fp32 ? 1 : 0;
@@ -733,21 +764,24 @@ int uvh_uv_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, 
     const size_t nn = size_t(m.nn);
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
-    if (!ctx->d_fields) chk(dalloc(&ctx->d_fields, 18 * nn));
+    if (!ctx->d_fields) chk(dalloc(&ctx->d_fields, kHostFields * nn));
     if (!ctx->d_R) chk(dalloc(&ctx->d_R, 3 * nn));
     if (!ctx->d_T) chk(dalloc(&ctx->d_T, 3 * nn));
     if (!ctx->d_F) chk(dalloc(&ctx->d_F, 3 * nn));
     if (!ronly && Kval && !ctx->d_K) chk(dalloc(&ctx->d_K, size_t(9) * m.nnzN()));
     if (e != hipSuccess) return ctx->fail(UVH_E_NOMEM, "uvh_uv_assemble: device allocation failed");
     hipStream_t s = ctx->own_stream;
-    const double *dst[11];
-    for (int k = 0; k < 11; ++k) {
+    const double *all[13] = {src[0], src[1], src[2], src[3], src[4], src[5], src[6], src[7], src[8], src[9], src[10],
+                             hf->q, hf->muk};
+    const double *dst[13];
+    for (int k = 0; k < 13; ++k) {
         double *d = ctx->d_fields + k * nn;
-        dst[k] = d;
-        chk(hipMemcpyAsync(d, src[k], nn * sizeof(double), hipMemcpyHostToDevice, s));
+        dst[k] = all[k] ? d : nullptr;
+        if (all[k]) chk(hipMemcpyAsync(d, all[k], nn * sizeof(double), hipMemcpyHostToDevice, s));
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_uv_assemble: field upload");
-    const uvh_uv_fields df{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8], dst[9], dst[10]};
+    const uvh_uv_fields df{dst[0], dst[1], dst[2], dst[3],  dst[4],  dst[5], dst[6],
+                           dst[7], dst[8], dst[9], dst[10], dst[11], dst[12]};
     double *dK = (!ronly && Kval) ? ctx->d_K : nullptr;
     if (!ronly && !Kval) flags |= UVH_RONLY;
     int st = uvh_uv_assemble_device(ctx, ctrl, uvc, &df, flags, ctx->d_R, dK, ctx->d_T, ctx->d_F, s);
@@ -968,6 +1002,9 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
     for (int i = 0; i < nm; ++i) {
         if (!fields_complete(dfields[i]) || !dR[i] || (!ronly && !dK[i]))
             return b->fail(UVH_E_ARG, "uvhb_assemble: member " + std::to_string(i) + ": null field or output");
+        std::string why;
+        const int st = check_law(*ctrl, dfields[i].q, dfields[i].muk, why);
+        if (st != UVH_OK) return b->fail(st, "uvhb_assemble: member " + std::to_string(i) + ": " + why);
     }
     hipSetDevice(b->device);
     hipStream_t s = static_cast<hipStream_t>(stream);

@@ -133,13 +133,18 @@ struct Params {
     // thickness rows; dvisk scales the viscosity-derivative terms (NRviscosity; 1 for uvh)
     double cmin, aglenmin, dvisk;
     int32_t uv, pad;
+    // sliding law (enum uvh_sliding_law) and the lower bound of the effective pressure
+    double nzero;
+    int32_t law, pad2;
 };
 
-// Device pointers of the nodal fields, in the order of uvh_fields (dadh may be null).
+// Device pointers of the nodal fields, in the order of uvh_fields (dadh, q and muk may be
+// null; q and muk are read only by the sliding laws that use them).
 struct FieldPtrs {
     const double *h, *u, *v, *S, *B, *rho, *C, *m, *n, *A, *as1, *ab1;
     const double *h0, *u0, *v0, *as0, *ab0;
     const double *dadh;
+    const double *q, *muk;
 };
 
 // One mesh of a batched assembly, as the batched kernels read it (device memory). The

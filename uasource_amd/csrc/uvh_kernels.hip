@@ -35,7 +35,7 @@ namespace uvh {
 // nodal values staged per element in LDS
 enum : int {
     NV_H, NV_U, NV_V, NV_S, NV_B, NV_RHO, NV_C, NV_M, NV_N, NV_A, NV_AS1, NV_AB1,
-    NV_H0, NV_U0, NV_V0, NV_AS0, NV_AB0, NV_DADH, NV_BL, NV_X, NV_Y, NV_COUNT
+    NV_H0, NV_U0, NV_V0, NV_AS0, NV_AB0, NV_DADH, NV_BL, NV_X, NV_Y, NV_Q, NV_MUK, NV_COUNT
 };
 
 // per-(element, integration point) coefficient record; W = detJ * weight folded in where noted
@@ -170,6 +170,63 @@ __device__ __forceinline__ void floatation(T h, T hf, T H, T rho, const Params &
 // differences (exact in FP64), and the floatation terms and the hydrostatic combinations
 // built on them are evaluated in FP64 at every point: on floating ice rho h - rhow d cancels
 // to zero, which FP32 cannot resolve.
+// x^y as MATLAB's x.^y for x >= 0: 0^0 = 1, 0^y = 0 (y > 0) or Inf (y < 0)
+template <typename T>
+__device__ __forceinline__ T pow_nonneg(T x, T y) {
+    return y == T(0) ? T(1) : Fn<T>::exp(y * Fn<T>::log(x));
+}
+
+// Budd, Coulomb and Tsai sliding (BasalDrag.m:193-235 with the helpers Budd :448-498,
+// Coulomb :365-408 and N0 :500-521), replacing the Weertman values in tau and its
+// derivatives; beta2 and Db are Weertman's, He is clamped at zero. The effective pressure
+// N = rho g Dh + Nzero with Dh = max(h - hf, 0) and hf = max(rhow H / rho, 0) (both cut at
+// eps), dN/dh = rho g where Dh > 0 and 0 elsewhere, in FP64 from the point values.
+template <int NOD, typename T>
+__device__ __forceinline__ void other_sliding_law(const double *__restrict__ nv, const double *__restrict__ tN,
+                                                   const Params &p, double hd, double Hd, double rhod, T u, T v, T m,
+                                               T U2, T beta2, T Db, T Hec, T dlc, T &taux, T &tauy, T &dtxdu,
+                                               T &dtydv, T &dtxdv, T &dtxdh, T &dtydh) {
+    constexpr double kEps = 2.220446049250313e-16;
+    T q = 0, muk = 0;
+    for (int i = 0; i < NOD; ++i) {
+        const T Ni = T(tN[i]);
+        q += Ni * T(nv[NV_Q * NOD + i]);
+        muk += Ni * T(nv[NV_MUK * NOD + i]);
+    }
+    double hf = p.rhow * Hd / rhod;
+    if (hf < kEps) hf = 0.0;
+    double Dh = hd - hf;
+    const bool dry = Dh < kEps;   // no grounded ice above floatation
+    if (dry) Dh = 0.0;
+    const T N = T(rhod * p.g * Dh + p.nzero);
+    const T dNdh = dry ? T(0) : T(rhod * p.g);
+    if (p.law == 1) {   // Budd: tau = He N^(q/m) beta2 (u, v)
+        const T qm = q / m;
+        const T Nqm = pow_nonneg(N, qm);
+        const T Tb = Hec * Nqm * beta2;
+        taux = Tb * u; tauy = Tb * v;
+        dtxdu = Hec * Nqm * (beta2 + Db * u * u);
+        dtydv = Hec * Nqm * (beta2 + Db * v * v);
+        dtxdv = Hec * Nqm * Db * u * v;
+        // the reference's form (:489): delta times the whole of T, so He enters twice
+        const T E = dlc * Tb + Hec * qm * pow_nonneg(N, qm - T(1)) * dNdh * beta2;
+        dtxdh = E * u; dtydh = E * v;
+        return;
+    }
+    // Coulomb: tau = muk N (u, v) / |u|
+    const T speed = Fn<T>::sqrt(U2);
+    const T Tau = muk * N;
+    const T cx = Tau * u / speed, cy = Tau * v / speed;
+    const T cube = speed * speed * speed;
+    const T cdu = Tau * (T(1) / speed - u * u / cube), cdv = Tau * (T(1) / speed - v * v / cube);
+    const T cuv = -Tau * u * v / cube;
+    const T Ec = dry ? T(0) : muk * dNdh / speed;
+    if (p.law == 3 && !(cx * cx + cy * cy < taux * taux + tauy * tauy)) return;   // Tsai keeps Weertman here
+    taux = cx; tauy = cy;
+    dtxdu = cdu; dtydv = cdv; dtxdv = cuv;
+    dtxdh = Ec * u; dtydh = Ec * v;
+}
+
 template <int NOD, typename T>
 __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in LDS
                            const double *__restrict__ tN, const double *__restrict__ tXi,
@@ -264,10 +321,13 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     const T rm = T(1) / m;
     const T beta2 = Fn<T>::exp(-Fn<T>::log(C + T(p.czero)) * rm + (rm - T(1)) * (T(0.5) * Fn<T>::log(U2)));
     const T Db = (rm - T(1)) * beta2 / U2;
-    const T taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
-    const T dtxdu = Hec * (beta2 + Db * u * u), dtydv = Hec * (beta2 + Db * v * v);
-    const T dtxdv = Hec * Db * u * v;
-    const T dtxdh = dlc * beta2 * u, dtydh = dlc * beta2 * v;
+    T taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
+    T dtxdu = Hec * (beta2 + Db * u * u), dtydv = Hec * (beta2 + Db * v * v);
+    T dtxdv = Hec * Db * u * v;
+    T dtxdh = dlc * beta2 * u, dtydh = dlc * beta2 * v;
+    if (p.law != 0)   // the same law at every point: a uniform branch
+        other_sliding_law<NOD, T>(nv, tN, p, hd, Hd, rhod, u, v, m, U2, beta2, Db, Hec, dlc, taux, tauy, dtxdu,
+                                  dtydv, dtxdv, dtxdh, dtydh);
 
     // SUPG (uvhAssemblyIntPointImplicitSUPG.m:323-390, SUPGtau.m:24-77)
     int c0i, c1i, c2i;
@@ -354,6 +414,8 @@ __device__ __forceinline__ void gather_node(const int32_t *__restrict__ conn, co
     nv[NV_DADH * NOD + a] = f.dadh ? f.dadh[k] : 0.0;
     nv[NV_BL * NOD + a] = bnod[k];
     nv[NV_X * NOD + a] = xn[k]; nv[NV_Y * NOD + a] = yn[k];
+    nv[NV_Q * NOD + a] = f.q ? f.q[k] : 0.0;
+    nv[NV_MUK * NOD + a] = f.muk ? f.muk[k] : 0.0;
 }
 
 // Row i, block row BI (0: x, 1: y, 2: h) of the element: T and F of that row and its
@@ -883,6 +945,7 @@ struct FieldPtrsG {
     const UVH_G double *h, *u, *v, *S, *B, *rho, *C, *m, *n, *A, *as1, *ab1;
     const UVH_G double *h0, *u0, *v0, *as0, *ab0;
     const UVH_G double *dadh;
+    const UVH_G double *q, *muk;
 };
 struct BatchMemberG {
     const UVH_G int32_t *conn;
@@ -911,7 +974,8 @@ __device__ __forceinline__ FieldPtrs generic_fields(const UVH_G FieldPtrsG *g) {
                      (const double *)g->B, (const double *)g->rho, (const double *)g->C, (const double *)g->m,
                      (const double *)g->n, (const double *)g->A, (const double *)g->as1, (const double *)g->ab1,
                      (const double *)g->h0, (const double *)g->u0, (const double *)g->v0, (const double *)g->as0,
-                     (const double *)g->ab0, (const double *)g->dadh};
+                     (const double *)g->ab0, (const double *)g->dadh, (const double *)g->q,
+                     (const double *)g->muk};
 }
 
 // next item of a persistent workgroup: thread 0 takes it, the block reads it from LDS

@@ -48,6 +48,7 @@ def default_ctrlvar() -> Struct:
         Enforce_bAboveB=False,                    # :2643
         FlowApproximation="SSTREAM",              # :139
         SlidingLaw="Weertman",                    # :171
+        Nzero=1e-20,                              # :445, lower bound of the effective pressure
         TriNodes=3,                               # :239
         uvGroupAssembly=False,                    # :611
         uvhGroupAssembly=False,                   # :612

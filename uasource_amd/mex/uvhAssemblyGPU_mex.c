@@ -9,10 +9,11 @@
  *   connectivity  MUA.connectivity, Nele x nod double, 1-based (UaMain/CreateMUA.m:31-61)
  *   coordinates   MUA.coordinates, Nn x 2 double
  *   points        MUA.points, nip x 2 double;  weights  MUA.weights, nip x 1 double
- *   ctrl          1 x 17 double: dt theta SUPG.beta0 uvh.SUPG.tauMultiplier SpeedZero EpsZero
+ *   ctrl          1 x 19 double: dt theta SUPG.beta0 uvh.SUPG.tauMultiplier SpeedZero EpsZero
  *                 etaZero Czero kH Hh0 F.g F.rhow F.alpha ThickMin tau(0..4) resetThickness
- *                 Enforce_bAboveB   (the fields of uvh_ctrl, in order)
- *   fields        Nn x 18 double: h ub vb S B rho C m n AGlen as ab (F1), h ub vb as ab (F0), dadh
+ *                 Enforce_bAboveB slidingLaw(0..3) Nzero   (the fields of uvh_ctrl, in order)
+ *   fields        Nn x 20 double: h ub vb S B rho C m n AGlen as ab (F1), h ub vb as ab (F0), dadh,
+ *                 q muk (F1; read only by the sliding laws that use them)
  *   flags         double: UVH_RONLY | UVH_ZEROFIELDS | UVH_FP32 bits of include/uvh.h
  *
  *   R      3Nn x 1 double, [x; y; h] rows, R = Tint - Fext (UaMain/uvhMatrixAssembly.m:455)
@@ -111,20 +112,20 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     if (id != g_mesh_id) set_mesh(id, prhs[1], prhs[2], prhs[3], prhs[4]);
 
     nn = mxGetM(prhs[2]);
-    require(mxGetNumberOfElements(prhs[5]) == 17, "ctrl must have 17 entries");
-    require(mxGetM(prhs[6]) == nn && mxGetN(prhs[6]) == 18, "fields must be Nn x 18");
+    require(mxGetNumberOfElements(prhs[5]) == 19, "ctrl must have 19 entries");
+    require(mxGetM(prhs[6]) == nn && mxGetN(prhs[6]) == 20, "fields must be Nn x 20");
     cv = mxGetDoubles(prhs[5]);
     memset(&ctrl, 0, sizeof(ctrl));
     ctrl.dt = cv[0]; ctrl.theta = cv[1]; ctrl.supg_beta0 = cv[2]; ctrl.supg_tau_multiplier = cv[3];
     ctrl.speed_zero = cv[4]; ctrl.eps_zero = cv[5]; ctrl.eta_zero = cv[6]; ctrl.czero = cv[7];
     ctrl.kH = cv[8]; ctrl.Hh0 = cv[9]; ctrl.g = cv[10]; ctrl.rhow = cv[11]; ctrl.alpha = cv[12];
     ctrl.thick_min = cv[13]; ctrl.supg_tau = (int32_t)cv[14]; ctrl.reset_thickness = (int32_t)cv[15];
-    ctrl.enforce_b_above_B = (int32_t)cv[16];
+    ctrl.enforce_b_above_B = (int32_t)cv[16]; ctrl.sliding_law = (int32_t)cv[17]; ctrl.nzero = cv[18];
     F = mxGetDoubles(prhs[6]);
     f.h = F; f.ub = F + nn; f.vb = F + 2 * nn; f.S = F + 3 * nn; f.B = F + 4 * nn; f.rho = F + 5 * nn;
     f.C = F + 6 * nn; f.m = F + 7 * nn; f.n = F + 8 * nn; f.AGlen = F + 9 * nn; f.as = F + 10 * nn;
     f.ab = F + 11 * nn; f.h0 = F + 12 * nn; f.ub0 = F + 13 * nn; f.vb0 = F + 14 * nn; f.as0 = F + 15 * nn;
-    f.ab0 = F + 16 * nn; f.dadh = F + 17 * nn;
+    f.ab0 = F + 16 * nn; f.dadh = F + 17 * nn; f.q = F + 18 * nn; f.muk = F + 19 * nn;
     flags = (uint32_t)mxGetScalar(prhs[7]);
     ronly = (flags & UVH_RONLY) != 0;
 

@@ -97,6 +97,11 @@ class GpuAssembler:
             setattr(fd1, k, trial.get(k, self.f1[k]))
         for k in uvh.F0_KEYS:
             setattr(fd0, k, self.f0[k])
+        for k in ("q", "muk"):   # read by the sliding laws other than Weertman
+            a = getattr(F1, k, None)
+            if a is not None and np.size(a) > 0:
+                self.f1[k] = t(a)
+                setattr(fd1, k, self.f1[k])
         self.fields = uvh.make_fields(fd0, fd1)
         self.ctrl = uvh.make_ctrl(CtrlVar, F1)
         self.R = torch.zeros(3 * n, dtype=torch.float64, device=self.dev)

@@ -59,16 +59,23 @@ class UvhCtrl(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dt", "theta", "supg_beta0", "supg_tau_multiplier", "speed_zero",
                                           "eps_zero", "eta_zero", "czero", "kH", "Hh0", "g", "rhow", "alpha",
                                           "thick_min")] + \
-               [(n, C.c_int32) for n in ("supg_tau", "reset_thickness", "enforce_b_above_B", "reserved")]
+               [(n, C.c_int32) for n in ("supg_tau", "reset_thickness", "enforce_b_above_B", "sliding_law")] + \
+               [("nzero", C.c_double)]
+
+
+# CtrlVar.SlidingLaw names (UaMain/BasalDrag.m:182-257) -> enum uvh_sliding_law
+SLIDING_LAWS = {"Weertman": 0, "W": 0, "Budd": 1, "W-N0": 1, "Coulomb": 2, "C": 2, "Tsai": 3, "minCW-N0": 3}
+# the nodal fields each law reads beyond Weertman's (F.q, F.muk)
+SLIDING_FIELDS = {0: (), 1: ("q",), 2: ("muk",), 3: ("muk",)}
 
 
 class UvhFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("h", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "as_", "ab",
-                                          "h0", "ub0", "vb0", "as0", "ab0", "dadh")]
+                                          "h0", "ub0", "vb0", "as0", "ab0", "dadh", "q", "muk")]
 
 
 class UvhUvFields(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("h", "s", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen")]
+    _fields_ = [(n, C.c_void_p) for n in ("h", "s", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "q", "muk")]
 
 
 class UvhUvCtrl(C.Structure):
@@ -95,7 +102,21 @@ def make_uv_fields(F, keep=None) -> UvhUvFields:
         if keep is not None:
             keep.append(a)
         setattr(f, k, _ptr(a))
+    _sliding_fields(f, F, keep)
     return f
+
+
+def _sliding_fields(f, F, keep):
+    """q and muk of F, when F has them (the sliding laws that read them check for them)."""
+    for k in ("q", "muk"):
+        a = getattr(F, k, None)
+        if a is None or (isinstance(a, np.ndarray) and a.size == 0):
+            continue
+        if isinstance(a, np.ndarray):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+        if keep is not None:
+            keep.append(a)
+        setattr(f, k, _ptr(a))
 
 
 _lib = None
@@ -170,7 +191,7 @@ def check_supported(CtrlVar) -> None:
     reasons = []
     if str(cv.FlowApproximation).lower() != "sstream":
         reasons.append(f"FlowApproximation={cv.FlowApproximation}")
-    if str(cv.SlidingLaw) not in ("Weertman", "W"):
+    if str(cv.SlidingLaw) not in SLIDING_LAWS:
         reasons.append(f"SlidingLaw={cv.SlidingLaw}")
     if cv.IncludeMelangeModelPhysics:
         reasons.append("IncludeMelangeModelPhysics")
@@ -215,6 +236,8 @@ def make_ctrl(CtrlVar, F1) -> UvhCtrl:
     c.supg_tau = TAU_OPTIONS[CtrlVar.uvh.SUPG.tau]
     c.reset_thickness = int(bool(CtrlVar.ResetThicknessToMinThickness) and bool(CtrlVar.ResetThicknessInNonLinLoop))
     c.enforce_b_above_B = int(bool(CtrlVar.Enforce_bAboveB))
+    c.sliding_law = SLIDING_LAWS.get(str(CtrlVar.SlidingLaw), -1)
+    c.nzero = float(CtrlVar.Nzero)
     return c
 
 
@@ -243,6 +266,7 @@ def make_fields(F0, F1, dadh=None, keep=None) -> UvhFields:
         keep.extend(arrs)
         if dadh is not None:
             keep.append(dadh)
+    _sliding_fields(f, F1, keep)
     return f
 
 
