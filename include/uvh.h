@@ -58,14 +58,18 @@ enum uvh_supg_tau { UVH_TAU_S = 0, UVH_TAU_T = 1, UVH_TAU_1 = 2, UVH_TAU_2 = 3, 
 
 /* CtrlVar.SlidingLaw (UaMain/BasalDrag.m:182-257): "Weertman"/"W"; "Budd"/"W-N0" (needs q);
  * "Coulomb"/"C" (needs muk); "Tsai"/"minCW-N0" (needs muk: the smaller in magnitude of the
- * Weertman and Coulomb drag at each point). The effective pressure of the last three is N0
- * (BasalDrag.m:500-521): rho g max(h - hf, 0) + CtrlVar.Nzero. Other laws return
- * UVH_E_UNSUPPORTED. */
+ * Weertman and Coulomb drag at each point); "Cornford"/"rpCW-N0" (rpCWN0.m, needs muk);
+ * "Umbi"/"rCW-N0" (rCWN0.m, needs muk); "Joughin"/"rCW-v0" (rCWV0.m, needs V0). The laws
+ * with muk use the effective pressure N0 (BasalDrag.m:500-521): rho g max(h - hf, 0) +
+ * CtrlVar.Nzero. Other codes return UVH_E_UNSUPPORTED. */
 enum uvh_sliding_law {
     UVH_SLIDING_WEERTMAN = 0,
     UVH_SLIDING_BUDD = 1,
     UVH_SLIDING_COULOMB = 2,
-    UVH_SLIDING_TSAI = 3
+    UVH_SLIDING_TSAI = 3,
+    UVH_SLIDING_CORNFORD = 4,
+    UVH_SLIDING_UMBI = 5,
+    UVH_SLIDING_JOUGHIN = 6
 };
 
 /* Scalars the assembly reads from CtrlVar and F (defaults: UaMain/Ua2D_DefaultParameters.m). */
@@ -92,13 +96,13 @@ typedef struct {
 } uvh_ctrl;
 
 /* Nodal fields of F1 (current) and F0 (previous time step); each nnodes long.
- * dadh may be NULL (zero: MassBalanceGeometryFeedback < 2). q (F1.q, Budd) and muk (F1.muk,
- * Coulomb and Tsai) are read only by the sliding laws that use them (NULL otherwise). */
+ * dadh may be NULL (zero: MassBalanceGeometryFeedback < 2). q (F1.q), muk (F1.muk) and V0
+ * (F1.V0) are read only by the sliding laws that use them (NULL otherwise). */
 typedef struct {
     const double *h, *ub, *vb, *S, *B, *rho, *C, *m, *n, *AGlen, *as, *ab;
     const double *h0, *ub0, *vb0, *as0, *ab0;
     const double *dadh;
-    const double *q, *muk;
+    const double *q, *muk, *V0;
 } uvh_fields;
 
 /* Context on HIP device `device` (one per process and device; not thread-safe). */
@@ -161,7 +165,7 @@ int uvh_kernel_times(uvh_ctx *ctx, const char **names, double *total_ms, int64_t
  * as ub, vb directly. */
 typedef struct {
     const double *h, *s, *ub, *vb, *S, *B, *rho, *C, *m, *n, *AGlen;
-    const double *q, *muk;   /* as in uvh_fields */
+    const double *q, *muk, *V0;   /* as in uvh_fields */
 } uvh_uv_fields;
 
 typedef struct {

@@ -27,7 +27,8 @@ class O2Params(C.Structure):
                 ("reset_thickness", C.c_int), ("thick_min", C.c_double), ("law", C.c_int), ("nzero", C.c_double)]
 
 # CtrlVar.SlidingLaw (UaMain/BasalDrag.m:182-257) -> the restatement's law code
-SLIDING_LAWS = {"Weertman": 0, "W": 0, "Budd": 1, "W-N0": 1, "Coulomb": 2, "C": 2, "Tsai": 3, "minCW-N0": 3}
+SLIDING_LAWS = {"Weertman": 0, "W": 0, "Budd": 1, "W-N0": 1, "Coulomb": 2, "C": 2, "Tsai": 3, "minCW-N0": 3,
+                "Cornford": 4, "rpCW-N0": 4, "Umbi": 5, "rCW-N0": 5, "Joughin": 6, "rCW-v0": 6}
 
 
 def _optional(F, k):
@@ -156,9 +157,9 @@ def assemble(CtrlVar, MUA, F0, F1, mesh: O2Mesh | None = None, nthreads: int = 0
     nele, nod, nn = MUA.Nele, MUA.nod, MUA.Nnodes
     f1 = [np.ascontiguousarray(getattr(F1, k), dtype=np.float64)
           for k in ("h", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "as_", "ab")]
-    f1 += [_optional(F1, "q"), _optional(F1, "muk")]
+    f1 += [_optional(F1, "q"), _optional(F1, "muk"), _optional(F1, "V0")]
     f0 = [np.ascontiguousarray(getattr(F0, k), dtype=np.float64) for k in ("h", "ub", "vb", "as_", "ab")]
-    f1p = (C.c_void_p * 14)(*[None if a is None else a.ctypes.data for a in f1])
+    f1p = (C.c_void_p * 15)(*[None if a is None else a.ctypes.data for a in f1])
     f0p = (C.c_void_p * 5)(*[a.ctypes.data for a in f0])
     R = np.zeros(3 * nn)
     Tint = np.zeros(3 * nn)
@@ -212,7 +213,7 @@ def assemble_uv(CtrlVar, MUA, F, mesh: O2Mesh | None = None, ronly: bool = False
     dvisk = 0.0 if CtrlVar.Picard else float(CtrlVar.NRviscosity)
     nn = MUA.Nnodes
     arrs = [np.ascontiguousarray(getattr(F, k), dtype=np.float64) for k in UV_FIELDS]
-    arrs += [_optional(F, "q"), _optional(F, "muk")]
+    arrs += [_optional(F, "q"), _optional(F, "muk"), _optional(F, "V0")]
     fp = (C.c_void_p * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
     R, Tint, Fext = np.zeros(2 * nn), np.zeros(2 * nn), np.zeros(2 * nn)
     Kval = np.zeros(4 * mesh.nnzN) if (want_K and not ronly) else None

@@ -41,12 +41,13 @@ typedef struct {
     int zero_fields;
     int reset_thickness; /* ResetThicknessToMinThickness && ResetThicknessInNonLinLoop */
     double thick_min;
-    int law;             /* CtrlVar.SlidingLaw: 0 Weertman, 1 Budd, 2 Coulomb, 3 Tsai */
+    int law;             /* CtrlVar.SlidingLaw: 0 Weertman, 1 Budd, 2 Coulomb, 3 Tsai, 4 Cornford,
+                            5 Umbi, 6 Joughin */
     double nzero;        /* CtrlVar.Nzero */
 } o2_params;
 
 /* field order in the `f1` array of pointers: h ub vb S B rho C m n AGlen as ab ; f0: h ub vb as ab */
-enum { F1_H, F1_U, F1_V, F1_S, F1_B, F1_RHO, F1_C, F1_M, F1_N, F1_A, F1_AS, F1_AB, F1_Q, F1_MUK, NF1 };
+enum { F1_H, F1_U, F1_V, F1_S, F1_B, F1_RHO, F1_C, F1_M, F1_N, F1_A, F1_AS, F1_AB, F1_Q, F1_MUK, F1_V0, NF1 };
 enum { F0_H, F0_U, F0_V, F0_AS, F0_AB, NF0 };
 
 #define MAXNOD 10
@@ -139,15 +140,18 @@ static void nodal_b(const o2_params *P, int64_t nn, const double *h, const doubl
     }
 }
 
-/* Sliding laws other than Weertman, UaMain/BasalDrag.m:182-235: Budd (:448-498), Coulomb
- * (:365-408) and Tsai (the Coulomb values where |tau_C|^2 < |tau_W|^2), with the effective
- * pressure N0 (:500-521). Inputs at the point: h, H = S - B, rho, u, v, m, q, muk, and the
- * Weertman quantities (U2, beta2, Dbeta2; He and delta clamped at zero). The outputs hold the
- * Weertman values on entry and are replaced where the law says so. */
-static void other_drag(const o2_params *P, double h, double H, double rho, double u, double v, double m, double q,
-                       double muk, double U2, double beta2, double Dbeta2, double He, double delta,
-                       double *taux, double *tauy, double *dtxdu, double *dtydv, double *dtxdv, double *dtxdh,
-                       double *dtydh)
+/* Sliding laws other than Weertman, UaMain/BasalDrag.m:182-257: Budd (:448-498), Coulomb
+ * (:365-408), Tsai (the Coulomb values where |tau_C|^2 < |tau_W|^2), Cornford (rpCWN0.m), Umbi
+ * (rCWN0.m) and Joughin (rCWV0.m), with the effective pressure N0 (:500-521). Inputs at the
+ * point: h, H = S - B, rho, u, v, m, C + Czero, q, muk, V0, and the Weertman quantities (U2,
+ * beta2, Dbeta2; He and delta clamped at zero). The outputs hold the Weertman values on entry
+ * and are replaced where the law says so. The last three are written as the reference's
+ * generated expressions evaluate: tau = f(U) (u, v) with U = |(u, v, u0)|; their delta terms
+ * take He = 1/2 and Umbi's dN/dh term carries He once. */
+static void other_drag(const o2_params *P, double h, double H, double rho, double u, double v, double m, double Cp,
+                       double q, double muk, double V0, double U2, double beta2, double Dbeta2, double He,
+                       double delta, double *taux, double *tauy, double *dtxdu, double *dtydv, double *dtxdv,
+                       double *dtxdh, double *dtydh)
 {
     double hf = P->rhow * H / rho;
     if (hf < DBL_EPSILON) hf = 0.0;
@@ -166,7 +170,47 @@ static void other_drag(const o2_params *P, double h, double H, double rho, doubl
         *dtxdh = E * u; *dtydh = E * v;
         return;
     }
-    const double speed = sqrt(U2), Tau = muk * N;
+    const double speed = sqrt(U2);
+    if (P->law >= 4) {
+        double f, fU;   /* tau = f (u, v); fU = df/dU */
+        double dTdh;     /* d|tau|/dh (reference form) */
+        if (P->law == 4) {
+            /* |tau| = tauC tauW / (tauW^m + tauC^m)^(1/m), tauC = muk N, tauW = He C'^(-1/m) U^(1/m) */
+            const double tC = muk * N, w = pow(Cp, -1.0 / m) * pow(speed, 1.0 / m);
+            const double D = pow(tC, m) + pow(He * w, m);
+            const double mag = tC * He * w * pow(D, -1.0 / m);
+            f = mag / speed;
+            /* d mag / dU = mag (1/(m U)) (1 - (He w)^m / D) */
+            fU = mag / (m * speed) * (1.0 - pow(He * w, m) / D) / speed - mag / (speed * speed);
+            const double Dh = pow(tC, m) + pow(0.5 * w, m);   /* He = 1/2 */
+            const double dmag_dHe = tC * w * pow(Dh, -1.0 / m) * (1.0 - pow(0.5 * w, m) / Dh);
+            const double dmag_dN = muk * He * w * pow(D, -1.0 / m) * (1.0 - pow(tC, m) / D);
+            dTdh = delta * dmag_dHe + dNdh * dmag_dN;
+        } else if (P->law == 5) {
+            /* |tau| = tauC tauW / (tauC + tauW) */
+            const double tC = muk * N, tW = He * pow(Cp, -1.0 / m) * pow(speed, 1.0 / m);
+            const double mag = tC * tW / (tC + tW);
+            f = mag / speed;
+            fU = (tC * tC / ((tC + tW) * (tC + tW))) * tW / (m * speed) / speed - mag / (speed * speed);
+            const double tW2 = 0.5 * pow(Cp, -1.0 / m) * pow(speed, 1.0 / m);   /* He = 1/2 */
+            const double dmag_dHe = tC * tC * pow(Cp, -1.0 / m) * pow(speed, 1.0 / m) / ((tC + tW2) * (tC + tW2));
+            const double dmag_dN = muk * tW * tW / ((tC + tW) * (tC + tW)) / (He > 0 ? He : 1.0);
+            dTdh = delta * dmag_dHe + dNdh * (He > 0 ? dmag_dN : 0.0);
+        } else {
+            /* |tau| = He C'^(-1/m) (U / (U + V0))^(1/m) */
+            const double base = pow(Cp, -1.0 / m) * pow(speed / (speed + V0), 1.0 / m);
+            const double mag = He * base;
+            f = mag / speed;
+            fU = mag * (1.0 / m) * (1.0 / speed - 1.0 / (speed + V0)) / speed - mag / (speed * speed);
+            dTdh = delta * base;
+        }
+        const double g = fU / speed;
+        *taux = f * u; *tauy = f * v;
+        *dtxdu = f + u * u * g; *dtydv = f + v * v * g; *dtxdv = u * v * g;
+        *dtxdh = dTdh * u / speed; *dtydh = dTdh * v / speed;
+        return;
+    }
+    const double Tau = muk * N;
     const double cx = Tau * u / speed, cy = Tau * v / speed;
     if (P->law == 3 && !(cx * cx + cy * cy < *taux * *taux + *tauy * *tauy)) return;
     const double temp = speed * speed * speed;
@@ -184,7 +228,7 @@ static void element(const o2_params *P, int nod, int nip, const double *pts, con
                     const double *xn, const double *yn, const double nv[][MAXNOD], double area,
                     double *Rel, double *Kel)
 {
-    enum { H, U, V, S, B, RHO, C, M, NN, A, AS1, AB1, H0, U0, V0, AS0, AB0, DADH, BL, QQ, MUK };
+    enum { H, U, V, S, B, RHO, C, M, NN, A, AS1, AB1, H0, U0, V0, AS0, AB0, DADH, BL, QQ, MUK, JV0 };
     const double ca = cos(P->alpha), sa = sin(P->alpha);
     const double g = P->g, rhow = P->rhow, theta = P->theta, dt = P->dt;
     double Tx[MAXNOD] = {0}, Fx[MAXNOD] = {0}, Ty[MAXNOD] = {0}, Fy[MAXNOD] = {0}, Th[MAXNOD] = {0}, Fh[MAXNOD] = {0};
@@ -202,8 +246,8 @@ static void element(const o2_params *P, int nod, int nip, const double *pts, con
             Nx[i] = (J22 * Nxi[i] - J12 * Neta[i]) / det;
             Ny[i] = (-J21 * Nxi[i] + J11 * Neta[i]) / det;
         }
-        double val[MUK + 1];
-        for (int f = 0; f <= MUK; ++f) {
+        double val[JV0 + 1];
+        for (int f = 0; f <= JV0; ++f) {
             double s = 0;
             for (int i = 0; i < nod; ++i) s += N[i] * nv[f][i];
             val[f] = s;
@@ -249,8 +293,8 @@ static void element(const o2_params *P, int nod, int nip, const double *pts, con
         double dtxdv = Hec * Dbeta2 * u * v, dtydu = dtxdv;
         double dtxdh = dlc * beta2 * u, dtydh = dlc * beta2 * v;
         if (P->law != 0) {
-            other_drag(P, h, Hth, rho, u, v, m, val[QQ], val[MUK], U2, beta2, Dbeta2, Hec, dlc, &taux, &tauy,
-                       &dtxdu, &dtydv, &dtxdv, &dtxdh, &dtydh);
+            other_drag(P, h, Hth, rho, u, v, m, Cq, val[QQ], val[MUK], val[JV0], U2, beta2, Dbeta2, Hec, dlc,
+                       &taux, &tauy, &dtxdu, &dtydv, &dtxdv, &dtxdh, &dtydh);
             dtydu = dtxdv;
         }
         /* SUPG */
@@ -419,7 +463,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
     const int *cn = corners[nod == 3 ? 0 : (nod == 6 ? 1 : 2)];
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t e = 0; e < nele; ++e) {
-        double xn[MAXNOD], yn[MAXNOD], nv[21][MAXNOD];
+        double xn[MAXNOD], yn[MAXNOD], nv[22][MAXNOD];
         for (int a = 0; a < nod; ++a) {
             int32_t k = conn[e * nod + a];
             xn[a] = x[k]; yn[a] = y[k];
@@ -433,6 +477,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
             nv[18][a] = bwork[k];
             nv[19][a] = f1[F1_Q] ? f1[F1_Q][k] : 0.0;
             nv[20][a] = f1[F1_MUK] ? f1[F1_MUK][k] : 0.0;
+            nv[21][a] = f1[F1_V0] ? f1[F1_V0][k] : 0.0;
         }
         double x1 = xn[cn[0]], x2 = xn[cn[1]], x3 = xn[cn[2]];
         double y1 = yn[cn[0]], y2 = yn[cn[1]], y3 = yn[cn[2]];
@@ -508,7 +553,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
  * R = Tint - Fext (2nn, [x; y]); K (4*nnzN values) on the CSR rows r = blk*nn + node with
  * row = [adj(node); adj + nn], symmetrised as (K + K')/2 (:545-549).
  */
-enum { UV_H, UV_S, UV_U, UV_V, UV_SS, UV_B, UV_RHO, UV_C, UV_M, UV_N, UV_A, UV_Q, UV_MUK, NUV };
+enum { UV_H, UV_S, UV_U, UV_V, UV_SS, UV_B, UV_RHO, UV_C, UV_M, UV_N, UV_A, UV_Q, UV_MUK, UV_V0, NUV };
 
 static void uv_element(const o2_params *P, double cmin, double aglenmin, double dvisk, int nod, int nip,
                        const double *pts, const double *wts, const double *xn, const double *yn,
@@ -569,8 +614,8 @@ static void uv_element(const o2_params *P, double cmin, double aglenmin, double 
         if (P->law != 0) {   /* the thickness derivatives are not used by the uv system */
             double dxh = 0.0, dyh = 0.0;
             const double dl = soft_delta(P->kH, h - hf, P->Hh0);
-            other_drag(P, h, Hth, rho, u, v, m, val[UV_Q], val[UV_MUK], U2, beta2, Dbeta2, Hec, dl < 0 ? 0 : dl,
-                       &taux, &tauy, &dtxdu, &dtydv, &dtxdv, &dxh, &dyh);
+            other_drag(P, h, Hth, rho, u, v, m, Cz, val[UV_Q], val[UV_MUK], val[UV_V0], U2, beta2, Dbeta2, Hec,
+                       dl < 0 ? 0 : dl, &taux, &tauy, &dtxdu, &dtydv, &dtxdv, &dxh, &dyh);
             dtydu = dtxdv;
         }
         const double e = sqrt(P->eps_zero * P->eps_zero + exx * exx + eyy * eyy + exx * eyy + exy * exy);

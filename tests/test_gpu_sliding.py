@@ -1,10 +1,12 @@
-"""Sliding laws other than Weertman on the GPU (Budd, Coulomb, Tsai: UaMain/BasalDrag.m:182-235)
-against the oracle O2, through the C ABI, to the bars of the Weertman assembly: R within 1e-12
-of max(||Tint||, ||Fext||) and K within 1e-12 normwise in FP64; 1e-6 (R) and 1e-5 (K) in the
-precision form; the uv assembly likewise. muk is used at two scales: the drawn one (Coulomb
-stress far above Weertman's, so Tsai is Weertman everywhere) and 2 % of it (the stresses
-comparable, so Tsai mixes the two laws). A law whose field is missing is refused
-(UVH_E_ARG), an unknown law is unsupported, and the Weertman path is unchanged."""
+"""Sliding laws other than Weertman on the GPU (Budd, Coulomb, Tsai, Cornford, Umbi, Joughin:
+UaMain/BasalDrag.m:182-257) against the oracle O2, through the C ABI, to the bars of the
+Weertman assembly: R within 1e-12 of max(||Tint||, ||Fext||) and K within 1e-12 normwise in
+FP64; 1e-6 (R) and 1e-5 (K) in the precision form; the uv assembly likewise. muk is used at
+two scales: the drawn one (Coulomb stress far above Weertman's, so Tsai is Weertman
+everywhere) and 2 % of it (the stresses comparable, so Tsai mixes the two laws and the
+regularised laws sit between their limits). A law whose field is missing is refused
+(UVH_E_ARG); an unknown law, and any law but Weertman in the batched assembly, is
+unsupported; the Weertman path is unchanged."""
 import numpy as np
 import pytest
 
@@ -25,6 +27,7 @@ def case(name, muk_scale=1.0):
         rng = np.random.default_rng(3)
         F1.q = rng.uniform(0.5, 1.5, M.Nnodes)
         F1.muk = rng.uniform(0.3, 0.7, M.Nnodes)
+        F1.V0 = rng.uniform(50.0, 200.0, M.Nnodes)
         _cache[name] = (M, F0, F1)
     M, F0, F1 = _cache[name]
     G = F1.copy()
@@ -36,7 +39,8 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-CASES = [(law, s) for law in ("Budd", "Coulomb", "Tsai") for s in (1.0, 0.02) if not (law == "Budd" and s != 1.0)]
+CASES = [(law, s) for law in ("Budd", "Coulomb", "Tsai", "Cornford", "Umbi", "Joughin") for s in (1.0, 0.02)
+         if not (law in ("Budd", "Joughin") and s != 1.0)]
 
 
 @pytest.mark.parametrize("law,scale", CASES)
@@ -96,7 +100,8 @@ def test_missing_fields_and_unknown_laws_are_refused(o2lib):
     ctx = uvh.UvhContext(0)
     try:
         ctx.set_mesh(M)
-        for law, missing in (("Budd", "q"), ("Coulomb", "muk"), ("Tsai", "muk")):
+        for law, missing in (("Budd", "q"), ("Coulomb", "muk"), ("Tsai", "muk"), ("Cornford", "muk"), ("Umbi", "muk"),
+                             ("Joughin", "V0")):
             cv = default_ctrlvar()
             cv.SlidingLaw = law
             G = F1.copy()
@@ -112,12 +117,44 @@ def test_missing_fields_and_unknown_laws_are_refused(o2lib):
         st = ctx.lib.uvh_assemble(ctx.h, uvh.C.byref(ctrl), uvh.C.byref(fields), uvh.UVH_RONLY, uvh._ptr(R), None,
                                   None, None)
         assert st == uvh.UVH_E_UNSUPPORTED
-        # Weertman ignores q and muk: the same bits with and without them
+        # Weertman ignores q, muk and V0: the same bits with and without them
         cv = default_ctrlvar()
         a = ctx.assemble(cv, F0, F1)
         G = F1.copy()
-        G.q, G.muk = None, None
+        G.q, G.muk, G.V0 = None, None, None
         b = ctx.assemble(cv, F0, G)
         assert np.array_equal(a["R"], b["R"]) and np.array_equal(a["Kval"], b["Kval"])
     finally:
         ctx.close()
+
+
+def test_batched_assembly_runs_weertman_only():
+    import torch
+    from uasource_amd.batch import UvhBatch
+    print("[gpu] sliding law in the batched assembly", flush=True)
+    M, F0, F1 = case("tiny_p2")
+    dev = torch.device("cuda:0")
+    b = UvhBatch(0)
+    try:
+        b.add_mesh(M)
+        keep = []
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)  # noqa: E731
+
+        class _F:
+            pass
+        f1, f0 = _F(), _F()
+        for k in uvh.F1_KEYS:
+            setattr(f1, k, t(getattr(F1, k)))
+        for k in uvh.F0_KEYS:
+            setattr(f0, k, t(getattr(F0, k)))
+        f1.muk = t(F1.muk)
+        keep += [f1, f0]
+        fields = [uvh.make_fields(f0, f1)]
+        R = [torch.zeros(3 * M.Nnodes, dtype=torch.float64, device=dev)]
+        K = [torch.zeros(b.members[0].nnz, dtype=torch.float64, device=dev)]
+        cv = default_ctrlvar()
+        cv.SlidingLaw = "Coulomb"
+        with pytest.raises(uvh.UvhUnsupported):
+            b.assemble_device(uvh.make_ctrl(cv, F1), fields, 0, R, K)
+    finally:
+        b.close()

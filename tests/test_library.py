@@ -42,11 +42,12 @@ def test_unsupported_options_are_refused_before_any_device_work():
     from uasource_amd.ctrlvar import default_ctrlvar
     cv = default_ctrlvar()
     uvh.check_supported(cv)
-    for law in ("Budd", "W-N0", "Coulomb", "C", "Tsai", "minCW-N0"):
+    for law in ("Budd", "W-N0", "Coulomb", "C", "Tsai", "minCW-N0", "Cornford", "rpCW-N0", "Umbi", "rCW-N0",
+                "Joughin", "rCW-v0"):
         c = cv.copy()
         c.SlidingLaw = law
         uvh.check_supported(c)
-    for attr, val in [("SlidingLaw", "Cornford"), ("SlidingLaw", "Joughin"), ("FlowApproximation", "SSHEET"),
+    for attr, val in [("SlidingLaw", "Schoof"), ("FlowApproximation", "SSHEET"),
                       ("ThicknessPenalty", True)]:
         c = cv.copy()
         setattr(c, attr, val)

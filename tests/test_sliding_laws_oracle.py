@@ -1,10 +1,13 @@
-"""Sliding laws other than Weertman in the oracle (O2, UaMain/BasalDrag.m:182-235: Budd,
-Coulomb, Tsai, with the effective pressure N0 of :500-521), pinned without retyping their
-formulas:
+"""Sliding laws other than Weertman in the oracle (O2, UaMain/BasalDrag.m:182-257: Budd,
+Coulomb, Tsai, Cornford (rpCWN0.m), Umbi (rCWN0.m) and Joughin (rCWV0.m), with the effective
+pressure N0 of :500-521), pinned without retyping their formulas:
 
-* K is the Jacobian of R (central differences along random directions in (u, v, h)) for Budd
-  and Coulomb, on the tiny meshes, which are grounded everywhere (there the reference's Budd
-  h-derivative, delta times the whole of T, is exact: delta = 0 and He = 1);
+* K is the Jacobian of R (central differences along random directions in (u, v, h)) for every
+  law but Tsai (whose switch is not smooth), on the tiny meshes, which are grounded everywhere
+  (there the reference's h-derivatives, which differ from the exact ones only through He and
+  delta, are exact: delta = 0 and He = 1);
+* Cornford and Umbi tend to Weertman as muk N grows and their drag tends to Coulomb's as it
+  shrinks; Joughin tends to Weertman with C V0 as V0 grows;
 * Budd with q = 0 is Weertman (N^0 = 1), bit for bit on grounded meshes;
 * Coulomb's drag is muk N u/|u|: R is affine in muk, R(2 muk) - 2 R(muk) + R(0) = 0;
 * Tsai is Weertman where the Coulomb stress is the larger (muk large) and Coulomb where it is
@@ -29,6 +32,7 @@ def setup(name, law, seed=3):
     rng = np.random.default_rng(seed)
     F1.q = rng.uniform(0.5, 1.5, M.Nnodes)
     F1.muk = rng.uniform(0.3, 0.7, M.Nnodes)
+    F1.V0 = rng.uniform(50.0, 200.0, M.Nnodes)
     cv = default_ctrlvar()
     cv.SlidingLaw = law
     return cv, M, F0, F1
@@ -41,7 +45,7 @@ def R_of(cv, M, F0, F1, x):
     return o2.assemble(cv, M, F0, F, ronly=True)["R"]
 
 
-@pytest.mark.parametrize("law", ["Budd", "Coulomb"])
+@pytest.mark.parametrize("law", ["Budd", "Coulomb", "Cornford", "Umbi", "Joughin"])
 @pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3"])
 def test_K_is_jacobian_of_R(name, law):
     cv, M, F0, F1 = setup(name, law)
@@ -119,7 +123,46 @@ def test_tsai_takes_the_smaller_stress(name):
     assert mixed
 
 
-@pytest.mark.parametrize("law", ["Budd", "Coulomb", "Tsai"])
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("law", ["Cornford", "Umbi"])
+@pytest.mark.parametrize("name", ["tiny_p2", "tiny_p3"])
+def test_regularised_coulomb_limits(name, law):
+    cv, M, F0, F1 = setup(name, law)
+    cw, cc = cv.copy(), cv.copy()
+    cw.SlidingLaw, cc.SlidingLaw = "Weertman", "Coulomb"
+    # muk N far above the Weertman stress: Weertman
+    G = F1.copy()
+    G.muk = 1e8 * F1.muk
+    a, b = o2.assemble(cv, M, F0, G), o2.assemble(cw, M, F0, G)
+    assert _rel(a["R"], b["R"]) < 1e-8 and _rel(a["Kval"], b["Kval"]) < 1e-8
+    # muk N far below it: the drag (R minus R without drag) is Coulomb's
+    G.muk = 1e-8 * F1.muk
+    Z = F1.copy()
+    Z.muk = 0.0 * F1.muk
+    r0 = o2.assemble(cc, M, F0, Z, ronly=True)["R"]
+    d_law = o2.assemble(cv, M, F0, G, ronly=True)["R"] - r0
+    d_c = o2.assemble(cc, M, F0, G, ronly=True)["R"] - r0
+    assert np.linalg.norm(d_c) > 0 and _rel(d_law, d_c) < 1e-4
+
+
+@pytest.mark.parametrize("name", ["tiny_p2", "tiny_p3"])
+def test_joughin_with_large_V0_is_weertman_with_C_V0(name):
+    cv, M, F0, F1 = setup(name, "Joughin")
+    G = F1.copy()
+    G.V0 = np.full(M.Nnodes, 1e10)
+    a = o2.assemble(cv, M, F0, G)
+    cw = cv.copy()
+    cw.SlidingLaw = "Weertman"
+    W = F1.copy()
+    W.C = F1.C * 1e10
+    b = o2.assemble(cw, M, F0, W)
+    assert _rel(a["R"], b["R"]) < 1e-6 and _rel(a["Kval"], b["Kval"]) < 1e-6
+
+
+@pytest.mark.parametrize("law", ["Budd", "Coulomb", "Tsai", "Cornford", "Umbi", "Joughin"])
 @pytest.mark.parametrize("name", ["tiny_p2", "medium_p3"])
 def test_uv_equals_the_uvh_momentum_part(name, law):
     cv, M, F0, F1 = setup(name, law)

@@ -102,6 +102,8 @@ class UvhBatch:
             raise uvh.UaError("uvhMatrixAssembly:NaNinR", msg)
         if st == uvh.UVH_E_NAN_K:
             raise uvh.UaError("uvhMatrixAssembly:NaNinK", msg)
+        if st == uvh.UVH_E_UNSUPPORTED:
+            raise uvh.UvhUnsupported("uvhAssembly:unsupported", msg)
         raise uvh.UvhLibraryError(f"{what}: status {st}: {msg}")
 
     def add_mesh(self, MUA) -> int:

@@ -211,7 +211,7 @@ uvh::Params make_params(const uvh_ctrl &c, uint32_t flags) {
 }
 
 // UVH_OK, or why the sliding law cannot run: an unknown law, or a field it reads is missing
-int check_law(const uvh_ctrl &c, const double *q, const double *muk, std::string &why) {
+int check_law(const uvh_ctrl &c, const double *q, const double *muk, const double *V0, std::string &why) {
     switch (c.sliding_law) {
         case UVH_SLIDING_WEERTMAN: return UVH_OK;
         case UVH_SLIDING_BUDD:
@@ -220,14 +220,20 @@ int check_law(const uvh_ctrl &c, const double *q, const double *muk, std::string
             return UVH_E_ARG;
         case UVH_SLIDING_COULOMB:
         case UVH_SLIDING_TSAI:
+        case UVH_SLIDING_CORNFORD:
+        case UVH_SLIDING_UMBI:
             if (muk) return UVH_OK;
-            why = "sliding laws Coulomb and Tsai need muk";
+            why = "sliding laws Coulomb, Tsai, Cornford and Umbi need muk";
+            return UVH_E_ARG;
+        case UVH_SLIDING_JOUGHIN:
+            if (V0) return UVH_OK;
+            why = "sliding law Joughin needs V0";
             return UVH_E_ARG;
         default: why = "sliding law " + std::to_string(c.sliding_law) + " is not supported"; return UVH_E_UNSUPPORTED;
     }
 }
 
-constexpr int kHostFields = 20;   // uvh_fields arrays the host entry points upload
+constexpr int kHostFields = 21;   // uvh_fields arrays the host entry points upload
 
 bool fields_complete(const uvh_fields &f) {
     const double *req[] = {f.h, f.ub, f.vb, f.S, f.B, f.rho, f.C, f.m, f.n, f.AGlen, f.as, f.ab,
@@ -239,7 +245,7 @@ bool fields_complete(const uvh_fields &f) {
 
 uvh::FieldPtrs to_ptrs(const uvh_fields &f) {
     return uvh::FieldPtrs{f.h, f.ub, f.vb, f.S, f.B, f.rho, f.C, f.m, f.n, f.AGlen, f.as, f.ab,
-                          f.h0, f.ub0, f.vb0, f.as0, f.ab0, f.dadh, f.q, f.muk};
+                          f.h0, f.ub0, f.vb0, f.as0, f.ab0, f.dadh, f.q, f.muk, f.V0};
 }
 
 // Chunk plan of the pipelined assembly for element tiles of `epb` elements: tile boundaries
@@ -539,8 +545,10 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     if (!fields_complete(*df)) return ctx->fail(UVH_E_ARG, "uvh_assemble: a required field pointer is null");
     {
         std::string why;
-        const int st = check_law(*ctrl, df->q, df->muk, why);
+        const int st = check_law(*ctrl, df->q, df->muk, df->V0, why);
         if (st != UVH_OK) return ctx->fail(st, "uvh_assemble: " + why);
+        if (ctrl->sliding_law != UVH_SLIDING_WEERTMAN && !ctx->two_pass)
+            return ctx->fail(UVH_E_UNSUPPORTED, "uvh_assemble: sliding laws other than Weertman need the two-pass assembly");
     }
     const bool fp32 = (flags & UVH_FP32) != 0;
     if (fp32 && !ctx->two_pass)
@@ -666,7 +674,7 @@ int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *hf, uint3
     if (e != hipSuccess) return ctx->fail(UVH_E_NOMEM, "uvh_assemble: device allocation failed");
     const double *src[kHostFields] = {hf->h, hf->ub, hf->vb, hf->S, hf->B, hf->rho, hf->C, hf->m, hf->n, hf->AGlen,
                                       hf->as, hf->ab, hf->h0, hf->ub0, hf->vb0, hf->as0, hf->ab0, hf->dadh,
-                                      hf->q, hf->muk};
+                                      hf->q, hf->muk, hf->V0};
     const double *dst[kHostFields];
     hipStream_t s = ctx->own_stream;
     for (int k = 0; k < kHostFields; ++k) {
@@ -676,7 +684,8 @@ int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *hf, uint3
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: field upload");
     uvh_fields df{dst[0],  dst[1],  dst[2],  dst[3],  dst[4],  dst[5],  dst[6],  dst[7],  dst[8],  dst[9],
-                  dst[10], dst[11], dst[12], dst[13], dst[14], dst[15], dst[16], dst[17], dst[18], dst[19]};
+                  dst[10], dst[11], dst[12], dst[13], dst[14], dst[15], dst[16], dst[17], dst[18], dst[19],
+                  dst[20]};
     double *dK = (!ronly && Kval) ? ctx->d_K : nullptr;
     if (!ronly && !Kval) flags |= UVH_RONLY;
     int st = uvh_assemble_device(ctx, ctrl, &df, flags, ctx->d_R, dK, ctx->d_T, ctx->d_F, s);
@@ -702,7 +711,7 @@ int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl
         return ctx->fail(UVH_E_UNSUPPORTED, "uvh_uv_assemble: ZeroFields (pass the zeroed velocities instead)");
     {
         std::string why;
-        const int st = check_law(*ctrl, df->q, df->muk, why);
+        const int st = check_law(*ctrl, df->q, df->muk, df->V0, why);
         if (st != UVH_OK) return ctx->fail(st, "uvh_uv_assemble: " + why);
     }
     if (!ctx->two_pass) return ctx->fail(UVH_E_UNSUPPORTED, "uvh_uv_assemble: needs the two-pass assembly");
@@ -728,7 +737,7 @@ int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl
     p.dvisk = uvc->nrviscosity;
     const double *z = ctx->d_zero;
     const uvh::FieldPtrs f{df->h, df->ub, df->vb, df->S, df->B, df->rho, df->C, df->m, df->n, df->AGlen, z, z,
-                           df->h, df->ub, df->vb, z, z, nullptr, df->q, df->muk};
+                           df->h, df->ub, df->vb, z, z, nullptr, df->q, df->muk, df->V0};
     const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
     const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
     const int t = fp32 ? 1 : 0;
@@ -771,17 +780,17 @@ int uvh_uv_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, 
     if (!ronly && Kval && !ctx->d_K) chk(dalloc(&ctx->d_K, size_t(9) * m.nnzN()));
     if (e != hipSuccess) return ctx->fail(UVH_E_NOMEM, "uvh_uv_assemble: device allocation failed");
     hipStream_t s = ctx->own_stream;
-    const double *all[13] = {src[0], src[1], src[2], src[3], src[4], src[5], src[6], src[7], src[8], src[9], src[10],
-                             hf->q, hf->muk};
-    const double *dst[13];
-    for (int k = 0; k < 13; ++k) {
+    const double *all[14] = {src[0], src[1], src[2], src[3], src[4], src[5], src[6], src[7], src[8], src[9], src[10],
+                             hf->q, hf->muk, hf->V0};
+    const double *dst[14];
+    for (int k = 0; k < 14; ++k) {
         double *d = ctx->d_fields + k * nn;
         dst[k] = all[k] ? d : nullptr;
         if (all[k]) chk(hipMemcpyAsync(d, all[k], nn * sizeof(double), hipMemcpyHostToDevice, s));
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_uv_assemble: field upload");
-    const uvh_uv_fields df{dst[0], dst[1], dst[2], dst[3],  dst[4],  dst[5], dst[6],
-                           dst[7], dst[8], dst[9], dst[10], dst[11], dst[12]};
+    const uvh_uv_fields df{dst[0], dst[1], dst[2],  dst[3],  dst[4],  dst[5],  dst[6],
+                           dst[7], dst[8], dst[9], dst[10], dst[11], dst[12], dst[13]};
     double *dK = (!ronly && Kval) ? ctx->d_K : nullptr;
     if (!ronly && !Kval) flags |= UVH_RONLY;
     int st = uvh_uv_assemble_device(ctx, ctrl, uvc, &df, flags, ctx->d_R, dK, ctx->d_T, ctx->d_F, s);
@@ -1003,8 +1012,10 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
         if (!fields_complete(dfields[i]) || !dR[i] || (!ronly && !dK[i]))
             return b->fail(UVH_E_ARG, "uvhb_assemble: member " + std::to_string(i) + ": null field or output");
         std::string why;
-        const int st = check_law(*ctrl, dfields[i].q, dfields[i].muk, why);
+        const int st = check_law(*ctrl, dfields[i].q, dfields[i].muk, dfields[i].V0, why);
         if (st != UVH_OK) return b->fail(st, "uvhb_assemble: member " + std::to_string(i) + ": " + why);
+        if (ctrl->sliding_law != UVH_SLIDING_WEERTMAN)
+            return b->fail(UVH_E_UNSUPPORTED, "uvhb_assemble: the batched assembly runs the Weertman law only");
     }
     hipSetDevice(b->device);
     hipStream_t s = static_cast<hipStream_t>(stream);

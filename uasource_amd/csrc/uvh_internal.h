@@ -138,13 +138,13 @@ struct Params {
     int32_t law, pad2;
 };
 
-// Device pointers of the nodal fields, in the order of uvh_fields (dadh, q and muk may be
-// null; q and muk are read only by the sliding laws that use them).
+// Device pointers of the nodal fields, in the order of uvh_fields (dadh, q, muk and V0 may be
+// null; q, muk and V0 are read only by the sliding laws that use them).
 struct FieldPtrs {
     const double *h, *u, *v, *S, *B, *rho, *C, *m, *n, *A, *as1, *ab1;
     const double *h0, *u0, *v0, *as0, *ab0;
     const double *dadh;
-    const double *q, *muk;
+    const double *q, *muk, *V0;
 };
 
 // One mesh of a batched assembly, as the batched kernels read it (device memory). The

@@ -35,7 +35,7 @@ namespace uvh {
 // nodal values staged per element in LDS
 enum : int {
     NV_H, NV_U, NV_V, NV_S, NV_B, NV_RHO, NV_C, NV_M, NV_N, NV_A, NV_AS1, NV_AB1,
-    NV_H0, NV_U0, NV_V0, NV_AS0, NV_AB0, NV_DADH, NV_BL, NV_X, NV_Y, NV_Q, NV_MUK, NV_COUNT
+    NV_H0, NV_U0, NV_V0, NV_AS0, NV_AB0, NV_DADH, NV_BL, NV_X, NV_Y, NV_Q, NV_MUK, NV_JV0, NV_COUNT
 };
 
 // per-(element, integration point) coefficient record; W = detJ * weight folded in where noted
@@ -176,58 +176,113 @@ __device__ __forceinline__ T pow_nonneg(T x, T y) {
     return y == T(0) ? T(1) : Fn<T>::exp(y * Fn<T>::log(x));
 }
 
-// Budd, Coulomb and Tsai sliding (BasalDrag.m:193-235 with the helpers Budd :448-498,
-// Coulomb :365-408 and N0 :500-521), replacing the Weertman values in tau and its
-// derivatives; beta2 and Db are Weertman's, He is clamped at zero. The effective pressure
+// Sliding laws other than Weertman (BasalDrag.m:182-257): Budd (W-N0, helper :448-498),
+// Coulomb (:365-408), Tsai (minCW-N0: the Coulomb values where |tau_C|^2 < |tau_W|^2),
+// Cornford (rpCW-N0, rpCWN0.m), Umbi (rCW-N0, rCWN0.m) and Joughin (rCW-v0, rCWV0.m). They
+// replace the Weertman values of tau and its derivatives; beta2 and Db are Weertman's, He and
+// delta are clamped at zero. The effective pressure (N0, BasalDrag.m:500-521) is
 // N = rho g Dh + Nzero with Dh = max(h - hf, 0) and hf = max(rhow H / rho, 0) (both cut at
-// eps), dN/dh = rho g where Dh > 0 and 0 elsewhere, in FP64 from the point values.
+// eps); dN/dh = rho g where Dh > 0, 0 elsewhere. Everything here is FP64, also in the precision
+// form: (mu N)^m of Cornford underflows FP32 on floating ice.
+//
+// With U = |(u, v, u0)|, C' = C + C0 and tau = f (u, v), the u, v derivatives follow from
+// f'(U): d tau_u/du = f + u^2 f'/U, d tau_u/dv = u v f'/U. The h-derivatives are the
+// reference's: Cornford's and Umbi's delta terms are taken at He = 1/2 (the value of the step
+// where its derivative lives), and Umbi's dN/dh term carries He once.
 template <int NOD, typename T>
 __device__ __forceinline__ void other_sliding_law(const double *__restrict__ nv, const double *__restrict__ tN,
-                                                   const Params &p, double hd, double Hd, double rhod, T u, T v, T m,
-                                               T U2, T beta2, T Db, T Hec, T dlc, T &taux, T &tauy, T &dtxdu,
-                                               T &dtydv, T &dtxdv, T &dtxdh, T &dtydh) {
+                                                  const Params &p, double hd, double Hd, double rhod, T uT, T vT,
+                                                  T mT, T U2T, T beta2T, T DbT, T HecT, T dlcT, T &taux, T &tauy,
+                                                  T &dtxdu, T &dtydv, T &dtxdv, T &dtxdh, T &dtydh) {
     constexpr double kEps = 2.220446049250313e-16;
-    T q = 0, muk = 0;
+    double q = 0, muk = 0, V0 = 0, C = 0;
     for (int i = 0; i < NOD; ++i) {
-        const T Ni = T(tN[i]);
-        q += Ni * T(nv[NV_Q * NOD + i]);
-        muk += Ni * T(nv[NV_MUK * NOD + i]);
+        const double Ni = tN[i];
+        q += Ni * nv[NV_Q * NOD + i];
+        muk += Ni * nv[NV_MUK * NOD + i];
+        V0 += Ni * nv[NV_JV0 * NOD + i];
+        C += Ni * nv[NV_C * NOD + i];
     }
+    if (p.uv && C < p.cmin) C = p.cmin;
+    const double u = double(uT), v = double(vT), m = double(mT), U2 = double(U2T);
+    const double beta2 = double(beta2T), Db = double(DbT), He = double(HecT), dl = double(dlcT);
+    double tx = double(taux), ty = double(tauy), txu = double(dtxdu), tyv = double(dtydv), txv = double(dtxdv);
+    double txh = double(dtxdh), tyh = double(dtydh);
     double hf = p.rhow * Hd / rhod;
     if (hf < kEps) hf = 0.0;
     double Dh = hd - hf;
     const bool dry = Dh < kEps;   // no grounded ice above floatation
     if (dry) Dh = 0.0;
-    const T N = T(rhod * p.g * Dh + p.nzero);
-    const T dNdh = dry ? T(0) : T(rhod * p.g);
-    if (p.law == 1) {   // Budd: tau = He N^(q/m) beta2 (u, v)
-        const T qm = q / m;
-        const T Nqm = pow_nonneg(N, qm);
-        const T Tb = Hec * Nqm * beta2;
-        taux = Tb * u; tauy = Tb * v;
-        dtxdu = Hec * Nqm * (beta2 + Db * u * u);
-        dtydv = Hec * Nqm * (beta2 + Db * v * v);
-        dtxdv = Hec * Nqm * Db * u * v;
-        // the reference's form (:489): delta times the whole of T, so He enters twice
-        const T E = dlc * Tb + Hec * qm * pow_nonneg(N, qm - T(1)) * dNdh * beta2;
-        dtxdh = E * u; dtydh = E * v;
-        return;
+    const double N = rhod * p.g * Dh + p.nzero;
+    const double dNdh = dry ? 0.0 : rhod * p.g;
+    const double U = sqrt(U2), rm = 1.0 / m, Cp = C + p.czero;
+    // tau = f (u, v) with f'(U) / U = g f: the u, v derivatives
+    auto set_uv = [&](double f, double g) {
+        tx = f * u; ty = f * v;
+        txu = f * (1.0 + u * u * g); tyv = f * (1.0 + v * v * g); txv = f * u * v * g;
+    };
+    switch (p.law) {
+        case 1: {   // Budd: tau = He N^(q/m) beta2 (u, v)
+            const double qm = q / m, Nqm = pow_nonneg(N, qm), Tb = He * Nqm * beta2;
+            tx = Tb * u; ty = Tb * v;
+            txu = He * Nqm * (beta2 + Db * u * u);
+            tyv = He * Nqm * (beta2 + Db * v * v);
+            txv = He * Nqm * Db * u * v;
+            // the reference's form (:489): delta times the whole of T, so He enters twice
+            const double E = dl * Tb + He * qm * pow_nonneg(N, qm - 1.0) * dNdh * beta2;
+            txh = E * u; tyh = E * v;
+        } break;
+        case 2:
+        case 3: {   // Coulomb: tau = muk N (u, v) / U; Tsai keeps Weertman where that is smaller
+            const double Tau = muk * N;
+            const double cx = Tau * u / U, cy = Tau * v / U;
+            if (p.law == 3 && !(cx * cx + cy * cy < tx * tx + ty * ty)) break;
+            const double cube = U * U * U;
+            tx = cx; ty = cy;
+            txu = Tau * (1.0 / U - u * u / cube);
+            tyv = Tau * (1.0 / U - v * v / cube);
+            txv = -Tau * u * v / cube;
+            const double E = dry ? 0.0 : muk * dNdh / U;
+            txh = E * u; tyh = E * v;
+        } break;
+        case 4: {   // Cornford: |tau| = tauC tauW / (tauW^m + tauC^m)^(1/m), tauC = muk N, tauW = He w
+            const double w = exp((log(U) - log(Cp)) * rm);        // C'^(-1/m) U^(1/m)
+            const double P = pow_nonneg(muk * N, m), wm = U / Cp;  // (muk N)^m, w^m
+            const double Q = pow_nonneg(He, m) * wm;
+            const double S = P + Q;
+            const double f = muk * N * He * w * pow_nonneg(S, -rm) / U;
+            set_uv(f, (P / (m * S) - 1.0) / U2);
+            const double dHe = pow_nonneg(2.0, m + 1.0) * pow_nonneg(muk * N, m + 1.0) * w *
+                               pow_nonneg(pow_nonneg(2.0, m) * P + wm, -(m + 1.0) * rm);
+            const double dN = muk * pow_nonneg(He, m + 1.0) * pow_nonneg(w, m + 1.0) * pow_nonneg(S, -(m + 1.0) * rm);
+            const double E = (dl * dHe + dNdh * dN) / U;
+            txh = E * u; tyh = E * v;
+        } break;
+        case 5: {   // Umbi: |tau| = tauC tauW / (tauW + tauC)
+            const double Um = exp(log(U) * rm);                    // U^(1/m)
+            const double a = He * Um, b = muk * N * exp(log(Cp) * rm);
+            const double f = muk * N * He * Um / ((a + b) * U);
+            set_uv(f, (b / (m * (a + b)) - 1.0) / U2);
+            const double h2 = 2.0 * b + Um;
+            const double E = (muk * dNdh * He * Um * Um / ((a + b) * (a + b)) +
+                              4.0 * muk * N * b * dl * Um / (h2 * h2)) / U;
+            txh = E * u; tyh = E * v;
+        } break;
+        case 6: {   // Joughin: |tau| = He C'^(-1/m) (U / (U + V0))^(1/m)
+            const double base = exp(-log(Cp) * rm) * exp(-log(U + V0) * rm) * exp((rm - 1.0) * log(U));
+            const double f = He * base;
+            set_uv(f, -(m * U + (m - 1.0) * V0) / (m * U2 * (U + V0)));
+            const double E = dl * base;
+            txh = E * u; tyh = E * v;
+        } break;
+        default: break;
     }
-    // Coulomb: tau = muk N (u, v) / |u|
-    const T speed = Fn<T>::sqrt(U2);
-    const T Tau = muk * N;
-    const T cx = Tau * u / speed, cy = Tau * v / speed;
-    const T cube = speed * speed * speed;
-    const T cdu = Tau * (T(1) / speed - u * u / cube), cdv = Tau * (T(1) / speed - v * v / cube);
-    const T cuv = -Tau * u * v / cube;
-    const T Ec = dry ? T(0) : muk * dNdh / speed;
-    if (p.law == 3 && !(cx * cx + cy * cy < taux * taux + tauy * tauy)) return;   // Tsai keeps Weertman here
-    taux = cx; tauy = cy;
-    dtxdu = cdu; dtydv = cdv; dtxdv = cuv;
-    dtxdh = Ec * u; dtydh = Ec * v;
+    taux = T(tx); tauy = T(ty); dtxdu = T(txu); dtydv = T(tyv); dtxdv = T(txv); dtxdh = T(txh); dtydh = T(tyh);
 }
 
-template <int NOD, typename T>
+// SL: the sliding laws other than Weertman are compiled in (a separate instantiation, so
+// that their registers do not weigh on the Weertman kernels)
+template <int NOD, typename T, bool SL = false>
 __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in LDS
                            const double *__restrict__ tN, const double *__restrict__ tXi,
                            const double *__restrict__ tEta, double wq, const Params &p,
@@ -325,9 +380,11 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     T dtxdu = Hec * (beta2 + Db * u * u), dtydv = Hec * (beta2 + Db * v * v);
     T dtxdv = Hec * Db * u * v;
     T dtxdh = dlc * beta2 * u, dtydh = dlc * beta2 * v;
-    if (p.law != 0)   // the same law at every point: a uniform branch
-        other_sliding_law<NOD, T>(nv, tN, p, hd, Hd, rhod, u, v, m, U2, beta2, Db, Hec, dlc, taux, tauy, dtxdu,
-                                  dtydv, dtxdv, dtxdh, dtydh);
+    if constexpr (SL) {
+        if (p.law != 0)   // the same law at every point: a uniform branch
+            other_sliding_law<NOD, T>(nv, tN, p, hd, Hd, rhod, u, v, m, U2, beta2, Db, Hec, dlc, taux, tauy, dtxdu,
+                                      dtydv, dtxdv, dtxdh, dtydh);
+    }
 
     // SUPG (uvhAssemblyIntPointImplicitSUPG.m:323-390, SUPGtau.m:24-77)
     int c0i, c1i, c2i;
@@ -416,6 +473,7 @@ __device__ __forceinline__ void gather_node(const int32_t *__restrict__ conn, co
     nv[NV_X * NOD + a] = xn[k]; nv[NV_Y * NOD + a] = yn[k];
     nv[NV_Q * NOD + a] = f.q ? f.q[k] : 0.0;
     nv[NV_MUK * NOD + a] = f.muk ? f.muk[k] : 0.0;
+    nv[NV_JV0 * NOD + a] = f.V0 ? f.V0[k] : 0.0;
 }
 
 // Row i, block row BI (0: x, 1: y, 2: h) of the element: T and F of that row and its
@@ -601,7 +659,7 @@ __device__ __forceinline__ int row_table_doubles(int tstride) {
 // rows. The caller guarantees epb*NOD <= 256 and epb*nip <= 256. T is the arithmetic type of
 // the physics and the element rows (double, or float for the precision form); the rows are
 // stored in T at their position in the node-incidence order.
-template <int NOD, typename T, bool RONLY, bool FULL>
+template <int NOD, typename T, bool RONLY, bool FULL, bool SL = false>
 __device__ __forceinline__ void element_block(double *lds, int nele, int nip, int epb, int lb,
                                               const int32_t *__restrict__ conn, const double *__restrict__ xn,
                                               const double *__restrict__ yn, const FieldPtrs &f,
@@ -646,7 +704,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     T c[kNC];
     const int pel = tid / nip, pq = tid - pel * nip;
     const bool phys = tid < epb * nip && e0 + pel < nele;
-    if (phys) ip_physics<NOD, T>(work + pel * nv_stride<NOD>(), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
+    if (phys) ip_physics<NOD, T, SL>(work + pel * nv_stride<NOD>(), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
                                  sW[pq], p, c);
     __syncthreads();  // the nodal staging is dead from here; the coefficient records reuse it
     T *rec = reinterpret_cast<T *>(work);
@@ -709,7 +767,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     }
 }
 
-template <int NOD, typename T, bool RONLY, bool FULL>
+template <int NOD, typename T, bool RONLY, bool FULL, bool SL = false>
 __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, const int32_t *__restrict__ conn,
                                                  const double *__restrict__ xn, const double *__restrict__ yn,
                                                  FieldPtrs f, const double *__restrict__ bnod,
@@ -720,7 +778,8 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     extern __shared__ double lds[];
     const int lb = lb0 + xcd_block(blockIdx.x, gridDim.x);   // tiles lb0 .. lb1-1 of this launch
     if (lb >= lb1) return;  // whole block
-    element_block<NOD, T, RONLY, FULL>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB, RE);
+    element_block<NOD, T, RONLY, FULL, SL>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB,
+                                           RE);
 }
 
 // Node rows: row_lanes(NOD) lanes per node, 64 / row_lanes nodes per wave. The incident element
@@ -945,7 +1004,7 @@ struct FieldPtrsG {
     const UVH_G double *h, *u, *v, *S, *B, *rho, *C, *m, *n, *A, *as1, *ab1;
     const UVH_G double *h0, *u0, *v0, *as0, *ab0;
     const UVH_G double *dadh;
-    const UVH_G double *q, *muk;
+    const UVH_G double *q, *muk, *V0;
 };
 struct BatchMemberG {
     const UVH_G int32_t *conn;
@@ -975,7 +1034,7 @@ __device__ __forceinline__ FieldPtrs generic_fields(const UVH_G FieldPtrsG *g) {
                      (const double *)g->n, (const double *)g->A, (const double *)g->as1, (const double *)g->ab1,
                      (const double *)g->h0, (const double *)g->u0, (const double *)g->v0, (const double *)g->as0,
                      (const double *)g->ab0, (const double *)g->dadh, (const double *)g->q,
-                     (const double *)g->muk};
+                     (const double *)g->muk, (const double *)g->V0};
 }
 
 // next item of a persistent workgroup: thread 0 takes it, the block reads it from LDS
@@ -1322,6 +1381,10 @@ hipError_t launch_element_t(int nele, int nip, int epb, bool full, const int32_t
     constexpr bool can_full = NOD <= 6;
     auto kern = ronly ? (full && can_full ? k_element<NOD, T, true, can_full> : k_element<NOD, T, true, false>)
                       : (full && can_full ? k_element<NOD, T, false, can_full> : k_element<NOD, T, false, false>);
+    if (p.law != 0)
+        kern = ronly ? (full && can_full ? k_element<NOD, T, true, can_full, true> : k_element<NOD, T, true, false, true>)
+                     : (full && can_full ? k_element<NOD, T, false, can_full, true>
+                                         : k_element<NOD, T, false, false, true>);
     lds_attr(reinterpret_cast<const void *>(kern), lds);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
                        static_cast<T *>(EB), static_cast<T *>(RE), lb0, lb1);

@@ -11,9 +11,9 @@
  *   points        MUA.points, nip x 2 double;  weights  MUA.weights, nip x 1 double
  *   ctrl          1 x 19 double: dt theta SUPG.beta0 uvh.SUPG.tauMultiplier SpeedZero EpsZero
  *                 etaZero Czero kH Hh0 F.g F.rhow F.alpha ThickMin tau(0..4) resetThickness
- *                 Enforce_bAboveB slidingLaw(0..3) Nzero   (the fields of uvh_ctrl, in order)
- *   fields        Nn x 20 double: h ub vb S B rho C m n AGlen as ab (F1), h ub vb as ab (F0), dadh,
- *                 q muk (F1; read only by the sliding laws that use them)
+ *                 Enforce_bAboveB slidingLaw(0..6) Nzero   (the fields of uvh_ctrl, in order)
+ *   fields        Nn x 21 double: h ub vb S B rho C m n AGlen as ab (F1), h ub vb as ab (F0), dadh,
+ *                 q muk V0 (F1; read only by the sliding laws that use them)
  *   flags         double: UVH_RONLY | UVH_ZEROFIELDS | UVH_FP32 bits of include/uvh.h
  *
  *   R      3Nn x 1 double, [x; y; h] rows, R = Tint - Fext (UaMain/uvhMatrixAssembly.m:455)
@@ -113,7 +113,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
 
     nn = mxGetM(prhs[2]);
     require(mxGetNumberOfElements(prhs[5]) == 19, "ctrl must have 19 entries");
-    require(mxGetM(prhs[6]) == nn && mxGetN(prhs[6]) == 20, "fields must be Nn x 20");
+    require(mxGetM(prhs[6]) == nn && mxGetN(prhs[6]) == 21, "fields must be Nn x 21");
     cv = mxGetDoubles(prhs[5]);
     memset(&ctrl, 0, sizeof(ctrl));
     ctrl.dt = cv[0]; ctrl.theta = cv[1]; ctrl.supg_beta0 = cv[2]; ctrl.supg_tau_multiplier = cv[3];
@@ -126,6 +126,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     f.C = F + 6 * nn; f.m = F + 7 * nn; f.n = F + 8 * nn; f.AGlen = F + 9 * nn; f.as = F + 10 * nn;
     f.ab = F + 11 * nn; f.h0 = F + 12 * nn; f.ub0 = F + 13 * nn; f.vb0 = F + 14 * nn; f.as0 = F + 15 * nn;
     f.ab0 = F + 16 * nn; f.dadh = F + 17 * nn; f.q = F + 18 * nn; f.muk = F + 19 * nn;
+    f.V0 = F + 20 * nn;
     flags = (uint32_t)mxGetScalar(prhs[7]);
     ronly = (flags & UVH_RONLY) != 0;
 

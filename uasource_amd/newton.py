@@ -97,7 +97,7 @@ class GpuAssembler:
             setattr(fd1, k, trial.get(k, self.f1[k]))
         for k in uvh.F0_KEYS:
             setattr(fd0, k, self.f0[k])
-        for k in ("q", "muk"):   # read by the sliding laws other than Weertman
+        for k in ("q", "muk", "V0"):   # read by the sliding laws other than Weertman
             a = getattr(F1, k, None)
             if a is not None and np.size(a) > 0:
                 self.f1[k] = t(a)

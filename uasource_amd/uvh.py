@@ -64,18 +64,20 @@ class UvhCtrl(C.Structure):
 
 
 # CtrlVar.SlidingLaw names (UaMain/BasalDrag.m:182-257) -> enum uvh_sliding_law
-SLIDING_LAWS = {"Weertman": 0, "W": 0, "Budd": 1, "W-N0": 1, "Coulomb": 2, "C": 2, "Tsai": 3, "minCW-N0": 3}
-# the nodal fields each law reads beyond Weertman's (F.q, F.muk)
-SLIDING_FIELDS = {0: (), 1: ("q",), 2: ("muk",), 3: ("muk",)}
+SLIDING_LAWS = {"Weertman": 0, "W": 0, "Budd": 1, "W-N0": 1, "Coulomb": 2, "C": 2, "Tsai": 3, "minCW-N0": 3,
+                "Cornford": 4, "rpCW-N0": 4, "Umbi": 5, "rCW-N0": 5, "Joughin": 6, "rCW-v0": 6}
+# the nodal fields each law reads beyond Weertman's (F.q, F.muk, F.V0)
+SLIDING_FIELDS = {0: (), 1: ("q",), 2: ("muk",), 3: ("muk",), 4: ("muk",), 5: ("muk",), 6: ("V0",)}
 
 
 class UvhFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("h", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "as_", "ab",
-                                          "h0", "ub0", "vb0", "as0", "ab0", "dadh", "q", "muk")]
+                                          "h0", "ub0", "vb0", "as0", "ab0", "dadh", "q", "muk", "V0")]
 
 
 class UvhUvFields(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("h", "s", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "q", "muk")]
+    _fields_ = [(n, C.c_void_p) for n in ("h", "s", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "q", "muk",
+                                          "V0")]
 
 
 class UvhUvCtrl(C.Structure):
@@ -107,8 +109,8 @@ def make_uv_fields(F, keep=None) -> UvhUvFields:
 
 
 def _sliding_fields(f, F, keep):
-    """q and muk of F, when F has them (the sliding laws that read them check for them)."""
-    for k in ("q", "muk"):
+    """q, muk and V0 of F, when F has them (the sliding laws that read them check for them)."""
+    for k in ("q", "muk", "V0"):
         a = getattr(F, k, None)
         if a is None or (isinstance(a, np.ndarray) and a.size == 0):
             continue
