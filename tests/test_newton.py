@@ -41,6 +41,25 @@ def test_oracle_newton_converges_quadratically(o2lib):
     assert np.array_equal(x0[fixed], x1[fixed])
 
 
+def test_host_solvers_agree(o2lib):
+    """The host solves give the same Newton iteration: minimum-degree direct (the default),
+    COLAMD with partial pivoting (its fall-back), and the LU kept across steps as the GMRES
+    preconditioner; the same iteration counts (within one), rForce < 1e-15 and the same state."""
+    import scipy.sparse.linalg as spla
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    colamd = lambda K, rhs: spla.splu(K.tocsc(), permc_spec="COLAMD").solve(rhs)  # noqa: E731
+    lagged = newton.LaggedLU()
+    for solver in (colamd, lagged):
+        res = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, linear_solver=solver, fixed=fixed)
+        assert res.converged and res.rForce < 1e-15, res.reason
+        assert abs(res.iterations - ref.iterations) <= 1
+        for x, y in ((res.ub, ref.ub), (res.vb, ref.vb), (res.h, ref.h)):
+            assert rel(x, y) < 1e-9
+    # the LU was reused for some steps (GMRES on an older factorization)
+    assert lagged.factorizations < ref.iterations and max(lagged.krylov) > 0
+
+
 def test_line_search_accepts_full_step_and_backtracks():
     cv = default_ctrlvar()
     # full step below NewtonAcceptRatio * r0
