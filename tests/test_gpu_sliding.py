@@ -4,9 +4,9 @@ Weertman assembly: R within 1e-12 of max(||Tint||, ||Fext||) and K within 1e-12 
 FP64; 1e-6 (R) and 1e-5 (K) in the precision form; the uv assembly likewise. muk is used at
 two scales: the drawn one (Coulomb stress far above Weertman's, so Tsai is Weertman
 everywhere) and 2 % of it (the stresses comparable, so Tsai mixes the two laws and the
-regularised laws sit between their limits). A law whose field is missing is refused
-(UVH_E_ARG); an unknown law, and any law but Weertman in the batched assembly, is
-unsupported; the Weertman path is unchanged."""
+regularised laws sit between their limits). The batched assembly gives the single-mesh
+results. A law whose field is missing is refused (UVH_E_ARG), an unknown law is unsupported,
+and the Weertman path is unchanged."""
 import numpy as np
 import pytest
 
@@ -128,33 +128,48 @@ def test_missing_fields_and_unknown_laws_are_refused(o2lib):
         ctx.close()
 
 
-def test_batched_assembly_runs_weertman_only():
+@pytest.mark.parametrize("law", ["Budd", "Tsai", "Cornford", "Joughin"])
+def test_batched_assembly_matches_single_meshes(law):
     import torch
     from uasource_amd.batch import UvhBatch
-    print("[gpu] sliding law in the batched assembly", flush=True)
-    M, F0, F1 = case("tiny_p2")
+    print(f"[gpu] sliding law {law} in the batched assembly", flush=True)
     dev = torch.device("cuda:0")
+    cv = default_ctrlvar()
+    cv.SlidingLaw = law
+    names = ["tiny_p1", "tiny_p2", "medium_p2"]
+    cases = [case(n, 0.02) for n in names]
     b = UvhBatch(0)
+    keep = []
     try:
-        b.add_mesh(M)
-        keep = []
+        for M, _, _ in cases:
+            b.add_mesh(M)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)  # noqa: E731
-
-        class _F:
-            pass
-        f1, f0 = _F(), _F()
-        for k in uvh.F1_KEYS:
-            setattr(f1, k, t(getattr(F1, k)))
-        for k in uvh.F0_KEYS:
-            setattr(f0, k, t(getattr(F0, k)))
-        f1.muk = t(F1.muk)
-        keep += [f1, f0]
-        fields = [uvh.make_fields(f0, f1)]
-        R = [torch.zeros(3 * M.Nnodes, dtype=torch.float64, device=dev)]
-        K = [torch.zeros(b.members[0].nnz, dtype=torch.float64, device=dev)]
-        cv = default_ctrlvar()
-        cv.SlidingLaw = "Coulomb"
-        with pytest.raises(uvh.UvhUnsupported):
-            b.assemble_device(uvh.make_ctrl(cv, F1), fields, 0, R, K)
+        fields, R, K = [], [], []
+        for (M, F0, F1), mv in zip(cases, b.members):
+            class _F:
+                pass
+            f1, f0 = _F(), _F()
+            for k in uvh.F1_KEYS:
+                setattr(f1, k, t(getattr(F1, k)))
+            for k in uvh.F0_KEYS:
+                setattr(f0, k, t(getattr(F0, k)))
+            for k in ("q", "muk", "V0"):
+                setattr(f1, k, t(getattr(F1, k)))
+            keep += [f1, f0]
+            fields.append(uvh.make_fields(f0, f1))
+            R.append(torch.zeros(3 * M.Nnodes, dtype=torch.float64, device=dev))
+            K.append(torch.zeros(mv.nnz, dtype=torch.float64, device=dev))
+        b.assemble_device(uvh.make_ctrl(cv, cases[0][2]), fields, 0, R, K)
+        torch.cuda.synchronize()
+        for name, (M, F0, F1), r, k in zip(names, cases, R, K):
+            ctx = uvh.UvhContext(0)
+            try:
+                ctx.set_mesh(M)
+                ref = ctx.assemble(cv, F0, F1)
+            finally:
+                ctx.close()
+            eR, eK = rel(r.cpu().numpy(), ref["R"]), rel(k.cpu().numpy(), ref["Kval"])
+            print(f"   {name}: R {eR:.1e}, K {eK:.1e}", flush=True)
+            assert eR <= 1e-13 and eK <= 1e-13
     finally:
         b.close()

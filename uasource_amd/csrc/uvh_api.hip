@@ -1014,8 +1014,9 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
         std::string why;
         const int st = check_law(*ctrl, dfields[i].q, dfields[i].muk, dfields[i].V0, why);
         if (st != UVH_OK) return b->fail(st, "uvhb_assemble: member " + std::to_string(i) + ": " + why);
-        if (ctrl->sliding_law != UVH_SLIDING_WEERTMAN)
-            return b->fail(UVH_E_UNSUPPORTED, "uvhb_assemble: the batched assembly runs the Weertman law only");
+        if (ctrl->sliding_law != UVH_SLIDING_WEERTMAN && !b->flat)
+            return b->fail(UVH_E_UNSUPPORTED,
+                           "uvhb_assemble: the persistent batch schedule runs the Weertman law only");
     }
     hipSetDevice(b->device);
     hipStream_t s = static_cast<hipStream_t>(stream);

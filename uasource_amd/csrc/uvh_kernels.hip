@@ -1104,8 +1104,8 @@ __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__rest
 
 // The same work list as k_element_batch, one workgroup per item instead of persistent
 // workgroups: item blockIdx (XCD-aware), no work counter, the register budget of the
-// single-mesh kernel.
-template <int NOD, typename T, bool RONLY>
+// single-mesh kernel. SL: the sliding laws other than Weertman compiled in.
+template <int NOD, typename T, bool RONLY, bool SL = false>
 __global__ __launch_bounds__(256) void k_element_batch_flat(const BatchMember *__restrict__ mem,
                                                             const int2 *__restrict__ items, int nitems, Params p) {
     extern __shared__ double lds[];
@@ -1119,7 +1119,7 @@ __global__ __launch_bounds__(256) void k_element_batch_flat(const BatchMember *_
     else
         rtab = (const ShapeTables32 *)(const __attribute__((address_space(4))) ShapeTables32 *)M->tab32;
     const FieldPtrs f = generic_fields(&M->f);
-    element_block<NOD, T, RONLY, (NOD <= 6 && sizeof(T) == sizeof(float))>(
+    element_block<NOD, T, RONLY, (NOD <= 6 && sizeof(T) == sizeof(float)), SL>(
         lds, M->nele, M->nip, M->epb, w.y, (const int32_t *)M->conn, (const double *)M->x, (const double *)M->y, f,
         (const double *)M->b, (const ShapeTables *)M->tab, rtab, p, (const int32_t *)M->inc_pos, (T *)M->EB,
         (T *)M->RE);
@@ -1539,7 +1539,7 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
     do {                                                                                                   \
         const int n_it = eb[K + 1] - eb[K];                                                                \
         if (flat) {                                                                                        \
-            auto kern = k_element_batch_flat<NODV, T, RO>;                                                 \
+            auto kern = p.law ? k_element_batch_flat<NODV, T, RO, true> : k_element_batch_flat<NODV, T, RO>; \
             lds_attr(reinterpret_cast<const void *>(kern), elem_lds[K]);                                   \
             const int grid = ((n_it + kXcds - 1) / kXcds) * kXcds;                                         \
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), elem_lds[K], s, mem, elem_items + eb[K], n_it, p); \
