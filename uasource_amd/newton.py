@@ -13,8 +13,12 @@ without boundary-condition constraints (``L`` empty):
 * the Newton step ``K d = -R`` (``:356-372``) and the line search in the Newton direction of
   ``UaMain/rLineminUa.m:60-135`` with the backtracking of ``UaMain/BackTracking.m``
   (full step accepted below ``NewtonAcceptRatio * r0``, else guarded quadratic fits down to
-  a step of 1e-3). The Cauchy and steepest-descent fall-backs of ``rLineminUa`` are not
-  restated: the iteration stops as "backtracking stagnated" where they would be tried.
+  a step of 1e-3). The other directions of ``rLineminUa``'s default "-Auto-" mode are not
+  restated: where the Newton line search ends below gamma = 0.75 with r/r0 above 0.7
+  (``rLineminUa.m:94-103``), the reference also tries a Cauchy step with the mass matrix in
+  place of the Hessian, the path from that point towards the Newton point (``fminbnd``) and
+  steepest descent, and keeps the best point; this driver keeps the Newton point there, and
+  stops as "backtracking stagnated" where the Newton direction gives no reduction.
 
 The residual cost is device-resident (the R-only assemblies, the axpy to the trial state
 and the dot products stay in HBM; three scalars come back per evaluation) and the residual
