@@ -202,12 +202,21 @@ def uv_csr_pattern(mesh: O2Mesh):
 
 
 def assemble_uv(CtrlVar, MUA, F, mesh: O2Mesh | None = None, ronly: bool = False, nthreads: int = 0,
-                want_K: bool = True):
+                want_K: bool = True, BCs=None):
     """uv (diagnostic) assembly, UaMain/uvMatrixAssemblySSTREAM.m. ``F`` carries the surface
-    ``s`` besides the uvh fields. Returns dict(R, Tint, Fext (2nn), Kval, K (CSR or None),
-    status, mesh)."""
+    ``s`` besides the uvh fields. ZeroFields (:24-28): ub = vb = 0, then the fixed values of
+    ``BCs`` at their nodes. Returns dict(R, Tint, Fext (2nn), Kval, K (CSR or None), status,
+    mesh)."""
     if mesh is None:
         mesh = O2Mesh(MUA)
+    zf = CtrlVar.get("uvAssembly")
+    if zf is not None and zf.get("ZeroFields", False):
+        F = F.copy()
+        F.ub = F.ub * 0
+        F.vb = F.vb * 0
+        if BCs is not None:
+            F.ub[BCs.ubFixedNode] = BCs.ubFixedValue
+            F.vb[BCs.vbFixedNode] = BCs.vbFixedValue
     P = params_from(CtrlVar, F, ronly=ronly, zero_fields=False)
     P.reset_thickness = 0
     dvisk = 0.0 if CtrlVar.Picard else float(CtrlVar.NRviscosity)

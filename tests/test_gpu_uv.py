@@ -4,8 +4,9 @@ pattern bit-exact; R within 1e-12 of max(||Tint||, ||Fext||); K within 1e-12 nor
 oracle's K is the reference's (K + K')/2; the library's is symmetric to rounding, checked to
 1e-14); R-only gives the R of the R+K call bit for bit; runs repeat bit for bit; the
 precision form (FP32 element arithmetic, FP64 accumulation) within 1e-6 (R) and 1e-5 (K); the
-Cmin/AGlenmin floors and Picard (no viscosity-derivative terms) as in the oracle; ZeroFields
-is refused."""
+Cmin/AGlenmin floors and Picard (no viscosity-derivative terms) as in the oracle; the
+ZeroFields flag is refused by the C ABI, and CtrlVar.uvAssembly.ZeroFields through the Python
+mirror (which zeroes the velocities but for the fixed values of BCs) matches the oracle."""
 import numpy as np
 import pytest
 
@@ -111,5 +112,27 @@ def test_uv_floors_picard_and_refusals(o2lib):
                                      uvh.C.byref(fields), uvh.UVH_ZEROFIELDS | uvh.UVH_RONLY, uvh._ptr(R), None,
                                      None, None)
         assert st == uvh.UVH_E_UNSUPPORTED
+    finally:
+        ctx.close()
+
+
+def test_uv_zero_fields_through_the_mirror(o2lib):
+    print("[gpu] uv ZeroFields", flush=True)
+    from uasource_amd.fields import BoundaryConditions
+    M, F, mesh = case(o2lib, "medium_p2")
+    x = M.coordinates[:, 0]
+    inflow = np.flatnonzero(x <= x.min() + 1.0)
+    BCs = BoundaryConditions(ubFixedNode=inflow, ubFixedValue=np.full(inflow.size, 100.0),
+                             vbFixedNode=inflow, vbFixedValue=np.zeros(inflow.size))
+    cv = default_ctrlvar()
+    cv.uvAssembly.ZeroFields = True
+    ref = o2lib.assemble_uv(cv, M, F, mesh=mesh, BCs=BCs)
+    ctx = uvh.UvhContext(0)
+    try:
+        ctx.set_mesh(M)
+        out = ctx.assemble_uv(cv, F, want_tf=True, BCs=BCs)
+        scale = max(np.linalg.norm(ref["Tint"]), np.linalg.norm(ref["Fext"]))
+        assert np.linalg.norm(out["R"] - ref["R"]) / scale <= 1e-12
+        assert rel(out["Kval"], ref["Kval"]) <= 1e-12
     finally:
         ctx.close()

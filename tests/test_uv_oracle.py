@@ -98,3 +98,23 @@ def test_uv_clips_and_picard(o2lib):
     assert np.array_equal(pk["R"], base["R"])
     assert not np.array_equal(pk["Kval"], base["Kval"])
     assert (pk["K"] != pk["K"].T).nnz == 0
+
+
+def test_uv_zero_fields(o2lib):
+    """ZeroFields (uvMatrixAssemblySSTREAM.m:24-28): u = v = 0 except the fixed values at the
+    fixed nodes; the same as passing those velocities."""
+    from uasource_amd.fields import BoundaryConditions
+    cv, M, F, ref = _setup(o2lib, "tiny_p2")
+    x = M.coordinates[:, 0]
+    inflow = np.flatnonzero(x <= x.min() + 1.0)
+    BCs = BoundaryConditions(ubFixedNode=inflow, ubFixedValue=np.full(inflow.size, 100.0),
+                             vbFixedNode=inflow, vbFixedValue=np.zeros(inflow.size))
+    c = cv.copy()
+    c.uvAssembly.ZeroFields = True
+    a = o2lib.assemble_uv(c, M, F, mesh=ref["mesh"], BCs=BCs)
+    G = F.copy()
+    G.ub, G.vb = np.zeros(M.Nnodes), np.zeros(M.Nnodes)
+    G.ub[inflow] = 100.0
+    b = o2lib.assemble_uv(cv, M, G, mesh=ref["mesh"])
+    assert np.array_equal(a["R"], b["R"]) and np.array_equal(a["Kval"], b["Kval"])
+    assert not np.array_equal(a["R"], o2lib.assemble_uv(cv, M, F, mesh=ref["mesh"])["R"])

@@ -376,11 +376,20 @@ class UvhContext:
         self._check(self.lib.uvh_uv_get_pattern(self.h, _ptr(rowptr), _ptr(colind)), "uvh_uv_get_pattern")
         return rowptr, colind
 
-    def assemble_uv(self, CtrlVar, F, ronly=False, want_tf=False, nan_check=True, fp32=False):
+    def assemble_uv(self, CtrlVar, F, ronly=False, want_tf=False, nan_check=True, fp32=False, BCs=None):
         """uv (diagnostic) assembly from host arrays (``F`` with the upper surface ``s``);
-        returns dict(R (2Nn), Kval (4 nnzN) or None, Tint, Fext)."""
+        returns dict(R (2Nn), Kval (4 nnzN) or None, Tint, Fext). With
+        ``CtrlVar.uvAssembly.ZeroFields`` the velocities are zero except at the fixed nodes of
+        ``BCs``, which take their fixed values (``uvMatrixAssemblySSTREAM.m:24-28``); the
+        library assembles those velocities."""
         ctrl = make_ctrl(CtrlVar, F)
         uvc = make_uv_ctrl(CtrlVar)
+        if CtrlVar.uvAssembly.ZeroFields:
+            F = F.copy()
+            F.ub, F.vb = np.zeros(self.nn), np.zeros(self.nn)
+            if BCs is not None:
+                F.ub[np.asarray(BCs.ubFixedNode, dtype=np.int64)] = BCs.ubFixedValue
+                F.vb[np.asarray(BCs.vbFixedNode, dtype=np.int64)] = BCs.vbFixedValue
         keep = []
         fields = make_uv_fields(F, keep)
         flags = (UVH_RONLY if ronly else 0) | (0 if nan_check else UVH_NO_NAN_CHECK) | (UVH_FP32 if fp32 else 0)
