@@ -82,6 +82,10 @@ def main(argv=None):
     ap.add_argument("--uv", action="store_true", help="uv (diagnostic) assembly of the config-2 mesh")
     ap.add_argument("--device-solver", action="store_true",
                     help="with --newton: linear solves on the GPU (block ILU(0) BiCGStab)")
+    ap.add_argument("--checkpoint", default="", metavar="FILE",
+                    help="with --newton: FP64 solve of --workload in calls of --time-limit seconds, resumed "
+                         "from FILE (.npz) until it converges")
+    ap.add_argument("--time-limit", type=float, default=900.0, help="with --checkpoint: seconds per call")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -111,6 +115,8 @@ def main(argv=None):
             line = bench_modes.bench_serve(log=log)
         elif args.uv:
             line = bench_modes.bench_uv(args.steps, args.warmup, cpu_seconds=args.cpu_seconds, log=log)
+        elif args.checkpoint:
+            line = bench_modes.bench_newton_long(args.workload, args.time_limit, args.checkpoint, log=log)
         else:
             line = bench_modes.bench_newton("newton_p2" if args.workload == "plain_p2" and not args.device_solver
                                             else args.workload, device_solver=args.device_solver, log=log)

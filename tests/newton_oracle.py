@@ -18,6 +18,9 @@ class OracleAssembler:
         self.n_rk = self.n_r = 0
         self.free = None
 
+    def set_state(self, x):
+        self.x = np.array(x, dtype=float)
+
     def set_fixed(self, fixed):
         self.free = None
         if fixed is not None and len(fixed):

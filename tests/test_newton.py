@@ -60,6 +60,26 @@ def test_host_solvers_agree(o2lib):
     assert lagged.factorizations < ref.iterations and max(lagged.krylov) > 0
 
 
+def test_checkpoint_and_resume_give_the_uninterrupted_run(o2lib):
+    """A run stopped at its time limit and resumed from the checkpoint, one iteration per call,
+    ends where the uninterrupted run ends, bit for bit."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    ck, calls = None, 0
+    while True:
+        res = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, fixed=fixed, time_limit=0.0, resume=ck)
+        calls += 1
+        if res.checkpoint is None:
+            break
+        ck = res.checkpoint
+        assert res.reason == "time limit" and calls < 100
+    assert calls == ref.iterations + 1 or calls == ref.iterations
+    assert res.converged and res.iterations == ref.iterations and res.rForce == ref.rForce
+    for x, y in ((res.ub, ref.ub), (res.vb, ref.vb), (res.h, ref.h)):
+        assert np.array_equal(x, y)
+    assert [h[:4] for h in res.history] == [h[:4] for h in ref.history]
+
+
 def test_line_search_accepts_full_step_and_backtracks():
     cv = default_ctrlvar()
     # full step below NewtonAcceptRatio * r0

@@ -273,6 +273,58 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
     }
 
 
+def bench_newton_long(workload: str, time_limit: float, checkpoint: str, log=print) -> dict:
+    """FP64 Newton solve of a workload too large for one call (the full config-2 mesh: ~65 s
+    per host factorization): runs for ``time_limit`` seconds, then saves the iteration's state
+    to ``checkpoint`` (.npz of plain arrays); the next call resumes from it. The line reports the
+    totals over all calls so far and ``"status": "converged"`` once done."""
+    import os
+    from . import newton
+    w = WORKLOADS[workload]
+    M = build_mesh(w)
+    F0, F1, BCs, fixed = newton_problem(M, w)
+    cv = default_ctrlvar()
+    ck, acc = None, {"seconds": 0.0, "assembly_s": 0.0, "solve_s": 0.0, "linesearch_s": 0.0, "rk": 0, "r": 0,
+                     "calls": 0}
+    if os.path.exists(checkpoint):
+        z = np.load(checkpoint)
+        ck = {k: z[k] for k in ("iteration", "gamma", "r", "r0", "rForce", "rWork", "residual64", "backtrack_ok",
+                                "ratios", "history", "x")}
+        acc = {k: float(z["acc_" + k]) for k in acc}
+        log(f"newton {workload}: resuming at iteration {int(ck['iteration'])}")
+    t = time.perf_counter()
+    asm = newton.GpuAssembler(cv, M, F0, F1)
+    log(f"newton fp64 on {workload} ({M.Nele} elements, {3 * M.Nnodes} unknowns), time limit {time_limit:.0f} s")
+    res = newton.solve(asm, cv, fixed=fixed, verbose=True, log=log, time_limit=time_limit, resume=ck)
+    acc["seconds"] += time.perf_counter() - t
+    acc["assembly_s"] += res.t_assembly
+    acc["solve_s"] += res.t_solve
+    acc["linesearch_s"] += res.t_linesearch
+    acc["rk"] += res.n_rk
+    acc["r"] += res.n_r
+    acc["calls"] += 1
+    asm.ctx.close()
+    if res.checkpoint is not None:
+        c = res.checkpoint
+        np.savez(checkpoint, **{k: np.asarray(c[k], dtype=float) for k in c},
+                 **{"acc_" + k: np.float64(v) for k, v in acc.items()})
+    elif os.path.exists(checkpoint):
+        os.remove(checkpoint)
+    hist = [[float(v) for v in h[:4]] for h in res.history]
+    log(f"newton fp64: {res.iterations} iterations, {res.reason}, {acc['seconds']:.0f} s over {int(acc['calls'])} calls")
+    return {
+        "metric": "uvh Newton solve to rForce < 1e-15", "value": acc["seconds"], "unit": "s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": acc["seconds"] * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, inflow conditions)",
+        "config": {"workload": workload, "Nele": M.Nele, "Nnodes": M.Nnodes,
+                   "linear_solver": "SciPy SuperLU (host, minimum degree)", "parallelism": "1 GPU"},
+        "status": "converged" if res.converged else res.reason, "iterations": int(res.iterations),
+        "rForce": float(res.rForce), "calls": int(acc["calls"]), "assembly_s": acc["assembly_s"],
+        "solve_s": acc["solve_s"], "linesearch_s": acc["linesearch_s"], "rk_assemblies": int(acc["rk"]),
+        "r_assemblies": int(acc["r"]), "history": hist,
+    }
+
+
 def bench_uv(steps: int, warmup: int, workload: str = "plain_p2", cpu_seconds: float = 10.0, log=print) -> dict:
     """uv (diagnostic SSTREAM) assembly, R and the 2 x 2-block K, of the config-2 mesh (same
     fields as the uvh workload, s from the floatation of h): FP64 and the precision form, with

@@ -61,6 +61,7 @@ class NewtonResult:
     n_rk: int = 0
     n_r: int = 0
     linear_iterations: list = field(default_factory=list)   # Krylov iterations per Newton step (device solver)
+    checkpoint: dict | None = None   # set when the run stopped at its time limit: pass as ``resume``
     ub: np.ndarray | None = None
     vb: np.ndarray | None = None
     h: np.ndarray | None = None
@@ -263,6 +264,10 @@ class GpuAssembler:
         n = self.nn
         return x[:n].copy(), x[n:2 * n].copy(), x[2 * n:].copy()
 
+    def set_state(self, x: np.ndarray):
+        """Newton state [ub; vb; h] (resuming a run)."""
+        self.x.copy_(self.torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)))
+
 
 def _criteria(CtrlVar, gamma, rWork, rForce, iteration):
     """Exit test of SSTREAM_TransientImplicit.m:269-287."""
@@ -324,9 +329,14 @@ def eliminate(linear_solver, fixed: np.ndarray | None):
 
 
 def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, fp64_switch: float = 1e-8,
-          fixed: np.ndarray | None = None, verbose: bool = False, log=print) -> NewtonResult:
+          fixed: np.ndarray | None = None, verbose: bool = False, log=print, time_limit: float | None = None,
+          resume: dict | None = None) -> NewtonResult:
     """Newton-Raphson uvh iteration (SSTREAM_TransientImplicit.m:224-418). ``fixed``: degrees of
-    freedom held at their current values (their residual rows are left out of the cost)."""
+    freedom held at their current values (their residual rows are left out of the cost).
+    ``time_limit`` (seconds): once at least one iteration has run in this call, stop before
+    starting an iteration past it, with the loop state in ``NewtonResult.checkpoint``;
+    ``resume`` (such a checkpoint) continues that run where it stopped, with the same result as
+    one uninterrupted run."""
     on_device = getattr(linear_solver, "on_device", False)
     if on_device:
         # the device solver holds its own fixed set (identity rows and columns)
@@ -346,6 +356,15 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
     residual64 = not fp32
     ratios = np.zeros(3)
     backtrack_ok = True
+    if resume is not None:
+        iteration, gamma, r, r0 = int(resume["iteration"]), float(resume["gamma"]), float(resume["r"]), float(resume["r0"])
+        rForce, rWork = float(resume["rForce"]), float(resume["rWork"])
+        residual64, backtrack_ok = bool(resume["residual64"]), bool(resume["backtrack_ok"])
+        ratios = np.array(resume["ratios"], dtype=float)
+        res.history = [tuple(h) for h in resume["history"]]
+        assembler.set_state(resume["x"])
+    t_start = time.perf_counter()
+    done_here = 0
     while True:
         rw = 0.0 if np.isnan(rWork) else rWork
         if residual64 and _criteria(CtrlVar, gamma, rw, rForce, iteration):
@@ -373,7 +392,14 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             res.converged = rw < 1e-14
             res.reason = "stagnated"
             break
+        if time_limit is not None and done_here > 0 and time.perf_counter() - t_start > time_limit:
+            res.reason = "time limit"
+            res.checkpoint = dict(iteration=iteration, gamma=gamma, r=r, r0=r0, rForce=rForce, rWork=rWork,
+                                  residual64=residual64, backtrack_ok=backtrack_ok, ratios=ratios.copy(),
+                                  history=list(res.history), x=np.concatenate(assembler.state()))
+            break
         iteration += 1
+        done_here += 1
         if fp32 and not residual64 and rForce < fp64_switch:
             residual64 = True   # from here on the residual, the step and the exit test are FP64
         fp32_res = fp32 and not residual64
