@@ -86,6 +86,9 @@ def main(argv=None):
                     help="with --newton: FP64 solve of --workload in calls of --time-limit seconds, resumed "
                          "from FILE (.npz) until it converges")
     ap.add_argument("--time-limit", type=float, default=900.0, help="with --checkpoint: seconds per call")
+    ap.add_argument("--line-search", choices=("newton", "auto"), default="newton",
+                    help='with --checkpoint: the Newton direction only, or rLineminUa\'s "-Auto-" with the '
+                         "Cauchy step with the mass matrix")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -116,7 +119,8 @@ def main(argv=None):
         elif args.uv:
             line = bench_modes.bench_uv(args.steps, args.warmup, cpu_seconds=args.cpu_seconds, log=log)
         elif args.checkpoint:
-            line = bench_modes.bench_newton_long(args.workload, args.time_limit, args.checkpoint, log=log)
+            line = bench_modes.bench_newton_long(args.workload, args.time_limit, args.checkpoint,
+                                                 "-Auto-" if args.line_search == "auto" else "-Newton Step-", log=log)
         else:
             line = bench_modes.bench_newton("newton_p2" if args.workload == "plain_p2" and not args.device_solver
                                             else args.workload, device_solver=args.device_solver, log=log)

@@ -8,6 +8,7 @@ EPS = np.finfo(float).eps
 class OracleAssembler:
     def __init__(self, o2, CtrlVar, MUA, F0, F1, nthreads=8):
         self.o2, self.cv, self.M, self.F0 = o2, CtrlVar, MUA, F0
+        self.mua = MUA
         self.F1 = F1.copy()
         self.mesh = o2.O2Mesh(MUA)
         self.nthreads = nthreads

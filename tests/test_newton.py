@@ -80,6 +80,28 @@ def test_checkpoint_and_resume_give_the_uninterrupted_run(o2lib):
     assert [h[:4] for h in res.history] == [h[:4] for h in ref.history]
 
 
+def test_auto_line_search_takes_the_cauchy_step_where_newton_fails(o2lib):
+    """rLineMinUa "-Auto-": with a linear solve that returns the reversed Newton step, the
+    Newton direction gives no reduction; the Newton-only line search stops at once, while
+    the Cauchy step with the mass matrix (rLineminUa.m:137-259) still reduces rForce at every
+    iteration. Where the Newton step does well, both give the same iteration."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    cv.NRitmax = 3
+    reverse = lambda K, rhs: -newton.superlu_solve(K, rhs)  # noqa: E731
+    plain = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, linear_solver=reverse, fixed=fixed)
+    assert plain.reason == "backtracking stagnated" and plain.iterations == 1
+    cv.rLineMinUa = "-Auto-"
+    auto = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, linear_solver=reverse, fixed=fixed)
+    assert auto.cauchy_steps == auto.iterations >= 3
+    r = [h[2] for h in auto.history]
+    assert all(b < a for a, b in zip(r, r[1:]))
+    c = default_ctrlvar()
+    c.rLineMinUa = "-Auto-"
+    same = newton.solve(OracleAssembler(o2lib, c, M, F0, F1), c, fixed=fixed)
+    assert same.iterations == ref.iterations and same.rForce == ref.rForce
+
+
 def test_line_search_accepts_full_step_and_backtracks():
     cv = default_ctrlvar()
     # full step below NewtonAcceptRatio * r0

@@ -273,7 +273,8 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
     }
 
 
-def bench_newton_long(workload: str, time_limit: float, checkpoint: str, log=print) -> dict:
+def bench_newton_long(workload: str, time_limit: float, checkpoint: str, line_search: str = "-Newton Step-",
+                      log=print) -> dict:
     """FP64 Newton solve of a workload too large for one call (the full config-2 mesh: ~65 s
     per host factorization): runs for ``time_limit`` seconds, then saves the iteration's state
     to ``checkpoint`` (.npz of plain arrays); the next call resumes from it. The line reports the
@@ -284,8 +285,9 @@ def bench_newton_long(workload: str, time_limit: float, checkpoint: str, log=pri
     M = build_mesh(w)
     F0, F1, BCs, fixed = newton_problem(M, w)
     cv = default_ctrlvar()
+    cv.rLineMinUa = line_search
     ck, acc = None, {"seconds": 0.0, "assembly_s": 0.0, "solve_s": 0.0, "linesearch_s": 0.0, "rk": 0, "r": 0,
-                     "calls": 0}
+                     "calls": 0, "cauchy": 0}
     if os.path.exists(checkpoint):
         z = np.load(checkpoint)
         ck = {k: z[k] for k in ("iteration", "gamma", "r", "r0", "rForce", "rWork", "residual64", "backtrack_ok",
@@ -303,6 +305,7 @@ def bench_newton_long(workload: str, time_limit: float, checkpoint: str, log=pri
     acc["rk"] += res.n_rk
     acc["r"] += res.n_r
     acc["calls"] += 1
+    acc["cauchy"] = acc.get("cauchy", 0.0) + res.cauchy_steps
     asm.ctx.close()
     if res.checkpoint is not None:
         c = res.checkpoint
@@ -321,7 +324,8 @@ def bench_newton_long(workload: str, time_limit: float, checkpoint: str, log=pri
         "status": "converged" if res.converged else res.reason, "iterations": int(res.iterations),
         "rForce": float(res.rForce), "calls": int(acc["calls"]), "assembly_s": acc["assembly_s"],
         "solve_s": acc["solve_s"], "linesearch_s": acc["linesearch_s"], "rk_assemblies": int(acc["rk"]),
-        "r_assemblies": int(acc["r"]), "history": hist,
+        "r_assemblies": int(acc["r"]), "line_search": line_search, "cauchy_steps": int(acc["cauchy"]),
+        "history": hist,
     }
 
 

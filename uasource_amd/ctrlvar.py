@@ -74,6 +74,10 @@ def default_ctrlvar() -> Struct:
         NRitmax=50,                               # :597
         NRitmin=1,                                # :600
         NewtonAcceptRatio=0.5,                    # :601
+        # line search of the uvh Newton step: "-Newton Step-" (the Newton direction only) or
+        # "-Auto-", which adds rLineminUa.m's Cauchy step with the mass matrix where the Newton
+        # direction falls short (rLineminUa.m:38 sets "-Auto-" in the reference)
+        rLineMinUa="-Newton Step-",
         BacktrackingGammaMin=1e-10,               # :604
         BackTrackBeta=0.1,                        # :616
         BackTrackMaxIterations=50,                # :617

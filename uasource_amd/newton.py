@@ -13,12 +13,14 @@ without boundary-condition constraints (``L`` empty):
 * the Newton step ``K d = -R`` (``:356-372``) and the line search in the Newton direction of
   ``UaMain/rLineminUa.m:60-135`` with the backtracking of ``UaMain/BackTracking.m``
   (full step accepted below ``NewtonAcceptRatio * r0``, else guarded quadratic fits down to
-  a step of 1e-3). The other directions of ``rLineminUa``'s default "-Auto-" mode are not
-  restated: where the Newton line search ends below gamma = 0.75 with r/r0 above 0.7
-  (``rLineminUa.m:94-103``), the reference also tries a Cauchy step with the mass matrix in
-  place of the Hessian, the path from that point towards the Newton point (``fminbnd``) and
-  steepest descent, and keeps the best point; this driver keeps the Newton point there, and
-  stops as "backtracking stagnated" where the Newton direction gives no reduction.
+  a step of 1e-3). ``rLineminUa`` always runs its "-Auto-" mode (``rLineminUa.m:38``): where
+  the Newton line search ends below gamma = 0.75 with r/r0 above 0.7 (``:94-103``), it also
+  tries a Cauchy step with the mass matrix in place of the Hessian (``:137-259``), the path
+  from that point towards the Newton point (``fminbnd``) and steepest descent, and keeps the
+  best point. With ``CtrlVar.rLineMinUa = "-Auto-"`` this driver adds the Cauchy step (host
+  linear solves only; the other two are not restated); the default here, "-Newton Step-",
+  keeps the Newton point and stops as "backtracking stagnated" where the Newton direction
+  gives no reduction.
 
 The residual cost is device-resident (the R-only assemblies, the axpy to the trial state
 and the dot products stay in HBM; three scalars come back per evaluation) and the residual
@@ -62,6 +64,7 @@ class NewtonResult:
     n_r: int = 0
     linear_iterations: list = field(default_factory=list)   # Krylov iterations per Newton step (device solver)
     checkpoint: dict | None = None   # set when the run stopped at its time limit: pass as ``resume``
+    cauchy_steps: int = 0            # iterations that took the Cauchy step (rLineMinUa "-Auto-")
     ub: np.ndarray | None = None
     vb: np.ndarray | None = None
     h: np.ndarray | None = None
@@ -162,6 +165,7 @@ class GpuAssembler:
         uvh.check_supported(CtrlVar)
         self.ctx = ctx or uvh.UvhContext(device)
         self.ctx.set_mesh(MUA)
+        self.mua = MUA
         self.nn = MUA.Nnodes
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.dev)
         self.f1 = {k: t(getattr(F1, k)) for k in uvh.F1_KEYS}
@@ -269,6 +273,104 @@ class GpuAssembler:
         self.x.copy_(self.torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)))
 
 
+# Lagrange shape functions in Úa's local node order (UaMain/shape_fun.m:63-82), as barycentric
+# multi-indices (c1, c2, c3) with c1 = xi, c2 = eta, c3 = 1 - xi - eta
+_NODE_INDEX = {
+    3: [(1, 0, 0), (0, 0, 1), (0, 1, 0)],
+    6: [(2, 0, 0), (1, 0, 1), (0, 0, 2), (0, 1, 1), (0, 2, 0), (1, 1, 0)],
+    10: [(3, 0, 0), (2, 0, 1), (1, 0, 2), (0, 0, 3), (0, 1, 2), (0, 2, 1), (0, 3, 0), (1, 2, 0), (2, 1, 0), (1, 1, 1)],
+}
+_CORNERS = {3: (0, 1, 2), 6: (0, 2, 4), 10: (0, 3, 6)}   # nodes at c1 = 1, c3 = 1, c2 = 1
+
+
+def shape_functions(nod: int, points: np.ndarray) -> np.ndarray:
+    """N (npoints x nod) at (xi, eta) points: the degree-p Lagrange basis on the triangle,
+    prod_i prod_{j < a_i} (p c_i - j) / (j + 1) for node multi-index a (|a| = p)."""
+    p = {3: 1, 6: 2, 10: 3}[nod]
+    c = np.stack([points[:, 0], points[:, 1], 1.0 - points[:, 0] - points[:, 1]], axis=1)
+    N = np.ones((points.shape[0], nod))
+    for k, a in enumerate(_NODE_INDEX[nod]):
+        for i in range(3):
+            for j in range(a[i]):
+                N[:, k] *= (p * c[:, i] - j) / (j + 1)
+    return N
+
+
+def mass_matrix(MUA) -> sp.csr_matrix:
+    """Nodal mass matrix M_ij = integral of N_i N_j (UaMain/MassMatrix2D1dof.m), with MUA's
+    quadrature; the elements are straight-sided, so each element's matrix is its detJ times
+    that of the reference triangle. Symmetrised as (M + M')/2, as there."""
+    nod = MUA.nod
+    N = shape_functions(nod, np.asarray(MUA.points, dtype=float))
+    Mref = (N * np.asarray(MUA.weights, dtype=float)[:, None]).T @ N
+    conn = np.asarray(MUA.connectivity, dtype=np.int64)
+    x, y = MUA.coordinates[:, 0], MUA.coordinates[:, 1]
+    a, b, c = _CORNERS[nod]
+    detJ = (x[conn[:, a]] - x[conn[:, b]]) * (y[conn[:, c]] - y[conn[:, b]]) - \
+           (y[conn[:, a]] - y[conn[:, b]]) * (x[conn[:, c]] - x[conn[:, b]])
+    rows = np.repeat(conn, nod, axis=1).ravel()
+    cols = np.tile(conn, (1, nod)).ravel()
+    vals = (detJ[:, None] * Mref.ravel()[None, :]).ravel()
+    n = MUA.Nnodes
+    M = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    return ((M + M.T) * 0.5).tocsr()
+
+
+class _CauchyM:
+    """The Cauchy step with the mass matrix in place of the Hessian (rLineminUa.m:137-259):
+    direction s = Mblock^-1 frhs on the free degrees of freedom (Mblock = diag(M, M, M)), the
+    minimum of the quadratic model along it as the first trial, and backtracking with
+    NewtonAcceptRatio 0.9 down to a step of 1e-10."""
+
+    def __init__(self, MUA, fixed):
+        self.nn = MUA.Nnodes
+        M = mass_matrix(MUA)
+        free = np.ones(3 * self.nn, dtype=bool)
+        if fixed is not None and len(fixed):
+            free[np.asarray(fixed, dtype=np.int64)] = False
+        self.free = free
+        self.lu = []
+        for b in range(3):
+            f = free[b * self.nn:(b + 1) * self.nn]
+            self.lu.append((f, spla.splu(M[f][:, f].tocsc())))
+
+    def direction(self, frhs):
+        s = np.zeros(3 * self.nn)
+        for b, (f, lu) in enumerate(self.lu):
+            blk = slice(b * self.nn, (b + 1) * self.nn)
+            sb = np.zeros(self.nn)
+            sb[f] = lu.solve(frhs[blk][f])
+            s[blk] = sb
+        return s
+
+    def step(self, CtrlVar, K, frhs, cost, r0, N):
+        """(trial step along s that the backtracking returned, its cost triple, s, the quadratic
+        model's step) or None when the model gives no usable slope."""
+        free = self.free
+        s = self.direction(frhs)
+        Kf = K[free][:, free]
+        Hs = Kf @ s[free]
+        Rf = frhs[free]
+        RHs = float(Rf @ Hs)
+        HsHs = float(Hs @ Hs)
+        if HsHs == 0.0:
+            return None
+        slope0 = -2.0 * RHs / N
+        gC = RHs / HsHs
+        if slope0 > 0:   # the other way along s (rLineminUa.m:180-199)
+            s, RHs = -s, -RHs
+            slope0, gC = -2.0 * RHs / N, RHs / HsHs
+        if slope0 > 0:
+            slope0 = -slope0
+            gC = -0.1 * r0 / slope0
+        if gC < 0:
+            gC = -0.1 * r0 / slope0
+        c1 = cost(gC, s)
+        g, best, _ = _line_search(CtrlVar, lambda g: cost(g, s), r0, c1, slope0=slope0, b0=gC, gamma_min=1e-10,
+                                  accept_ratio=0.9)
+        return g, best, s, gC
+
+
 def _criteria(CtrlVar, gamma, rWork, rForce, iteration):
     """Exit test of SSTREAM_TransientImplicit.m:269-287."""
     if gamma > max(CtrlVar.uvhExitBackTrackingStepLength, CtrlVar.BacktrackingGammaMin):
@@ -280,17 +382,19 @@ def _criteria(CtrlVar, gamma, rWork, rForce, iteration):
     return (rWork < aw and rForce < af) and (rWork < ow or rForce < of) and iteration >= CtrlVar.NRitmin
 
 
-def _line_search(CtrlVar, cost, r0, r1):
-    """Newton-direction backtracking (rLineminUa.m:60-135, BackTracking.m): returns
-    (gamma, (rForce, rWork, D2) at gamma, number of trial evaluations)."""
-    slope0 = -2.0 * r0
-    target = max(CtrlVar.NewtonAcceptRatio * r0, min(r0 + CtrlVar.BackTrackBeta * slope0, CtrlVar.NewtonAcceptRatio * r0))
-    best_g, best = 1.0, r1
+def _line_search(CtrlVar, cost, r0, r1, slope0=None, b0=1.0, gamma_min=1e-3, accept_ratio=None):
+    """Backtracking (BackTracking.m) from the trial step ``b0`` with value ``r1``: returns
+    (gamma, (rForce, rWork, D2) at gamma, number of trial evaluations). The defaults are the
+    Newton direction's (rLineminUa.m:60-135: slope -2 r0, first trial 1, floor 1e-3)."""
+    if slope0 is None:
+        slope0 = -2.0 * r0
+    ratio = CtrlVar.NewtonAcceptRatio if accept_ratio is None else accept_ratio
+    target = max(ratio * r0, min(r0 + CtrlVar.BackTrackBeta * slope0 * b0, ratio * r0))
+    best_g, best = b0, r1
     if r1[0] < target:
-        return 1.0, r1, 0
-    gamma_min = 1e-3                      # rLineminUa.m:90 in the Newton direction
+        return b0, r1, 0
     lower, upper = CtrlVar.BackTrackGuardLower, CtrlVar.BackTrackGuardUpper
-    b, fb = 1.0, r1[0]
+    b, fb = b0, r1[0]
     evals = 0
     for _ in range(CtrlVar.BackTrackMaxIterations):
         # minimum of the quadratic through f(0) = r0, f'(0) = slope0 and f(b) = fb (BackTracking.m:268)
@@ -347,6 +451,11 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             raise ValueError("the device linear solver was built for a different set of fixed degrees of freedom")
     else:
         linear_solver = eliminate(linear_solver, fixed)
+    cauchy = None
+    if "-Auto-" in str(CtrlVar.get("rLineMinUa", "")):
+        if on_device:
+            raise ValueError('rLineMinUa "-Auto-" needs K on the host (the host linear solver)')
+        cauchy = _CauchyM(assembler.mua, fixed)
     assembler.set_fixed(fixed)
     N = assembler.normalisation()
     res = NewtonResult(False, 0, "", float("inf"), float("nan"))
@@ -420,9 +529,25 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
         c0 = cost(0.0)
         c1 = cost(1.0)
         gamma, best, _ = _line_search(CtrlVar, cost, c0[0], c1)
+        step = gamma   # the length taken along the direction the assembler holds
+        rmin = best[0] if gamma > 0 else c0[0]
+        if cauchy is not None and gamma < 0.75 and rmin / c0[0] > 0.7:
+            # rLineminUa.m:94-103: the Newton step fell short; try the Cauchy step with the mass matrix
+            def cost_along(g, s):
+                assembler.set_step(s)
+                return assembler.cost(g, fp32_res, N)
+            out = cauchy.step(CtrlVar, K, rhs, cost_along, c0[0], N)
+            if out is not None and out[1][0] < rmin:
+                gC, bestC, s, gModel = out
+                assembler.set_step(s)
+                step, best = gC, (bestC[0], float("nan"), float("nan"))
+                gamma = gC / gModel   # returned normalised by the quadratic model's step, as there
+                res.cauchy_steps += 1
+            else:
+                assembler.set_step(d)
         backtrack_ok = gamma > 0
         if gamma > 0:
-            assembler.accept(gamma)
+            assembler.accept(step)
             rForce, rWork = best[0], best[1]
             r = rForce if CtrlVar.uvhMinimisationQuantity == "Force Residuals" else rWork
         r0 = c0[0] if CtrlVar.uvhMinimisationQuantity == "Force Residuals" else c0[1]
