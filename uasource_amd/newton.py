@@ -17,8 +17,9 @@ without boundary-condition constraints (``L`` empty):
   the Newton line search ends below gamma = 0.75 with r/r0 above 0.7 (``:94-103``), it also
   tries a Cauchy step with the mass matrix in place of the Hessian (``:137-259``), the path
   from that point towards the Newton point (``fminbnd``) and steepest descent, and keeps the
-  best point. With ``CtrlVar.rLineMinUa = "-Auto-"`` this driver adds the Cauchy step (host
-  linear solves only; the other two are not restated); the default here, "-Newton Step-",
+  best point. With ``CtrlVar.rLineMinUa = "-Auto-"`` this driver adds the Cauchy step and the
+  Cauchy-to-Newton path (host linear solves only; steepest descent, which the reference tries
+  where the Cauchy step gives no reduction, is not restated); the default here, "-Newton Step-",
   keeps the Newton point and stops as "backtracking stagnated" where the Newton direction
   gives no reduction.
 
@@ -64,7 +65,7 @@ class NewtonResult:
     n_r: int = 0
     linear_iterations: list = field(default_factory=list)   # Krylov iterations per Newton step (device solver)
     checkpoint: dict | None = None   # set when the run stopped at its time limit: pass as ``resume``
-    cauchy_steps: int = 0            # iterations that took the Cauchy step (rLineMinUa "-Auto-")
+    cauchy_steps: int = 0            # iterations that took a Cauchy or Cauchy-to-Newton point (rLineMinUa "-Auto-")
     ub: np.ndarray | None = None
     vb: np.ndarray | None = None
     h: np.ndarray | None = None
@@ -75,17 +76,26 @@ def superlu_solve(K: sp.csr_matrix, rhs: np.ndarray, rtol: float = 1e-10) -> np.
     order is minimum degree on A'A + A with diagonal pivots kept unless one falls below 1e-3
     of its column (at 12,800 P2 elements: 1.0 s and 15 M factor entries, against 4-5.5 s and
     53 M with COLAMD and partial pivoting). If that factorization fails or leaves a relative
-    residual above ``rtol``, the COLAMD factorization with partial pivoting solves instead."""
+    residual above ``rtol``, the COLAMD factorization with partial pivoting solves instead, and
+    so for every later system of the same pattern (the Newton systems of one run share it: at
+    the full 200k-element size the diagonal-pivot factorization fails from the second step on,
+    after ~40-90 s of factoring)."""
     A = K.tocsc()
-    try:
-        x = spla.splu(A, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=1e-3,
-                      options=dict(SymmetricMode=True)).solve(rhs)
-        bn = np.linalg.norm(rhs)
-        if bn == 0 or np.linalg.norm(rhs - A @ x) <= rtol * bn:
-            return x
-    except RuntimeError:   # a singular pivot in the diagonal-pivot factorization
-        pass
+    key = (A.shape[0], A.nnz, hash(A.indices[:: max(1, A.nnz // 4096)].tobytes()))
+    if key not in _DIAGONAL_PIVOTS_FAILED:
+        try:
+            x = spla.splu(A, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=1e-3,
+                          options=dict(SymmetricMode=True)).solve(rhs)
+            bn = np.linalg.norm(rhs)
+            if bn == 0 or np.linalg.norm(rhs - A @ x) <= rtol * bn:
+                return x
+        except RuntimeError:   # a singular pivot in the diagonal-pivot factorization
+            pass
+        _DIAGONAL_PIVOTS_FAILED.add(key)
     return spla.splu(A, permc_spec="COLAMD").solve(rhs)
+
+
+_DIAGONAL_PIVOTS_FAILED: set = set()   # patterns whose diagonal-pivot factorization has failed
 
 
 class LaggedLU:
@@ -537,14 +547,32 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
                 assembler.set_step(s)
                 return assembler.cost(g, fp32_res, N)
             out = cauchy.step(CtrlVar, K, rhs, cost_along, c0[0], N)
-            if out is not None and out[1][0] < rmin:
+            chosen = d
+            if out is not None:
                 gC, bestC, s, gModel = out
-                assembler.set_step(s)
-                step, best = gC, (bestC[0], float("nan"), float("nan"))
-                gamma = gC / gModel   # returned normalised by the quadratic model's step, as there
-                res.cauchy_steps += 1
-            else:
-                assembler.set_step(d)
+                if bestC[0] < rmin:
+                    chosen, rmin = s, bestC[0]
+                    step, best = gC, (bestC[0], float("nan"), float("nan"))
+                    gamma = gC / gModel   # returned normalised by the quadratic model's step, as there
+                    res.cauchy_steps += 1
+                if bestC[0] < c0[0] and rmin / c0[0] >= 0.7:
+                    # rLineminUa.m:270-300, :420-470: from the Cauchy point towards the full Newton
+                    # step, x + xC + t (xN - xC) for t in [0, 1], minimised by Brent's bounded method
+                    # (MATLAB's fminbnd, TolX 0.02)
+                    from scipy.optimize import minimize_scalar
+                    xC = gC * s
+
+                    def along(t):
+                        assembler.set_step(xC + t * (d - xC))
+                        return assembler.cost(1.0, fp32_res, N)[0]
+                    o = minimize_scalar(along, bounds=(0.0, 1.0), method="bounded", options={"xatol": 0.02})
+                    if o.fun < rmin:
+                        chosen = xC + float(o.x) * (d - xC)
+                        assembler.set_step(chosen)
+                        step, best, gamma = 1.0, assembler.cost(1.0, fp32_res, N), float(o.x)
+                        rmin = best[0]
+                        res.cauchy_steps += 1
+            assembler.set_step(chosen)
         backtrack_ok = gamma > 0
         if gamma > 0:
             assembler.accept(step)
