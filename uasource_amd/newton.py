@@ -485,7 +485,9 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
     t_start = time.perf_counter()
     done_here = 0
     while True:
-        rw = 0.0 if np.isnan(rWork) else rWork
+        # rWork is NaN after a Cauchy point (no work measure there): as in MATLAB, every
+        # comparison with it is false, so such a point passes no test that involves rWork
+        rw = rWork
         if residual64 and _criteria(CtrlVar, gamma, rw, rForce, iteration):
             res.converged, res.reason = True, "converged"
             break
