@@ -134,6 +134,29 @@ def test_gpu_newton_matches_oracle_newton(o2lib, name):
 
 
 @pytest.mark.gpu
+def test_gpu_newton_auto_line_search_matches_oracle(o2lib):
+    """rLineMinUa "-Auto-" on the GPU assembly: with the reversed Newton step only the Cauchy
+    points reduce rForce; the GPU-driven run takes them like the oracle-driven one, and with
+    the true step both converge the same way."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    print("[gpu] newton Auto line search cpu_reference", flush=True)
+    cv = default_ctrlvar()
+    cv.rLineMinUa = "-Auto-"
+    cv.NRitmax = 3
+    reverse = lambda K, rhs: -newton.superlu_solve(K, rhs)  # noqa: E731
+    a = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, linear_solver=reverse, fixed=fixed)
+    g = newton.solve(newton.GpuAssembler(cv, M, F0, F1), cv, linear_solver=reverse, fixed=fixed)
+    print(f"   oracle {[h[2] for h in a.history]}, gpu {[h[2] for h in g.history]}", flush=True)
+    assert g.cauchy_steps == a.cauchy_steps == g.iterations == a.iterations
+    for x, y in zip(g.history, a.history):
+        assert abs(x[2] - y[2]) <= 1e-9 * y[2]
+    c = default_ctrlvar()
+    c.rLineMinUa = "-Auto-"
+    res = newton.solve(newton.GpuAssembler(c, M, F0, F1), c, fixed=fixed)
+    assert res.converged and abs(res.iterations - ref.iterations) <= 1 and res.rForce < 1e-15
+
+
+@pytest.mark.gpu
 def test_gpu_newton_precision_form_reaches_fp64_solution(o2lib):
     M, F0, F1, fixed, ref = oracle_solution(o2lib, "medium_p2")
     cv = default_ctrlvar()
