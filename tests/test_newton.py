@@ -157,6 +157,37 @@ def test_gpu_newton_auto_line_search_matches_oracle(o2lib):
 
 
 @pytest.mark.gpu
+def test_gpu_newton_with_other_sliding_laws(o2lib):
+    """Newton under Budd converges on the GPU assembly as on the oracle's (same iterations within
+    one, rForce < 1e-15, same u, v, h); under Cornford (with the Auto line search; the
+    Coulomb-type laws do not converge from this start, on either assembly) the first five
+    iterations follow the oracle-driven ones."""
+    M = build_mesh(WORKLOADS["cpu_reference"])
+    F0, F1, _, fixed = newton_problem(M, WORKLOADS["cpu_reference"])
+    rng = np.random.default_rng(3)
+    F1.q = rng.uniform(0.5, 1.5, M.Nnodes)
+    F1.muk = 0.02 * rng.uniform(0.3, 0.7, M.Nnodes)
+    print("[gpu] newton with the Budd and Cornford sliding laws", flush=True)
+    cv = default_ctrlvar()
+    cv.SlidingLaw = "Budd"
+    a = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, fixed=fixed)
+    g = newton.solve(newton.GpuAssembler(cv, M, F0, F1), cv, fixed=fixed)
+    print(f"   Budd: oracle {a.iterations}, gpu {g.iterations} iterations, rForce {g.rForce:.1e}", flush=True)
+    assert a.converged and g.converged and abs(a.iterations - g.iterations) <= 1 and g.rForce < 1e-15
+    for x, y in ((g.ub, a.ub), (g.vb, a.vb), (g.h, a.h)):
+        assert rel(x, y) < 1e-7
+    cv = default_ctrlvar()
+    cv.SlidingLaw, cv.rLineMinUa, cv.NRitmax = "Cornford", "-Auto-", 5
+    a = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, fixed=fixed)
+    g = newton.solve(newton.GpuAssembler(cv, M, F0, F1), cv, fixed=fixed)
+    ra, rg = [h[2] for h in a.history], [h[2] for h in g.history]
+    print(f"   Cornford: oracle {ra}, gpu {rg}", flush=True)
+    assert len(ra) == len(rg) >= 5
+    for x, y in zip(rg, ra):
+        assert abs(x - y) <= 1e-6 * y
+
+
+@pytest.mark.gpu
 def test_gpu_newton_precision_form_reaches_fp64_solution(o2lib):
     M, F0, F1, fixed, ref = oracle_solution(o2lib, "medium_p2")
     cv = default_ctrlvar()
