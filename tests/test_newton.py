@@ -102,6 +102,29 @@ def test_auto_line_search_takes_the_cauchy_step_where_newton_fails(o2lib):
     assert same.iterations == ref.iterations and same.rForce == ref.rForce
 
 
+def test_steepest_descent_stage(o2lib):
+    """The steepest-descent stage of the Auto line search on its own: its direction is frhs with
+    the fixed degrees of freedom held, and what it returns is never worse than the start."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    asm = OracleAssembler(o2lib, cv, M, F0, F1)
+    asm.set_fixed(fixed)
+    N = asm.normalisation()
+    K, rhs = asm.assemble_rk(False, False)
+    r0 = asm.cost(0.0, False, N)[0]
+
+    def cost(g, s):
+        asm.set_step(s)
+        return asm.cost(g, False, N)
+    out = newton._CauchyM(M, fixed).steepest(cv, K, rhs, cost, r0, N)
+    assert out is not None
+    g, best, s, gD = out
+    free = np.ones(3 * M.Nnodes, dtype=bool)
+    free[fixed] = False
+    assert np.all(s[~free] == 0) and np.array_equal(s[free], rhs[free])
+    assert gD > 0 and best[0] <= r0 and (g == 0.0 or best[0] < r0)
+
+
 def test_line_search_accepts_full_step_and_backtracks():
     cv = default_ctrlvar()
     # full step below NewtonAcceptRatio * r0

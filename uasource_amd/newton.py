@@ -17,9 +17,9 @@ without boundary-condition constraints (``L`` empty):
   the Newton line search ends below gamma = 0.75 with r/r0 above 0.7 (``:94-103``), it also
   tries a Cauchy step with the mass matrix in place of the Hessian (``:137-259``), the path
   from that point towards the Newton point (``fminbnd``) and steepest descent, and keeps the
-  best point. With ``CtrlVar.rLineMinUa = "-Auto-"`` this driver adds the Cauchy step and the
-  Cauchy-to-Newton path (host linear solves only; steepest descent, which the reference tries
-  where the Cauchy step gives no reduction, is not restated); the default here, "-Newton Step-",
+  best point. With ``CtrlVar.rLineMinUa = "-Auto-"`` this driver adds these three stages (host
+  linear solves only; BackTracking's extrapolation, which the steepest-descent stage may use
+  there, is not restated); the default here, "-Newton Step-",
   keeps the Newton point and stops as "backtracking stagnated" where the Newton direction
   gives no reduction.
 
@@ -380,6 +380,30 @@ class _CauchyM:
                                   accept_ratio=0.9)
         return g, best, s, gC
 
+    def steepest(self, CtrlVar, K, frhs, cost, r0, N):
+        """The steepest-descent step (rLineminUa.m:333-400), tried where the Cauchy step gives no
+        reduction: direction frhs on the free degrees of freedom (the identity in place of the
+        mass matrix), the quadratic model's step as the first trial (a positive slope is negated,
+        the direction kept, as there), backtracking with NewtonAcceptRatio 0.999 down to 1/1000
+        of that step. BackTracking's extrapolation, allowed there, is not restated."""
+        free = self.free
+        s = np.where(free, frhs, 0.0)
+        Hs = K[free][:, free] @ s[free]
+        RHs = float(frhs[free] @ Hs)
+        HsHs = float(Hs @ Hs)
+        if HsHs == 0.0:
+            return None
+        slope0 = -2.0 * RHs / N
+        gD = RHs / HsHs
+        if slope0 > 0:
+            slope0 = -slope0
+        if gD < 0:
+            gD = -0.1 * r0 / slope0
+        c1 = cost(gD, s)
+        g, best, _ = _line_search(CtrlVar, lambda g: cost(g, s), r0, c1, slope0=slope0, b0=gD, gamma_min=gD / 1000,
+                                  accept_ratio=0.999)
+        return g, best, s, gD
+
 
 def _criteria(CtrlVar, gamma, rWork, rForce, iteration):
     """Exit test of SSTREAM_TransientImplicit.m:269-287."""
@@ -550,6 +574,12 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
                 return assembler.cost(g, fp32_res, N)
             out = cauchy.step(CtrlVar, K, rhs, cost_along, c0[0], N)
             chosen = d
+            if out is None or out[1][0] >= c0[0]:
+                # no reduction along the mass-matrix direction: steepest descent (rLineminUa.m:320),
+                # whose point then takes the Cauchy point's place if it is the better one
+                sd = cauchy.steepest(CtrlVar, K, rhs, cost_along, c0[0], N)
+                if sd is not None and (out is None or sd[1][0] < out[1][0]):
+                    out = sd
             if out is not None:
                 gC, bestC, s, gModel = out
                 if bestC[0] < rmin:
