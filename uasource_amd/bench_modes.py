@@ -268,7 +268,7 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, inflow conditions)",
         "config": {"workload": workload, "Nele": M.Nele, "Nnodes": M.Nnodes,
                    "linear_solver": "block ILU(0) BiCGStab (GPU), LinSolveTol 1e-8" if device_solver
-                   else "SciPy SuperLU (host)", "parallelism": "1 GPU"},
+                   else "SciPy SuperLU (host; minimum-degree order, COLAMD fall-back)", "parallelism": "1 GPU"},
         "fp64": out["fp64"], "fp32_form": out["fp32"], "fp32_form_rel_err_vs_fp64": err,
     }
 
