@@ -155,8 +155,16 @@ class LaggedLU:
         x = self.lu.solve(rhs)
         bn = np.linalg.norm(rhs)
         if bn > 0 and np.linalg.norm(rhs - K @ x) / bn > self.rtol:   # refine with the fresh LU
-            x, its, _ = self._gmres(K, rhs)
+            x, its, ok = self._gmres(K, rhs)
             self.krylov.append(its)
+            if not ok:
+                # the diagonal-pivot factorization is unstable on this K: solve it with partial
+                # pivoting (COLAMD), and keep that LU for the following steps
+                t = time.perf_counter()
+                self.lu = spla.splu(K.tocsc(), permc_spec="COLAMD")
+                self.factorizations += 1
+                self.t_factor += time.perf_counter() - t
+                x = self.lu.solve(rhs)
         else:
             self.krylov.append(0)
         return x
