@@ -211,6 +211,32 @@ def test_gpu_newton_with_other_sliding_laws(o2lib):
 
 
 @pytest.mark.gpu
+def test_gpu_time_steps_with_resident_fields(o2lib):
+    """Two implicit time steps on one GpuAssembler (next_time_step: F0 takes the converged
+    state, nothing is uploaded again) equal the second step solved on a fresh assembler built
+    from that state, bit for bit."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    print("[gpu] newton two time steps, resident fields", flush=True)
+    asm = newton.GpuAssembler(cv, M, F0, F1)
+    newton.solve(asm, cv, fixed=fixed)
+    asm.next_time_step()
+    r2 = newton.solve(asm, cv, fixed=fixed)
+    G0 = F0.copy()
+    G0.ub, G0.vb, G0.h = ref.ub, ref.vb, ref.h
+    G0.as_, G0.ab = F1.as_, F1.ab
+    G1 = F1.copy()
+    G1.ub, G1.vb, G1.h = ref.ub, ref.vb, ref.h
+    fresh = newton.solve(newton.GpuAssembler(cv, M, G0, G1), cv, fixed=fixed)
+    print(f"   step 2: {r2.iterations} iterations (fresh {fresh.iterations}), rForce {r2.rForce:.1e}", flush=True)
+    assert r2.converged and r2.iterations == fresh.iterations
+    for x, y in ((r2.ub, fresh.ub), (r2.vb, fresh.vb), (r2.h, fresh.h)):
+        assert rel(x, y) < 1e-12
+    # the second step moved the state on from the first (h evolves with dt)
+    assert rel(r2.h, ref.h) > 1e-12
+
+
+@pytest.mark.gpu
 def test_gpu_newton_precision_form_reaches_fp64_solution(o2lib):
     M, F0, F1, fixed, ref = oracle_solution(o2lib, "medium_p2")
     cv = default_ctrlvar()

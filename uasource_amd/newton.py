@@ -216,6 +216,28 @@ class GpuAssembler:
         self.n_rk = self.n_r = 0
         self.free = None
 
+    def next_time_step(self, F1=None, dt: float | None = None):
+        """Moves to the next implicit time step with the mesh, pattern and fields resident: F0
+        takes the current state (u, v, h) and F1's a_s, a_b, as a converged step hands over to
+        the next (``SSTREAM_TransientImplicit.m``: F0 = F1); with ``F1``, its fields (and its
+        u, v, h as the first guess) replace the resident F1 ones in place, else the last
+        solution is the first guess; ``dt`` replaces CtrlVar.dt."""
+        n = self.nn
+        self.f0["ub"].copy_(self.x[:n])
+        self.f0["vb"].copy_(self.x[n:2 * n])
+        self.f0["h"].copy_(self.x[2 * n:])
+        self.f0["as_"].copy_(self.f1["as_"])
+        self.f0["ab"].copy_(self.f1["ab"])
+        if F1 is not None:
+            t = self.torch
+            for k, v in self.f1.items():
+                a = getattr(F1, k, None)
+                if a is not None and k not in ("ub", "vb", "h"):
+                    v.copy_(t.from_numpy(np.ascontiguousarray(a, dtype=np.float64)))
+            self.set_state(np.concatenate([F1.ub, F1.vb, F1.h]))
+        if dt is not None:
+            self.ctrl.dt = float(dt)
+
     def set_fixed(self, fixed):
         """Residual rows of fixed degrees of freedom are left out of the cost."""
         if fixed is None or len(fixed) == 0:
