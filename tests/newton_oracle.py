@@ -18,6 +18,10 @@ class OracleAssembler:
         self.R0 = None
         self.n_rk = self.n_r = 0
         self.free = None
+        self.shift = None
+
+    def set_shift(self, a0=None, a1=None):
+        self.shift = None if a0 is None else (np.asarray(a0, dtype=float), np.asarray(a1, dtype=float))
 
     def set_state(self, x):
         self.x = np.array(x, dtype=float)
@@ -59,6 +63,8 @@ class OracleAssembler:
         else:
             R = self._assemble(self.x + gamma * self.d, True)["R"]
             self.n_r += 1
+        if self.shift is not None:
+            R = R + self.shift[0] + gamma * self.shift[1]
         if self.free is not None:
             R = R * self.free
         D2 = float(-R @ self.d)

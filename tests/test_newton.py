@@ -125,6 +125,91 @@ def test_steepest_descent_stage(o2lib):
     assert gD > 0 and best[0] <= r0 and (g == 0.0 or best[0] < r0)
 
 
+def _rows(idx, n3):
+    """Rows of the identity at ``idx`` (m x n3): L x = x[idx]."""
+    import scipy.sparse as sp
+    idx = np.asarray(idx)
+    return sp.csr_matrix((np.ones(idx.size), (np.arange(idx.size), idx)), shape=(idx.size, n3))
+
+
+def _infeasible_start(F1, fixed, seed=3):
+    """F1 with the constrained values moved off their prescribed ones."""
+    x = np.concatenate([F1.ub, F1.vb, F1.h])
+    rng = np.random.default_rng(seed)
+    x[fixed] += 1e-3 * (np.abs(x[fixed]) + 1.0) * rng.standard_normal(len(fixed))
+    G = F1.copy()
+    n = F1.h.size
+    G.ub, G.vb, G.h = x[:n].copy(), x[n:2 * n].copy(), x[2 * n:].copy()
+    return G
+
+
+def test_constraints_as_L_give_the_eliminated_solution(o2lib):
+    """Dirichlet conditions given as L x = cuvh (Lagrange multipliers and the saddle-point
+    system of solveKApe.m), started off the constraints, reach the solution of the eliminated
+    form; at convergence frhs = -R - L'luvh = 0, so the multipliers are the reactions,
+    luvh = -R on the constrained rows."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    x0 = np.concatenate([F1.ub, F1.vb, F1.h])
+    asm = OracleAssembler(o2lib, cv, M, F0, _infeasible_start(F1, fixed))
+    res = newton.solve(asm, cv, L=_rows(fixed, x0.size), cuvh=x0[fixed])
+    assert res.converged and res.rForce < 1e-15, (res.reason, res.rForce)
+    # both runs stop at rForce < 1e-15 (|R| / |Fext0| < 3e-8), which fixes the state [u; v; h]
+    # to ~1e-9 of its norm (vb, a hundredth of ub here, is compared at that scale too)
+    x1 = np.concatenate([res.ub, res.vb, res.h])
+    assert rel(x1, np.concatenate([ref.ub, ref.vb, ref.h])) < 1e-9
+    # feasibility as SSTREAM_TransientImplicit.m:210 measures it: |L x - cuvh| / |cuvh|
+    assert np.linalg.norm(x1[fixed] - x0[fixed]) < 1e-10 * np.linalg.norm(x0[fixed])
+    R = asm._assemble(x1, True)["R"]
+    assert rel(res.luvh, -R[fixed]) < 1e-8
+    # the free rows are in equilibrium
+    free = np.ones(x0.size, dtype=bool)
+    free[fixed] = False
+    assert np.linalg.norm(R[free]) < 1e-7 * np.linalg.norm(R[fixed])
+    with pytest.raises(ValueError):
+        newton.solve(asm, cv, L=_rows(fixed, x0.size), cuvh=x0[fixed], fixed=fixed)
+    with pytest.raises(ValueError):
+        newton.solve(asm, cv, L=_rows(fixed, x0.size)[:, :-1], cuvh=x0[fixed])
+
+
+def _tied(M, F1, fixed):
+    """The Dirichlet rows plus one tie, h at two neighbouring free nodes equal: (L, cuvh, a, b)."""
+    import scipy.sparse as sp
+    n = M.Nnodes
+    x0 = np.concatenate([F1.ub, F1.vb, F1.h])
+    isfixed = np.zeros(3 * n, dtype=bool)
+    isfixed[fixed] = True
+    a, b = next((int(e[0]), int(e[1])) for e in M.connectivity
+                if not isfixed[2 * n + e[0]] and not isfixed[2 * n + e[1]] and abs(F1.h[e[0]] - F1.h[e[1]]) > 1e-3)
+    tie = sp.csr_matrix(([1.0, -1.0], ([0, 0], [2 * n + a, 2 * n + b])), shape=(1, 3 * n))
+    return sp.vstack([_rows(fixed, 3 * n), tie]).tocsr(), np.concatenate([x0[fixed], [0.0]]), a, b
+
+
+def test_tie_constraint_and_resume_with_multipliers(o2lib):
+    """A constraint that is not a boundary condition, h at two neighbouring nodes tied equal
+    (a row with +1 and -1), on top of the Dirichlet rows: the solution meets every constraint to
+    rounding, differs from the untied one, and a run stopped at its time limit and resumed from
+    the checkpoint (state and multipliers) ends where the uninterrupted run ends, bit for bit."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    L, cuvh, a, b = _tied(M, F1, fixed)
+    res = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, L=L, cuvh=cuvh)
+    assert res.converged and res.rForce < 1e-15, (res.reason, res.rForce)
+    x1 = np.concatenate([res.ub, res.vb, res.h])
+    assert np.linalg.norm(L @ x1 - cuvh) < 1e-10 * np.linalg.norm(cuvh)
+    assert abs(ref.h[a] - ref.h[b]) > 1e-6 and rel(res.h, ref.h) > 1e-12
+    ck, calls = None, 0
+    while True:
+        r = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, L=L, cuvh=cuvh, time_limit=0.0, resume=ck)
+        calls += 1
+        if r.checkpoint is None:
+            break
+        ck = r.checkpoint
+        assert "luvh" in ck and calls < 100
+    assert r.iterations == res.iterations and r.rForce == res.rForce
+    assert np.array_equal(r.h, res.h) and np.array_equal(r.luvh, res.luvh)
+
+
 def test_line_search_accepts_full_step_and_backtracks():
     cv = default_ctrlvar()
     # full step below NewtonAcceptRatio * r0
@@ -208,6 +293,28 @@ def test_gpu_newton_with_other_sliding_laws(o2lib):
     assert len(ra) == len(rg) >= 5
     for x, y in zip(rg, ra):
         assert abs(x - y) <= 1e-6 * y
+
+
+@pytest.mark.gpu
+def test_gpu_newton_with_linear_constraints_matches_oracle(o2lib):
+    """Linear constraints (Dirichlet rows and a tie) with Lagrange multipliers, from an
+    infeasible start: the device-resident cost (R + L'luvh + gamma L'dl on the GPU) against
+    the oracle-driven run."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    cv = default_ctrlvar()
+    L, cuvh, a, b = _tied(M, F1, fixed)
+    G1 = _infeasible_start(F1, fixed)
+    print("[gpu] newton with linear constraints L (cpu_reference)", flush=True)
+    o = newton.solve(OracleAssembler(o2lib, cv, M, F0, G1), cv, L=L, cuvh=cuvh)
+    asm = newton.GpuAssembler(cv, M, F0, G1)
+    g = newton.solve(asm, cv, L=L, cuvh=cuvh)
+    print(f"   gpu: {g.iterations} iterations ({o.iterations} oracle), {g.reason}, rForce {g.rForce:.2e}", flush=True)
+    assert o.converged and g.converged, (o.reason, g.reason)
+    assert abs(g.iterations - o.iterations) <= 1 and g.rForce < 1e-15
+    xo, xg = np.concatenate([o.ub, o.vb, o.h]), np.concatenate([g.ub, g.vb, g.h])
+    assert rel(xg, xo) < 1e-9 and rel(g.luvh, o.luvh) < 1e-7
+    assert np.linalg.norm(L @ xg - cuvh) < 1e-10 * np.linalg.norm(cuvh)
+    assert asm.shift is None   # the assembler's cost is the unconstrained one again
 
 
 @pytest.mark.gpu

@@ -66,6 +66,7 @@ class NewtonResult:
     linear_iterations: list = field(default_factory=list)   # Krylov iterations per Newton step (device solver)
     checkpoint: dict | None = None   # set when the run stopped at its time limit: pass as ``resume``
     cauchy_steps: int = 0            # iterations that took a Cauchy or Cauchy-to-Newton point (rLineMinUa "-Auto-")
+    luvh: np.ndarray | None = None   # Lagrange multipliers of the linear constraints L x = cuvh
     ub: np.ndarray | None = None
     vb: np.ndarray | None = None
     h: np.ndarray | None = None
@@ -215,6 +216,7 @@ class GpuAssembler:
         self.stream = torch.cuda.current_stream(self.dev).cuda_stream
         self.n_rk = self.n_r = 0
         self.free = None
+        self.shift = None
 
     def next_time_step(self, F1=None, dt: float | None = None):
         """Moves to the next implicit time step with the mesh, pattern and fields resident: F0
@@ -246,6 +248,15 @@ class GpuAssembler:
             m = np.ones(3 * self.nn)
             m[fixed] = 0.0
             self.free = self.torch.from_numpy(m).to(self.dev)
+
+    def set_shift(self, a0=None, a1=None):
+        """Terms added to R in the cost, R + a0 + gamma a1: L'luvh and L'dl with linear
+        constraints L x = cuvh (CalcCostFunctionNRuvh.m:47, frhs = -R - L'luvh)."""
+        if a0 is None:
+            self.shift = None
+        else:
+            t = self.torch
+            self.shift = tuple(t.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.dev) for a in (a0, a1))
 
     def _assemble(self, flags, K=None):
         self.ctx.assemble_device(self.ctrl, self.fields, flags, self.R, K, None, None, stream=self.stream)
@@ -293,6 +304,8 @@ class GpuAssembler:
             self._assemble(uvh.UVH_RONLY | (uvh.UVH_FP32 if fp32 else 0))
             self.n_r += 1
             R = self.R
+        if self.shift is not None:
+            R = (R + self.shift[0]).add_(self.shift[1], alpha=gamma)
         if self.free is not None:
             R = R * self.free
         frhs_frhs = self.torch.dot(R, R)
@@ -498,14 +511,35 @@ def eliminate(linear_solver, fixed: np.ndarray | None):
 
 def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, fp64_switch: float = 1e-8,
           fixed: np.ndarray | None = None, verbose: bool = False, log=print, time_limit: float | None = None,
-          resume: dict | None = None) -> NewtonResult:
+          resume: dict | None = None, L=None, cuvh: np.ndarray | None = None,
+          luvh: np.ndarray | None = None) -> NewtonResult:
     """Newton-Raphson uvh iteration (SSTREAM_TransientImplicit.m:224-418). ``fixed``: degrees of
     freedom held at their current values (their residual rows are left out of the cost).
     ``time_limit`` (seconds): once at least one iteration has run in this call, stop before
     starting an iteration past it, with the loop state in ``NewtonResult.checkpoint``;
     ``resume`` (such a checkpoint) continues that run where it stopped, with the same result as
-    one uninterrupted run."""
+    one uninterrupted run.
+
+    ``L`` (m x 3Nn sparse), ``cuvh`` (m): linear constraints L [ub; vb; h] = cuvh held with
+    Lagrange multipliers ``luvh`` (start value, zero by default), as Úa does
+    (SSTREAM_TransientImplicit.m:195, :373-383, :481): each step solves
+    [K L'; L 0] [d; dl] = [-R - L'luvh; cuvh - L x] (solveKApe.m) with the host linear solver,
+    and the cost carries both residual blocks (CalcCostFunctionNRuvh.m:45-85). The start need
+    not be feasible."""
     on_device = getattr(linear_solver, "on_device", False)
+    if L is not None:
+        if on_device:
+            raise ValueError("linear constraints L need the host linear solver")
+        if fixed is not None and len(fixed):
+            raise ValueError("give the boundary conditions either as fixed degrees of freedom or as L, not both")
+        if "-Auto-" in str(CtrlVar.get("rLineMinUa", "")):
+            raise ValueError('rLineMinUa "-Auto-" is not available with linear constraints L')
+        L = sp.csr_matrix(L, dtype=np.float64)
+        cuvh = np.asarray(cuvh, dtype=np.float64).ravel()
+        if L.shape[1] != 3 * assembler.nn or cuvh.shape != (L.shape[0],):
+            raise ValueError(f"L must be m x {3 * assembler.nn} and cuvh of length m; got {L.shape} and {cuvh.shape}")
+        lam = np.zeros(L.shape[0]) if luvh is None else np.array(luvh, dtype=np.float64).ravel()
+        LT = L.T.tocsr()
     if on_device:
         # the device solver holds its own fixed set (identity rows and columns)
         same = (linear_solver.fixed is None and (fixed is None or len(fixed) == 0)) or (
@@ -536,6 +570,8 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
         ratios = np.array(resume["ratios"], dtype=float)
         res.history = [tuple(h) for h in resume["history"]]
         assembler.set_state(resume["x"])
+        if L is not None:
+            lam = np.array(resume["luvh"], dtype=np.float64)
     t_start = time.perf_counter()
     done_here = 0
     while True:
@@ -572,6 +608,8 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             res.checkpoint = dict(iteration=iteration, gamma=gamma, r=r, r0=r0, rForce=rForce, rWork=rWork,
                                   residual64=residual64, backtrack_ok=backtrack_ok, ratios=ratios.copy(),
                                   history=list(res.history), x=np.concatenate(assembler.state()))
+            if L is not None:
+                res.checkpoint["luvh"] = lam.copy()
             break
         iteration += 1
         done_here += 1
@@ -585,13 +623,34 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             assembler.step_on_device(linear_solver)
             t2 = time.perf_counter()
             res.linear_iterations.append(linear_solver.last_iterations)
+        elif L is not None:
+            K, rhs = assembler.assemble_rk(fp32, fp32_res)
+            t1 = time.perf_counter()
+            n3 = K.shape[0]
+            G0 = cuvh - L @ np.concatenate(assembler.state())
+            sol = linear_solver(sp.bmat([[K, LT], [L, None]], format="csr"),
+                                np.concatenate([rhs - LT @ lam, G0]))
+            d, dl = sol[:n3], sol[n3:]
+            t2 = time.perf_counter()
+            assembler.set_step(d)
+            assembler.set_shift(LT @ lam, LT @ dl)
+            G1 = L @ d
         else:
             K, rhs = assembler.assemble_rk(fp32, fp32_res)
             t1 = time.perf_counter()
             d = linear_solver(K, rhs)
             t2 = time.perf_counter()
             assembler.set_step(d)
-        cost = lambda g: assembler.cost(g, fp32_res, N)
+        if L is None:
+            cost = lambda g: assembler.cost(g, fp32_res, N)
+        else:
+            def cost(g):
+                # the assembler's terms hold frhs = -(R + L'(luvh + g dl)); the constraint block
+                # grhs = cuvh - L (x + g d) is added here
+                c = assembler.cost(g, fp32_res, N)
+                gr = G0 - g * G1
+                D2 = c[2] + float(gr @ dl)
+                return c[0] + float(gr @ gr) / N, D2 * D2, D2
         c0 = cost(0.0)
         c1 = cost(1.0)
         gamma, best, _ = _line_search(CtrlVar, cost, c0[0], c1)
@@ -638,6 +697,8 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
         backtrack_ok = gamma > 0
         if gamma > 0:
             assembler.accept(step)
+            if L is not None:
+                lam = lam + step * dl
             rForce, rWork = best[0], best[1]
             r = rForce if CtrlVar.uvhMinimisationQuantity == "Force Residuals" else rWork
         r0 = c0[0] if CtrlVar.uvhMinimisationQuantity == "Force Residuals" else c0[1]
@@ -655,4 +716,7 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
     res.iterations, res.rForce, res.rWork = iteration, rForce, rWork
     res.n_rk, res.n_r = assembler.n_rk, assembler.n_r
     res.ub, res.vb, res.h = assembler.state()
+    if L is not None:
+        assembler.set_shift(None)
+        res.luvh = lam
     return res
