@@ -225,7 +225,7 @@ def test_line_search_accepts_full_step_and_backtracks():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cpu_reference", "medium_p2"])
+@pytest.mark.parametrize("name", ["cpu_reference", "medium_p2", "medium_p3"])
 def test_gpu_newton_matches_oracle_newton(o2lib, name):
     import sys
     M, F0, F1, fixed, ref = oracle_solution(o2lib, name)
@@ -344,10 +344,11 @@ def test_gpu_time_steps_with_resident_fields(o2lib):
 
 
 @pytest.mark.gpu
-def test_gpu_newton_precision_form_reaches_fp64_solution(o2lib):
-    M, F0, F1, fixed, ref = oracle_solution(o2lib, "medium_p2")
+@pytest.mark.parametrize("name", ["medium_p2", "medium_p3"])
+def test_gpu_newton_precision_form_reaches_fp64_solution(o2lib, name):
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, name)
     cv = default_ctrlvar()
-    print("[gpu] newton precision form medium_p2", flush=True)
+    print(f"[gpu] newton precision form {name}", flush=True)
     res = newton.solve(newton.GpuAssembler(cv, M, F0, F1), cv, fixed=fixed, fp32=True)
     # velocities against the speed scale ||(u, v)|| (v is near zero here, so ||dv||/||v|| says
     # little), thickness against ||h||
