@@ -319,28 +319,44 @@ def test_gpu_newton_with_linear_constraints_matches_oracle(o2lib):
 
 @pytest.mark.gpu
 def test_gpu_time_steps_with_resident_fields(o2lib):
-    """Two implicit time steps on one GpuAssembler (next_time_step: F0 takes the converged
-    state, nothing is uploaded again) equal the second step solved on a fresh assembler built
-    from that state, bit for bit."""
+    """Implicit time steps on one GpuAssembler (next_time_step: F0 takes the converged state in
+    place, nothing is uploaded again; then new forcing, a new dt and a given first guess) equal
+    each step solved on a fresh assembler built from the previous step's result, bit for bit."""
     M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
     cv = default_ctrlvar()
-    print("[gpu] newton two time steps, resident fields", flush=True)
+    print("[gpu] newton three time steps, resident fields", flush=True)
     asm = newton.GpuAssembler(cv, M, F0, F1)
-    newton.solve(asm, cv, fixed=fixed)
+    r1 = newton.solve(asm, cv, fixed=fixed)
     asm.next_time_step()
     r2 = newton.solve(asm, cv, fixed=fixed)
     G0 = F0.copy()
-    G0.ub, G0.vb, G0.h = ref.ub, ref.vb, ref.h
+    G0.ub, G0.vb, G0.h = r1.ub, r1.vb, r1.h
     G0.as_, G0.ab = F1.as_, F1.ab
     G1 = F1.copy()
-    G1.ub, G1.vb, G1.h = ref.ub, ref.vb, ref.h
+    G1.ub, G1.vb, G1.h = r1.ub, r1.vb, r1.h
     fresh = newton.solve(newton.GpuAssembler(cv, M, G0, G1), cv, fixed=fixed)
     print(f"   step 2: {r2.iterations} iterations (fresh {fresh.iterations}), rForce {r2.rForce:.1e}", flush=True)
     assert r2.converged and r2.iterations == fresh.iterations
     for x, y in ((r2.ub, fresh.ub), (r2.vb, fresh.vb), (r2.h, fresh.h)):
-        assert rel(x, y) < 1e-12
+        assert np.array_equal(x, y)
     # the second step moved the state on from the first (h evolves with dt)
     assert rel(r2.h, ref.h) > 1e-12
+    # a third step with new forcing (a_s up by half), a shorter dt and a given first guess
+    H1 = F1.copy()
+    H1.as_ = 1.5 * F1.as_
+    H1.ub, H1.vb, H1.h = r2.ub, r2.vb, r2.h
+    asm.next_time_step(F1=H1, dt=0.5 * cv.dt)
+    r3 = newton.solve(asm, cv, fixed=fixed)
+    cv3 = cv.copy()
+    cv3.dt = 0.5 * cv.dt
+    H0 = F0.copy()
+    H0.ub, H0.vb, H0.h, H0.as_, H0.ab = r2.ub, r2.vb, r2.h, F1.as_, F1.ab
+    fresh3 = newton.solve(newton.GpuAssembler(cv3, M, H0, H1), cv3, fixed=fixed)
+    print(f"   step 3: {r3.iterations} iterations (fresh {fresh3.iterations})", flush=True)
+    assert r3.converged and r3.iterations == fresh3.iterations
+    for x, y in ((r3.ub, fresh3.ub), (r3.vb, fresh3.vb), (r3.h, fresh3.h)):
+        assert np.array_equal(x, y)
+    assert rel(r3.h, r2.h) > 1e-12
 
 
 @pytest.mark.gpu
