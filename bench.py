@@ -223,6 +223,19 @@ def main(argv=None):
         if sorted(tinfo.get("kernels", [])) == sorted(kernels):
             traffic = tinfo.get("hbm_bytes_per_assembly")
 
+    # VALU use of the element kernel under the factored flop model of the element arithmetic,
+    # nip (42 nod^2 + 90 nod + 150) per element for R+K (the arithmetic is not counted
+    # instruction by instruction), against the vector peak of the element arithmetic's type
+    valu = None
+    ek = [k for k in kernels if k.startswith("element")]
+    if ek and not args.ronly:
+        flops = M.Nele * M.nip * (42 * M.nod ** 2 + 90 * M.nod + 150)
+        tflops = flops / (sum(kernels[k] for k in ek) * 1e-3) / 1e12
+        vpeak = 157.3 if args.fp32 else 78.6
+        valu = {"kernel": "+".join(ek), "model_flops": flops, "achieved": tflops, "peak": vpeak,
+                "unit": "TFLOP/s", "frac": tflops / vpeak,
+                "model": "nip*(42*nod^2 + 90*nod + 150) flops per element"}
+
     line = {
         "metric": "uvh elements assembled/s (R+K), HBM GB/s fraction; p50/p99 request latency",
         "value": M.Nele * args.steps / (ms_total * 1e-3),
@@ -242,6 +255,7 @@ def main(argv=None):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "uvh assembly (all kernels of one evaluation, between events on its stream)",
                      "algorithmic_bytes": nbytes, "assembly_us": assembly_ms * 1e3},
+        "valu": valu,
         "kernels_us": {k: v * 1e3 for k, v in kernels.items()},
         "kernel_launches_per_assembly": launches,
         "precision": precision,
