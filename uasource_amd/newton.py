@@ -221,9 +221,10 @@ class GpuAssembler:
     def next_time_step(self, F1=None, dt: float | None = None):
         """Moves to the next implicit time step with the mesh, pattern and fields resident: F0
         takes the current state (u, v, h) and F1's a_s, a_b, as a converged step hands over to
-        the next (``SSTREAM_TransientImplicit.m``: F0 = F1); with ``F1``, its fields (and its
-        u, v, h as the first guess) replace the resident F1 ones in place, else the last
-        solution is the first guess; ``dt`` replaces CtrlVar.dt."""
+        the next (``SSTREAM_TransientImplicit.m``: F0 = F1); with ``F1``, its nodal fields
+        (those the assembler holds, q, muk and V0 included), its scalars g, rhow, alpha, and its
+        u, v, h as the first guess replace the resident ones in place, else the last solution is
+        the first guess; ``dt`` replaces CtrlVar.dt."""
         n = self.nn
         self.f0["ub"].copy_(self.x[:n])
         self.f0["vb"].copy_(self.x[n:2 * n])
@@ -237,6 +238,7 @@ class GpuAssembler:
                 if a is not None and k not in ("ub", "vb", "h"):
                     v.copy_(t.from_numpy(np.ascontiguousarray(a, dtype=np.float64)))
             self.set_state(np.concatenate([F1.ub, F1.vb, F1.h]))
+            self.ctrl.g, self.ctrl.rhow, self.ctrl.alpha = float(F1.g), float(F1.rhow), float(F1.alpha)
         if dt is not None:
             self.ctrl.dt = float(dt)
 
