@@ -175,7 +175,8 @@ class GpuAssembler:
     """R and K of one mesh on the GPU, with F0, F1 and the Newton state resident in HBM.
 
     The interface the driver uses: ``normalisation()``, ``assemble_rk(fp32)``,
-    ``set_step(d)``, ``cost(gamma, fp32)``, ``accept(gamma)``, ``state()``."""
+    ``set_step(d)``, ``cost(gamma, fp32)``, ``accept(gamma)``, ``state()``, ``set_state(x)``,
+    ``set_fixed(fixed)`` and ``set_shift(a0, a1)`` (constraint terms added to R)."""
 
     def __init__(self, CtrlVar, MUA, F0, F1, device: int = 0, ctx: uvh.UvhContext | None = None):
         import torch
@@ -557,6 +558,7 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             raise ValueError('rLineMinUa "-Auto-" needs K on the host (the host linear solver)')
         cauchy = _CauchyM(assembler.mua, fixed)
     assembler.set_fixed(fixed)
+    assembler.set_shift(None)   # no constraint terms left from a solve that ended in an error
     N = assembler.normalisation()
     res = NewtonResult(False, 0, "", float("inf"), float("nan"))
     iteration, gamma = 0, 1.0
