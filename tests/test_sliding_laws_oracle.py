@@ -8,15 +8,17 @@ pressure N0 of :500-521), pinned without retyping their formulas:
   delta, are exact: delta = 0 and He = 1);
 * Cornford and Umbi tend to Weertman as muk N grows and their drag tends to Coulomb's as it
   shrinks; Joughin tends to Weertman with C V0 as V0 grows;
-* Budd with q = 0 is Weertman (N^0 = 1), bit for bit on grounded meshes;
+* Budd with q = 0 is Weertman (N^0 = 1), bit for bit on grounded meshes, and with N uniform
+  Budd is Weertman with C N^(-q), N = rho g (h - rhow (S - B) / rho) computed from its
+  definition (this fixes the power q/m of N and the effective pressure);
 * Coulomb's drag is muk N u/|u|: R is affine in muk, R(2 muk) - 2 R(muk) + R(0) = 0;
 * Tsai is Weertman where the Coulomb stress is the larger (muk large) and Coulomb where it is
   the smaller (muk small), bit for bit;
 * the uv restatement gives the uvh momentum part under each law (the uv and uvh assemblies
   follow different reference files, uvMatrixAssemblySSTREAM.m and uvhNodalLoopSSTREAM.m).
 
-The power N^(q/m) of Budd's law is checked only through these and the GPU-oracle agreement
-(the device kernel is a separate restatement): no reference vector holds it."""
+No reference vector holds these laws; the device kernel, a separate restatement, is checked
+against this oracle."""
 import numpy as np
 import pytest
 
@@ -175,3 +177,30 @@ def test_uv_equals_the_uvh_momentum_part(name, law):
     assert np.linalg.norm(uv["R"] - ref["R"][:n2]) <= 1e-13 * scale
     Kuvh = ref["K"][:n2, :n2]
     assert abs(uv["K"] - Kuvh).max() <= 1e-13 * abs(Kuvh).max()
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3"])
+def test_budd_is_weertman_with_C_scaled_by_N_to_the_minus_q(name):
+    """Budd's drag is N^(q/m) times Weertman's, C^(-1/m) U^(1/m-1) u (BasalDrag.m:448-452), so
+    where N is uniform (h, S, B and rho uniform, grounded) Budd with C equals Weertman with
+    C N^(-q), for any m: R to rounding. N = rho g (h - hf), hf = rhow (S - B) / rho
+    (BasalDrag.m:505-512; Nzero = 1e-20 is below rounding here). A wrong power of N (q, 1/m)
+    or a wrong N is off by orders of magnitude at N ~ 4e3 kPa."""
+    cv, M, F0, F1 = setup(name, "Budd")
+    n = M.Nnodes
+    F1.h, F1.S, F1.B, F1.rho = np.full(n, 1000.0), np.zeros(n), np.full(n, -500.0), np.full(n, 917.0)
+    q = 1.3
+    F1.q = np.full(n, q)
+    N = 917.0 * F1.g * (1000.0 - F1.rhow * (0.0 + 500.0) / 917.0)
+    assert N > 1e3
+    Rb = o2.assemble(cv, M, F0, F1, ronly=True)["R"]
+    cw = cv.copy()
+    cw.SlidingLaw = "Weertman"
+    G = F1.copy()
+    G.C = F1.C * N ** (-q)
+    Rw = o2.assemble(cw, M, F0, G, ronly=True)["R"]
+    scale = np.linalg.norm(Rb)
+    assert np.linalg.norm(Rb - Rw) < 1e-12 * scale
+    # the factor is not negligible: Weertman with the unscaled C differs
+    Rw0 = o2.assemble(cw, M, F0, F1, ronly=True)["R"]
+    assert np.linalg.norm(Rw0 - Rb) > 1e-3 * scale
