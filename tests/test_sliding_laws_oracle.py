@@ -8,6 +8,8 @@ pressure N0 of :500-521), pinned without retyping their formulas:
   delta, are exact: delta = 0 and He = 1);
 * Cornford and Umbi tend to Weertman as muk N grows and their drag tends to Coulomb's as it
   shrinks; Joughin tends to Weertman with C V0 as V0 grows;
+* on a uniform flow the x-momentum rows of R add up to the basal drag times the area
+  (Weertman and Coulomb, from their definitions);
 * Budd with q = 0 is Weertman (N^0 = 1), bit for bit on grounded meshes, and with N uniform
   Budd is Weertman with C N^(-q), N = rho g (h - rhow (S - B) / rho) computed from its
   definition (this fixes the power q/m of N and the effective pressure);
@@ -204,3 +206,32 @@ def test_budd_is_weertman_with_C_scaled_by_N_to_the_minus_q(name):
     # the factor is not negligible: Weertman with the unscaled C differs
     Rw0 = o2.assemble(cw, M, F0, F1, ronly=True)["R"]
     assert np.linalg.norm(Rw0 - Rb) > 1e-3 * scale
+
+
+@pytest.mark.parametrize("law", ["Weertman", "Coulomb"])
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3"])
+def test_total_basal_drag_on_a_uniform_flow(name, law):
+    """With every field uniform (flat surface, u = (u0, 0)), each term of the momentum rows of
+    the form int dN_i/dx f dA sums to zero over the nodes (sum_i N_i = 1), so the x-momentum
+    rows of R add up to the basal drag times the domain's area and the y rows to zero:
+    Weertman tau = C^(-1/m) speed^(1/m-1) u0 (BasalDrag.m:54-77), Coulomb tau = muk N u0 /
+    speed (BasalDrag.m:373-375), speed = sqrt(u0^2 + SpeedZero^2), N = rho g (h - hf)."""
+    w = WORKLOADS[name]
+    cv, M, F0, F1 = setup(name, law)
+    n = M.Nnodes
+    u0, C, m, muk = 120.0, 2e-2, 3.0, 0.4
+    F1.h, F1.S, F1.B, F1.rho = np.full(n, 1000.0), np.zeros(n), np.full(n, -500.0), np.full(n, 917.0)
+    F1.ub, F1.vb = np.full(n, u0), np.zeros(n)
+    F1.C, F1.m, F1.muk = np.full(n, C), np.full(n, m), np.full(n, muk)
+    R = o2.assemble(cv, M, F0, F1, ronly=True)["R"]
+    speed = np.sqrt(u0 ** 2 + cv.SpeedZero ** 2)
+    if law == "Weertman":
+        tau = C ** (-1.0 / m) * speed ** (1.0 / m - 1.0) * u0
+    else:
+        N = 917.0 * F1.g * (1000.0 - F1.rhow * 500.0 / 917.0)
+        tau = muk * N * u0 / speed
+    area = w.Lx * w.Ly
+    fx, fy = R[:n].sum(), R[n:2 * n].sum()
+    # the drag is internal resistance: in Tint, so +tau area in R = Tint - Fext for u0 > 0
+    assert abs(fx - tau * area) < 1e-11 * tau * area, (fx, tau * area)
+    assert abs(fy) < 1e-11 * tau * area
