@@ -6,7 +6,8 @@ two scales: the drawn one (Coulomb stress far above Weertman's, so Tsai is Weert
 everywhere) and 2 % of it (the stresses comparable, so Tsai mixes the two laws and the
 regularised laws sit between their limits). The batched assembly gives the single-mesh
 results. A law whose field is missing is refused (UVH_E_ARG), an unknown law is unsupported,
-and the Weertman path is unchanged."""
+and the Weertman path is unchanged. On a uniform flow the device R alone meets the closed
+form of the total basal drag (Weertman, Budd, Coulomb)."""
 import numpy as np
 import pytest
 
@@ -173,3 +174,46 @@ def test_batched_assembly_matches_single_meshes(law):
             assert eR <= 1e-13 and eK <= 1e-13
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("law", ["Weertman", "Budd", "Coulomb"])
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3"])
+def test_total_basal_drag_on_a_uniform_flow(name, law):
+    """The device kernel against a closed form, without the oracle: with every field uniform
+    (flat surface, grounded, u = (u0, 0)) the int dN_i/dx f terms of the momentum rows sum to
+    zero over the nodes, so the x rows of R add up to +tau area and the y rows to zero, with
+    tau from the law's definition (BasalDrag.m:54-77, :373-375, :448-452): Weertman
+    C^(-1/m) speed^(1/m-1) u0, Budd N^(q/m) times that, Coulomb muk N u0 / speed;
+    speed = sqrt(u0^2 + SpeedZero^2), N = rho g (h - rhow (S - B) / rho). FP64 to 1e-11 of
+    tau area, the precision form to 1e-6 of the rows' sum |R|."""
+    print(f"[gpu] uniform-flow drag {law} {name}", flush=True)
+    w = WORKLOADS[name]
+    M, F0, F1 = case(name)
+    F1 = F1.copy()
+    n = M.Nnodes
+    u0, C, m, muk, q = 120.0, 2e-2, 3.0, 0.4, 1.3
+    F1.h, F1.S, F1.B, F1.rho = np.full(n, 1000.0), np.zeros(n), np.full(n, -500.0), np.full(n, 917.0)
+    F1.ub, F1.vb = np.full(n, u0), np.zeros(n)
+    F1.C, F1.m, F1.muk, F1.q = np.full(n, C), np.full(n, m), np.full(n, muk), np.full(n, q)
+    cv = default_ctrlvar()
+    cv.SlidingLaw = law
+    speed = np.sqrt(u0 ** 2 + cv.SpeedZero ** 2)
+    N = 917.0 * F1.g * (1000.0 - F1.rhow * 500.0 / 917.0)
+    tau = {"Weertman": C ** (-1.0 / m) * speed ** (1.0 / m - 1.0) * u0,
+           "Budd": N ** (q / m) * C ** (-1.0 / m) * speed ** (1.0 / m - 1.0) * u0,
+           "Coulomb": muk * N * u0 / speed}[law]
+    total = tau * w.Lx * w.Ly
+    ctx = uvh.UvhContext(0)
+    try:
+        ctx.set_mesh(M)
+        for fp32, tol in ((False, 1e-11), (True, 1e-6)):
+            R = ctx.assemble(cv, F0, F1, fp32=fp32)["R"]
+            fx, fy = R[:n].sum(), R[n:2 * n].sum()
+            # the rows' sums cancel large boundary terms (rho g h^2 / 2), whose FP32 rounding in the
+            # precision form scales with the summands: its bar is relative to sum |R| of those rows
+            bar = tol * (np.abs(R[:2 * n]).sum() if fp32 else total)
+            print(f"   {'precision form' if fp32 else 'FP64'}: sum Rx / (tau area) - 1 = {fx / total - 1:.1e}, "
+                  f"sum Ry / (tau area) = {fy / total:.1e}, bar / (tau area) = {bar / total:.1e}", flush=True)
+            assert abs(fx - total) < bar and abs(fy) < bar
+    finally:
+        ctx.close()
