@@ -217,3 +217,44 @@ def test_total_basal_drag_on_a_uniform_flow(name, law):
             assert abs(fx - total) < bar and abs(fy) < bar
     finally:
         ctx.close()
+
+
+def _sloped_bed_case(M, F1, SpeedZero):
+    """Uniform h and flow u = (u0, 0) over a linearly sloped bed below sea level, grounded
+    everywhere, Weertman drag: the fields and the closed-form sums of the x and y momentum rows
+    of R. Summed over the nodes (sum_i N_i = 1) the momentum rows keep only the drag and the
+    net of gravity and the water pressure on the bed, g (rho h - rhow (S - b)) grad b, whose
+    integral over the rectangle is its value at the centre times the area (b is linear)."""
+    n = M.Nnodes
+    x, y = M.coordinates[:, 0], M.coordinates[:, 1]
+    u0, C, m, sx, sy = 120.0, 2e-2, 3.0, 1e-3, -4e-4
+    F1.h, F1.S, F1.rho = np.full(n, 1000.0), np.zeros(n), np.full(n, 917.0)
+    F1.B = -500.0 + sx * x + sy * y
+    F1.ub, F1.vb = np.full(n, u0), np.zeros(n)
+    F1.C, F1.m = np.full(n, C), np.full(n, m)
+    area = (x.max() - x.min()) * (y.max() - y.min())
+    tau = C ** (-1.0 / m) * np.sqrt(u0 ** 2 + SpeedZero ** 2) ** (1.0 / m - 1.0) * u0
+    bc = -500.0 + sx * (x.min() + x.max()) / 2 + sy * (y.min() + y.max()) / 2
+    P = F1.g * (917.0 * 1000.0 - F1.rhow * (0.0 - bc))
+    return (tau + P * sx) * area, P * sy * area, P * abs(sy) * area
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3"])
+def test_gravity_and_water_pressure_on_a_sloped_bed(name):
+    """The device R alone against the closed form of the summed momentum rows on a sloped bed
+    below sea level (drag plus g (rho h - rhow (S - b)) grad b times the area): FP64 to 1e-11."""
+    print(f"[gpu] sloped-bed force balance {name}", flush=True)
+    M, F0, F1 = case(name)
+    F1 = F1.copy()
+    cv = default_ctrlvar()
+    fx, fy, scale = _sloped_bed_case(M, F1, cv.SpeedZero)
+    n = M.Nnodes
+    ctx = uvh.UvhContext(0)
+    try:
+        ctx.set_mesh(M)
+        R = ctx.assemble(cv, F0, F1)["R"]
+    finally:
+        ctx.close()
+    ex, ey = R[:n].sum() / fx - 1, (R[n:2 * n].sum() - fy) / scale
+    print(f"   sum Rx / closed form - 1 = {ex:.1e}, sum Ry error / scale = {ey:.1e}", flush=True)
+    assert abs(ex) < 1e-11 and abs(ey) < 1e-11
