@@ -268,3 +268,34 @@ def test_gravity_and_water_pressure_on_a_sloped_bed(name):
     n = M.Nnodes
     assert abs(R[:n].sum() - fx) < 1e-12 * abs(fx)
     assert abs(R[n:2 * n].sum() - fy) < 1e-12 * scale
+
+
+def _mass_balance_case(M, F0, F1, cv):
+    """Uniform fields with h1 != h0 and non-zero a_s, a_b at both times, dt = 0.5: the
+    closed-form sum of the h rows of R. Summed over the nodes the SUPG weights
+    N_i + tau u . grad N_i add up to 1 and the flux divergences vanish, leaving
+    rho (h1 - h0 - dt ((1 - theta) a0 + theta a1)) times the area
+    (uvhNodalLoopSSTREAM.m:213-217: th = -dhdt, fh = accterm - qterm, R = T - F)."""
+    n = M.Nnodes
+    x, y = M.coordinates[:, 0], M.coordinates[:, 1]
+    F1.h, F1.S, F1.B, F1.rho = np.full(n, 1000.0), np.zeros(n), np.full(n, -500.0), np.full(n, 917.0)
+    F1.ub, F1.vb = np.full(n, 120.0), np.full(n, -30.0)
+    F1.as_, F1.ab = np.full(n, 0.3), np.full(n, -0.1)
+    F0.h, F0.ub, F0.vb = np.full(n, 990.0), np.full(n, 100.0), np.full(n, 10.0)
+    F0.as_, F0.ab = np.full(n, 0.5), np.zeros(n)
+    cv.dt = 0.5
+    area = (x.max() - x.min()) * (y.max() - y.min())
+    a0, a1 = 0.5, 0.3 - 0.1
+    return 917.0 * area * ((1000.0 - 990.0) - cv.dt * ((1 - cv.theta) * a0 + cv.theta * a1))
+
+
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3"])
+def test_mass_balance_sum_of_the_thickness_rows(name):
+    """The h rows of R summed over the nodes equal the closed-form mass balance (to 1e-12):
+    this fixes rho, dt, theta and the signs of the time and accumulation terms."""
+    cv, M, F0, F1 = setup(name, "Weertman")
+    F0 = F0.copy()
+    total = _mass_balance_case(M, F0, F1, cv)
+    R = o2.assemble(cv, M, F0, F1, ronly=True)["R"]
+    n = M.Nnodes
+    assert abs(R[2 * n:].sum() / total - 1) < 1e-12
