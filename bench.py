@@ -125,7 +125,7 @@ def main(argv=None):
         return 2
 
     if args.batch or args.serve or args.newton or args.uv:
-        from uasource_amd import bench_modes
+        from measure import bench_modes
         log = lambda m: print(m, file=sys.stderr, flush=True)
         if args.batch:
             line = bench_modes.bench_batch(args.steps, args.warmup, log=log)

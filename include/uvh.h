@@ -138,10 +138,23 @@ int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *fields, u
 /* Device-buffer assembly, enqueued on `stream` (a hipStream_t; NULL is the HIP null stream).
  * All pointers are device pointers; nothing is copied to or from the host and the call
  * returns once the work is enqueued. NaN checks are deferred: uvh_sync_status() waits
- * for the stream and returns UVH_E_NAN_R / UVH_E_NAN_K if the last assembly met a NaN. */
+ * for the context's last assembly (not for the device) and returns UVH_E_NAN_R / UVH_E_NAN_K
+ * if it met a NaN.
+ * One context runs one assembly at a time: its calls share the context's work buffers and
+ * status word. A call on another stream than the context's previous call is ordered after
+ * that call (the library makes the new stream wait for it), except under graph capture,
+ * where the caller orders the replays. The status word holds the last call's NaN flags only.
+ * Same seam as uvh_assemble (uvhMatrixAssembly.m:6-585), with device buffers. */
 int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *dfields, uint32_t flags,
                         double *dR, double *dKval, double *dTint, double *dFext, void *stream);
 int uvh_sync_status(uvh_ctx *ctx);
+
+/* Enqueues on `stream` a copy of the context's status word (bit 0: NaN in R, bit 1: NaN on
+ * the diagonal of K, as uvhMatrixAssembly.m:575-584 tests) into `dst` (pinned host or device
+ * memory), for callers that read each assembly's errors without waiting for the device (the
+ * serving driver). Enqueue it on the stream of the assembly it reports, right after it. No
+ * counterpart in the reference: it splits uvhMatrixAssembly's error check from the wait. */
+int uvh_status_async(uvh_ctx *ctx, int32_t *dst, void *stream);
 
 /* Per-kernel timing with HIP events on the launch stream (for measurement).
  * uvh_profile(ctx, 1) starts recording; uvh_kernel_times fills up to `cap` entries of
@@ -180,7 +193,8 @@ int uvh_uv_get_pattern(const uvh_ctx *ctx, int64_t *rowptr, int32_t *colind);
 /* Host buffers (R 2Nn required; Kval, Tint, Fext may be NULL); blocks until done. */
 int uvh_uv_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *fields,
                     uint32_t flags, double *R, double *Kval, double *Tint, double *Fext);
-/* Device buffers, enqueued on `stream`; NaN checks deferred to uvh_sync_status. */
+/* Device buffers, enqueued on `stream`; NaN checks deferred to uvh_sync_status; ordered with the
+ * context's other calls as uvh_assemble_device is. */
 int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc,
                            const uvh_uv_fields *dfields, uint32_t flags, double *dR, double *dKval, double *dTint,
                            double *dFext, void *stream);

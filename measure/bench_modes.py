@@ -6,9 +6,9 @@ import time
 
 import numpy as np
 
-from . import uvh
-from .ctrlvar import default_ctrlvar
-from .fields import WORKLOADS, batch_workloads, build_mesh, make_fields, newton_problem
+from uasource_amd import uvh
+from uasource_amd.ctrlvar import default_ctrlvar
+from uasource_amd.fields import WORKLOADS, batch_workloads, build_mesh, make_fields, newton_problem
 
 
 def _device_fields(torch, dev, F0, F1):
@@ -45,7 +45,7 @@ def bench_batch(steps: int, warmup: int, n_members: int = 64, log=print) -> dict
     accumulation and FP64, autotuned tiles, eager and HIP-graph replay; against the same work
     as 64 single-mesh assemblies."""
     import torch
-    from .batch import UvhBatch, autotune
+    from uasource_amd.batch import UvhBatch, autotune
     dev = torch.device("cuda:0")
     cv = default_ctrlvar()
     ws = batch_workloads(n=n_members)
@@ -126,7 +126,7 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
     """500 R/K requests (Newton step : line search = 1:3) over meshes of 2k-300k elements
     (lognormal, P1/P2/P3 = 50/35/15 %), Poisson arrivals at fractions of the saturated rate."""
     import torch
-    from .serve import UvhServer, make_requests
+    from uasource_amd.serve import UvhServer, make_requests
     rng = np.random.default_rng(17)
     sizes = np.clip(np.exp(rng.normal(np.log(20000), 1.0, n_meshes)), 2000, 300000)
     types = [((3, 2), (6, 5), (10, 8))[rng.choice(3, p=(0.5, 0.35, 0.15))] for _ in range(n_meshes)]
@@ -164,7 +164,7 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
     for frac in (0.3, 0.5, 0.7, 0.9):
         srv.run(at_rate(paced, frac * warm_rate))
     # service time of each (mesh, kind) alone: the median of five after one untimed launch
-    from .serve import Request
+    from uasource_amd.serve import Request
     alone = {}
     for mi in range(n_meshes):
         st = [torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
@@ -231,8 +231,8 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
     """Full Newton uvh solve (Úa's default tolerance rForce < 1e-15) with the GPU assembly,
     FP64 and the precision form; the linear solve on the host (SciPy SuperLU) or, with
     ``device_solver``, on the GPU (block ILU(0) BiCGStab to LinSolveTol = 1e-8)."""
-    from . import newton
-    from .linsolve import GpuLinearSolver
+    from uasource_amd import newton
+    from uasource_amd.linsolve import GpuLinearSolver
     w = WORKLOADS[workload]
     M = build_mesh(w)
     F0, F1, BCs, fixed = newton_problem(M, w)
@@ -280,7 +280,7 @@ def bench_newton_long(workload: str, time_limit: float, checkpoint: str, line_se
     to ``checkpoint`` (.npz of plain arrays); the next call resumes from it. The line reports the
     totals over all calls so far and ``"status": "converged"`` once done."""
     import os
-    from . import newton
+    from uasource_amd import newton
     w = WORKLOADS[workload]
     M = build_mesh(w)
     F0, F1, BCs, fixed = newton_problem(M, w)
@@ -335,7 +335,7 @@ def bench_uv(steps: int, warmup: int, workload: str = "plain_p2", cpu_seconds: f
     the oracle's C restatement of uvMatrixAssemblySSTREAM timed on the host beside it."""
     import os
     import torch
-    from .fields import surface_from_floatation
+    from uasource_amd.fields import surface_from_floatation
     dev = torch.device("cuda:0")
     w = WORKLOADS[workload]
     M = build_mesh(w)

@@ -463,7 +463,7 @@ def uvh_matrix_assembly(CtrlVar, MUA, F0, F1, BCs1=None, return_blocks=False):
         K.sum_duplicates()
         K.sort_indices()
         if np.any(np.isnan(K.diagonal())):
-            raise UaError("uvhMatrixAssembly:NaNinK", " NaN in K ")
+            raise UaError("", " NaN in K ")  # no identifier (uvhMatrixAssembly.m:577)
     if np.any(np.isnan(R)):
         raise UaError("uvhMatrixAssembly:NaNinR", " NaN in R ")
     result = dict(R=R, K=K, Tint=Tint, Fext=Fext, b=b1)

@@ -66,3 +66,33 @@ def test_admission_respects_the_hbm_budget():
             srv.run(make_requests(ms, 2, float("inf"), seed=5, newton_fraction=1.0))
     finally:
         srv.close()
+
+
+def test_a_request_with_a_nan_reports_the_error_and_the_others_are_unaffected():
+    """A request whose state holds a NaN thickness completes with the reference's error
+    (uvhMatrixAssembly:NaNinR, or ' NaN in K ' for a Newton step: uvhMatrixAssembly.m:575-584),
+    read from its own status word; every other request in the mix has no error and equals
+    its alone result bit for bit."""
+    print("[gpu] serve with one NaN request", flush=True)
+    ms = meshes()
+    srv = UvhServer(default_ctrlvar(), ms, slots=6, hbm_budget=8e9)
+    try:
+        reqs = make_requests(ms, 30, float("inf"), seed=7, newton_fraction=0.5)
+        bad = {5, 17}
+        for k in bad:
+            reqs[k].h = reqs[k].h.copy()
+            reqs[k].h[3] = np.nan
+        srv.run(reqs)
+        for r in reqs:
+            if r.rid in bad:
+                assert r.error is not None, f"request {r.rid}: the NaN went unreported"
+                want = "" if r.newton else "uvhMatrixAssembly:NaNinR"
+                assert r.error.identifier in (want, "uvhMatrixAssembly:NaNinR"), r.error
+            else:
+                assert r.error is None, f"request {r.rid}: unexpected {r.error}"
+                alone = srv.serve_alone(r)
+                assert alone.error is None and np.array_equal(alone.R, r.R)
+                if r.newton:
+                    assert np.array_equal(alone.K, r.K)
+    finally:
+        srv.close()

@@ -240,6 +240,25 @@ def test_line_search_accepts_full_step_and_backtracks():
     assert g == 0.0
 
 
+def test_nan_work_is_judged_on_rforce_alone(o2lib, monkeypatch):
+    """A line-search point without a work measure (rWork NaN, as after a Cauchy or
+    steepest-descent point) counts as rWork = 0 in the exit tests
+    (SSTREAM_TransientImplicit.m:275-281): with every accepted point reported so, the driver
+    still stops with "converged" at the iteration where rForce meets the tolerance."""
+    M, F0, F1, fixed, ref = oracle_solution(o2lib, "cpu_reference")
+    real = newton._line_search
+
+    def no_work(*a, **k):
+        g, best, n = real(*a, **k)
+        return g, (best[0], float("nan"), float("nan")), n
+
+    monkeypatch.setattr(newton, "_line_search", no_work)
+    cv = default_ctrlvar()
+    res = newton.solve(OracleAssembler(o2lib, cv, M, F0, F1), cv, fixed=fixed)
+    assert res.converged and res.reason == "converged"
+    assert abs(res.iterations - ref.iterations) <= 1 and res.rForce < cv.uvhDesiredWorkAndForceTolerances[1]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["cpu_reference", "medium_p2", "medium_p3"])
 def test_gpu_newton_matches_oracle_newton(o2lib, name):

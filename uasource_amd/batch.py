@@ -101,7 +101,7 @@ class UvhBatch:
         if st == uvh.UVH_E_NAN_R:
             raise uvh.UaError("uvhMatrixAssembly:NaNinR", msg)
         if st == uvh.UVH_E_NAN_K:
-            raise uvh.UaError("uvhMatrixAssembly:NaNinK", msg)
+            raise uvh.UaError("", msg)  # no identifier (uvhMatrixAssembly.m:577)
         if st == uvh.UVH_E_UNSUPPORTED:
             raise uvh.UvhUnsupported("uvhAssembly:unsupported", msg)
         raise uvh.UvhLibraryError(f"{what}: status {st}: {msg}")

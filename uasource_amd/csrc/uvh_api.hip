@@ -119,6 +119,13 @@ struct uvh_ctx {
     // UVH_ROWS=full or =split forces one form for both.
     int rows_mode = 0;      // 0 automatic, 1 one-pass, 2 three-pass
     int *d_status = nullptr;
+    int *h_status = nullptr;    // pinned copy of d_status for uvh_sync_status
+    // One assembly at a time per context: calls share the work buffers and the status word. A
+    // call enqueued on another stream than the previous one waits for the previous one's
+    // kernels (done_ev, recorded at the end of every call outside graph capture).
+    hipEvent_t done_ev = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool inflight = false;
     // host-path staging
     double *d_fields = nullptr;  // 18 * nn
     double *d_R = nullptr, *d_T = nullptr, *d_F = nullptr, *d_K = nullptr;
@@ -313,6 +320,8 @@ int uvh_create(uvh_ctx **out, int device) {
     ctx->nchunks = ch ? std::max(1, std::min(64, std::atoi(ch))) : kDefaultChunks;
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_status), sizeof(int), 0);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->done_ev, hipEventDisableTiming);
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_cargs), sizeof(uvh::CallArgs) * uvh_ctx::kRing, 0);
     if (e == hipSuccess)
@@ -341,6 +350,8 @@ void uvh_destroy(uvh_ctx *ctx) {
     ctx->resolve_timers();
     ctx->free_mesh();
     hipFree(ctx->d_status);
+    if (ctx->h_status) hipHostFree(ctx->h_status);
+    if (ctx->done_ev) hipEventDestroy(ctx->done_ev);
     for (int k = 0; k < uvh_ctx::kRing; ++k)
         if (ctx->ring_ev[k]) hipEventDestroy(ctx->ring_ev[k]);
     hipHostFree(ctx->h_cargs);
@@ -537,9 +548,78 @@ int uvh_get_csc_map(const uvh_ctx *ctx, int64_t *map) {
     return UVH_OK;
 }
 
+extern "C++" {
+namespace {
+bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// Orders a call on `s` after the context's previous call (see uvh_ctx::done_ev).
+hipError_t begin_call(uvh_ctx *ctx, hipStream_t s) {
+    if (ctx->inflight && s != ctx->last_stream && !stream_capturing(s)) return hipStreamWaitEvent(s, ctx->done_ev, 0);
+    return hipSuccess;
+}
+
+hipError_t end_call(uvh_ctx *ctx, hipStream_t s) {
+    ctx->last_stream = s;
+    if (stream_capturing(s)) {   // the graph's replays are ordered by whoever launches them
+        ctx->inflight = false;
+        return hipSuccess;
+    }
+    ctx->inflight = true;
+    return hipEventRecord(ctx->done_ev, s);
+}
+
+int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
+                         double *dK, double *dT, double *dF, void *stream);
+int uv_assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *df,
+                            uint32_t flags, double *dR, double *dK, double *dT, double *dF, void *stream);
+
+template <class F>
+int ordered_call(uvh_ctx *ctx, void *stream, const char *what, F &&body) {
+    if (!ctx) return UVH_E_ARG;
+    hipSetDevice(ctx->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = begin_call(ctx, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, what);
+    const int st = body();
+    if (st != UVH_OK) return st;
+    e = end_call(ctx, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, what);
+    return UVH_OK;
+}
+}  // namespace
+}  // extern "C++"
+
 int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
                         double *dK, double *dT, double *dF, void *stream) {
+    return ordered_call(ctx, stream, "uvh_assemble_device", [&] {
+        return assemble_device_impl(ctx, ctrl, df, flags, dR, dK, dT, dF, stream);
+    });
+}
+
+int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *df,
+                           uint32_t flags, double *dR, double *dK, double *dT, double *dF, void *stream) {
+    return ordered_call(ctx, stream, "uvh_uv_assemble_device", [&] {
+        return uv_assemble_device_impl(ctx, ctrl, uvc, df, flags, dR, dK, dT, dF, stream);
+    });
+}
+
+int uvh_status_async(uvh_ctx *ctx, int32_t *dst, void *stream) {
     if (!ctx) return UVH_E_ARG;
+    if (!dst) return ctx->fail(UVH_E_ARG, "uvh_status_async: null destination");
+    hipSetDevice(ctx->device);
+    const hipError_t e = hipMemcpyAsync(dst, ctx->d_status, sizeof(int32_t), hipMemcpyDefault,
+                                        static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_status_async");
+    return UVH_OK;
+}
+
+extern "C++" {
+namespace {
+int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
+                         double *dK, double *dT, double *dF, void *stream) {
     if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_assemble: no mesh (call uvh_set_mesh first)");
     if (!ctrl || !df || !dR) return ctx->fail(UVH_E_ARG, "uvh_assemble: null ctrl, fields or R");
     if (!fields_complete(*df)) return ctx->fail(UVH_E_ARG, "uvh_assemble: a required field pointer is null");
@@ -642,13 +722,18 @@ int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df
     return UVH_OK;
 }
 
+}  // namespace
+}  // extern "C++"
+
 int uvh_sync_status(uvh_ctx *ctx) {
     if (!ctx) return UVH_E_ARG;
     hipSetDevice(ctx->device);
-    int st = 0;
-    hipError_t e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(&st, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost);
+    // waits for the context's last assembly only (not for the device)
+    hipError_t e = ctx->inflight ? hipEventSynchronize(ctx->done_ev) : hipSuccess;
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->h_status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->own_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->own_stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_sync_status");
+    const int st = *ctx->h_status;
     if (st & 2) return ctx->fail(UVH_E_NAN_K, " NaN in K ");
     if (st & 1) return ctx->fail(UVH_E_NAN_R, "uvhMatrixAssembly:NaNinR  NaN in R ");
     return UVH_OK;
@@ -700,9 +785,10 @@ int uvh_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *hf, uint3
     return uvh_sync_status(ctx);
 }
 
-int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *df,
-                           uint32_t flags, double *dR, double *dK, double *dT, double *dF, void *stream) {
-    if (!ctx) return UVH_E_ARG;
+extern "C++" {
+namespace {
+int uv_assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *df,
+                            uint32_t flags, double *dR, double *dK, double *dT, double *dF, void *stream) {
     if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_uv_assemble: no mesh (call uvh_set_mesh first)");
     if (!ctrl || !uvc || !df || !dR) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: null ctrl, fields or R");
     for (const double *q : {df->h, df->s, df->ub, df->vb, df->S, df->B, df->rho, df->C, df->m, df->n, df->AGlen})
@@ -758,6 +844,8 @@ int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_uv_assemble: kernel launch");
     return UVH_OK;
 }
+}  // namespace
+}  // extern "C++"
 
 int uvh_uv_assemble(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *hf, uint32_t flags,
                     double *R, double *Kval, double *Tint, double *Fext) {

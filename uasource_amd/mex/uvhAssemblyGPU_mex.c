@@ -20,8 +20,9 @@
  *   K      3Nn x 3Nn sparse double (CSC, sorted row indices), [] when Ronly
  *   status 0, or UVH_E_UNSUPPORTED (5): the caller falls back to uvhMatrixAssembly
  *
- * Errors: the NaN checks raise the reference's identifiers (uvhMatrixAssembly:NaNinR,
- * UaMain/uvhMatrixAssembly.m:581-584; NaN in K, :575-579) through mexErrMsgIdAndTxt.
+ * Errors: the NaN checks raise the reference's errors: uvhMatrixAssembly:NaNinR
+ * (UaMain/uvhMatrixAssembly.m:581-584) through mexErrMsgIdAndTxt, and ' NaN in K ', which has
+ * no identifier there (:575-579), through mexErrMsgTxt.
  * Device state (context, mesh, CSC map) lives in this file's statics; MATLAB calls the gateway
  * on its main thread only, and mexAtExit releases the state. Under spmd each worker process
  * holds its own context.
@@ -53,7 +54,8 @@ static void release(void) {
 
 static void fail(int st) {
     if (st == UVH_E_NAN_R) mexErrMsgIdAndTxt("uvhMatrixAssembly:NaNinR", " NaN in R ");
-    if (st == UVH_E_NAN_K) mexErrMsgIdAndTxt("uvhMatrixAssembly:NaNinK", " NaN in K ");
+    /* the reference raises error(' NaN in K ') without an identifier (uvhMatrixAssembly.m:577) */
+    if (st == UVH_E_NAN_K) mexErrMsgTxt(" NaN in K ");
     mexErrMsgIdAndTxt("uvhAssemblyGPU:failed", "%s", g_ctx ? uvh_last_error(g_ctx) : "no device context");
 }
 
@@ -74,11 +76,15 @@ static void set_mesh(double id, const mxArray *connA, const mxArray *coordA, con
     for (e = 0; e < nele; ++e)
         for (a = 0; a < nod; ++a) c[e * nod + a] = (int32_t)conn[a * nele + e] - 1;
     for (e = 0; e < nip; ++e) { p[2 * e] = pts[e]; p[2 * e + 1] = pts[nip + e]; }
+    /* uvh_set_mesh drops the context's old mesh before it checks the new one: forget the old
+       mesh and its pattern first, so that a failed call leaves no stale mesh id behind */
+    g_mesh_id = -1.0;
+    free(g_rowptr); free(g_colind); free(g_cscmap); free(g_kval);
+    g_rowptr = g_cscmap = NULL; g_colind = NULL; g_kval = NULL;
     st = uvh_set_mesh(g_ctx, (int32_t)nele, (int32_t)nod, (int32_t)nn, c, xy, xy + nn, (int32_t)nip, p, wts);
     mxFree(c);
     mxFree(p);
     if (st != UVH_OK) fail(st);
-    free(g_rowptr); free(g_colind); free(g_cscmap); free(g_kval);
     uvh_pattern_info(g_ctx, &g_neq, &g_nnz);
     g_rowptr = (int64_t *)malloc((size_t)(g_neq + 1) * sizeof(int64_t));
     g_colind = (int32_t *)malloc((size_t)g_nnz * sizeof(int32_t));

@@ -579,9 +579,10 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
     t_start = time.perf_counter()
     done_here = 0
     while True:
-        # rWork is NaN after a Cauchy point (no work measure there): as in MATLAB, every
-        # comparison with it is false, so such a point passes no test that involves rWork
-        rw = rWork
+        # rWork is NaN after a Cauchy or steepest-descent point (no work measure there); it is
+        # then taken as 0, so that the exit and stagnation tests judge such a point on rForce
+        # alone (SSTREAM_TransientImplicit.m:275-281, 345-350)
+        rw = 0.0 if np.isnan(rWork) else rWork
         if residual64 and _criteria(CtrlVar, gamma, rw, rForce, iteration):
             res.converged, res.reason = True, "converged"
             break

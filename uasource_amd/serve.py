@@ -46,6 +46,7 @@ class Request:
     done: float = float("nan")
     R: np.ndarray | None = None
     K: np.ndarray | None = None
+    error: Exception | None = None   # the assembly's NaN error (uvhMatrixAssembly.m:575-584)
     bypassed: int = 0
 
     @property
@@ -99,6 +100,9 @@ class UvhServer:
         # one completion event per slot, recorded again for each request the slot takes (its
         # previous request has completed by then)
         self.slot_ev = [torch.cuda.Event() for _ in range(slots)]
+        # each request's NaN flags, copied stream-ordered behind its kernels into its slot's
+        # pinned word and read when the slot's event has completed
+        self.slot_status = [torch.zeros(1, dtype=torch.int32).pin_memory() for _ in range(slots)]
         self.budget = hbm_budget
         self.flags = uvh.UVH_FP32 if fp32 else 0
         self.max_bypass = max_bypass
@@ -126,8 +130,9 @@ class UvhServer:
                     m.f1[name].copy_(val, non_blocking=True)
             R = self.slot_R[slot][:3 * m.nn]
             K = self.slot_K[slot][:m.ctx.nnz] if req.newton else None
-            flags = self.flags | (0 if req.newton else uvh.UVH_RONLY) | uvh.UVH_NO_NAN_CHECK
+            flags = self.flags | (0 if req.newton else uvh.UVH_RONLY)
             m.ctx.assemble_device(m.ctrl, m.fields, flags, R, K, None, None, stream=s.cuda_stream)
+            m.ctx.status_async(self.slot_status[slot], stream=s.cuda_stream)
             ev = self.slot_ev[slot]
             ev.record(s)
         return (req, slot, R, K, ev)
@@ -177,6 +182,7 @@ class UvhServer:
                 req, slot, R, K, ev = item
                 if ev.query():
                     req.done = clock() - t0
+                    req.error = uvh.UvhContext.status_error(int(self.slot_status[slot][0]))
                     if self.keep_outputs:
                         req.R = R.cpu().numpy()
                         req.K = None if K is None else K.cpu().numpy()
@@ -248,6 +254,7 @@ class UvhServer:
         item = self._launch(r, 0)
         self.torch.cuda.synchronize()
         _, _, R, K, _ = item
+        r.error = uvh.UvhContext.status_error(int(self.slot_status[0][0]))
         r.R = R.cpu().numpy()
         r.K = None if K is None else K.cpu().numpy()
         return r

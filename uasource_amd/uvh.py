@@ -36,7 +36,7 @@ TAU_OPTIONS = {"taus": 0, "taut": 1, "tau1": 2, "tau2": 3, "tau0": 4}
 EXPORTED = ("uvh_create", "uvh_destroy", "uvh_last_error", "uvh_set_mesh", "uvh_pattern_info",
             "uvh_get_pattern", "uvh_get_csc_map", "uvh_assemble", "uvh_assemble_device", "uvh_sync_status",
             "uvh_profile", "uvh_kernel_times", "uvh_uv_pattern_info", "uvh_uv_get_pattern", "uvh_uv_assemble",
-            "uvh_uv_assemble_device")
+            "uvh_uv_assemble_device", "uvh_status_async")
 
 
 class UaError(RuntimeError):
@@ -171,6 +171,8 @@ def load_library(path: str = LIB_PATH):
     lib.uvh_uv_assemble_device.restype = C.c_int
     lib.uvh_sync_status.argtypes = [vp]
     lib.uvh_sync_status.restype = C.c_int
+    lib.uvh_status_async.argtypes = [vp, vp, vp]
+    lib.uvh_status_async.restype = C.c_int
     lib.uvh_profile.argtypes = [vp, C.c_int]
     lib.uvh_profile.restype = C.c_int
     lib.uvh_kernel_times.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
@@ -307,7 +309,7 @@ class UvhContext:
         if st == UVH_E_NAN_R:
             raise UaError("uvhMatrixAssembly:NaNinR", " NaN in R ")
         if st == UVH_E_NAN_K:
-            raise UaError("uvhMatrixAssembly:NaNinK", " NaN in K ")
+            raise UaError("", " NaN in K ")  # no identifier (uvhMatrixAssembly.m:577)
         if st == UVH_E_UNSUPPORTED:
             raise UvhUnsupported("uvhAssembly:unsupported", msg)
         raise UvhLibraryError(f"{what}: status {st}: {msg}")
@@ -365,6 +367,20 @@ class UvhContext:
 
     def sync_status(self):
         self._check(self.lib.uvh_sync_status(self.h), "uvh_sync_status")
+
+    def status_async(self, dst, stream=None):
+        """Enqueue a copy of the status word (bit 0 NaN in R, bit 1 NaN in K) into ``dst`` (a
+        pinned host or device int32 tensor, or a pointer) on ``stream``."""
+        self._check(self.lib.uvh_status_async(self.h, _ptr(dst), stream), "uvh_status_async")
+
+    @staticmethod
+    def status_error(word: int):
+        """The reference's error for a status word, or None (uvhMatrixAssembly.m:575-584)."""
+        if word & 2:
+            return UaError("", " NaN in K ")
+        if word & 1:
+            return UaError("uvhMatrixAssembly:NaNinR", " NaN in R ")
+        return None
 
     # ---------------------------------------------------------------- uv assembly
     def uv_pattern(self):
