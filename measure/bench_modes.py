@@ -8,7 +8,7 @@ import numpy as np
 
 from uasource_amd import uvh
 from uasource_amd.ctrlvar import default_ctrlvar
-from uasource_amd.fields import WORKLOADS, batch_workloads, build_mesh, make_fields, newton_problem
+from uasource_amd.fields import WORKLOADS, batch_workloads, build_mesh, make_fields, newton_problem, serve_workloads
 
 
 def _device_fields(torch, dev, F0, F1):
@@ -127,14 +127,9 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
     (lognormal, P1/P2/P3 = 50/35/15 %), Poisson arrivals at fractions of the saturated rate."""
     import torch
     from uasource_amd.serve import UvhServer, make_requests
-    rng = np.random.default_rng(17)
-    sizes = np.clip(np.exp(rng.normal(np.log(20000), 1.0, n_meshes)), 2000, 300000)
-    types = [((3, 2), (6, 5), (10, 8))[rng.choice(3, p=(0.5, 0.35, 0.15))] for _ in range(n_meshes)]
     meshes = []
     t0 = time.perf_counter()
-    for k, (nele, t) in enumerate(zip(sizes, types)):
-        w = batch_workloads(n=1, seed=100 + k, lo=nele, hi=nele * 1.0001, types=(t,), name=f"srv{k}",
-                            C_sigma=0.2, A_sigma=0.0)[0]
+    for w in serve_workloads(n_meshes, seed=17):
         M = build_mesh(w)
         F0, F1, _ = make_fields(M, w)
         meshes.append((M, F0, F1))
@@ -221,7 +216,8 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
         "ms_per_step": at70["p50_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded meshes and Poisson arrivals)",
         "config": {"workload": "serve500", "meshes": n_meshes, "slots": slots,
-                   "Nele_range": [int(sizes.min()), int(sizes.max())], "parallelism": "1 GPU"},
+                   "Nele_range": [int(min(m[0].Nele for m in meshes)), int(max(m[0].Nele for m in meshes))],
+                   "parallelism": "1 GPU"},
         "saturated_requests_per_s": sat_rate, "saturated_elements_per_s": elements / makespan,
         "saturated_makespans_ms": [float(s * 1e3) for s in spans], "sweep": sweep,
     }

@@ -341,3 +341,31 @@ def test_single_element_mesh(o2lib, tri):
                 assert np.linalg.norm(gpu["R"] - ref["R"]) <= 1e-6 * scale
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("tri,degree,grid", [(3, 4, (61, 31)), (6, 8, (41, 21)), (10, 10, (31, 16))])
+def test_reference_default_quadrature_many_tiles(o2lib, tri, degree, grid):
+    """The reference's default rules (QuadratureRuleDegree.m:15,21,28: degree 4/8/10, 6/16/25
+    points for 3/6/10-node elements) on meshes of many tiles (at 25 points a tile holds about
+    ten 10-node elements), against the oracle to the FP64 bar, with R only and the
+    precision form beside."""
+    from uasource_amd.mesh import CreateMUA, jittered_delaunay
+    xy, el = jittered_delaunay(100e3, 50e3, grid[0], grid[1], 0.1, seed=60 + tri)
+    M = CreateMUA(xy, el, TriNodes=tri, QuadratureRuleDegree=degree)
+    say(f"[gpu] default quadrature nod={tri} nip={M.nip} Nele={M.Nele}")
+    F0, F1, _ = make_fields(M, WORKLOADS["medium_p2"])
+    cv = default_ctrlvar()
+    ref = o2lib.assemble(cv, M, F0, F1)
+    ctx = uvh.UvhContext(0)
+    try:
+        ctx.set_mesh(M)
+        gpu = ctx.assemble(cv, F0, F1, want_tf=True)
+        assert_parity(ctx, M, gpu, ref)
+        r_only = ctx.assemble(cv, F0, F1, ronly=True)
+        assert np.array_equal(r_only["R"], gpu["R"])
+        lo = ctx.assemble(cv, F0, F1, fp32=True)
+        scale = max(np.linalg.norm(ref["Tint"]), np.linalg.norm(ref["Fext"]))
+        assert np.linalg.norm(lo["R"] - ref["R"]) <= 1e-6 * scale
+        assert np.linalg.norm(lo["Kval"] - ref["Kval"]) <= 1e-5 * np.linalg.norm(ref["Kval"])
+    finally:
+        ctx.close()

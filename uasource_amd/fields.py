@@ -163,6 +163,16 @@ def batch_workloads(n: int = 64, seed: int = 4, lo: int = 5000, hi: int = 80000,
     return out
 
 
+def serve_workloads(n_meshes: int = 24, seed: int = 17) -> list:
+    """Meshes of the serving workload: element counts lognormal around 20k (sigma 1) clipped to
+    [2k, 300k], element types P1/P2/P3 = 50/35/15 % with 3/7/16 points, C lognormal (sigma 0.2)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(rng.normal(np.log(20000), 1.0, n_meshes)), 2000, 300000)
+    types = [((3, 2), (6, 5), (10, 8))[rng.choice(3, p=(0.5, 0.35, 0.15))] for _ in range(n_meshes)]
+    return [batch_workloads(n=1, seed=100 + k, lo=nele, hi=nele * 1.0001, types=(t,), name=f"srv{k}",
+                            C_sigma=0.2, A_sigma=0.0)[0] for k, (nele, t) in enumerate(zip(sizes, types))]
+
+
 def surface_from_floatation(F, CtrlVar):
     """Upper surface s = b + h with b from the floatation of h over the bed B and sea level S
     (UaMain/Calc_bs_From_hBS.m:71-79): the input the uv assembly takes as ``F.s``."""

@@ -47,6 +47,7 @@ class Request:
     R: np.ndarray | None = None
     K: np.ndarray | None = None
     error: Exception | None = None   # the assembly's NaN error (uvhMatrixAssembly.m:575-584)
+    keep: bool = True                # copy R (and K) to the host at completion (with keep_outputs)
     bypassed: int = 0
 
     @property
@@ -183,7 +184,7 @@ class UvhServer:
                 if ev.query():
                     req.done = clock() - t0
                     req.error = uvh.UvhContext.status_error(int(self.slot_status[slot][0]))
-                    if self.keep_outputs:
+                    if self.keep_outputs and req.keep:
                         req.R = R.cpu().numpy()
                         req.K = None if K is None else K.cpu().numpy()
                     m = self.meshes[req.mesh]
