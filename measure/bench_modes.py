@@ -269,6 +269,51 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
     }
 
 
+def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = True, log=print) -> dict:
+    """The start of a transient run as Úa makes it (Ua2D.m:598-611): a uv (diagnostic) Newton
+    solve on F1's geometry with the GPU uv assembly (host SuperLU for its steps), then the
+    implicit uvh Newton solve (SSTREAM_TransientImplicit.m) from that balanced state with the GPU
+    assembly and, with ``device_solver``, the device block-ILU(0) Krylov solve (else host
+    SuperLU), to Úa's default tolerance rForce < 1e-15. FP64 throughout."""
+    from uasource_amd import newton
+    from uasource_amd.linsolve import GpuLinearSolver
+    w = WORKLOADS[workload]
+    M = build_mesh(w)
+    F0, F1, BCs, fixed = newton_problem(M, w)
+    cv = default_ctrlvar()
+    ctx = uvh.UvhContext(0)
+    t0 = time.perf_counter()
+    log(f"transient start on {workload} ({M.Nele} elements): uv solve ({2 * M.Nnodes} unknowns)")
+    G0, G1, ruv = newton.transient_start(cv, M, F0, F1, fixed,
+                                         lambda c, m, F: newton.GpuUvAssembler(c, m, F, ctx=ctx), verbose=True, log=log)
+    t_uv = time.perf_counter() - t0
+    log(f"uv: {ruv.iterations} iterations, {ruv.reason}, rForce {ruv.rForce:.3e}, {t_uv:.1f} s")
+    t1 = time.perf_counter()
+    asm = newton.GpuAssembler(cv, M, G0, G1, ctx=ctx)
+    solver = GpuLinearSolver(asm.ctx, fixed, rtol=cv.LinSolveTol) if device_solver else newton.superlu_solve
+    res = newton.solve(asm, cv, linear_solver=solver, fixed=fixed, verbose=True, log=log)
+    t_uvh = time.perf_counter() - t1
+    log(f"uvh: {res.iterations} iterations, {res.reason}, rForce {res.rForce:.3e}, {t_uvh:.1f} s")
+    if device_solver:
+        solver.close()
+    ctx.close()
+    hist = lambda r: [[float(v) for v in h[:4]] for h in r.history]  # noqa: E731
+    return {
+        "metric": "uvh Newton solve to rForce < 1e-15 (transient start)", "value": t_uv + t_uvh, "unit": "s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": (t_uv + t_uvh) * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, inflow conditions)",
+        "config": {"workload": workload, "Nele": M.Nele, "Nnodes": M.Nnodes,
+                   "linear_solver": ("block ILU(0) Krylov (GPU)" if device_solver else "SciPy SuperLU (host)")
+                   + " for uvh; SciPy SuperLU (host) for uv", "parallelism": "1 GPU"},
+        "uv": {"converged": bool(ruv.converged), "iterations": int(ruv.iterations), "rForce": float(ruv.rForce),
+               "seconds": t_uv, "history": hist(ruv)},
+        "uvh": {"converged": bool(res.converged), "iterations": int(res.iterations), "rForce": float(res.rForce),
+                "reason": res.reason, "seconds": t_uvh, "assembly_s": res.t_assembly, "solve_s": res.t_solve,
+                "linesearch_s": res.t_linesearch, "history": hist(res),
+                "krylov_iterations": res.linear_iterations if device_solver else None},
+    }
+
+
 def bench_newton_long(workload: str, time_limit: float, checkpoint: str, line_search: str = "-Newton Step-",
                       log=print) -> dict:
     """FP64 Newton solve of a workload too large for one call (the full config-2 mesh: ~65 s

@@ -216,24 +216,42 @@ def test_full_size_plain_p2_parity(ctx, o2lib):
     assert_parity(ctx, M, gpu, ref)
 
 
-@pytest.mark.parametrize("name", ["tiny_p1", "medium_p2", "medium_p3", "plain_p2"])
-def test_fused_path_matches_oracle(o2lib, monkeypatch, name):
+@pytest.mark.parametrize("name", ["tiny_p1", "tiny_p2", "tiny_p3", "medium_p2", "medium_p3", "plain_p2"])
+def test_fused_path_matches_oracle_and_two_pass(o2lib, monkeypatch, name):
     """A/B check in the manner of the reference's CtrlVar.Parallel.isTest: the fused
-    LDS-window path (UVH_ASSEMBLY=fused) meets the oracle like the default two-pass path."""
+    LDS-window path (UVH_ASSEMBLY=fused: one kernel, no element rows in HBM) meets the oracle
+    like the default two-pass path, and the two agree to rounding (they sum the diagonal
+    blocks and residual entries of vertices in different orders, every other entry exactly
+    alike); each path's R alone equals its R of R+K bit for bit, and the fused path is
+    bitwise reproducible."""
     M, F0, F1, BCs = mesh_fields(name)
     cv = default_ctrlvar()
     ref = o2lib.assemble(cv, M, F0, F1, nthreads=16)
     monkeypatch.setenv("UVH_ASSEMBLY", "fused")
-    c2 = uvh.UvhContext(0)
+    c1 = uvh.UvhContext(0)
     monkeypatch.delenv("UVH_ASSEMBLY")
+    c2 = uvh.UvhContext(0)
     try:
-        say(f"[gpu] fused path {name}")
+        say(f"[gpu] fused and two-pass paths {name}")
         c2.set_mesh(M)
-        out2 = c2.assemble(cv, F0, F1, want_tf=True)
-        assert_parity(c2, M, out2, ref)
-        r_only = c2.assemble(cv, F0, F1, ronly=True)
-        assert np.array_equal(r_only["R"], out2["R"])
+        c1.set_mesh(M)
+        c1.profile(True)
+        fused = c1.assemble(cv, F0, F1, want_tf=True)
+        assert "fused_rk" in c1.kernel_times(), "UVH_ASSEMBLY=fused did not select the fused kernel"
+        two = c2.assemble(cv, F0, F1, want_tf=True)
+        assert_parity(c2, M, two, ref)
+        assert_parity(c1, M, fused, ref)
+        scale = max(np.linalg.norm(two["Tint"]), np.linalg.norm(two["Fext"]))
+        assert np.linalg.norm(fused["R"] - two["R"]) <= 1e-14 * scale
+        assert np.linalg.norm(fused["Kval"] - two["Kval"]) <= 1e-14 * np.linalg.norm(two["Kval"])
+        for c, out in ((c1, fused), (c2, two)):
+            r_only = c.assemble(cv, F0, F1, ronly=True)
+            assert np.array_equal(r_only["R"], out["R"])
+        again = c1.assemble(cv, F0, F1, want_tf=True)
+        for k in ("R", "Kval", "Tint", "Fext"):
+            assert np.array_equal(again[k], fused[k]), k
     finally:
+        c1.close()
         c2.close()
 
 

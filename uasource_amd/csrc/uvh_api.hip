@@ -36,7 +36,7 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
 int element_epb_default(int nod, int nip, bool fp32);
 size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
-                        int64_t nnzN, int nn, bool ronly, int check_nan, hipStream_t s);
+                        int64_t nnzN, int nn, bool sl, bool ronly, int check_nan, hipStream_t s);
 }  // namespace uvh
 
 namespace {
@@ -94,13 +94,16 @@ struct uvh_ctx {
     std::vector<int> plan_tiles[2];        // nchunks+1 tile boundaries
     std::vector<int> plan_boff[2];         // nchunks+1 offsets into the block list
     int32_t *d_plan_blocks[2] = {nullptr, nullptr};
-    // fused-assembly schedule
-    bool two_pass = true;   // false: fused LDS-window kernel (UVH_ASSEMBLY=fused)
+    // fused-assembly schedule (UVH_ASSEMBLY=fused): FP64 R+K and R assemblies take the fused
+    // kernel when the mesh has a schedule (every node in an element, at most two elements per
+    // pair of nodes); the precision form and the uv assembly take the two-pass kernels
+    bool fused_wanted = false;
+    bool fused_ok = false;      // this mesh has a fused schedule
+    std::string fused_why;      // why not, when it has none
     uvh::Schedule sched;
     uvh::FusedLayout layout{};
-    int32_t *d_pbp = nullptr, *d_belem = nullptr, *d_easlot = nullptr, *d_zptr = nullptr, *d_zslot = nullptr;
-    int32_t *d_zlen = nullptr, *d_fptr = nullptr, *d_fnode = nullptr, *d_fslot = nullptr;
-    uint8_t *d_earank = nullptr, *d_bmax = nullptr;
+    uvh::PackedSchedule psched;       // host copy of the packed schedule (kept for its sizes)
+    std::vector<void *> d_sched;      // the packed schedule's device arrays
     uvh::MeshArgs *d_margs = nullptr;
     // per-call argument blocks: a ring of pinned host copies and device copies; a slot is
     // reused only after the event recorded behind its upload has completed
@@ -147,8 +150,10 @@ struct uvh_ctx {
         d_tab32 = nullptr;
         hipFree(d_inc_ptr); hipFree(d_inc); hipFree(d_pos); hipFree(d_b); hipFree(d_RE); hipFree(d_EB);
         hipFree(d_fields); hipFree(d_R); hipFree(d_T); hipFree(d_F); hipFree(d_K);
-        hipFree(d_diag); hipFree(d_pbp); hipFree(d_belem); hipFree(d_easlot); hipFree(d_zptr); hipFree(d_zslot);
-        hipFree(d_zlen); hipFree(d_fptr); hipFree(d_fnode); hipFree(d_fslot); hipFree(d_earank); hipFree(d_bmax);
+        hipFree(d_diag);
+        for (void *q : d_sched) hipFree(q);
+        d_sched.clear();
+        fused_ok = false;
         hipFree(d_margs); hipFree(d_pos_inc); hipFree(d_inc_pos);
         hipFree(d_nzrow); hipFree(d_imap_ptr); hipFree(d_imap); hipFree(d_zero);
         d_nzrow = nullptr; d_imap_ptr = nullptr; d_imap = nullptr; d_zero = nullptr;
@@ -159,8 +164,7 @@ struct uvh_ctx {
         }
         d_margs = nullptr; d_pos_inc = nullptr; d_inc_pos = nullptr;
         last_slot = -1;
-        d_diag = d_earank = d_bmax = nullptr;
-        d_pbp = d_belem = d_easlot = d_zptr = d_zslot = d_zlen = d_fptr = d_fnode = d_fslot = nullptr;
+        d_diag = nullptr;
         d_conn = nullptr; d_x = d_y = nullptr; d_tab = nullptr; d_rowptr_n = nullptr;
         d_inc_ptr = d_inc = nullptr; d_pos = nullptr; d_b = d_RE = d_EB = nullptr;
         d_fields = d_R = d_T = d_F = d_K = nullptr;
@@ -308,10 +312,11 @@ int uvh_create(uvh_ctx **out, int device) {
     if (hipSetDevice(device) != hipSuccess) return UVH_E_HIP;
     auto *ctx = new uvh_ctx();
     ctx->device = device;
-    // UVH_ASSEMBLY=fused selects the single-kernel LDS-window assembly; the default is the
-    // two-pass path (element rows through HBM, then one wave per node).
+    // UVH_ASSEMBLY=fused selects the single-kernel LDS-window assembly for FP64; the default is
+    // the two-pass path (element rows through HBM, then one lane group per node), which is
+    // faster on MI355X (measured: 0.59 ms against 1.27-1.9 ms at 199,712 6-node elements).
     const char *am = std::getenv("UVH_ASSEMBLY");
-    ctx->two_pass = !(am && std::strcmp(am, "fused") == 0);
+    ctx->fused_wanted = am && std::strcmp(am, "fused") == 0;
     const char *rows = std::getenv("UVH_ROWS");
     ctx->rows_mode = !rows ? 0 : std::strcmp(rows, "full") == 0 ? 1 : std::strcmp(rows, "split") == 0 ? 2 : 0;
     const char *sc = std::getenv("UVH_SCATTER");
@@ -385,16 +390,33 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     m.y.assign(y, y + nnodes);
     std::string msg = uvh::fill_shape_tables(m.tab, nod, nip, pts, wts);
     if (msg.empty()) msg = uvh::build_mesh_host(m);
+    if (!msg.empty()) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: " + msg);
     ctx->layout = uvh::fused_layout(nod, nip);
     ctx->sched = uvh::Schedule{};
-    if (msg.empty() && !ctx->two_pass && m.has_isolated)
-        msg = "the fused assembly (UVH_ASSEMBLY=fused) needs every node in an element";
-    if (msg.empty() && !ctx->two_pass) {  // the fused path's patch schedule
-        const char *pe = std::getenv("UVH_PATCHES");
-        const int target = pe ? std::max(1, std::atoi(pe)) : 256;
-        msg = uvh::build_schedule(m, ctx->layout.epb, ctx->layout.window_doubles, target, ctx->sched);
+    ctx->fused_why.clear();
+    bool fused = ctx->fused_wanted;
+    if (fused && m.has_isolated) {
+        fused = false;
+        ctx->fused_why = "a node belongs to no element";
     }
-    if (!msg.empty()) return ctx->fail(UVH_E_ARG, "uvh_set_mesh: " + msg);
+    if (fused) {  // the fused kernel's patch schedule
+        int pelems = 0, band = 0;
+        uvh::fused_patch_shape(nod, pelems, band);
+        std::string why = uvh::build_schedule(m, ctx->layout.epb, ctx->layout.window_doubles, pelems, band, ctx->sched);
+        if (why.empty() && ctx->sched.max_stage > ctx->layout.stage_items) {
+            // more ordered items in a batch than the staging holds: grow it, shrink the window
+            const int grow = ctx->sched.max_stage - ctx->layout.stage_items;
+            ctx->layout.stage_items = ctx->sched.max_stage;
+            ctx->layout.window_doubles -= ((int64_t(grow) * uvh::kStageValues + 1) / 2) * 2;
+            why = uvh::build_schedule(m, ctx->layout.epb, ctx->layout.window_doubles, pelems, band, ctx->sched);
+            if (why.empty() && ctx->sched.max_stage > ctx->layout.stage_items) why = "ordered staging does not fit";
+        }
+        if (!why.empty()) {
+            fused = false;
+            ctx->fused_why = why;
+            ctx->sched = uvh::Schedule{};
+        }
+    }
     hipError_t e = hipSuccess;
     const size_t ne = size_t(nele), nn = size_t(nnodes);
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
@@ -409,19 +431,26 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(dalloc(&ctx->d_pos, ne * nod * nod));
     chk(dalloc(&ctx->d_b, nn));
     chk(dalloc(&ctx->d_diag, nn));
-    const uvh::Schedule &S = ctx->sched;
-    chk(dalloc(&ctx->d_pbp, S.patch_batch_ptr.size()));
-    chk(dalloc(&ctx->d_belem, S.batch_elem.size()));
-    chk(dalloc(&ctx->d_easlot, S.ea_slot.size()));
-    chk(dalloc(&ctx->d_earank, S.ea_rank.size()));
-    chk(dalloc(&ctx->d_bmax, S.batch_maxrank.size()));
-    chk(dalloc(&ctx->d_zptr, S.zero_ptr.size()));
-    chk(dalloc(&ctx->d_zslot, std::max<size_t>(1, S.zero_slot.size())));
-    chk(dalloc(&ctx->d_zlen, std::max<size_t>(1, S.zero_len.size())));
-    chk(dalloc(&ctx->d_fptr, S.flush_ptr.size()));
-    chk(dalloc(&ctx->d_fnode, std::max<size_t>(1, S.flush_node.size())));
-    chk(dalloc(&ctx->d_fslot, std::max<size_t>(1, S.flush_slot.size())));
-    if (ctx->two_pass) {
+    // the packed schedule: patch ranges, batch headers, gather nodes, item/fixup/zero/write-out records
+    uvh::PackedSchedule &P = ctx->psched;
+    P = uvh::PackedSchedule{};
+    struct Arr { const void *src; size_t bytes; };
+    std::vector<Arr> arrs;
+    if (fused) {
+        uvh::pack_schedule(m, ctx->sched, P);
+        const uvh::Schedule &S = ctx->sched;
+        arrs = {{S.patch_batch_ptr.data(), S.patch_batch_ptr.size() * sizeof(int32_t)},
+                {P.hdr.data(), P.hdr.size() * sizeof(uvh::BatchHdr)},
+                {P.batch_node.data(), P.batch_node.size() * sizeof(int32_t)},
+                {P.items.data(), P.items.size() * sizeof(uvh::PatchItem)},
+                {P.fixes.data(), P.fixes.size() * sizeof(uvh::PatchFix)},
+                {P.zeros.data(), P.zeros.size() * sizeof(uvh::PatchZero)},
+                {P.flushes.data(), P.flushes.size() * sizeof(uvh::PatchFlush)}};
+        ctx->d_sched.assign(arrs.size(), nullptr);
+        for (size_t k = 0; k < arrs.size(); ++k)
+            chk(hipMalloc(&ctx->d_sched[k], std::max<size_t>(16, arrs[k].bytes)));
+    }
+    {
         chk(dalloc(&ctx->d_RE, ne * nod * 6));
         chk(dalloc(&ctx->d_inc_pos, ne * nod));
         chk(dalloc(&ctx->d_pos_inc, ne * nod * nod));
@@ -454,7 +483,7 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(hipMemcpy(ctx->d_inc, m.inc.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_pos, m.pos.data(), ne * nod * nod, hipMemcpyHostToDevice));
     chk(hipMemcpy(ctx->d_diag, m.diag.data(), nn, hipMemcpyHostToDevice));
-    if (ctx->two_pass) {
+    {
         chk(hipMemcpy(ctx->d_inc_pos, m.inc_pos.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
         chk(hipMemcpy(ctx->d_pos_inc, m.pos_inc.data(), ne * nod * nod, hipMemcpyHostToDevice));
         if (ctx->gather_rows) {
@@ -464,20 +493,23 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
             chk(hipMemcpy(ctx->d_imap, m.imap.data(), m.imap.size(), hipMemcpyHostToDevice));
         }
     }
-    auto up = [&](auto *dst, const auto &v) {
-        if (!v.empty()) chk(hipMemcpy(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
-    };
-    up(ctx->d_pbp, S.patch_batch_ptr); up(ctx->d_belem, S.batch_elem); up(ctx->d_easlot, S.ea_slot);
-    up(ctx->d_earank, S.ea_rank); up(ctx->d_bmax, S.batch_maxrank); up(ctx->d_zptr, S.zero_ptr);
-    up(ctx->d_zslot, S.zero_slot); up(ctx->d_zlen, S.zero_len); up(ctx->d_fptr, S.flush_ptr);
-    up(ctx->d_fnode, S.flush_node); up(ctx->d_fslot, S.flush_slot);
+    if (fused && e == hipSuccess)
+        for (size_t k = 0; k < arrs.size(); ++k)
+            if (arrs[k].bytes) chk(hipMemcpy(ctx->d_sched[k], arrs[k].src, arrs[k].bytes, hipMemcpyHostToDevice));
     chk(dalloc(&ctx->d_margs, 1));
     if (e == hipSuccess) {
         uvh::MeshArgs ma{};
-        ma.s = uvh::SchedPtrs{ctx->d_pbp, ctx->d_belem, ctx->d_easlot, ctx->d_earank, ctx->d_bmax, ctx->d_zptr,
-                              ctx->d_zslot, ctx->d_zlen, ctx->d_fptr, ctx->d_fnode, ctx->d_fslot};
-        ma.pos = ctx->d_pos; ma.diag = ctx->d_diag; ma.conn = ctx->d_conn; ma.x = ctx->d_x; ma.y = ctx->d_y;
-        ma.tab = ctx->d_tab; ma.rowptr_n = ctx->d_rowptr_n; ma.b = ctx->d_b; ma.status = ctx->d_status;
+        if (fused) {
+            ma.patch_batch_ptr = static_cast<const int32_t *>(ctx->d_sched[0]);
+            ma.hdr = static_cast<const uvh::BatchHdr *>(ctx->d_sched[1]);
+            ma.batch_node = static_cast<const int32_t *>(ctx->d_sched[2]);
+            ma.items = static_cast<const uvh::PatchItem *>(ctx->d_sched[3]);
+            ma.fixes = static_cast<const uvh::PatchFix *>(ctx->d_sched[4]);
+            ma.zeros = static_cast<const uvh::PatchZero *>(ctx->d_sched[5]);
+            ma.flushes = static_cast<const uvh::PatchFlush *>(ctx->d_sched[6]);
+        }
+        ma.x = ctx->d_x; ma.y = ctx->d_y;
+        ma.tab = ctx->d_tab; ma.b = ctx->d_b; ma.status = ctx->d_status;
         chk(hipMemcpy(ctx->d_margs, &ma, sizeof(ma), hipMemcpyHostToDevice));
     }
     if (e != hipSuccess) {
@@ -486,6 +518,7 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     }
     ctx->epb[0] = uvh::element_epb_default(nod, nip, false);
     ctx->epb[1] = uvh::element_epb_default(nod, nip, true);
+    ctx->fused_ok = fused;
     ctx->has_mesh = true;
     return UVH_OK;
 }
@@ -627,19 +660,18 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
         std::string why;
         const int st = check_law(*ctrl, df->q, df->muk, df->V0, why);
         if (st != UVH_OK) return ctx->fail(st, "uvh_assemble: " + why);
-        if (ctrl->sliding_law != UVH_SLIDING_WEERTMAN && !ctx->two_pass)
-            return ctx->fail(UVH_E_UNSUPPORTED, "uvh_assemble: sliding laws other than Weertman need the two-pass assembly");
     }
     const bool fp32 = (flags & UVH_FP32) != 0;
-    if (fp32 && !ctx->two_pass)
-        return ctx->fail(UVH_E_UNSUPPORTED, "uvh_assemble: UVH_FP32 needs the two-pass assembly");
     if (ctrl->supg_tau < 0 || ctrl->supg_tau > 4) return ctx->fail(UVH_E_ARG, "uvh_assemble: supg_tau out of range");
     const bool ronly = (flags & UVH_RONLY) != 0;
     if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_assemble: K output required unless UVH_RONLY");
     hipSetDevice(ctx->device);
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL is the HIP null stream
     const uvh::MeshHost &m = ctx->mesh;
-    if (ctx->two_pass && !ronly && !ctx->d_EB) {
+    // FP64 R+K and R on the fused kernel where the mesh has its schedule (R alone then sums in
+    // the same order as R of R+K); the precision form on the two-pass kernels
+    const bool fused = ctx->fused_ok && !fp32;
+    if (!fused && !ronly && !ctx->d_EB) {
         // FP64 rows; the FP32 rows of the precision form fit in the same buffer
         const size_t stride = uvh::element_row_stride(m.nod, false);
         if (dalloc(&ctx->d_EB, size_t(m.nele) * m.nod * stride) != hipSuccess)
@@ -649,7 +681,7 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
     const uvh::FieldPtrs f = to_ptrs(*df);
     const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
     hipError_t e = ctx->timed("nodal_b", s, [&] { return uvh::launch_nodal_b(m.nn, f, p, ctx->d_b, ctx->d_status, s); });
-    if (e == hipSuccess && !ctx->two_pass) {
+    if (e == hipSuccess && fused) {
         uvh::CallArgs ca{};
         ca.f = f; ca.p = p; ca.R = dR; ca.Tint = dT; ca.Fext = dF; ca.Kval = ronly ? nullptr : dK;
         int slot = ctx->last_slot;
@@ -667,7 +699,7 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
         if (e == hipSuccess)
             e = ctx->timed(ronly ? "fused_r" : "fused_rk", s, [&] {
                 return uvh::launch_fused(m.nod, m.nip, ctx->sched.npatch, ctx->layout, ctx->d_margs,
-                                         ctx->d_cargs + slot, m.nnzN(), m.nn, ronly, check, s);
+                                         ctx->d_cargs + slot, m.nnzN(), m.nn, p.law != 0, ronly, check, s);
             });
         if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble: kernel launch");
         return UVH_OK;
@@ -800,7 +832,6 @@ int uv_assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctr
         const int st = check_law(*ctrl, df->q, df->muk, df->V0, why);
         if (st != UVH_OK) return ctx->fail(st, "uvh_uv_assemble: " + why);
     }
-    if (!ctx->two_pass) return ctx->fail(UVH_E_UNSUPPORTED, "uvh_uv_assemble: needs the two-pass assembly");
     const bool fp32 = (flags & UVH_FP32) != 0, ronly = (flags & UVH_RONLY) != 0;
     if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_uv_assemble: K output required unless UVH_RONLY");
     hipSetDevice(ctx->device);
@@ -1037,7 +1068,7 @@ int uvhb_add_mesh(uvh_batch *b, int32_t nele, int32_t nod, int32_t nnodes, const
     uvh_ctx *c = nullptr;
     int st = uvh_create(&c, b->device);
     if (st != UVH_OK) return b->fail(st, "uvhb_add_mesh: cannot create a member context");
-    c->two_pass = true;
+    c->fused_wanted = false;   // the batched kernels are the two-pass ones
     st = uvh_set_mesh(c, nele, nod, nnodes, conn, x, y, nip, pts, wts);
     if (st != UVH_OK) {
         b->err = std::string("uvhb_add_mesh: ") + c->err;

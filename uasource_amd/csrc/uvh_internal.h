@@ -65,27 +65,43 @@ struct MeshHost {
 // elements per patch, LDS window slots (offsets in doubles) of the owned node rows.
 struct Schedule {
     int epb = 0;
-    int64_t window = 0;                 // LDS window capacity in doubles
+    int64_t window = 0;                    // LDS window capacity in doubles
     int32_t npatch = 0;
     std::vector<int32_t> patch_batch_ptr;  // npatch+1
     std::vector<int32_t> batch_elem;       // nbatch*epb, -1 padded
-    std::vector<int32_t> ea_slot;          // nbatch*epb*nod: slot of (element, local node), -1 if not owned
-    std::vector<uint8_t> ea_rank;          // nbatch*epb*nod: order among the batch's elements on that node
-    std::vector<uint8_t> batch_maxrank;    // nbatch
-    std::vector<int32_t> zero_ptr;         // nbatch+1
+    // row items of a batch: (element slot, local node) of owned nodes, el*nod + a
+    std::vector<int32_t> item_ptr;         // nbatch+1
+    std::vector<int32_t> item_ea, item_slot;
+    std::vector<int32_t> item_stage;       // staging index of the ordered values, or -1 (direct adds)
+    // fixup groups of a batch: nodes with several items whose ordered values are summed in order
+    std::vector<int32_t> fix_ptr;          // nbatch+1
+    std::vector<int32_t> fix_node, fix_slot;
+    std::vector<int32_t> fix_item_ptr;     // nfix+1
+    std::vector<int32_t> fix_item;         // staging indices in item order
+    std::vector<int32_t> zero_ptr;         // nbatch+1: rows whose first element is in the batch
     std::vector<int32_t> zero_slot, zero_len;
-    std::vector<int32_t> flush_ptr;        // nbatch+1
+    std::vector<int32_t> flush_ptr;        // nbatch+1: rows whose last element is in the batch
     std::vector<int32_t> flush_node, flush_slot;
     int64_t peak_window = 0;               // largest window use over the patches (doubles)
     int64_t patch_work = 0;                // element visits (>= nele: ghosts are visited twice)
-    int32_t nbatch() const { return int32_t(batch_maxrank.size()); }
+    int32_t max_stage = 0;                 // most staged items in one batch
+    int32_t max_items = 0;                 // most row items in one batch
+    int32_t nbatch() const { return int32_t(item_ptr.empty() ? 0 : item_ptr.size() - 1); }
 };
 
 // Row slot size in doubles: 9*deg K values, then Tx, Ty, Th, Fx, Fy, Fh.
 int64_t slot_doubles(int64_t deg);
-// Builds the schedule for `target_patches` patches; patches whose window would exceed
-// `window` doubles are split further. Needs rowptr_n, inc_ptr and inc.
-std::string build_schedule(MeshHost &m, int epb, int64_t window, int target_patches, Schedule &S);
+// Ordered values per staged item: the 3 diagonal entries of its block row's 3 blocks, then T, F,
+// for each of the 3 block rows.
+constexpr int kStageValues = 15;
+// "" if every pair of distinct nodes shares at most two elements (see uvh_schedule.cpp).
+std::string check_two_contributions(const MeshHost &m);
+// Builds the schedule with patches of about `patch_elems` elements in bands `band_cells`
+// element sizes high; patches whose window would exceed `window` doubles are split across
+// their sweep direction. Needs rowptr_n, inc_ptr, inc, pos and diag.
+std::string build_schedule(MeshHost &m, int epb, int64_t window, int patch_elems, int band_cells, Schedule &S);
+// Default patch shape of the fused kernel for `nod`-node elements: elements per patch, band height.
+void fused_patch_shape(int nod, int &patch_elems, int &band_cells);
 
 // Builds rowptr_n/colind_n, inc_ptr/inc and pos from conn; returns "" or an error message.
 std::string build_mesh_host(MeshHost &m);
@@ -95,33 +111,62 @@ void build_csc_map(const MeshHost &m, int64_t *map);
 // Fills the shape tables for nod nodes at the nip points pts (nip x 2) with weights wts.
 std::string fill_shape_tables(ShapeTables &t, int nod, int nip, const double *pts, const double *wts);
 
-// Schedule tables of the fused kernel (device pointers; see uvh_schedule.cpp).
-struct SchedPtrs {
-    const int32_t *patch_batch_ptr, *batch_elem, *ea_slot;
-    const uint8_t *ea_rank, *batch_maxrank;
-    const int32_t *zero_ptr, *zero_slot, *zero_len, *flush_ptr, *flush_node, *flush_slot;
+// The schedule as the fused kernel reads it: per batch one header, and records that carry
+// everything a thread needs in one load (no chains of dependent index loads).
+struct BatchHdr {             // ranges of the batch's items, zero, fixup and write-out records
+    int32_t item0, item1, zero0, zero1, fix0, fix1, flush0, flush1;
 };
+struct alignas(16) PatchItem {
+    int32_t slot, stage, ea, deg;   // window slot, staging index or -1, el*nod + a, node degree
+    uint8_t pos[16];                // column of each element node in the node's row
+};
+struct alignas(16) PatchFix {       // one node's staged items: stage0 .. stage0 + count - 1
+    int32_t slot, stage0, count, deg_dcol;   // deg | diagonal column << 16
+};
+struct alignas(8) PatchZero { int32_t slot, len; };
+struct alignas(16) PatchFlush {
+    int32_t node, slot, deg, dcol;
+    int64_t rp3, pad;               // 3 * rowptr_n[node]: the row's offset in each block row of K
+};
+struct PackedSchedule {
+    std::vector<BatchHdr> hdr;
+    std::vector<int32_t> batch_node;    // nbatch*epb*nod: node of (element slot, local node), -1 padded
+    std::vector<PatchItem> items;
+    std::vector<PatchFix> fixes;
+    std::vector<PatchZero> zeros;
+    std::vector<PatchFlush> flushes;
+};
+// Packs a schedule for the kernel.
+void pack_schedule(const MeshHost &m, const Schedule &S, PackedSchedule &P);
 
 // Argument blocks of the fused kernel, kept in device memory and read through one pointer
-// each: as kernel arguments the ~45 pointers would not fit in scalar registers.
+// each: as kernel arguments the ~30 pointers would not fit in scalar registers.
 struct MeshArgs {             // fixed per mesh
-    SchedPtrs s;
-    const uint8_t *pos, *diag;
-    const int32_t *conn;
+    const int32_t *patch_batch_ptr;
+    const BatchHdr *hdr;
+    const int32_t *batch_node;
+    const PatchItem *items;
+    const PatchFix *fixes;
+    const PatchZero *zeros;
+    const PatchFlush *flushes;
     const double *x, *y;
     const ShapeTables *tab;
-    const int64_t *rowptr_n;
     double *b;                // nodal lower surface, written by the pre-pass
     int *status;
 };
 
-// LDS budget of the fused kernel per workgroup (gfx950: 160 KiB per workgroup).
-constexpr int64_t kFusedLdsBytes = 160 * 1024;
+// LDS budget of the fused kernel per workgroup: two workgroups share a CU (160 KiB).
+constexpr int64_t kFusedLdsBytes = 80 * 1024;
 
-// LDS layout of the fused kernel (in doubles) and its elements per batch.
+// LDS layout of the fused kernel (in doubles) and its elements per batch: shape tables,
+// nodal staging, coefficient records, ordered staging (stage_items x kStageValues), window.
 struct FusedLayout {
-    int table_doubles, work_doubles, epb;
+    int table_doubles, nodal_doubles, rec_doubles, epb, stage_items;
     int64_t window_doubles;
+    int64_t lds_bytes() const {
+        return (int64_t(table_doubles) + nodal_doubles + rec_doubles + int64_t(stage_items) * kStageValues +
+                window_doubles) * int64_t(sizeof(double));
+    }
 };
 FusedLayout fused_layout(int nod, int nip);
 

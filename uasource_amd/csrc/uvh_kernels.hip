@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -449,11 +450,25 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
 
 // Gather of node a of element e into the LDS staging record nv[NV_COUNT][NOD].
 template <int NOD>
+__device__ __forceinline__ void gather_node_at(int k, int a, const double *__restrict__ xn,
+                                               const double *__restrict__ yn, const FieldPtrs &f,
+                                               const double *__restrict__ bnod, const Params &p,
+                                               double *__restrict__ nv);
+
+template <int NOD>
 __device__ __forceinline__ void gather_node(const int32_t *__restrict__ conn, const double *__restrict__ xn,
                                             const double *__restrict__ yn, const FieldPtrs &f,
                                             const double *__restrict__ bnod, const Params &p, int64_t e, int a,
                                             double *__restrict__ nv) {
-    const int k = conn[e * NOD + a];
+    gather_node_at<NOD>(conn[e * NOD + a], a, xn, yn, f, bnod, p, nv);
+}
+
+// Gather of node k, local node a of its element, into the LDS staging record nv[NV_COUNT][NOD].
+template <int NOD>
+__device__ __forceinline__ void gather_node_at(int k, int a, const double *__restrict__ xn,
+                                               const double *__restrict__ yn, const FieldPtrs &f,
+                                               const double *__restrict__ bnod, const Params &p,
+                                               double *__restrict__ nv) {
     double h = f.h[k], u = f.u[k], v = f.v[k], u0 = f.u0[k], v0 = f.v0[k];
     double as1 = f.as1[k], ab1 = f.ab1[k], as0 = f.as0[k], ab0 = f.ab0[k];
     const double h0 = f.h0[k];
@@ -645,6 +660,7 @@ __device__ __forceinline__ void element_row_full(int i, int nip, const T *__rest
 // element on gfx950's 32 four-byte banks).
 template <int NOD>
 __host__ __device__ constexpr int nv_stride() { return NV_COUNT * NOD + 1; }
+constexpr int nv_stride_host(int nod) { return NV_COUNT * nod + 1; }
 __host__ __device__ constexpr int rec_stride(int nip) { return nip * kNC + 1; }
 
 // Shape tables in LDS for the row stage, in type T: the FP64 tables themselves for the plain
@@ -1186,148 +1202,182 @@ __global__ __launch_bounds__(256) void k_rows_r_batch(const BatchMember *__restr
     if (check_nan && (isnan(r0) || isnan(r1) || isnan(r2))) atomicOr(status + w.x, 1);
 }
 
-// Fused assembly: one 512-thread workgroup per patch. Per batch of `epb` elements: zero the
-// window slots of rows that start here, gather nodal values, compute the integration-point
-// physics, compute the element rows as (element, node, block row) items, add the rows of
-// owned nodes into their window slots in rank order (one rank per barrier phase, so no two
-// threads add to the same slot at once and every sum has a fixed order), then write the rows
-// whose last element was in this batch to K, R, Tint and Fext.
-constexpr int kFusedThreads = 512;
-constexpr int kFusedWaves = kFusedThreads / 64;
+// Fused assembly: one 256-thread workgroup per patch, two workgroups per CU. Per batch of
+// `epb` elements, in three phases separated by barriers:
+//   A  zero the window slots of rows that start in the batch; add the previous batch's staged
+//      ordered sums (fixups) into the window
+//   B  integration-point physics into coefficient records; write out the rows whose last
+//      element was in the previous batch (K, R, Tint, Fext)
+//   C  row items ((element, local node) of owned nodes, per block row): the element's row in
+//      registers, added into the owning node's window slot, ordered values staged where the
+//      node has several items in the batch; on the remaining lanes, the gathers of the next
+//      batch's nodal values (their latency hides behind the row arithmetic)
+// The schedule records each carry what a thread needs in one load, and each phase's records
+// are loaded at the top of the iteration, so no phase waits on a chain of index loads.
+// Every K entry off a node's diagonal block, and every value of a node with at most two
+// incident elements, is a sum of at most two terms and is added with ds_add_f64 in any order
+// (a + b = b + a exactly); the diagonal block and T, F of a node with three or more incident
+// elements take one add per batch, in batch order. Results are bitwise reproducible.
+constexpr int kFusedThreads = 256;
+constexpr int kFlushLanes = 16;
 
-template <int NOD, bool RONLY>
+template <int NOD>
+__device__ __forceinline__ void gather_batch(const MeshArgs *__restrict__ MA, const FieldPtrs &f, const Params &p,
+                                             int b, int epb, int g, double *__restrict__ nodal) {
+    const int k = MA->batch_node[int64_t(b) * epb * NOD + g];
+    if (k >= 0) gather_node_at<NOD>(k, g % NOD, MA->x, MA->y, f, MA->b, p, nodal + (g / NOD) * nv_stride<NOD>());
+}
+
+template <int NOD, bool SL, bool RONLY>
 __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restrict__ MA,
                                                          const CallArgs *__restrict__ CA, int nip, int epb,
-                                                         int work_doubles, int64_t nnzN, int nn, int check_nan) {
+                                                         int table_doubles, int nodal_doubles, int rec_doubles,
+                                                         int stage_items, int npatch, int64_t nnzN, int nn,
+                                                         int check_nan) {
     extern __shared__ double lds[];
+    const int patch = xcd_block(blockIdx.x, gridDim.x);
+    if (patch >= npatch) return;   // whole workgroup
     const int tstride = nip * NOD;
     double *sN = lds;
     double *sXi = sN + tstride;
     double *sEta = sXi + tstride;
     double *sW = sEta + tstride;
-    double *work = lds + ((3 * tstride + nip + 1) & ~1);
-    double *win = work + work_doubles;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    {
-        const ShapeTables *tab = MA->tab;
-        for (int k = tid; k < tstride; k += kFusedThreads) {
-            const int q = k / NOD, i = k - q * NOD;
-            sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
-        }
-        if (tid < nip) sW[tid] = tab->w[tid];
+    double *nodal = lds + table_doubles;                 // [epb][nv_stride]
+    double *rec = nodal + nodal_doubles;                 // [epb][rec_stride]
+    double *stage = rec + rec_doubles;                   // [stage_items][kStageValues]
+    double *win = stage + int64_t(stage_items) * kStageValues;
+    const int tid = threadIdx.x;
+    const ShapeTables *tab = MA->tab;
+    for (int k = tid; k < tstride; k += kFusedThreads) {
+        const int q = k / NOD, i = k - q * NOD;
+        sN[k] = tab->N[q][i]; sXi[k] = tab->dxi[q][i]; sEta[k] = tab->deta[q][i];
     }
+    if (tid < nip) sW[tid] = tab->w[tid];
+    const Params &p = CA->p;      // read through the pointer (scalar loads), not held in registers
+    const FieldPtrs &f = CA->f;
+    constexpr int nvs = nv_stride<NOD>();
+    const int rs = rec_stride(nip);
+    const int gl = tid % kFlushLanes, grp = tid / kFlushLanes;
     int bad = 0;
-    const int nrow = epb * NOD;  // (element, node) pairs per batch
-    const int b0 = MA->s.patch_batch_ptr[blockIdx.x], b1 = MA->s.patch_batch_ptr[blockIdx.x + 1];
-    for (int b = b0; b < b1; ++b) {
-        // rows that start in this batch: zero their slots (one wave per row)
-        {
-            const int z0 = MA->s.zero_ptr[b], z1 = MA->s.zero_ptr[b + 1];
-            for (int z = z0 + wave; z < z1; z += kFusedWaves) {
-                double *w = win + MA->s.zero_slot[z];
-                const int len = MA->s.zero_len[z];
-                for (int k = lane; k < len; k += 64) w[k] = 0.0;
+    const int b0 = MA->patch_batch_ptr[patch], b1 = MA->patch_batch_ptr[patch + 1];
+    if (b0 < b1)   // the first batch's gathers (later batches are gathered in the phase C before)
+        for (int g = tid; g < epb * NOD; g += kFusedThreads) gather_batch<NOD>(MA, f, p, b0, epb, g, nodal);
+    BatchHdr H{}, Hp{};
+    if (b0 < b1) H = MA->hdr[b0];
+    for (int b = b0; b <= b1; ++b) {
+        const bool has = b < b1;
+        const BatchHdr Hn = b + 1 < b1 ? MA->hdr[b + 1] : BatchHdr{};
+        const int ni = H.item1 - H.item0;
+        // records of this iteration's phases, loaded before any of them runs
+        PatchItem it{};
+        if (tid < 3 * ni) it = MA->items[H.item0 + tid % ni];
+        PatchFlush fr{};
+        const bool fl0 = b > b0 && Hp.flush0 + grp < Hp.flush1;
+        if (fl0) fr = MA->flushes[Hp.flush0 + grp];
+        // ---- phase A
+        if (has) {
+            const int lane = tid & 63, wave = tid >> 6;
+            for (int z = H.zero0 + wave; z < H.zero1; z += kFusedThreads / 64) {
+                const PatchZero zr = MA->zeros[z];
+                double *w = win + zr.slot;
+                for (int k = lane; k < zr.len; k += 64) w[k] = 0.0;
             }
         }
-        // gathers
-        const int32_t *belem = MA->s.batch_elem + (int64_t)b * epb;
-        if (tid < nrow) {
-            const int el = tid / NOD, a = tid - el * NOD;
-            const int e = belem[el];
-            if (e >= 0)
-                gather_node<NOD>(MA->conn, MA->x, MA->y, CA->f, MA->b, CA->p, e, a, work + el * (NV_COUNT * NOD));
-        }
-        __syncthreads();
-        // physics
-        double c[kNC];
-        const int pel = tid / nip, pq = tid - pel * nip;
-        const bool phys = tid < epb * nip && belem[pel < epb ? pel : 0] >= 0;
-        if (phys) {
-            const Params p = CA->p;
-            ip_physics<NOD, double>(work + pel * (NV_COUNT * NOD), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
-                                    sW[pq], p, c);
-        }
-        __syncthreads();
-        if (phys) {
-            double *dst = work + (pel * nip + pq) * kNC;
-#pragma unroll
-            for (int k = 0; k < kNC; k += 2) {
-                double2 t;
-                t.x = c[k];
-                t.y = (k + 1 < C_COUNT) ? c[k + 1] : 0.0;
-                *reinterpret_cast<double2 *>(dst + k) = t;
+        if (b > b0) {   // the previous batch's ordered sums, in item order, one add per value
+            for (int t = tid; t < (Hp.fix1 - Hp.fix0) * kStageValues; t += kFusedThreads) {
+                const int gi = t / kStageValues, v = t - gi * kStageValues;
+                if (RONLY && v % 5 < 3) continue;   // R only: no K values staged
+                const PatchFix fx = MA->fixes[Hp.fix0 + gi];
+                double sum = 0.0;
+                for (int k = 0; k < fx.count; ++k) sum += stage[(fx.stage0 + k) * kStageValues + v];
+                const int deg = fx.deg_dcol & 0xFFFF, dcol = fx.deg_dcol >> 16;
+                const int bi = v / 5, wv = v - bi * 5;
+                const int off = wv < 3 ? (3 * bi + wv) * deg + dcol : 9 * deg + (wv == 3 ? bi : 3 + bi);
+                win[fx.slot + off] += sum;
             }
         }
         __syncthreads();
-        // element rows of owned nodes: item = bi * nrow + (el * NOD + i), block row bi
-        const int bi = tid / nrow;
-        const int rr = tid - bi * nrow;
-        const int el = rr / NOD, i = rr - el * NOD;
-        int slot = -1, rank = 0, e = -1, deg = 0;
-        if (bi < 3) {
-            e = belem[el];
-            const int64_t ea = ((int64_t)b * epb + el) * NOD + i;
-            slot = e >= 0 ? MA->s.ea_slot[ea] : -1;
-            rank = MA->s.ea_rank[ea];
-        }
-        double T = 0, F = 0, Kb[3][NOD];
-        if (slot >= 0) {
-            const double *crr = work + el * nip * kNC;
-            if (bi == 0) element_row_b<NOD, 0, double>(i, nip, crr, sN, sXi, sEta, MA->tab, T, F, Kb);
-            else if (bi == 1) element_row_b<NOD, 1, double>(i, nip, crr, sN, sXi, sEta, MA->tab, T, F, Kb);
-            else element_row_b<NOD, 2, double>(i, nip, crr, sN, sXi, sEta, MA->tab, T, F, Kb);
-            const int node = MA->conn[(int64_t)e * NOD + i];
-            deg = (int)(MA->rowptr_n[node + 1] - MA->rowptr_n[node]);
-        }
-        const int maxrank = MA->s.batch_maxrank[b];
-        const uint8_t *ps = MA->pos + ((int64_t)(e < 0 ? 0 : e) * NOD + i) * NOD;
-        for (int r = 0; r <= maxrank; ++r) {
-            if (slot >= 0 && rank == r) {
-                double *w = win + slot;
-                if (!RONLY) {
-                    double *wb = w + bi * 3 * deg;
+        // ---- phase B
+        if (has) {
+            for (int t = tid; t < epb * nip; t += kFusedThreads) {
+                const int el = t / nip, q = t - el * nip;
+                if (MA->batch_node[(int64_t(b) * epb + el) * NOD] < 0) continue;   // padding slot
+                double c[kNC];
+                ip_physics<NOD, double, SL>(nodal + el * nvs, sN + q * NOD, sXi + q * NOD, sEta + q * NOD, sW[q], p, c);
+                double *dst = rec + el * rs + q * kNC;
 #pragma unroll
-                    for (int j = 0; j < NOD; ++j) {
-                        const int cj = ps[j];
+                for (int k = C_COUNT; k < kNC; ++k) c[k] = 0.0;
 #pragma unroll
-                        for (int bj = 0; bj < 3; ++bj) lds_add(wb + bj * deg + cj, Kb[bj][j]);
-                    }
-                }
-                lds_add(w + 9 * deg + bi, T);       // Tx Ty Th
-                lds_add(w + 9 * deg + 3 + bi, F);   // Fx Fy Fh
+                for (int k = 0; k < kNC; ++k) dst[k] = c[k];
             }
-            __syncthreads();
         }
-        // rows whose last element was in this batch: write out (one wave per row)
-        {
-            const int f0 = MA->s.flush_ptr[b], f1 = MA->s.flush_ptr[b + 1];
+        if (b > b0) {   // rows whose last element was in the previous batch
             double *Kval = CA->Kval;
-            for (int fl = f0 + wave; fl < f1; fl += kFusedWaves) {
-                const int node = MA->s.flush_node[fl];
-                const double *w = win + MA->s.flush_slot[fl];
-                const int64_t rp = MA->rowptr_n[node];
-                const int dg = (int)(MA->rowptr_n[node + 1] - rp);
-                if (!RONLY && Kval) {
+            for (int fl = Hp.flush0 + grp; fl < Hp.flush1; fl += kFusedThreads / kFlushLanes) {
+                const PatchFlush r = fl == Hp.flush0 + grp ? fr : MA->flushes[fl];
+                const double *w = win + r.slot;
+                const int dg = r.deg;
+                if (!RONLY) {
 #pragma unroll
                     for (int bk = 0; bk < 3; ++bk) {
-                        double *dst = Kval + (int64_t)bk * 3 * nnzN + 3 * rp;
+                        double *dst = Kval + int64_t(bk) * 3 * nnzN + r.rp3;
                         const double *src = w + bk * 3 * dg;
-                        for (int k = lane; k < 3 * dg; k += 64) dst[k] = src[k];
+                        for (int k = gl; k < 3 * dg; k += kFlushLanes) dst[k] = src[k];
                     }
-                    // diagonal entry of block (lane, lane) (uvhMatrixAssembly.m:575-579)
-                    if (check_nan && lane < 3 && isnan(w[lane * 3 * dg + lane * dg + MA->diag[node]])) bad |= 2;
+                    // diagonal entry of block (gl, gl) (uvhMatrixAssembly.m:575-579)
+                    if (check_nan && gl < 3 && isnan(w[gl * 3 * dg + gl * dg + r.dcol])) bad |= 2;
                 }
-                if (lane < 3) {
-                    const double Tv = w[9 * dg + lane], Fv = w[9 * dg + 3 + lane];
+                if (gl < 3) {
+                    const double Tv = w[9 * dg + gl], Fv = w[9 * dg + 3 + gl];
                     const double rv = Tv - Fv;
-                    CA->R[(int64_t)lane * nn + node] = rv;
-                    if (CA->Tint) CA->Tint[(int64_t)lane * nn + node] = Tv;
-                    if (CA->Fext) CA->Fext[(int64_t)lane * nn + node] = Fv;
+                    CA->R[int64_t(gl) * nn + r.node] = rv;
+                    if (CA->Tint) CA->Tint[int64_t(gl) * nn + r.node] = Tv;
+                    if (CA->Fext) CA->Fext[int64_t(gl) * nn + r.node] = Fv;
                     if (check_nan && isnan(rv)) bad |= 1;
                 }
             }
         }
         __syncthreads();
+        // ---- phase C: row items of this batch, then (other lanes) the next batch's gathers
+        const int ngat = b + 1 < b1 ? epb * NOD : 0;
+        for (int t = tid; t < 3 * ni + ngat; t += kFusedThreads) {
+            if (t >= 3 * ni) {
+                gather_batch<NOD>(MA, f, p, b + 1, epb, t - 3 * ni, nodal);
+                continue;
+            }
+            const int bi = t / ni;
+            const PatchItem r = t == tid ? it : MA->items[H.item0 + (t - bi * ni)];
+            const int el = r.ea / NOD, a = r.ea - el * NOD;
+            const int deg = r.deg, st = r.stage;
+            const double *crr = rec + el * rs;
+            double Tr, Fr, Kb[3][NOD];
+            if (bi == 0) element_row_b<NOD, 0, double>(a, nip, crr, sN, sXi, sEta, tab, Tr, Fr, Kb);
+            else if (bi == 1) element_row_b<NOD, 1, double>(a, nip, crr, sN, sXi, sEta, tab, Tr, Fr, Kb);
+            else element_row_b<NOD, 2, double>(a, nip, crr, sN, sXi, sEta, tab, Tr, Fr, Kb);
+            double *w = win + r.slot + bi * 3 * deg;
+            double *sg = stage + st * kStageValues + bi * 5;
+            if (!RONLY) {   // (with R only the Jacobian terms above are dead code)
+#pragma unroll
+                for (int j = 0; j < NOD; ++j) {
+                    const int cj = r.pos[j];
+#pragma unroll
+                    for (int bj = 0; bj < 3; ++bj) {
+                        if (st >= 0 && j == a) sg[bj] = Kb[bj][j];
+                        else lds_add(w + bj * deg + cj, Kb[bj][j]);
+                    }
+                }
+            }
+            if (st >= 0) {
+                sg[3] = Tr;
+                sg[4] = Fr;
+            } else {
+                lds_add(win + r.slot + 9 * deg + bi, Tr);
+                lds_add(win + r.slot + 9 * deg + 3 + bi, Fr);
+            }
+        }
+        __syncthreads();
+        Hp = H;
+        H = Hn;
     }
     if (bad) atomicOr(MA->status, bad);
 }
@@ -1599,36 +1649,41 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
     return hipGetLastError();
 }
 
-// LDS layout of the fused kernel: shape tables, per-batch work area, row window.
+// LDS layout of the fused kernel: shape tables, nodal staging and coefficient records of one
+// batch, ordered staging, row window; within kFusedLdsBytes so that two workgroups share a CU.
+// The batch takes about 45 % of the budget (P1: 24 elements, P2: 12, P3: 4 at the configs'
+// rules); the ordered staging starts at three items per element (the corners) and grows if a
+// schedule needs more (set_mesh rebuilds it with the smaller window).
 FusedLayout fused_layout(int nod, int nip) {
     FusedLayout L{};
     L.table_doubles = (3 * nip * nod + nip + 1) & ~1;
-    const int per_el_bytes = std::max(NV_COUNT * nod, nip * kNC) * int(sizeof(double));
-    L.epb = std::min(kFusedThreads / nip, kFusedThreads / (3 * nod));
-    L.epb = std::min<int>(L.epb, int((kLdsLimit - L.table_doubles * sizeof(double)) / per_el_bytes));
-    const int per_el = std::max(NV_COUNT * nod, nip * kNC);
-    L.work_doubles = (L.epb * per_el + 1) & ~1;
-    const int64_t total = kFusedLdsBytes / int64_t(sizeof(double));
-    L.window_doubles = ((total - L.table_doubles - L.work_doubles) / 2) * 2;
+    const int per_el = nv_stride_host(nod) + rec_stride(nip);
+    const int64_t budget = kFusedLdsBytes / int64_t(sizeof(double)) - L.table_doubles;
+    L.epb = std::max(2, std::min(64, int(0.45 * double(budget) / double(per_el))));
+    if (const char *e = std::getenv("UVH_FUSED_EPB")) L.epb = std::max(2, std::min(64, std::atoi(e)));
+    L.nodal_doubles = (L.epb * nv_stride_host(nod) + 1) & ~1;
+    L.rec_doubles = (L.epb * rec_stride(nip) + 1) & ~1;
+    L.stage_items = L.epb * std::min(nod, 3);
+    L.window_doubles = ((budget - L.nodal_doubles - L.rec_doubles - int64_t(L.stage_items) * kStageValues) / 2) * 2;
     return L;
 }
 
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
-                        int64_t nnzN, int nn, bool ronly, int check_nan, hipStream_t s) {
+                        int64_t nnzN, int nn, bool sl, bool ronly, int check_nan, hipStream_t s) {
     if (npatch <= 0) return hipSuccess;
-    const size_t lds = size_t(L.table_doubles + L.work_doubles + L.window_doubles) * sizeof(double);
-    const dim3 grid(npatch), block(kFusedThreads);
-#define UVH_FUSED(NODV, RO)                                                                                     \
-    do {                                                                                                        \
-        auto kern = k_fused<NODV, RO>;                                                                          \
-        hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,  \
-                            int(lds));                                                                          \
-        hipLaunchKernelGGL(kern, grid, block, lds, s, MA, CA, nip, L.epb, L.work_doubles, nnzN, nn, check_nan);  \
+    const size_t lds = size_t(L.lds_bytes());
+    const dim3 grid(((npatch + kXcds - 1) / kXcds) * kXcds), block(kFusedThreads);
+#define UVH_FUSED(NODV, SLV)                                                                                  \
+    do {                                                                                                      \
+        auto kern = ronly ? k_fused<NODV, SLV, true> : k_fused<NODV, SLV, false>;                             \
+        lds_attr(reinterpret_cast<const void *>(kern), lds);                                                  \
+        hipLaunchKernelGGL(kern, grid, block, lds, s, MA, CA, nip, L.epb, L.table_doubles, L.nodal_doubles,   \
+                           L.rec_doubles, L.stage_items, npatch, nnzN, nn, check_nan);                        \
     } while (0)
     switch (nod) {
-        case 3: if (ronly) UVH_FUSED(3, true); else UVH_FUSED(3, false); break;
-        case 6: if (ronly) UVH_FUSED(6, true); else UVH_FUSED(6, false); break;
-        case 10: if (ronly) UVH_FUSED(10, true); else UVH_FUSED(10, false); break;
+        case 3: if (sl) UVH_FUSED(3, true); else UVH_FUSED(3, false); break;
+        case 6: if (sl) UVH_FUSED(6, true); else UVH_FUSED(6, false); break;
+        case 10: if (sl) UVH_FUSED(10, true); else UVH_FUSED(10, false); break;
         default: return hipErrorInvalidValue;
     }
 #undef UVH_FUSED

@@ -89,6 +89,12 @@ def default_ctrlvar() -> Struct:
         uvhDesiredWorkOrForceTolerances=[float("inf"), 1e-15],    # :498
         uvhExitBackTrackingStepLength=1e-3,       # :502
         uvhAcceptableWorkAndForceTolerances=[float("inf"), 1e-6], # :504
+        # uv (diagnostic) Newton exit tests (uvResidualsCriteria.m)
+        uvDesiredWorkAndForceTolerances=[float("inf"), 1e-15],    # :508
+        uvDesiredWorkOrForceTolerances=[float("inf"), 1e-15],     # :509
+        uvExitBackTrackingStepLength=1e-10,                       # :510
+        uvAcceptableWorkAndForceTolerances=[float("inf"), 1e-15], # :511
+        uvAcceptableWorkOrForceTolerances=[float("inf"), 1e-15],  # :512
         # uv (diagnostic) assembly, uvMatrixAssemblySSTREAM.m
         Cmin=1e-20,                               # :458 (FlowApproximation "SSTREAM")
         AGlenmin=1e-20,                           # :463

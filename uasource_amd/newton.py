@@ -184,7 +184,8 @@ class GpuAssembler:
         self.dev = torch.device(f"cuda:{device}")
         uvh.check_supported(CtrlVar)
         self.ctx = ctx or uvh.UvhContext(device)
-        self.ctx.set_mesh(MUA)
+        if ctx is None or ctx.mesh_key != uvh.mesh_key(MUA):
+            self.ctx.set_mesh(MUA)
         self.mua = MUA
         self.nn = MUA.Nnodes
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.dev)
@@ -329,6 +330,141 @@ class GpuAssembler:
         self.x.copy_(self.torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)))
 
 
+class GpuUvAssembler:
+    """R and K of the uv (diagnostic SSTREAM) system on the GPU (uvMatrixAssemblySSTREAM.m),
+    with the geometry resident and the state x = [ub; vb]: the interface of ``GpuAssembler``
+    for ``solve(..., system="uv")``. ``F`` carries h and the upper surface s besides the
+    uvh fields. The normalisation is the zero-fields residual, with the velocities zero except
+    at the fixed degrees of freedom, which keep their values (SSTREAM2dNR2.m:116-118,
+    uvMatrixAssemblySSTREAM.m:24-28)."""
+
+    def __init__(self, CtrlVar, MUA, F, device: int = 0, ctx: uvh.UvhContext | None = None):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(f"cuda:{device}")
+        self.ctx = ctx or uvh.UvhContext(device)
+        if ctx is None or ctx.mesh_key != uvh.mesh_key(MUA):
+            self.ctx.set_mesh(MUA)
+        self.mua = MUA
+        self.nn = n = MUA.Nnodes
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.dev)  # noqa: E731
+        self.f = {k: t(getattr(F, k)) for k in uvh.UV_KEYS}
+        self.x = torch.cat([self.f["ub"], self.f["vb"]])
+        self.xt = self.x.clone()
+
+        class _F:
+            pass
+        fd = _F()
+        for k in uvh.UV_KEYS:
+            setattr(fd, k, self.f[k])
+        fd.ub, fd.vb = self.xt[:n], self.xt[n:]
+        for k in ("q", "muk", "V0"):
+            a = getattr(F, k, None)
+            if a is not None and np.size(a) > 0:
+                self.f[k] = t(a)
+                setattr(fd, k, self.f[k])
+        self.fields = uvh.make_uv_fields(fd)
+        self.ctrl = uvh.make_ctrl(CtrlVar, F)
+        self.uvc = uvh.make_uv_ctrl(CtrlVar)
+        self.nnz = 4 * (self.ctx.nnz // 9)
+        self.R = torch.zeros(2 * n, dtype=torch.float64, device=self.dev)
+        self.R0 = torch.zeros_like(self.R)
+        self.K = torch.zeros(self.nnz, dtype=torch.float64, device=self.dev)
+        self.d = torch.zeros_like(self.R)
+        self.stream = torch.cuda.current_stream(self.dev).cuda_stream
+        self.n_rk = self.n_r = 0
+        self.free = None
+        self.fixed_mask = torch.zeros_like(self.R)
+
+    def set_fixed(self, fixed):
+        m = np.ones(2 * self.nn)
+        if fixed is not None and len(fixed):
+            m[fixed] = 0.0
+        self.free = None if fixed is None or len(fixed) == 0 else self.torch.from_numpy(m).to(self.dev)
+        self.fixed_mask = self.torch.from_numpy(1.0 - m).to(self.dev)
+
+    def set_shift(self, a0=None, a1=None):
+        if a0 is not None:
+            raise ValueError("the uv system takes no constraint terms")
+
+    def _assemble(self, flags, K=None):
+        self.ctx.assemble_uv_device(self.ctrl, self.uvc, self.fields, flags, self.R, K, stream=self.stream)
+        self.ctx.sync_status()
+
+    def normalisation(self) -> float:
+        self.torch.mul(self.x, self.fixed_mask, out=self.xt)
+        self._assemble(uvh.UVH_RONLY)
+        self.n_r += 1
+        return float(self.torch.dot(self.R, self.R).item()) + 1000 * EPS
+
+    def assemble_rk(self, fp32: bool, fp32_residual: bool):
+        self.xt.copy_(self.x)
+        self._assemble(uvh.UVH_FP32 if fp32 else 0, self.K)
+        self.n_rk += 1
+        if fp32 and not fp32_residual:
+            self._assemble(uvh.UVH_RONLY)
+            self.n_r += 1
+        self.R0.copy_(self.R)
+        return self.ctx.uv_K_matrix(self.K.cpu().numpy()), (-self.R0).cpu().numpy()
+
+    def set_step(self, d: np.ndarray):
+        self.d.copy_(self.torch.from_numpy(d))
+
+    def cost(self, gamma: float, fp32: bool, N: float):
+        if gamma == 0.0:
+            R = self.R0
+        else:
+            self.torch.add(self.x, self.d, alpha=gamma, out=self.xt)
+            self._assemble(uvh.UVH_RONLY | (uvh.UVH_FP32 if fp32 else 0))
+            self.n_r += 1
+            R = self.R
+        if self.free is not None:
+            R = R * self.free
+        frhs_frhs = self.torch.dot(R, R)
+        D2 = -self.torch.dot(R, self.d)
+        v = self.torch.stack([frhs_frhs, D2]).cpu().numpy()
+        return v[0] / N, v[1] * v[1], v[1]
+
+    def accept(self, gamma: float):
+        self.x.add_(self.d, alpha=gamma)
+
+    def state(self):
+        x = self.x.cpu().numpy()
+        n = self.nn
+        return x[:n].copy(), x[n:].copy(), None
+
+    def set_state(self, x: np.ndarray):
+        self.x.copy_(self.torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)))
+
+
+def uv_fixed(fixed_uvh: np.ndarray | None, nn: int):
+    """The velocity degrees of freedom among uvh fixed dofs (indices into [ub; vb])."""
+    if fixed_uvh is None:
+        return None
+    f = np.asarray(fixed_uvh)
+    return f[f < 2 * nn]
+
+
+def transient_start(CtrlVar, MUA, F0, F1, fixed, uv_assembler, linear_solver=superlu_solve, verbose=False,
+                    log=print):
+    """The start of a transient run (Ua2D.m:598-611, "always needed at a start of a transient
+    run"): a uv (diagnostic) solve on F1's geometry, whose velocities then become both F0's and
+    F1's, so that the first implicit uvh step starts from the balanced state. ``uv_assembler``
+    builds the uv assembler from (CtrlVar, MUA, F) with F carrying the upper surface s (from the
+    floatation of h). Returns (F0, F1, uv NewtonResult)."""
+    from .fields import surface_from_floatation
+    F = F1.copy()
+    F.s = surface_from_floatation(F1, CtrlVar)
+    asm = uv_assembler(CtrlVar, MUA, F)
+    res = solve(asm, CtrlVar, linear_solver=linear_solver, fixed=uv_fixed(fixed, MUA.Nnodes), verbose=verbose,
+                log=log, system="uv")
+    ub, vb, _ = res.ub, res.vb, None
+    G0, G1 = F0.copy(), F1.copy()
+    G0.ub, G0.vb = ub.copy(), vb.copy()
+    G1.ub, G1.vb = ub.copy(), vb.copy()
+    return G0, G1, res
+
+
 # Lagrange shape functions in Úa's local node order (UaMain/shape_fun.m:63-82), as barycentric
 # multi-indices (c1, c2, c3) with c1 = xi, c2 = eta, c3 = 1 - xi - eta
 _NODE_INDEX = {
@@ -451,14 +587,16 @@ class _CauchyM:
         return g, best, s, gD
 
 
-def _criteria(CtrlVar, gamma, rWork, rForce, iteration):
-    """Exit test of SSTREAM_TransientImplicit.m:269-287."""
-    if gamma > max(CtrlVar.uvhExitBackTrackingStepLength, CtrlVar.BacktrackingGammaMin):
-        aw, af = CtrlVar.uvhDesiredWorkAndForceTolerances
-        ow, of = CtrlVar.uvhDesiredWorkOrForceTolerances
+def _criteria(CtrlVar, gamma, rWork, rForce, iteration, system="uvh"):
+    """Exit test of SSTREAM_TransientImplicit.m:269-287 (``system="uvh"``), or of
+    uvResidualsCriteria.m for the uv system (the same form with the uv tolerances)."""
+    tol = lambda name: CtrlVar.get(system + name)  # noqa: E731
+    if gamma > max(tol("ExitBackTrackingStepLength"), CtrlVar.BacktrackingGammaMin):
+        aw, af = tol("DesiredWorkAndForceTolerances")
+        ow, of = tol("DesiredWorkOrForceTolerances")
     else:
-        aw, af = CtrlVar.uvhAcceptableWorkAndForceTolerances
-        ow, of = CtrlVar.uvhAcceptableWorkOrForceTolerances
+        aw, af = tol("AcceptableWorkAndForceTolerances")
+        ow, of = tol("AcceptableWorkOrForceTolerances")
     return (rWork < aw and rForce < af) and (rWork < ow or rForce < of) and iteration >= CtrlVar.NRitmin
 
 
@@ -515,7 +653,7 @@ def eliminate(linear_solver, fixed: np.ndarray | None):
 def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, fp64_switch: float = 1e-8,
           fixed: np.ndarray | None = None, verbose: bool = False, log=print, time_limit: float | None = None,
           resume: dict | None = None, L=None, cuvh: np.ndarray | None = None,
-          luvh: np.ndarray | None = None) -> NewtonResult:
+          luvh: np.ndarray | None = None, system: str = "uvh") -> NewtonResult:
     """Newton-Raphson uvh iteration (SSTREAM_TransientImplicit.m:224-418). ``fixed``: degrees of
     freedom held at their current values (their residual rows are left out of the cost).
     ``time_limit`` (seconds): once at least one iteration has run in this call, stop before
@@ -528,8 +666,16 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
     (SSTREAM_TransientImplicit.m:195, :373-383, :481): each step solves
     [K L'; L 0] [d; dl] = [-R - L'luvh; cuvh - L x] (solveKApe.m) with the host linear solver,
     and the cost carries both residual blocks (CalcCostFunctionNRuvh.m:45-85). The start need
-    not be feasible."""
+    not be feasible.
+
+    ``system="uv"``: the assembler is a ``GpuUvAssembler`` (state [ub; vb]) and the exit
+    tests take the uv tolerances (SSTREAM2dNR2.m with uvResidualsCriteria.m, whose loop has the
+    same normalisation, step, line search and exit order)."""
     on_device = getattr(linear_solver, "on_device", False)
+    if system not in ("uvh", "uv"):
+        raise ValueError(f"unknown system {system!r}")
+    if system == "uv" and (L is not None or "-Auto-" in str(CtrlVar.get("rLineMinUa", ""))):
+        raise ValueError("the uv system takes Dirichlet conditions and the Newton line search only")
     if L is not None:
         if on_device:
             raise ValueError("linear constraints L need the host linear solver")
@@ -583,7 +729,7 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
         # then taken as 0, so that the exit and stagnation tests judge such a point on rForce
         # alone (SSTREAM_TransientImplicit.m:275-281, 345-350)
         rw = 0.0 if np.isnan(rWork) else rWork
-        if residual64 and _criteria(CtrlVar, gamma, rw, rForce, iteration):
+        if residual64 and _criteria(CtrlVar, gamma, rw, rForce, iteration, system):
             res.converged, res.reason = True, "converged"
             break
         if iteration > CtrlVar.NRitmax:
