@@ -453,7 +453,7 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     {
         chk(dalloc(&ctx->d_RE, ne * nod * 6));
         chk(dalloc(&ctx->d_inc_pos, ne * nod));
-        chk(dalloc(&ctx->d_pos_inc, ne * nod * nod));
+        chk(dalloc(&ctx->d_pos_inc, ne * nod * uvh::pos_inc_stride(nod)));
         if (ctx->gather_rows) {
             chk(dalloc(&ctx->d_nzrow, m.nzrow.size()));
             chk(dalloc(&ctx->d_imap_ptr, m.imap_ptr.size()));
@@ -485,7 +485,7 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     chk(hipMemcpy(ctx->d_diag, m.diag.data(), nn, hipMemcpyHostToDevice));
     {
         chk(hipMemcpy(ctx->d_inc_pos, m.inc_pos.data(), ne * nod * sizeof(int32_t), hipMemcpyHostToDevice));
-        chk(hipMemcpy(ctx->d_pos_inc, m.pos_inc.data(), ne * nod * nod, hipMemcpyHostToDevice));
+        chk(hipMemcpy(ctx->d_pos_inc, m.pos_inc.data(), m.pos_inc.size(), hipMemcpyHostToDevice));
         if (ctx->gather_rows) {
             chk(hipMemcpy(ctx->d_nzrow, m.nzrow.data(), m.nzrow.size() * sizeof(int32_t), hipMemcpyHostToDevice));
             chk(hipMemcpy(ctx->d_imap_ptr, m.imap_ptr.data(), m.imap_ptr.size() * sizeof(int64_t),

@@ -15,6 +15,8 @@ constexpr int kMaxNip = 32;
 // element rows are 90 values long) and nodes per 256-thread workgroup (consecutive numbers).
 constexpr int row_lanes(int nod) { return nod >= 10 ? 32 : 16; }
 constexpr int row_nodes_per_block(int nod) { return 4 * (64 / row_lanes(nod)); }
+// Bytes per row of the incidence-ordered column table: nod padded to a multiple of 8.
+constexpr int pos_inc_stride(int nod) { return (nod + 7) & ~7; }
 
 // Shape-function values and local derivatives at the integration points, plus the
 // weights, for one element type and quadrature rule. Filled on the host from the
@@ -47,7 +49,7 @@ struct MeshHost {
     std::vector<uint8_t> pos;         // nele*nod*nod: pos[(e*nod+a)*nod+b] = index of conn[e][b] in adj(conn[e][a])
     std::vector<uint8_t> diag;        // nn: index of node k in adj(k)
     std::vector<int32_t> inc_pos;     // nele*nod: position of (e, a) in the incidence order (inverse of inc)
-    std::vector<uint8_t> pos_inc;     // nele*nod*nod: pos rows in the incidence order
+    std::vector<uint8_t> pos_inc;     // nele*nod*pos_inc_stride(nod): pos rows in the incidence order
     // gather form of the row scatter: nzrow[x] is the node of node-graph entry x; for node n,
     // incidence t (t0 <= t < t1) and column c < deg(n), imap[imap_ptr[n] + (t - t0)*deg + c]
     // is the local index in element(t) of node adj(n)[c], or 0xFF if that element lacks it

@@ -854,6 +854,48 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     constexpr int NV = NBK * NBK * NOD;                   // values used per element row
     constexpr int VPL = (NV + kRowLanes - 1) / kRowLanes; // values per lane
     constexpr int CH = 4;
+    constexpr int PST = pos_inc_stride(NOD);
+    if constexpr (NBK == 3 && sizeof(TE) == sizeof(double) && (9 * NOD) % 2 == 0) {
+        // FP64 uvh rows of 6- and 10-node elements are 16-byte aligned (54 or 90 values): each
+        // lane loads pairs of values (2 gl + 2 kRowLanes u), and the row's columns with one 8-byte
+        // load per 8 columns, 4 element rows at a time
+        constexpr int VP2 = (NV + 2 * kRowLanes - 1) / (2 * kRowLanes);   // pairs per lane
+        constexpr int PW = PST / 8;                                      // column words per row
+        for (int c0 = t0; c0 < t1; c0 += CH) {
+            const int cnt = min(CH, t1 - c0);
+            double2 vals[CH][VP2];
+            uint64_t cw[CH][PW];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                if (c < cnt) {
+                    const double2 *row = reinterpret_cast<const double2 *>(EB + (int64_t)(c0 + c) * eb_stride<NOD, TE>());
+#pragma unroll
+                    for (int u = 0; u < VP2; ++u)
+                        if (2 * (gl + kRowLanes * u) < NV) vals[c][u] = row[gl + kRowLanes * u];
+                    const uint64_t *pw = reinterpret_cast<const uint64_t *>(pos + (int64_t)(c0 + c) * PST);
+#pragma unroll
+                    for (int w = 0; w < PW; ++w) cw[c][w] = pw[w];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                if (c >= cnt) continue;
+#pragma unroll
+                for (int u = 0; u < VP2; ++u) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int v = 2 * (gl + kRowLanes * u) + h;
+                        if (v < NV) {
+                            const int bk = v / NOD, j = v - bk * NOD;
+                            const int bi = bk / 3, bj = bk - bi * 3;
+                            const int col = int((cw[c][j >> 3] >> (8 * (j & 7))) & 0xFF);
+                            lds_add(buf + bi * 3 * deg + bj * deg + col, h ? vals[c][u].y : vals[c][u].x);
+                        }
+                    }
+                }
+            }
+        }
+    } else
     for (int c0 = t0; c0 < t1; c0 += CH) {
         const int cnt = min(CH, t1 - c0);
         TE vals[CH][VPL];
@@ -867,7 +909,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                     const int bk = v / NOD, j = v - bk * NOD;
                     const int src = NBK == 3 ? v : ((bk / NBK) * 3 + bk % NBK) * NOD + j;
                     vals[c][u] = EB[(int64_t)(c0 + c) * eb_stride<NOD, TE>() + src];
-                    cols[c][u] = pos[(int64_t)(c0 + c) * NOD + j];
+                    cols[c][u] = pos[(int64_t)(c0 + c) * PST + j];
                 }
             }
         }

@@ -157,13 +157,16 @@ std::string build_mesh_host(MeshHost &m) {
     }
     // incidence-ordered copies: the element row (e, a) is stored at inc_pos[e*nod + a], so the
     // rows of each node are contiguous
+    // (pos_inc rows are padded to pos_inc_stride(nod) bytes, so that a lane reads a row's
+    // columns with one or two 8-byte loads)
+    const int pst = pos_inc_stride(nod);
     m.inc_pos.assign(size_t(nele) * nod, 0);
-    m.pos_inc.assign(size_t(nele) * nod * nod, 0);
+    m.pos_inc.assign(size_t(nele) * nod * pst, 0);
     for (int64_t t = 0; t < int64_t(nele) * nod; ++t) {
         const int32_t ea = m.inc[t];
         m.inc_pos[ea] = int32_t(t);
         std::copy(m.pos.begin() + size_t(ea) * nod, m.pos.begin() + size_t(ea + 1) * nod,
-                  m.pos_inc.begin() + size_t(t) * nod);
+                  m.pos_inc.begin() + size_t(t) * pst);
     }
     // gather tables of the row scatter
     m.nzrow.assign(size_t(m.nnzN()), 0);
