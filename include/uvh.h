@@ -156,6 +156,19 @@ int uvh_sync_status(uvh_ctx *ctx);
  * counterpart in the reference: it splits uvhMatrixAssembly's error check from the wait. */
 int uvh_status_async(uvh_ctx *ctx, int32_t *dst, void *stream);
 
+/* uvh_assemble_device without any allocation: the same arguments, kernels and results (bit for
+ * bit), with the work buffers (status word, nodal lower surface, element residual and Jacobian
+ * rows) in a 256 MiB device array of the library instead of the context's own buffers. No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). A mesh whose
+ * work buffers exceed 256 MiB (R+K: about 2.9 KB per 6-node element, so up to ~90k of them) is
+ * refused with UVH_E_ARG. Every call on one GPU shares the array: calls that overlap on one GPU
+ * race, whatever their contexts. NaN checks are deferred to uvh_sync_status as for
+ * uvh_assemble_device. Always the two-pass kernels (the UVH_ASSEMBLY, UVH_CHUNKS and UVH_SCATTER
+ * options do not apply), and no per-kernel timing. */
+int uvh_assemble_device_noalloc(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *dfields, uint32_t flags,
+                                double *dR, double *dKval, double *dTint, double *dFext, void *stream);
+
 /* Per-kernel timing with HIP events on the launch stream (for measurement).
  * uvh_profile(ctx, 1) starts recording; uvh_kernel_times fills up to `cap` entries of
  * (name, total ms, launches) accumulated since the last reset, and returns the count. */

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/uvh.h"
@@ -123,6 +124,8 @@ struct uvh_ctx {
     int rows_mode = 0;      // 0 automatic, 1 one-pass, 2 three-pass
     int *d_status = nullptr;
     int *h_status = nullptr;    // pinned copy of d_status for uvh_sync_status
+    int *last_status = nullptr; // status word of the last call (d_status, or the scratch array's)
+    int scratch_rc = UVH_E_HIP; // whether the scratch array's address was resolved at uvh_create
     // One assembly at a time per context: calls share the work buffers and the status word. A
     // call enqueued on another stream than the previous one waits for the previous one's
     // kernels (done_ev, recorded at the end of every call outside graph capture).
@@ -302,6 +305,56 @@ int build_plan(uvh_ctx *ctx, int t) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ allocation-free assembly
+// uvh_assemble_device_noalloc takes its work buffers (the status word, the nodal lower
+// surface b, the element residual rows and, for R+K, the element rows) from `scratch`, a
+// file-scope device array of kScratchBytes that every GPU the library runs on holds until the
+// process ends. Host code never passes `scratch` itself (on the host the name is only a
+// stand-in): uvh_create resolves its device address once per device, with that device
+// current, and the calls use that address. All calls on one GPU share the array, so calls
+// that overlap on one GPU race; the status word is cleared by the nodal pre-pass of each call,
+// and every other scratch value is written before it is read.
+namespace {
+constexpr size_t kScratchBytes = size_t(256) << 20;   // 256 MiB per GPU
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: the device address of this device's copy of `scratch`.
+int resolve_scratch(int device, std::string &err) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) {
+        err = "allocation-free assembly: device index out of range";
+        return UVH_E_ARG;
+    }
+    if (g_scratch_ptr[device]) return UVH_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        err = std::string("allocation-free assembly: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return UVH_E_HIP;
+    }
+    g_scratch_ptr[device] = ptr;
+    return UVH_OK;
+}
+
+struct NoallocLayout {
+    size_t status, b, RE, EB, total;   // byte offsets into the scratch array, and its use
+};
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+NoallocLayout noalloc_layout(const uvh::MeshHost &m, uint32_t flags) {
+    NoallocLayout L{};
+    L.status = 0;
+    L.b = 256;
+    L.RE = L.b + align256(size_t(m.nn) * sizeof(double));
+    L.EB = L.RE + align256(size_t(m.nele) * m.nod * 6 * sizeof(double));
+    const bool ronly = (flags & UVH_RONLY) != 0;
+    L.total = L.EB + (ronly ? 0 : align256(size_t(m.nele) * m.nod * uvh::element_row_stride(m.nod, false) * sizeof(double)));
+    return L;
+}
+}  // namespace
+
 extern "C" {
 
 int uvh_create(uvh_ctx **out, int device) {
@@ -327,6 +380,7 @@ int uvh_create(uvh_ctx **out, int device) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&ctx->d_status), sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_status), sizeof(int), 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->done_ev, hipEventDisableTiming);
+    if (e == hipSuccess) ctx->scratch_rc = resolve_scratch(device, ctx->err);
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_cargs), sizeof(uvh::CallArgs) * uvh_ctx::kRing, 0);
     if (e == hipSuccess)
@@ -606,6 +660,8 @@ hipError_t end_call(uvh_ctx *ctx, hipStream_t s) {
 
 int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
                          double *dK, double *dT, double *dF, void *stream);
+int assemble_noalloc_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
+                          double *dK, double *dT, double *dF, void *stream);
 int uv_assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *df,
                             uint32_t flags, double *dR, double *dK, double *dT, double *dF, void *stream);
 
@@ -628,13 +684,22 @@ int ordered_call(uvh_ctx *ctx, void *stream, const char *what, F &&body) {
 int uvh_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
                         double *dK, double *dT, double *dF, void *stream) {
     return ordered_call(ctx, stream, "uvh_assemble_device", [&] {
+        ctx->last_status = ctx->d_status;
         return assemble_device_impl(ctx, ctrl, df, flags, dR, dK, dT, dF, stream);
+    });
+}
+
+int uvh_assemble_device_noalloc(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
+                                double *dK, double *dT, double *dF, void *stream) {
+    return ordered_call(ctx, stream, "uvh_assemble_device_noalloc", [&] {
+        return assemble_noalloc_impl(ctx, ctrl, df, flags, dR, dK, dT, dF, stream);
     });
 }
 
 int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl *uvc, const uvh_uv_fields *df,
                            uint32_t flags, double *dR, double *dK, double *dT, double *dF, void *stream) {
     return ordered_call(ctx, stream, "uvh_uv_assemble_device", [&] {
+        ctx->last_status = ctx->d_status;
         return uv_assemble_device_impl(ctx, ctrl, uvc, df, flags, dR, dK, dT, dF, stream);
     });
 }
@@ -643,8 +708,8 @@ int uvh_status_async(uvh_ctx *ctx, int32_t *dst, void *stream) {
     if (!ctx) return UVH_E_ARG;
     if (!dst) return ctx->fail(UVH_E_ARG, "uvh_status_async: null destination");
     hipSetDevice(ctx->device);
-    const hipError_t e = hipMemcpyAsync(dst, ctx->d_status, sizeof(int32_t), hipMemcpyDefault,
-                                        static_cast<hipStream_t>(stream));
+    const hipError_t e = hipMemcpyAsync(dst, ctx->last_status ? ctx->last_status : ctx->d_status, sizeof(int32_t),
+                                        hipMemcpyDefault, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_status_async");
     return UVH_OK;
 }
@@ -754,6 +819,53 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
     return UVH_OK;
 }
 
+// The two-pass assembly with its work buffers in the scratch array: no allocation, no
+// profiling events; the same kernels, launch shapes and summation order as the default path.
+int assemble_noalloc_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *df, uint32_t flags, double *dR,
+                          double *dK, double *dT, double *dF, void *stream) {
+    if (!ctx->has_mesh) return ctx->fail(UVH_E_STATE, "uvh_assemble_device_noalloc: no mesh (call uvh_set_mesh first)");
+    if (!ctrl || !df || !dR) return ctx->fail(UVH_E_ARG, "uvh_assemble_device_noalloc: null ctrl, fields or R");
+    if (!fields_complete(*df)) return ctx->fail(UVH_E_ARG, "uvh_assemble_device_noalloc: a required field pointer is null");
+    {
+        std::string why;
+        const int st = check_law(*ctrl, df->q, df->muk, df->V0, why);
+        if (st != UVH_OK) return ctx->fail(st, "uvh_assemble_device_noalloc: " + why);
+    }
+    if (ctrl->supg_tau < 0 || ctrl->supg_tau > 4)
+        return ctx->fail(UVH_E_ARG, "uvh_assemble_device_noalloc: supg_tau out of range");
+    const bool fp32 = (flags & UVH_FP32) != 0, ronly = (flags & UVH_RONLY) != 0;
+    if (!ronly && !dK) return ctx->fail(UVH_E_ARG, "uvh_assemble_device_noalloc: K output required unless UVH_RONLY");
+    if (ctx->scratch_rc != UVH_OK || !g_scratch_ptr[ctx->device])
+        return ctx->fail(UVH_E_HIP, "uvh_assemble_device_noalloc: the scratch array's address was not resolved");
+    const uvh::MeshHost &m = ctx->mesh;
+    const NoallocLayout L = noalloc_layout(m, flags);
+    if (L.total > kScratchBytes)
+        return ctx->fail(UVH_E_ARG, "uvh_assemble_device_noalloc: this mesh needs " + std::to_string(L.total) +
+                                        " bytes of work buffers; the allocation-free entry point holds at most " +
+                                        std::to_string(kScratchBytes) + " (use uvh_assemble_device)");
+    unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[ctx->device]);
+    int *status = reinterpret_cast<int *>(base + L.status);
+    double *b = reinterpret_cast<double *>(base + L.b);
+    double *RE = reinterpret_cast<double *>(base + L.RE);
+    double *EB = ronly ? nullptr : reinterpret_cast<double *>(base + L.EB);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uvh::Params p = make_params(*ctrl, flags);
+    const uvh::FieldPtrs f = to_ptrs(*df);
+    const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
+    const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
+    const int t = fp32 ? 1 : 0;
+    ctx->last_status = status;
+    hipError_t e = uvh::launch_nodal_b(m.nn, f, p, b, status, s);
+    if (e == hipSuccess)
+        e = uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], fp32, full, ctx->d_conn, ctx->d_x, ctx->d_y, f, b,
+                                ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, EB, RE, 0, -1, s);
+    if (e == hipSuccess)
+        e = uvh::launch_rows(m.nod, m.nn, m.max_block_deg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
+                             ctx->d_diag, m.nnzN(), EB, RE, dR, dT, dF, ronly ? nullptr : dK, check, status, nullptr, 0, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "uvh_assemble_device_noalloc: kernel launch");
+    return UVH_OK;
+}
+
 }  // namespace
 }  // extern "C++"
 
@@ -762,7 +874,8 @@ int uvh_sync_status(uvh_ctx *ctx) {
     hipSetDevice(ctx->device);
     // waits for the context's last assembly only (not for the device)
     hipError_t e = ctx->inflight ? hipEventSynchronize(ctx->done_ev) : hipSuccess;
-    if (e == hipSuccess) e = hipMemcpyAsync(ctx->h_status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->own_stream);
+    const int *src = ctx->last_status ? ctx->last_status : ctx->d_status;
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->h_status, src, sizeof(int), hipMemcpyDeviceToHost, ctx->own_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->own_stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "uvh_sync_status");
     const int st = *ctx->h_status;

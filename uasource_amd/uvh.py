@@ -36,7 +36,7 @@ TAU_OPTIONS = {"taus": 0, "taut": 1, "tau1": 2, "tau2": 3, "tau0": 4}
 EXPORTED = ("uvh_create", "uvh_destroy", "uvh_last_error", "uvh_set_mesh", "uvh_pattern_info",
             "uvh_get_pattern", "uvh_get_csc_map", "uvh_assemble", "uvh_assemble_device", "uvh_sync_status",
             "uvh_profile", "uvh_kernel_times", "uvh_uv_pattern_info", "uvh_uv_get_pattern", "uvh_uv_assemble",
-            "uvh_uv_assemble_device", "uvh_status_async")
+            "uvh_uv_assemble_device", "uvh_status_async", "uvh_assemble_device_noalloc")
 
 
 class UaError(RuntimeError):
