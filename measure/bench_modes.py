@@ -271,10 +271,11 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
 
 def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = True, log=print) -> dict:
     """The start of a transient run as Úa makes it (Ua2D.m:598-611): a uv (diagnostic) Newton
-    solve on F1's geometry with the GPU uv assembly (host SuperLU for its steps), then the
-    implicit uvh Newton solve (SSTREAM_TransientImplicit.m) from that balanced state with the GPU
-    assembly and, with ``device_solver``, the device block-ILU(0) Krylov solve (else host
-    SuperLU), to Úa's default tolerance rForce < 1e-15. FP64 throughout."""
+    solve on F1's geometry with the GPU uv assembly, then the implicit uvh Newton solve
+    (SSTREAM_TransientImplicit.m) from that balanced state with the GPU assembly; with
+    ``device_solver`` both take the device block-ILU(0) Krylov solve (the uv steps on the uvh
+    solver with the thickness held), else host SuperLU; to Úa's default tolerance rForce < 1e-15.
+    FP64 throughout."""
     from uasource_amd import newton
     from uasource_amd.linsolve import GpuLinearSolver
     w = WORKLOADS[workload]
@@ -285,7 +286,8 @@ def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = Tru
     t0 = time.perf_counter()
     log(f"transient start on {workload} ({M.Nele} elements): uv solve ({2 * M.Nnodes} unknowns)")
     G0, G1, ruv = newton.transient_start(cv, M, F0, F1, fixed,
-                                         lambda c, m, F: newton.GpuUvAssembler(c, m, F, ctx=ctx), verbose=True, log=log)
+                                         lambda c, m, F: newton.GpuUvAssembler(c, m, F, ctx=ctx), verbose=True, log=log,
+                                         device_solver=device_solver)
     t_uv = time.perf_counter() - t0
     log(f"uv: {ruv.iterations} iterations, {ruv.reason}, rForce {ruv.rForce:.3e}, {t_uv:.1f} s")
     t1 = time.perf_counter()
@@ -304,7 +306,7 @@ def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = Tru
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, inflow conditions)",
         "config": {"workload": workload, "Nele": M.Nele, "Nnodes": M.Nnodes,
                    "linear_solver": ("block ILU(0) Krylov (GPU)" if device_solver else "SciPy SuperLU (host)")
-                   + " for uvh; SciPy SuperLU (host) for uv", "parallelism": "1 GPU"},
+                   + " for uv and uvh", "parallelism": "1 GPU"},
         "uv": {"converged": bool(ruv.converged), "iterations": int(ruv.iterations), "rForce": float(ruv.rForce),
                "seconds": t_uv, "history": hist(ruv)},
         "uvh": {"converged": bool(res.converged), "iterations": int(res.iterations), "rForce": float(res.rForce),

@@ -410,3 +410,26 @@ def test_gpu_newton_precision_form_reaches_fp64_solution(o2lib, name):
     assert res.converged and res.rForce < 1e-15
     assert res.history[-1][4], "the last step used an FP64 residual"
     assert max(errs) < 1e-6
+
+
+def test_uv_system_in_the_uvh_layout(o2lib):
+    """The index map that lets the uvh device solver take a uv step: the uv K (oracle O2's,
+    4 blocks) placed into the uvh layout has the uv K as its upper-left 2 x 2 blocks and zeros in
+    the thickness blocks, on the uvh pattern."""
+    import scipy.sparse as sp
+    from uasource_amd.fields import surface_from_floatation
+    M = build_mesh(WORKLOADS["tiny_p2"])
+    F0, F1, _, _ = newton_problem(M, WORKLOADS["tiny_p2"])
+    cv = default_ctrlvar()
+    G = F1.copy()
+    G.s = surface_from_floatation(F1, cv)
+    uv = o2lib.assemble_uv(cv, M, G)
+    full = o2lib.assemble(cv, M, F0, F1)
+    rp = full["mesh"].rowptr_n
+    idx = newton.uv_to_uvh_index(rp)
+    vals = np.where(idx >= 0, uv["Kval"][np.maximum(idx, 0)], 0.0)
+    K = sp.csr_matrix((vals, full["K"].indices, full["K"].indptr), shape=full["K"].shape)
+    n = M.Nnodes
+    assert abs(K[:2 * n, :2 * n] - uv["K"]).max() == 0
+    assert K[2 * n:, :].nnz == 0 or abs(K[2 * n:, :]).max() == 0
+    assert abs(K[:, 2 * n:]).max() == 0

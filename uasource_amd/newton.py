@@ -397,7 +397,8 @@ class GpuUvAssembler:
         self.n_r += 1
         return float(self.torch.dot(self.R, self.R).item()) + 1000 * EPS
 
-    def assemble_rk(self, fp32: bool, fp32_residual: bool):
+    def assemble_rk_device(self, fp32: bool, fp32_residual: bool):
+        """K (device, ``self.K``) and R(gamma = 0) (device, ``self.R0``) at the current state."""
         self.xt.copy_(self.x)
         self._assemble(uvh.UVH_FP32 if fp32 else 0, self.K)
         self.n_rk += 1
@@ -405,7 +406,45 @@ class GpuUvAssembler:
             self._assemble(uvh.UVH_RONLY)
             self.n_r += 1
         self.R0.copy_(self.R)
+
+    def assemble_rk(self, fp32: bool, fp32_residual: bool):
+        self.assemble_rk_device(fp32, fp32_residual)
         return self.ctx.uv_K_matrix(self.K.cpu().numpy()), (-self.R0).cpu().numpy()
+
+    def uvh_layout_index(self):
+        """Device copy of ``uv_to_uvh_index`` for this mesh, and its mask of uv values."""
+        if getattr(self, "_uvh_index", None) is None:
+            rowptr_n = self.ctx.pattern()[0][:self.nn + 1] // 3   # block row 0 starts rows at 3 rowptr_n
+            idx = uv_to_uvh_index(rowptr_n)
+            self._uvh_index = self.torch.from_numpy(idx).to(self.dev)
+            self._uvh_mask = (self._uvh_index >= 0)
+        return self._uvh_index, self._uvh_mask
+
+    def device_solver(self, fixed_uv, rtol: float):
+        """A ``linsolve.GpuLinearSolver`` for the uv steps: the uvh solver of the same mesh with
+        the uv fixed dofs and every thickness dof held (their rows and columns are eliminated)."""
+        from .linsolve import GpuLinearSolver
+        n = self.nn
+        held = np.concatenate([np.asarray(fixed_uv if fixed_uv is not None else [], dtype=np.int64),
+                               np.arange(2 * n, 3 * n)])
+        solver = GpuLinearSolver(self.ctx, held, rtol=rtol)
+        self._Kuvh = self.torch.zeros(self.ctx.nnz, dtype=self.torch.float64, device=self.dev)
+        self._rhs3 = self.torch.zeros(3 * n, dtype=self.torch.float64, device=self.dev)
+        self._d3 = self.torch.zeros(3 * n, dtype=self.torch.float64, device=self.dev)
+        self.uvh_layout_index()
+        return solver
+
+    def step_on_device(self, solver):
+        """d = K^-1 (-R0) with the solver of ``device_solver``, into ``self.d``."""
+        idx, mask = self._uvh_index, self._uvh_mask
+        torch = self.torch
+        torch.index_select(self.K, 0, idx.clamp(min=0), out=self._Kuvh)
+        self._Kuvh.mul_(mask)
+        n = self.nn
+        self._rhs3[:2 * n].copy_(self.R0).neg_()
+        solver.factor(self._Kuvh, self.stream)
+        solver.solve(self._rhs3, self._d3, self.stream)
+        self.d.copy_(self._d3[:2 * n])
 
     def set_step(self, d: np.ndarray):
         self.d.copy_(self.torch.from_numpy(d))
@@ -437,6 +476,22 @@ class GpuUvAssembler:
         self.x.copy_(self.torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)))
 
 
+def uv_to_uvh_index(rowptr_n: np.ndarray) -> np.ndarray:
+    """For each value of K in the uvh layout (9 blocks: row (bi, n) at bi 3 nnzN + 3 rowptr_n[n],
+    blocks bj = 0..2 of deg(n) columns each), the index of the same entry in the uv layout
+    (4 blocks: row (bi, n) at bi 2 nnzN + 2 rowptr_n[n]), or -1 in the thickness blocks."""
+    rp = np.asarray(rowptr_n, dtype=np.int64)
+    deg = np.diff(rp)
+    nnzN = int(rp[-1])
+    node = np.repeat(np.arange(len(deg)), deg)
+    c = np.arange(nnzN) - rp[node]
+    idx = np.full(9 * nnzN, -1, dtype=np.int64)
+    for bi in range(2):
+        for bj in range(2):
+            idx[bi * 3 * nnzN + 3 * rp[node] + bj * deg[node] + c] = bi * 2 * nnzN + 2 * rp[node] + bj * deg[node] + c
+    return idx
+
+
 def uv_fixed(fixed_uvh: np.ndarray | None, nn: int):
     """The velocity degrees of freedom among uvh fixed dofs (indices into [ub; vb])."""
     if fixed_uvh is None:
@@ -446,18 +501,24 @@ def uv_fixed(fixed_uvh: np.ndarray | None, nn: int):
 
 
 def transient_start(CtrlVar, MUA, F0, F1, fixed, uv_assembler, linear_solver=superlu_solve, verbose=False,
-                    log=print):
+                    log=print, device_solver: bool = False):
     """The start of a transient run (Ua2D.m:598-611, "always needed at a start of a transient
     run"): a uv (diagnostic) solve on F1's geometry, whose velocities then become both F0's and
     F1's, so that the first implicit uvh step starts from the balanced state. ``uv_assembler``
     builds the uv assembler from (CtrlVar, MUA, F) with F carrying the upper surface s (from the
-    floatation of h). Returns (F0, F1, uv NewtonResult)."""
+    floatation of h). With ``device_solver`` the uv steps take the device solver (the uv
+    assembler's ``device_solver``) instead of ``linear_solver``. Returns (F0, F1, uv NewtonResult)."""
     from .fields import surface_from_floatation
     F = F1.copy()
     F.s = surface_from_floatation(F1, CtrlVar)
     asm = uv_assembler(CtrlVar, MUA, F)
-    res = solve(asm, CtrlVar, linear_solver=linear_solver, fixed=uv_fixed(fixed, MUA.Nnodes), verbose=verbose,
-                log=log, system="uv")
+    fuv = uv_fixed(fixed, MUA.Nnodes)
+    solver = asm.device_solver(fuv, CtrlVar.LinSolveTol) if device_solver else linear_solver
+    try:
+        res = solve(asm, CtrlVar, linear_solver=solver, fixed=fuv, verbose=verbose, log=log, system="uv")
+    finally:
+        if device_solver:
+            solver.close()
     ub, vb, _ = res.ub, res.vb, None
     G0, G1 = F0.copy(), F1.copy()
     G0.ub, G0.vb = ub.copy(), vb.copy()
@@ -690,10 +751,13 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
         lam = np.zeros(L.shape[0]) if luvh is None else np.array(luvh, dtype=np.float64).ravel()
         LT = L.T.tocsr()
     if on_device:
-        # the device solver holds its own fixed set (identity rows and columns)
-        same = (linear_solver.fixed is None and (fixed is None or len(fixed) == 0)) or (
-            linear_solver.fixed is not None and fixed is not None
-            and np.array_equal(linear_solver.fixed, np.unique(np.asarray(fixed))))
+        # the device solver holds its own fixed set (identity rows and columns); for the uv
+        # system, which it solves in the uvh layout, that set also holds every thickness dof
+        want = np.unique(np.asarray(fixed if fixed is not None else [], dtype=np.int64))
+        if system == "uv":
+            want = np.unique(np.concatenate([want, np.arange(2 * assembler.nn, 3 * assembler.nn)]))
+        have = linear_solver.fixed
+        same = (have is None and want.size == 0) or (have is not None and np.array_equal(have, want))
         if not same:
             raise ValueError("the device linear solver was built for a different set of fixed degrees of freedom")
     else:
