@@ -230,17 +230,30 @@ def main(argv=None):
     assembly_ms = ms_per_step
     dominant = max(kernels, key=kernels.get)
 
+    # roofline of the dominant kernel: the algorithmic bytes of one assembly (all of which that
+    # kernel's one launch per assembly works through) over its average launch time (HIP events
+    # on its stream); the whole assembly's rate beside it
     nbytes = algorithmic_bytes(M.Nnodes, M.Nele, M.nod, ctx.nnz, args.ronly)
-    achieved = nbytes / (assembly_ms * 1e-3) / 1e9
-    traffic = None
+    dom_ms = kernels[dominant] / max(1.0, launches[dominant])
+    achieved = nbytes / (dom_ms * 1e-3) / 1e9
+    achieved_assembly = nbytes / (assembly_ms * 1e-3) / 1e9
+    traffic = traffic_assembly = None
+    traffic_source = None
     tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}{'_fp32' if args.fp32 else ''}"
                                            f"{'_ronly' if args.ronly else ''}.json")
     if os.path.exists(tfile):
+        from measure.profiling import kernel_source_hash
         with open(tfile) as fh:
             tinfo = json.load(fh)
-        # the committed measurement applies only to the kernels it was taken on
-        if sorted(tinfo.get("kernels", [])) == sorted(kernels):
-            traffic = tinfo.get("hbm_bytes_per_assembly")
+        # the committed counter measurement applies only to the build made from the same kernel
+        # sources, with the same kernels in the assembly
+        if tinfo.get("source_sha256") == kernel_source_hash() and sorted(tinfo.get("kernels", [])) == sorted(kernels):
+            sym = tinfo["symbols"].get(dominant)
+            if sym is not None:
+                b = tinfo["per_kernel_bytes"][sym]
+                traffic = b["fetch"] + b["write"]
+            traffic_assembly = tinfo.get("hbm_bytes_per_assembly")
+            traffic_source = os.path.relpath(tfile, REPO)
 
     # VALU use of the element kernel under the factored flop model of the element arithmetic,
     # nip (42 nod^2 + 90 nod + 150) per element for R+K (the arithmetic is not counted
@@ -272,8 +285,11 @@ def main(argv=None):
                    "nnz_K": ctx.nnz, "mode": "R only" if args.ronly else "R+K", "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "uvh assembly (all kernels of one evaluation, between events on its stream)",
-                     "algorithmic_bytes": nbytes, "assembly_us": assembly_ms * 1e3},
+                     "kernel": dominant, "kernel_us": dom_ms * 1e3, "algorithmic_bytes": nbytes,
+                     "traffic_source": traffic_source,
+                     "assembly": {"achieved": achieved_assembly, "frac": achieved_assembly / HBM_PEAK_GBS,
+                                  "us": assembly_ms * 1e3, "traffic": traffic_assembly,
+                                  "note": "all kernels of one assembly, between events on its stream"}},
         "valu": valu,
         "kernels_us": {k: v * 1e3 for k, v in kernels.items()},
         "kernel_launches_per_assembly": launches,

@@ -277,6 +277,14 @@ uvh_ctx *uvhb_member(uvh_batch *batch, int32_t member);
  * the FP64 (fp32 = 0) or FP32 element arithmetic; the autotuner's output. */
 int uvhb_set_tile(uvh_batch *batch, int32_t nod, int32_t fp32, int32_t epb);
 int uvhb_get_tile(const uvh_batch *batch, int32_t nod, int32_t fp32);
+/* Per member m of the batch: rr[m] = sum_i (w_i R_i)^2 and d2[m] = -sum_i w_i R_i d_i over the
+ * member's 3 Nn entries, with weights w (0 at fixed dofs, 1 elsewhere; w NULL or w[m] NULL: all 1):
+ * the numerator of rForce and the D2 of CalcCostFunctionNRuvh.m:45-85 for each member's Newton
+ * step d. R, d, w are arrays of the members' device pointers; rr and d2 device arrays of nmem
+ * doubles. Sums in a fixed order with no atomics and nothing cleared, so the call can be captured
+ * in a HIP graph (the first call sets up the batch's tables and must not be captured). */
+int uvhb_cost_device(uvh_batch *b, const double *const *dR, const double *const *dd, const double *const *dw,
+                     double *rr, double *d2, void *stream);
 /* Device-buffer assembly of every member: dfields, dR and dK are arrays of uvh_size()
  * entries (dK may be NULL with UVH_RONLY). Enqueued on `stream`; uvhb_sync_status waits and
  * fills member_status (may be NULL) with UVH_OK / UVH_E_NAN_R / UVH_E_NAN_K per member. */

@@ -75,6 +75,20 @@ def per_kernel(path: str, counter: str) -> dict:
     return {k: tot[k] / len(cnt[k]) for k in tot}
 
 
+KERNEL_SOURCES = ("uvh_kernels.hip", "uvh_api.hip", "uvh_internal.h")
+
+
+def kernel_source_hash() -> str:
+    """SHA-256 over the kernel sources of the library: a traffic record applies to the build
+    made from exactly these sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for fn in KERNEL_SOURCES:
+        with open(os.path.join(REPO, "uasource_amd", "csrc", fn), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
 def traffic(d: str, workload: str, ronly: bool, fp32: bool = False) -> dict:
     fetch = per_kernel(os.path.join(d, "fetch"), "FETCH_SIZE")
     write = per_kernel(os.path.join(d, "write"), "WRITE_SIZE")
@@ -84,9 +98,12 @@ def traffic(d: str, workload: str, ronly: bool, fp32: bool = False) -> dict:
     fscale = cf[copy_kernel] * 1024 / COPY_BYTES
     wscale = cw[copy_kernel] * 1024 / COPY_BYTES
     out = {"workload": workload, "mode": "R only" if ronly else "R+K",
-           "copy_calibration": {"kernel": copy_kernel, "bytes": COPY_BYTES, "fetch_scale": fscale,
-                                "write_scale": wscale},
-           "per_kernel_bytes": {}}
+           "source_sha256": kernel_source_hash(),
+           "copy_calibration": {"kernel": copy_kernel, "bytes": COPY_BYTES, "copies": 3,
+                                "fetch_scale": fscale, "write_scale": wscale,
+                                "note": "counter KiB x 1024 per true byte of a device copy of 1 GiB, "
+                                        "averaged over its 3 dispatches"},
+           "per_kernel_bytes": {}, "symbols": {}}
     total = 0.0
     names = set()
     for k in fetch:
@@ -96,6 +113,7 @@ def traffic(d: str, workload: str, ronly: bool, fp32: bool = False) -> dict:
         fb = fetch[k] * 1024 / fscale
         wb = write.get(k, 0.0) * 1024 / wscale
         out["per_kernel_bytes"][k] = {"fetch": fb, "write": wb}
+        out["symbols"][t] = k
         total += fb + wb
         names.add(t)
     out["hbm_bytes_per_assembly"] = total

@@ -17,7 +17,7 @@ import numpy as np
 from . import uvh
 
 EXPORTED_BATCH = ("uvhb_create", "uvhb_destroy", "uvhb_last_error", "uvhb_add_mesh", "uvhb_size", "uvhb_member",
-                  "uvhb_set_tile", "uvhb_get_tile", "uvhb_assemble_device", "uvhb_sync_status")
+                  "uvhb_set_tile", "uvhb_get_tile", "uvhb_assemble_device", "uvhb_sync_status", "uvhb_cost_device")
 
 _bound = False
 
@@ -48,6 +48,9 @@ def _bind(lib):
     lib.uvhb_assemble_device.restype = C.c_int
     lib.uvhb_sync_status.argtypes = [vp, C.POINTER(i32)]
     lib.uvhb_sync_status.restype = C.c_int
+    lib.uvhb_cost_device.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), vp, vp,
+                                     vp]
+    lib.uvhb_cost_device.restype = C.c_int
     _bound = True
     return lib
 
@@ -134,6 +137,16 @@ class UvhBatch:
         karr = None if K is None else (C.c_void_p * n)(*[uvh._ptr(k) for k in K])
         st = self.lib.uvhb_assemble_device(self.h, C.byref(ctrl), farr, int(flags), rarr, karr, stream)
         self._check(st, "uvhb_assemble_device")
+
+    def cost_device(self, R, d, w, rr, d2, stream=None):
+        """Per member: rr[m] = |w R|^2, d2[m] = -(w R) . d (device tensors; ``w`` may be None).
+        Enqueue only; capturable once a first call has set up the tables."""
+        n = len(self.members)
+        ra = (C.c_void_p * n)(*[uvh._ptr(r) for r in R])
+        da = (C.c_void_p * n)(*[uvh._ptr(x) for x in d])
+        wa = None if w is None else (C.c_void_p * n)(*[uvh._ptr(x) for x in w])
+        st = self.lib.uvhb_cost_device(self.h, ra, da, wa, uvh._ptr(rr), uvh._ptr(d2), stream)
+        self._check(st, "uvhb_cost_device")
 
     def sync_status(self):
         n = len(self.members)

@@ -38,6 +38,9 @@ int element_epb_default(int nod, int nip, bool fp32);
 size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
                         int64_t nnzN, int nn, bool sl, bool ronly, int check_nan, hipStream_t s);
+hipError_t launch_cost(const CostMember *mem, int nmem, const int2 *items, int nitems, double2 *partial, double *rr,
+                       double *d2, hipStream_t s);
+constexpr int kCostChunk = 32768;
 }  // namespace uvh
 
 namespace {
@@ -1083,12 +1086,29 @@ struct uvh_batch {
     // counter (UVH_BATCH=persistent): the persistent loop keeps more registers live (227 vs
     // 146 VGPRs for the FP64 element kernel) and runs 1.3-1.5x slower
     bool flat = true;
+    // batched residual cost (uvhb_cost_device): member table, (member, chunk) items, partials
+    uvh::CostMember *d_cost = nullptr, *h_cost = nullptr;
+    std::vector<uvh::CostMember> cost_last;
+    int2 *d_cost_items = nullptr;
+    int n_cost_items = 0;
+    double2 *d_cost_partial = nullptr;
+    hipEvent_t cost_upload = nullptr;
+    bool cost_pending = false;
     int fail(int code, const std::string &m) { err = m; return code; }
     void free_lists() {
         for (int t = 0; t < 2; ++t) {
             for (int k = 0; k < 3; ++k) { hipFree(d_items[t][k]); d_items[t][k] = nullptr; n_items[t][k] = 0; }
             built[t] = false;
         }
+        // the cost tables depend on the members too
+        if (cost_pending) hipEventSynchronize(cost_upload);
+        hipFree(d_cost); hipFree(d_cost_items); hipFree(d_cost_partial); hipHostFree(h_cost);
+        d_cost = nullptr; d_cost_items = nullptr; d_cost_partial = nullptr; h_cost = nullptr;
+        if (cost_upload) hipEventDestroy(cost_upload);
+        cost_upload = nullptr;
+        cost_pending = false;
+        cost_last.clear();
+        n_cost_items = 0;
     }
 };
 
@@ -1273,7 +1293,10 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
         M.rowptr_n = c->d_rowptr_n; M.b = c->d_b; M.EB = c->d_EB; M.RE = c->d_RE;
         M.nnzN = m.nnzN(); M.nele = m.nele; M.nn = m.nn; M.nod = m.nod; M.nip = m.nip; M.epb = c->epb[t];
         M.f = to_ptrs(dfields[i]);
-        M.R = dR[i]; M.Tint = nullptr; M.Fext = nullptr; M.Kval = ronly ? nullptr : dK[i];
+        // K pointers are kept with UVH_RONLY too (the R-only kernels do not touch them), so that
+        // R+K and R-only calls on the same buffers share one member table and a HIP graph
+        // captured after eager calls of both replays with no table upload
+        M.R = dR[i]; M.Tint = nullptr; M.Fext = nullptr; M.Kval = dK ? dK[i] : nullptr;
     }
     hipError_t e = hipSuccess;
     if (b->last.size() != cur.size() || std::memcmp(b->last.data(), cur.data(), sizeof(uvh::BatchMember) * nm) != 0) {
@@ -1291,6 +1314,51 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
                           b->d_items[t][2], b->rbeg[t], b->elem_lds[t], b->row_lds[t], fp32, ronly, check,
                           b->d_ctr, b->d_status, p, b->flat, s);
     if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_assemble: launch: ") + hipGetErrorString(e));
+    return UVH_OK;
+}
+
+int uvhb_cost_device(uvh_batch *b, const double *const *dR, const double *const *dd, const double *const *dw,
+                     double *rr, double *d2, void *stream) {
+    if (!b) return UVH_E_ARG;
+    const int nm = int(b->members.size());
+    if (nm == 0) return b->fail(UVH_E_STATE, "uvhb_cost: empty batch");
+    if (!dR || !dd || !rr || !d2) return b->fail(UVH_E_ARG, "uvhb_cost: null argument");
+    hipSetDevice(b->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<uvh::CostMember> cur(static_cast<size_t>(nm));
+    int first = 0;
+    for (int i = 0; i < nm; ++i) {
+        if (!dR[i] || !dd[i]) return b->fail(UVH_E_ARG, "uvhb_cost: member " + std::to_string(i) + ": null R or d");
+        const int64_t n = 3 * int64_t(b->members[i]->mesh.nn);
+        const int nch = int((n + uvh::kCostChunk - 1) / uvh::kCostChunk);
+        cur[size_t(i)] = uvh::CostMember{dR[i], dd[i], dw ? dw[i] : nullptr, n, first, nch};
+        first += nch;
+    }
+    if (!b->d_cost) {   // first call: tables and partials (later calls allocate nothing)
+        std::vector<int2> items;
+        for (int i = 0; i < nm; ++i)
+            for (int c = 0; c < cur[size_t(i)].nchunk; ++c) items.push_back(make_int2(i, c));
+        if (dalloc(&b->d_cost, size_t(nm)) != hipSuccess || dalloc(&b->d_cost_items, std::max<size_t>(1, items.size())) != hipSuccess ||
+            dalloc(&b->d_cost_partial, std::max<size_t>(1, items.size())) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void **>(&b->h_cost), sizeof(uvh::CostMember) * nm, 0) != hipSuccess ||
+            hipEventCreateWithFlags(&b->cost_upload, hipEventDisableTiming) != hipSuccess)
+            return b->fail(UVH_E_NOMEM, "uvhb_cost: cannot allocate its tables");
+        if (!items.empty() &&
+            hipMemcpy(b->d_cost_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+            return b->fail(UVH_E_HIP, "uvhb_cost: item upload");
+        b->n_cost_items = int(items.size());
+    }
+    hipError_t e = hipSuccess;
+    if (b->cost_last.size() != cur.size() || std::memcmp(b->cost_last.data(), cur.data(), sizeof(uvh::CostMember) * nm) != 0) {
+        if (b->cost_pending) hipEventSynchronize(b->cost_upload);
+        std::memcpy(b->h_cost, cur.data(), sizeof(uvh::CostMember) * nm);
+        e = hipMemcpyAsync(b->d_cost, b->h_cost, sizeof(uvh::CostMember) * nm, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(b->cost_upload, s);
+        b->cost_pending = true;
+        b->cost_last = cur;
+    }
+    if (e == hipSuccess) e = uvh::launch_cost(b->d_cost, nm, b->d_cost_items, b->n_cost_items, b->d_cost_partial, rr, d2, s);
+    if (e != hipSuccess) return b->fail(UVH_E_HIP, std::string("uvhb_cost: ") + hipGetErrorString(e));
     return UVH_OK;
 }
 

@@ -220,6 +220,14 @@ struct CallArgs {             // per assembly
     double *R, *Tint, *Fext, *Kval;
 };
 
+// One member of a batched residual cost (device memory): R, the step d and the weights w
+// (null: all 1) of n entries; its chunks are partial[first .. first + nchunk - 1].
+struct CostMember {
+    const double *R, *d, *w;
+    int64_t n;
+    int32_t first, nchunk;
+};
+
 // Read access to a context's mesh (null before uvh_set_mesh) and device, for the parts of
 // the library outside uvh_api.hip.
 const MeshHost *ctx_mesh(const uvh_ctx *ctx);

@@ -1424,6 +1424,59 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(const MeshArgs *__restr
     if (bad) atomicOr(MA->status, bad);
 }
 
+// ------------------------------------------------------------------ batched residual cost
+// Per member of a batch: rr = sum_i (w_i R_i)^2 and d2 = -sum_i w_i R_i d_i (the numerator of
+// CalcCostFunctionNRuvh.m's rForce and its D2), with w_i 0 at fixed dofs. Chunks of kCostChunk
+// entries per workgroup, a fixed-order tree in LDS per workgroup, then one thread per member
+// adds its chunks in order: deterministic, no atomics, nothing to clear (graph-capturable).
+constexpr int kCostChunk = 32768;
+
+__global__ __launch_bounds__(256) void k_cost_partial(const CostMember *__restrict__ mem, const int2 *__restrict__ items,
+                                                      double2 *__restrict__ partial) {
+    __shared__ double2 red[256];
+    const int2 it = items[blockIdx.x];
+    const CostMember M = mem[it.x];
+    const int64_t i0 = int64_t(it.y) * kCostChunk;
+    const int64_t i1 = i0 + kCostChunk < M.n ? i0 + kCostChunk : M.n;
+    double rr = 0.0, d2 = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+        const double r = M.w ? M.w[i] * M.R[i] : M.R[i];
+        rr = fma(r, r, rr);
+        d2 = fma(-r, M.d[i], d2);
+    }
+    red[threadIdx.x] = make_double2(rr, d2);
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            red[threadIdx.x].x += red[threadIdx.x + h].x;
+            red[threadIdx.x].y += red[threadIdx.x + h].y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_cost_final(const CostMember *__restrict__ mem, int nmem, const double2 *__restrict__ partial,
+                             double *__restrict__ rr, double *__restrict__ d2) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmem) return;
+    const CostMember M = mem[m];
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < M.nchunk; ++c) {
+        a += partial[M.first + c].x;
+        b += partial[M.first + c].y;
+    }
+    rr[m] = a;
+    d2[m] = b;
+}
+
+hipError_t launch_cost(const CostMember *mem, int nmem, const int2 *items, int nitems, double2 *partial, double *rr,
+                       double *d2, hipStream_t s) {
+    if (nitems > 0) hipLaunchKernelGGL(k_cost_partial, dim3(nitems), dim3(256), 0, s, mem, items, partial);
+    hipLaunchKernelGGL(k_cost_final, dim3((nmem + 63) / 64), dim3(64), 0, s, mem, nmem, partial, rr, d2);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ launchers
 
 namespace {

@@ -661,10 +661,12 @@ def _criteria(CtrlVar, gamma, rWork, rForce, iteration, system="uvh"):
     return (rWork < aw and rForce < af) and (rWork < ow or rForce < of) and iteration >= CtrlVar.NRitmin
 
 
-def _line_search(CtrlVar, cost, r0, r1, slope0=None, b0=1.0, gamma_min=1e-3, accept_ratio=None):
-    """Backtracking (BackTracking.m) from the trial step ``b0`` with value ``r1``: returns
-    (gamma, (rForce, rWork, D2) at gamma, number of trial evaluations). The defaults are the
-    Newton direction's (rLineminUa.m:60-135: slope -2 r0, first trial 1, floor 1e-3)."""
+def _line_search_steps(CtrlVar, r0, r1, slope0=None, b0=1.0, gamma_min=1e-3, accept_ratio=None):
+    """Backtracking (BackTracking.m) from the trial step ``b0`` with value ``r1``, as a generator:
+    it yields each further trial step g and is sent the cost (rForce, rWork, D2) at g; its
+    return value is (gamma, (rForce, rWork, D2) at gamma, number of trial evaluations). The
+    defaults are the Newton direction's (rLineminUa.m:60-135: slope -2 r0, first trial 1,
+    floor 1e-3)."""
     if slope0 is None:
         slope0 = -2.0 * r0
     ratio = CtrlVar.NewtonAcceptRatio if accept_ratio is None else accept_ratio
@@ -682,7 +684,7 @@ def _line_search(CtrlVar, cost, r0, r1, slope0=None, b0=1.0, gamma_min=1e-3, acc
         g = min(max(g, lower * b), upper * b)
         if g < gamma_min:
             break
-        rg = cost(g)
+        rg = yield g
         evals += 1
         if rg[0] < best[0]:
             best_g, best = g, rg
@@ -692,6 +694,17 @@ def _line_search(CtrlVar, cost, r0, r1, slope0=None, b0=1.0, gamma_min=1e-3, acc
     if best[0] >= r0:
         return 0.0, (r0, float("nan"), float("nan")), evals
     return best_g, best, evals
+
+
+def _line_search(CtrlVar, cost, r0, r1, slope0=None, b0=1.0, gamma_min=1e-3, accept_ratio=None):
+    """Backtracking with the cost function ``cost(g)``; see ``_line_search_steps``."""
+    gen = _line_search_steps(CtrlVar, r0, r1, slope0, b0, gamma_min, accept_ratio)
+    try:
+        g = next(gen)
+        while True:
+            g = gen.send(cost(g))
+    except StopIteration as stop:
+        return stop.value
 
 
 def eliminate(linear_solver, fixed: np.ndarray | None):
