@@ -184,6 +184,12 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
     sat_rate = n_requests / makespan
     elements = sum(meshes[r.mesh][0].Nele for r in sat_reqs)
     log(f"serve: saturated {sat_rate:.0f} requests/s, {elements / makespan / 1e6:.1f} M elements/s")
+    # size classes: the meshes in thirds by element count (latency percentiles of a mix whose
+    # service times span two orders of magnitude say more per class than overall)
+    by_size = sorted(range(n_meshes), key=lambda mi: meshes[mi][0].Nele)
+    third = n_meshes // 3
+    size_classes = {"small": set(by_size[:third]), "medium": set(by_size[third:2 * third]),
+                    "large": set(by_size[2 * third:])}
     sweep = []
     for frac, policy in ((0.3, "fifo"), (0.5, "fifo"), (0.7, "fifo"), (0.9, "fifo"), (0.7, "sjf")):
         srv.policy = policy
@@ -195,6 +201,17 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
         svc = np.array([alone[(r.mesh, r.newton)] for r in reqs])   # the mix's own service times
         span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
         el = sum(meshes[r.mesh][0].Nele for r in reqs)
+        classes = {}
+        for cname, members in size_classes.items():
+            sel = [k for k, r in enumerate(reqs) if r.mesh in members]
+            if not sel:
+                continue
+            cl, cs = lat[sel], slow[sel]
+            classes[cname] = {"meshes_Nele": [int(meshes[mi][0].Nele) for mi in sorted(members)], "requests": len(sel),
+                              "p50_ms": float(np.percentile(cl, 50) * 1e3), "p99_ms": float(np.percentile(cl, 99) * 1e3),
+                              "p99_over_p50": float(np.percentile(cl, 99) / np.percentile(cl, 50)),
+                              "slowdown_p50": float(np.percentile(cs, 50)), "slowdown_p99": float(np.percentile(cs, 99)),
+                              "service_alone_p99_over_p50": float(np.percentile(svc[sel], 99) / np.percentile(svc[sel], 50))}
         row = {"load": frac, "policy": policy, "rate_per_s": frac * sat_rate,
                "p50_ms": float(np.percentile(lat, 50) * 1e3),
                "p99_ms": float(np.percentile(lat, 99) * 1e3), "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
@@ -203,10 +220,14 @@ def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=p
                "service_alone_p99_ms": float(np.percentile(svc, 99) * 1e3),
                "service_alone_p99_over_p50": float(np.percentile(svc, 99) / np.percentile(svc, 50)),
                "elements_per_s": el / span, "peak_inflight": srv.peak_inflight,
-               "host_stalls_ms": [round(x * 1e3, 2) for x in srv.stalls]}
+               "host_stalls_ms": [round(x * 1e3, 2) for x in srv.stalls], "size_classes": classes}
         sweep.append(row)
         log(f"serve load {frac} ({policy}): p50 {row['p50_ms']:.3f} ms p99 {row['p99_ms']:.3f} ms, slowdown p50 "
             f"{row['slowdown_p50']:.2f} p99 {row['slowdown_p99']:.2f}, host stalls {row['host_stalls_ms']}")
+        for cname, c in classes.items():
+            log(f"   {cname}: p50 {c['p50_ms']:.3f} ms p99 {c['p99_ms']:.3f} ms (p99/p50 {c['p99_over_p50']:.2f}; "
+                f"alone {c['service_alone_p99_over_p50']:.2f}), slowdown p50 {c['slowdown_p50']:.2f} "
+                f"p99 {c['slowdown_p99']:.2f}")
     srv.close()
     srv.policy = "fifo"
     at70 = next(r for r in sweep if r["load"] == 0.7 and r["policy"] == "fifo")

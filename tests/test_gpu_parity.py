@@ -279,6 +279,32 @@ def test_gather_scatter_matches_lds_scatter(monkeypatch, name):
         c2.close()
 
 
+@pytest.mark.parametrize("name", ["tiny_p1", "medium_p2", "medium_p3"])
+@pytest.mark.parametrize("threads", [64, 192])
+def test_element_tile_threads_change_no_result(monkeypatch, name, threads):
+    """Element tiles of one to three waves (UVH_ELEMENT_THREADS) compute every element row in
+    the same thread-local order as the 256-thread default: bit-identical R and K."""
+    M, F0, F1, BCs = mesh_fields(name)
+    cv = default_ctrlvar()
+    ref = uvh.UvhContext(0)
+    monkeypatch.setenv("UVH_ELEMENT_THREADS", str(threads))
+    c2 = uvh.UvhContext(0)
+    monkeypatch.delenv("UVH_ELEMENT_THREADS")
+    try:
+        say(f"[gpu] element tile threads={threads} {name}")
+        ref.set_mesh(M)
+        c2.set_mesh(M)
+        for fp32 in (False, True):
+            for ronly in (False, True):
+                a = ref.assemble(cv, F0, F1, want_tf=True, fp32=fp32, ronly=ronly)
+                b = c2.assemble(cv, F0, F1, want_tf=True, fp32=fp32, ronly=ronly)
+                for k in ("R", "Tint", "Fext") + (() if ronly else ("Kval",)):
+                    assert np.array_equal(a[k], b[k]), (k, fp32, ronly)
+    finally:
+        ref.close()
+        c2.close()
+
+
 @pytest.mark.parametrize("name,chunks", [("medium_p2", 5), ("medium_p3", 3), ("plain_p2", 8)])
 def test_pipelined_assembly_matches_single_pass(monkeypatch, name, chunks):
     """The pipelined assembly (UVH_CHUNKS: element tiles in chunks on the caller's stream, the

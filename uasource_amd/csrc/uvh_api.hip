@@ -16,7 +16,7 @@
 namespace uvh {
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, int *status, hipStream_t s);
 hipError_t launch_nodal_s(int nn, const double *sfc, const double *h, double *b, int *status, hipStream_t s);
-hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
+hipError_t launch_element(int nod, int nele, int nip, int epb, int nthr, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
                           const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, hipStream_t s);
@@ -24,7 +24,7 @@ hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
                        const int32_t *blocks, int nblk, hipStream_t s, int nbk = 3);
-int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap);
+int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap, int nthr);
 hipError_t launch_rows_gather(int nod, int nn, int64_t nnzN, bool fp32, const int32_t *nzrow, const int64_t *rowptr_n,
                               const int32_t *inc_ptr, const int64_t *imap_ptr, const uint8_t *imap,
                               const uint8_t *diag, const void *EB, const void *RE, double *R, double *Tint,
@@ -34,7 +34,7 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
                         const int2 *elem_items, const int *eb, const int2 *row_items, const int *rb,
                         const size_t *elem_lds, const size_t *row_lds, bool fp32, bool ronly, int check_nan,
                         int *ctr, int *status, const Params &p, bool flat, hipStream_t s);
-int element_epb_default(int nod, int nip, bool fp32);
+int element_epb_default(int nod, int nip, bool fp32, int nthr);
 size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
                         int64_t nnzN, int nn, bool sl, bool ronly, int check_nan, hipStream_t s);
@@ -121,6 +121,10 @@ struct uvh_ctx {
     double *d_b = nullptr, *d_RE = nullptr, *d_EB = nullptr;
     double *d_zero = nullptr;   // uv assembly: zero nodal field (lazily allocated)
     int epb[2] = {0, 0};    // elements per block of the element kernel: [FP64, FP32]
+    // threads per element block (UVH_ELEMENT_THREADS=64/128/192: one- to three-wave tiles, each
+    // with as many elements as its stages fit; measured slower than 256 on 6- and 10-node meshes)
+    int el_threads[2] = {256, 256};
+    int bepb[2] = {0, 0};   // elements per 256-thread block as a batch member (uvhb_set_tile)
     // one-pass element rows (all nine blocks per thread) for 3- and 6-node elements: faster with
     // FP32 arithmetic; with FP64 the three-pass rows keep the registers for two waves per SIMD.
     // UVH_ROWS=full or =split forces one form for both.
@@ -377,6 +381,9 @@ int uvh_create(uvh_ctx **out, int device) {
     ctx->rows_mode = !rows ? 0 : std::strcmp(rows, "full") == 0 ? 1 : std::strcmp(rows, "split") == 0 ? 2 : 0;
     const char *sc = std::getenv("UVH_SCATTER");
     ctx->gather_rows = sc && std::strcmp(sc, "gather") == 0;
+    const char *et = std::getenv("UVH_ELEMENT_THREADS");
+    const int nt = et ? std::atoi(et) : 0;
+    if (nt == 64 || nt == 128 || nt == 192 || nt == 256) ctx->el_threads[0] = ctx->el_threads[1] = nt;
     const char *ch = std::getenv("UVH_CHUNKS");
     ctx->nchunks = ch ? std::max(1, std::min(64, std::atoi(ch))) : kDefaultChunks;
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
@@ -573,8 +580,10 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
         ctx->free_mesh();
         return ctx->hip_fail(e, "uvh_set_mesh: upload");
     }
-    ctx->epb[0] = uvh::element_epb_default(nod, nip, false);
-    ctx->epb[1] = uvh::element_epb_default(nod, nip, true);
+    for (int t = 0; t < 2; ++t) {
+        ctx->epb[t] = uvh::element_epb_default(nod, nip, t == 1, ctx->el_threads[t]);
+        ctx->bepb[t] = uvh::element_epb_default(nod, nip, t == 1, 256);
+    }
     ctx->fused_ok = fused;
     ctx->has_mesh = true;
     return UVH_OK;
@@ -778,7 +787,8 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
     const char *row_name = fp32 ? (ronly ? "rows32_r" : "rows32_rk") : (ronly ? "rows_r" : "rows_rk");
     auto elements = [&](int lb0, int lb1, hipStream_t st) {
         return ctx->timed(el_name, st, [&] {
-            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], fp32, full, ctx->d_conn, ctx->d_x, ctx->d_y,
+            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32, full, ctx->d_conn,
+                                       ctx->d_x, ctx->d_y,
                                        f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, ctx->d_EB,
                                        ctx->d_RE, lb0, lb1, st);
         });
@@ -860,7 +870,8 @@ int assemble_noalloc_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *
     ctx->last_status = status;
     hipError_t e = uvh::launch_nodal_b(m.nn, f, p, b, status, s);
     if (e == hipSuccess)
-        e = uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], fp32, full, ctx->d_conn, ctx->d_x, ctx->d_y, f, b,
+        e = uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32, full, ctx->d_conn,
+                                ctx->d_x, ctx->d_y, f, b,
                                 ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, EB, RE, 0, -1, s);
     if (e == hipSuccess)
         e = uvh::launch_rows(m.nod, m.nn, m.max_block_deg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
@@ -978,7 +989,8 @@ int uv_assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctr
     if (e == hipSuccess)
         e = ctx->timed(fp32 ? (ronly ? "uv_element32_r" : "uv_element32_rk") : (ronly ? "uv_element_r" : "uv_element_rk"),
                        s, [&] {
-                           return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], fp32, full, ctx->d_conn,
+                           return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32,
+                                                      full, ctx->d_conn,
                                                       ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p,
                                                       ctx->d_inc_pos, ronly, ctx->d_EB, ctx->d_RE, 0, -1, s);
                        });
@@ -1132,7 +1144,7 @@ int batch_build_lists(uvh_batch *b, int t) {
             const uvh_ctx *c = b->members[mi];
             const uvh::MeshHost &m = c->mesh;
             if (m.nod != nods[ty]) continue;
-            const int epb = c->epb[t];
+            const int epb = c->bepb[t];
             for (int k = 0; k < (m.nele + epb - 1) / epb; ++k) items[1].push_back(make_int2(mi, k));
             for (int k = 0; k < (m.nn + uvh::row_nodes_per_block(m.nod) - 1) / uvh::row_nodes_per_block(m.nod); ++k)
                 items[2].push_back(make_int2(mi, k));
@@ -1239,7 +1251,7 @@ int uvhb_set_tile(uvh_batch *b, int32_t nod, int32_t fp32, int32_t epb) {
             return b->fail(UVH_E_ARG, "uvhb_set_tile: tile does not fit (epb*nip, epb*nod <= 256, LDS <= 156 KiB)");
     }
     for (auto *c : b->members)
-        if (c->mesh.nod == nod) c->epb[fp32 ? 1 : 0] = epb;
+        if (c->mesh.nod == nod) c->bepb[fp32 ? 1 : 0] = epb;
     b->built[fp32 ? 1 : 0] = false;
     return UVH_OK;
 }
@@ -1247,7 +1259,7 @@ int uvhb_set_tile(uvh_batch *b, int32_t nod, int32_t fp32, int32_t epb) {
 int uvhb_get_tile(const uvh_batch *b, int32_t nod, int32_t fp32) {
     if (!b) return 0;
     for (auto *c : b->members)
-        if (c->mesh.nod == nod) return c->epb[fp32 ? 1 : 0];
+        if (c->mesh.nod == nod) return c->bepb[fp32 ? 1 : 0];
     return 0;
 }
 
@@ -1291,7 +1303,7 @@ int uvhb_assemble_device(uvh_batch *b, const uvh_ctrl *ctrl, const uvh_fields *d
         M.conn = c->d_conn; M.x = c->d_x; M.y = c->d_y; M.tab = c->d_tab; M.tab32 = c->d_tab32;
         M.inc_ptr = c->d_inc_ptr; M.inc_pos = c->d_inc_pos; M.pos_inc = c->d_pos_inc; M.diag = c->d_diag;
         M.rowptr_n = c->d_rowptr_n; M.b = c->d_b; M.EB = c->d_EB; M.RE = c->d_RE;
-        M.nnzN = m.nnzN(); M.nele = m.nele; M.nn = m.nn; M.nod = m.nod; M.nip = m.nip; M.epb = c->epb[t];
+        M.nnzN = m.nnzN(); M.nele = m.nele; M.nn = m.nn; M.nod = m.nod; M.nip = m.nip; M.epb = c->bepb[t];
         M.f = to_ptrs(dfields[i]);
         // K pointers are kept with UVH_RONLY too (the R-only kernels do not touch them), so that
         // R+K and R-only calls on the same buffers share one member table and a HIP graph

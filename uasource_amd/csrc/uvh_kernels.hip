@@ -676,7 +676,7 @@ __device__ __forceinline__ int row_table_doubles(int tstride) {
 // the physics and the element rows (double, or float for the precision form); the rows are
 // stored in T at their position in the node-incidence order.
 template <int NOD, typename T, bool RONLY, bool FULL, bool SL = false>
-__device__ __forceinline__ void element_block(double *lds, int nele, int nip, int epb, int lb,
+__device__ __forceinline__ void element_block(double *lds, int nele, int nip, int epb, int lb, int nthr,
                                               const int32_t *__restrict__ conn, const double *__restrict__ xn,
                                               const double *__restrict__ yn, const FieldPtrs &f,
                                               const double *__restrict__ bnod, const ShapeTables *__restrict__ tab,
@@ -698,7 +698,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     double *work = lds + ttab + ((row_table_doubles<T>(tstride) + 1) & ~1);  // 16-byte aligned union:
                                             // nodal [epb][NV_COUNT][NOD] / coef [epb][nip][kNC]
     const int tid = threadIdx.x;
-    for (int k = tid; k < tstride; k += 256) {
+    for (int k = tid; k < tstride; k += nthr) {
         const int q = k / NOD, i = k - q * NOD;
         const double vN = tab->N[q][i], vXi = tab->dxi[q][i], vEta = tab->deta[q][i];
         sN[k] = vN; sXi[k] = vXi; sEta[k] = vEta;
@@ -706,7 +706,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
             rN[k] = T(vN); rXi[k] = T(vXi); rEta[k] = T(vEta);
         }
     }
-    if (tid < nip) sW[tid] = tab->w[tid];
+    for (int k = tid; k < nip; k += nthr) sW[k] = tab->w[k];
     const int64_t e0 = (int64_t)lb * epb;
 
     // stage 1: gathers, one (element, node) per thread
@@ -736,7 +736,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     const int nrow = epb * NOD;
     if (FULL) {
         // stage 3, one pass: (element, row) items, all nine blocks per item
-        for (int it = tid; it < nrow; it += 256) {
+        for (int it = tid; it < nrow; it += nthr) {
             const int el = it / NOD, i = it - el * NOD;
             const int64_t e = e0 + el;
             if (e >= nele) continue;
@@ -758,7 +758,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     // stage 3: element rows, as (block row, element, row) items dealt round-robin to the
     // threads; each item accumulates 3 x NOD Jacobian terms and its T and F over the points
     const int nbr = p.uv ? 2 : 3;   // the uv assembly has no thickness rows
-    for (int it = tid; it < nbr * nrow; it += 256) {
+    for (int it = tid; it < nbr * nrow; it += nthr) {
         const int bi = it / nrow;
         const int rr = it - bi * nrow;
         const int el = rr / NOD, i = rr - el * NOD;
@@ -794,8 +794,8 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
     extern __shared__ double lds[];
     const int lb = lb0 + xcd_block(blockIdx.x, gridDim.x);   // tiles lb0 .. lb1-1 of this launch
     if (lb >= lb1) return;  // whole block
-    element_block<NOD, T, RONLY, FULL, SL>(lds, nele, nip, epb, lb, conn, xn, yn, f, bnod, tab, rtab, p, inc_pos, EB,
-                                           RE);
+    element_block<NOD, T, RONLY, FULL, SL>(lds, nele, nip, epb, lb, blockDim.x, conn, xn, yn, f, bnod, tab, rtab, p,
+                                           inc_pos, EB, RE);
 }
 
 // Node rows: row_lanes(NOD) lanes per node, 64 / row_lanes nodes per wave. The incident element
@@ -1153,7 +1153,7 @@ __global__ __launch_bounds__(256) void k_element_batch(const BatchMember *__rest
             rtab = (const ShapeTables32 *)(const __attribute__((address_space(4))) ShapeTables32 *)M->tab32;
         const FieldPtrs f = generic_fields(&M->f);
         element_block<NOD, T, RONLY, (NOD <= 6 && sizeof(T) == sizeof(float))>(
-            lds, M->nele, M->nip, M->epb, w.y, (const int32_t *)M->conn, (const double *)M->x,
+            lds, M->nele, M->nip, M->epb, w.y, 256, (const int32_t *)M->conn, (const double *)M->x,
             (const double *)M->y, f, (const double *)M->b, (const ShapeTables *)M->tab, rtab, p,
             (const int32_t *)M->inc_pos, (T *)M->EB, (T *)M->RE);
         __syncthreads();  // the LDS of this item is dead before the next one overwrites it
@@ -1178,7 +1178,7 @@ __global__ __launch_bounds__(256) void k_element_batch_flat(const BatchMember *_
         rtab = (const ShapeTables32 *)(const __attribute__((address_space(4))) ShapeTables32 *)M->tab32;
     const FieldPtrs f = generic_fields(&M->f);
     element_block<NOD, T, RONLY, (NOD <= 6 && sizeof(T) == sizeof(float)), SL>(
-        lds, M->nele, M->nip, M->epb, w.y, (const int32_t *)M->conn, (const double *)M->x, (const double *)M->y, f,
+        lds, M->nele, M->nip, M->epb, w.y, 256, (const int32_t *)M->conn, (const double *)M->x, (const double *)M->y, f,
         (const double *)M->b, (const ShapeTables *)M->tab, rtab, p, (const int32_t *)M->inc_pos, (T *)M->EB,
         (T *)M->RE);
 }
@@ -1511,7 +1511,7 @@ size_t element_per_el_bytes(int nod, int nip, bool fp32) {
 }
 
 template <int NOD, typename T>
-hipError_t launch_element_t(int nele, int nip, int epb, bool full, const int32_t *conn, const double *x,
+hipError_t launch_element_t(int nele, int nip, int epb, int nthr, bool full, const int32_t *conn, const double *x,
                             const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
                             const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly,
                             void *EB, void *RE, int lb0, int lb1, hipStream_t s) {
@@ -1531,17 +1531,17 @@ hipError_t launch_element_t(int nele, int nip, int epb, bool full, const int32_t
                      : (full && can_full ? k_element<NOD, T, false, can_full, true>
                                          : k_element<NOD, T, false, false, true>);
     lds_attr(reinterpret_cast<const void *>(kern), lds);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(nthr), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
                        static_cast<T *>(EB), static_cast<T *>(RE), lb0, lb1);
     return hipGetLastError();
 }
 }  // namespace
 
-// Elements per 256-thread block: every stage maps at most one item per thread and the LDS
-// (shape tables + the larger of nodal staging and coefficient records) stays within
+// Elements per block of `nthr` threads: every stage maps at most one item per thread and the
+// LDS (shape tables + the larger of nodal staging and coefficient records) stays within
 // `lds_cap` bytes (at most 160 KiB, the gfx950 workgroup limit).
-int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap) {
-    int epb = std::min(256 / nip, 256 / nod);
+int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap, int nthr) {
+    int epb = std::min(nthr / nip, nthr / nod);
     const size_t tables = element_table_bytes(nod, nip, fp32);
     if (lds_cap <= tables) return 0;
     epb = std::min<int>(epb, int((lds_cap - tables) / element_per_el_bytes(nod, nip, fp32)));
@@ -1551,8 +1551,8 @@ int element_epb_max(int nod, int nip, bool fp32, size_t lds_cap) {
 // FP64 tiles of 3- and 6-node elements under 52 KiB, so that three workgroups share a CU (the
 // register budget allows three waves per SIMD): 5 % faster on 6-node meshes than two
 // workgroups of 80 KiB; 10-node and FP32 tiles keep the larger cap (measured).
-int element_epb_default(int nod, int nip, bool fp32) {
-    return element_epb_max(nod, nip, fp32, (!fp32 && nod <= 6) ? kElementLdsSmall : kElementLds);
+int element_epb_default(int nod, int nip, bool fp32, int nthr) {
+    return element_epb_max(nod, nip, fp32, (!fp32 && nod <= 6) ? kElementLdsSmall : kElementLds, nthr);
 }
 
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, int *status, hipStream_t s) {
@@ -1566,17 +1566,18 @@ hipError_t launch_nodal_s(int nn, const double *sfc, const double *h, double *b,
 }
 
 // Element tiles lb0 .. lb1-1 (tiles of epb elements); lb1 < 0 means all tiles.
-hipError_t launch_element(int nod, int nele, int nip, int epb, bool fp32, bool full, const int32_t *conn,
+hipError_t launch_element(int nod, int nele, int nip, int epb, int nthr, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
                           const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, hipStream_t s) {
-    if (epb < 1 || epb * nip > 256 || epb * nod > 256) return hipErrorInvalidConfiguration;
+    if (nthr < 64 || nthr > 256 || nthr % 64 != 0) return hipErrorInvalidConfiguration;
+    if (epb < 1 || epb * nip > nthr || epb * nod > nthr) return hipErrorInvalidConfiguration;
     const int ntiles = (nele + epb - 1) / epb;
     if (lb1 < 0 || lb1 > ntiles) lb1 = ntiles;
     if (lb0 < 0) lb0 = 0;
 #define UVH_EL(NODV)                                                                                           \
-    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s) \
-                : launch_element_t<NODV, double>(nele, nip, epb, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s)
+    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, nthr, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s) \
+                : launch_element_t<NODV, double>(nele, nip, epb, nthr, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s)
     switch (nod) {
         case 3: UVH_EL(3);
         case 6: UVH_EL(6);
