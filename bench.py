@@ -105,7 +105,8 @@ def main(argv=None):
                     help='with --checkpoint: the Newton direction only, or rLineminUa\'s "-Auto-" with the '
                          "Cauchy step with the mass matrix")
     ap.add_argument("--transient-start", action="store_true",
-                    help="with --newton: a uv solve first, then the uvh solve from that state (Ua2D.m:598-611)")
+                    help="with --newton: a uv solve first, then the uvh solve from that state (Ua2D.m:598-611); "
+                         "with --fp32 the uvh solve again in the precision form, with its u, v, h errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -136,7 +137,8 @@ def main(argv=None):
         elif args.uv:
             line = bench_modes.bench_uv(args.steps, args.warmup, cpu_seconds=args.cpu_seconds, log=log)
         elif args.transient_start:
-            line = bench_modes.bench_newton_transient(args.workload, device_solver=args.device_solver, log=log)
+            line = bench_modes.bench_newton_transient(args.workload, device_solver=args.device_solver, log=log,
+                                                      precision=args.fp32)
         elif args.checkpoint:
             line = bench_modes.bench_newton_long(args.workload, args.time_limit, args.checkpoint,
                                                  "-Auto-" if args.line_search == "auto" else "-Newton Step-", log=log)

@@ -290,15 +290,12 @@ def bench_newton(workload: str = "newton_p2", device_solver: bool = False, log=p
     }
 
 
-def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = True, log=print) -> dict:
+def transient_start_state(workload: str = "plain_p2", device_solver: bool = True, log=print) -> dict:
     """The start of a transient run as Úa makes it (Ua2D.m:598-611): a uv (diagnostic) Newton
-    solve on F1's geometry with the GPU uv assembly, then the implicit uvh Newton solve
-    (SSTREAM_TransientImplicit.m) from that balanced state with the GPU assembly; with
-    ``device_solver`` both take the device block-ILU(0) Krylov solve (the uv steps on the uvh
-    solver with the thickness held), else host SuperLU; to Úa's default tolerance rForce < 1e-15.
-    FP64 throughout."""
+    solve on F1's geometry with the GPU uv assembly; its velocities become F0's and F1's.
+    Returns the mesh, the Dirichlet set, CtrlVar, the balanced (G0, G1), the uv result and its
+    time."""
     from uasource_amd import newton
-    from uasource_amd.linsolve import GpuLinearSolver
     w = WORKLOADS[workload]
     M = build_mesh(w)
     F0, F1, BCs, fixed = newton_problem(M, w)
@@ -306,22 +303,65 @@ def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = Tru
     ctx = uvh.UvhContext(0)
     t0 = time.perf_counter()
     log(f"transient start on {workload} ({M.Nele} elements): uv solve ({2 * M.Nnodes} unknowns)")
-    G0, G1, ruv = newton.transient_start(cv, M, F0, F1, fixed,
-                                         lambda c, m, F: newton.GpuUvAssembler(c, m, F, ctx=ctx), verbose=True, log=log,
-                                         device_solver=device_solver)
+    try:
+        G0, G1, ruv = newton.transient_start(cv, M, F0, F1, fixed,
+                                             lambda c, m, F: newton.GpuUvAssembler(c, m, F, ctx=ctx), verbose=True,
+                                             log=log, device_solver=device_solver)
+    finally:
+        ctx.close()
     t_uv = time.perf_counter() - t0
     log(f"uv: {ruv.iterations} iterations, {ruv.reason}, rForce {ruv.rForce:.3e}, {t_uv:.1f} s")
+    return {"M": M, "fixed": fixed, "cv": cv, "G0": G0, "G1": G1, "uv": ruv, "t_uv": t_uv, "workload": workload}
+
+
+def uvh_solve(start: dict, fp32: bool = False, device_solver: bool = True, log=print):
+    """The implicit uvh Newton solve (SSTREAM_TransientImplicit.m) from a transient start's
+    balanced state, FP64 or in the precision form; returns (NewtonResult, seconds)."""
+    from uasource_amd import newton
+    from uasource_amd.linsolve import GpuLinearSolver
+    cv, fixed = start["cv"], start["fixed"]
     t1 = time.perf_counter()
-    asm = newton.GpuAssembler(cv, M, G0, G1, ctx=ctx)
+    asm = newton.GpuAssembler(cv, start["M"], start["G0"], start["G1"])
     solver = GpuLinearSolver(asm.ctx, fixed, rtol=cv.LinSolveTol) if device_solver else newton.superlu_solve
-    res = newton.solve(asm, cv, linear_solver=solver, fixed=fixed, verbose=True, log=log)
-    t_uvh = time.perf_counter() - t1
-    log(f"uvh: {res.iterations} iterations, {res.reason}, rForce {res.rForce:.3e}, {t_uvh:.1f} s")
-    if device_solver:
-        solver.close()
-    ctx.close()
+    try:
+        res = newton.solve(asm, cv, linear_solver=solver, fixed=fixed, verbose=True, log=log, fp32=fp32)
+    finally:
+        if device_solver:
+            solver.close()
+        asm.ctx.close()
+    t = time.perf_counter() - t1
+    log(f"uvh{' (precision form)' if fp32 else ''}: {res.iterations} iterations, {res.reason}, "
+        f"rForce {res.rForce:.3e}, {t:.1f} s")
+    return res, t
+
+
+def solution_errors(a, b) -> dict:
+    """Relative errors of the converged u, v, h of ``a`` against ``b``: normwise, and the
+    largest entry over the largest magnitude; u and v against the speed scale of (u, v) (v is
+    near zero in these problems, so its own norm says little), h against h."""
+    speed = np.linalg.norm(np.concatenate([b.ub, b.vb]))
+    smax = max(np.max(np.abs(b.ub)), np.max(np.abs(b.vb)))
+    out = {}
+    for k in ("ub", "vb", "h"):
+        x, y = getattr(a, k), getattr(b, k)
+        n, m = (speed, smax) if k != "h" else (np.linalg.norm(y), np.max(np.abs(y)))
+        out[k] = {"normwise": float(np.linalg.norm(x - y) / n), "max": float(np.max(np.abs(x - y)) / m)}
+    return out
+
+
+def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = True, log=print,
+                           precision: bool = False) -> dict:
+    """The transient start (``transient_start_state``), then the implicit uvh Newton solve from
+    that balanced state with the GPU assembly; with ``device_solver`` both take the device
+    block-ILU(0) Krylov solve (the uv steps on the uvh solver with the thickness held), else host
+    SuperLU; to Úa's default tolerance rForce < 1e-15. FP64; with ``precision`` the uvh solve is
+    repeated in the precision form from the same start, and its converged u, v, h are compared
+    with the FP64 ones."""
+    start = transient_start_state(workload, device_solver, log)
+    ruv, t_uv, M = start["uv"], start["t_uv"], start["M"]
+    res, t_uvh = uvh_solve(start, False, device_solver, log)
     hist = lambda r: [[float(v) for v in h[:4]] for h in r.history]  # noqa: E731
-    return {
+    out = {
         "metric": "uvh Newton solve to rForce < 1e-15 (transient start)", "value": t_uv + t_uvh, "unit": "s",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": (t_uv + t_uvh) * 1e3, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, inflow conditions)",
@@ -335,6 +375,13 @@ def bench_newton_transient(workload: str = "plain_p2", device_solver: bool = Tru
                 "linesearch_s": res.t_linesearch, "history": hist(res),
                 "krylov_iterations": res.linear_iterations if device_solver else None},
     }
+    if precision:
+        r32, t32 = uvh_solve(start, True, device_solver, log)
+        out["uvh_precision_form"] = {"converged": bool(r32.converged), "iterations": int(r32.iterations),
+                                     "rForce": float(r32.rForce), "reason": r32.reason, "seconds": t32,
+                                     "history": [[float(v) for v in h[:4]] + [bool(h[4])] for h in r32.history],
+                                     "errors_vs_fp64": solution_errors(r32, res)}
+    return out
 
 
 def bench_newton_long(workload: str, time_limit: float, checkpoint: str, line_search: str = "-Newton Step-",

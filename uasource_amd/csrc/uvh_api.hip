@@ -124,6 +124,7 @@ struct uvh_ctx {
     // threads per element block (UVH_ELEMENT_THREADS=64/128/192: one- to three-wave tiles, each
     // with as many elements as its stages fit; measured slower than 256 on 6- and 10-node meshes)
     int el_threads[2] = {256, 256};
+    int epb_override = 0;   // UVH_EPB: elements per element block, where the tile fits
     int bepb[2] = {0, 0};   // elements per 256-thread block as a batch member (uvhb_set_tile)
     // one-pass element rows (all nine blocks per thread) for 3- and 6-node elements: faster with
     // FP32 arithmetic; with FP64 the three-pass rows keep the registers for two waves per SIMD.
@@ -384,6 +385,8 @@ int uvh_create(uvh_ctx **out, int device) {
     const char *et = std::getenv("UVH_ELEMENT_THREADS");
     const int nt = et ? std::atoi(et) : 0;
     if (nt == 64 || nt == 128 || nt == 192 || nt == 256) ctx->el_threads[0] = ctx->el_threads[1] = nt;
+    const char *eo = std::getenv("UVH_EPB");
+    ctx->epb_override = eo ? std::atoi(eo) : 0;
     const char *ch = std::getenv("UVH_CHUNKS");
     ctx->nchunks = ch ? std::max(1, std::min(64, std::atoi(ch))) : kDefaultChunks;
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
@@ -582,6 +585,10 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
     }
     for (int t = 0; t < 2; ++t) {
         ctx->epb[t] = uvh::element_epb_default(nod, nip, t == 1, ctx->el_threads[t]);
+        const int o = ctx->epb_override;
+        if (o > 0 && o * nip <= ctx->el_threads[t] && o * nod <= ctx->el_threads[t] &&
+            uvh::element_lds_bytes(nod, nip, o, t == 1) <= 156 * 1024)
+            ctx->epb[t] = o;
         ctx->bepb[t] = uvh::element_epb_default(nod, nip, t == 1, 256);
     }
     ctx->fused_ok = fused;

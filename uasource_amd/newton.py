@@ -30,11 +30,14 @@ solve is either on the host (SciPy's SuperLU, the default, as Úa's direct ``sol
 branches) or, with a ``linsolve.GpuLinearSolver``, on the device (``solveKApe``'s iterative
 branch: block ILU(0) and BiCGStab), in which case K and R never leave HBM.
 
-Precision form: with ``fp32=True`` the Jacobian is always assembled with FP32 element
-arithmetic, and so is the residual while ``rForce`` is above ``fp64_switch`` and the FP32 line
-search still reduces it; after that the residual (and with it the step's right-hand side, the
-line search and the convergence test) is FP64, so the iteration converges to the FP64 solution
-and the final convergence check is FP64.
+Precision form: with ``fp32=True`` the Jacobian and the residual are assembled with FP32
+element arithmetic while ``rForce`` is above ``fp64_switch``, the FP32 line search still
+reduces it and no two steps in a row are shorter than 0.25; after that both are FP64 (the step's right-hand side, the line search and the
+convergence test with the residual), so the iteration ends with FP64 Newton steps, converges to
+the FP64 solution and makes its final convergence check in FP64. (An FP32 Jacobian kept to the
+end converges, linearly, on small meshes, but on the 199,712-element config-2 mesh its steps
+stall at gamma ~0.2 with rForce ~1e-7: the Jacobian's FP32 error is too large a part of the
+ill-conditioned system's steps.)
 """
 from __future__ import annotations
 
@@ -789,12 +792,14 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
     rForce, rWork = float("inf"), float("nan")
     residual64 = not fp32
     ratios = np.zeros(3)
+    short32 = 0
     backtrack_ok = True
     if resume is not None:
         iteration, gamma, r, r0 = int(resume["iteration"]), float(resume["gamma"]), float(resume["r"]), float(resume["r0"])
         rForce, rWork = float(resume["rForce"]), float(resume["rWork"])
         residual64, backtrack_ok = bool(resume["residual64"]), bool(resume["backtrack_ok"])
         ratios = np.array(resume["ratios"], dtype=float)
+        short32 = int(resume.get("short32", 0))
         res.history = [tuple(h) for h in resume["history"]]
         assembler.set_state(resume["x"])
         if L is not None:
@@ -834,25 +839,26 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
         if time_limit is not None and done_here > 0 and time.perf_counter() - t_start > time_limit:
             res.reason = "time limit"
             res.checkpoint = dict(iteration=iteration, gamma=gamma, r=r, r0=r0, rForce=rForce, rWork=rWork,
-                                  residual64=residual64, backtrack_ok=backtrack_ok, ratios=ratios.copy(),
+                                  residual64=residual64, backtrack_ok=backtrack_ok, ratios=ratios.copy(), short32=short32,
                                   history=list(res.history), x=np.concatenate(assembler.state()))
             if L is not None:
                 res.checkpoint["luvh"] = lam.copy()
             break
         iteration += 1
         done_here += 1
-        if fp32 and not residual64 and rForce < fp64_switch:
-            residual64 = True   # from here on the residual, the step and the exit test are FP64
-        fp32_res = fp32 and not residual64
+        if fp32 and not residual64 and (rForce < fp64_switch or short32 >= 2):
+            # from here on the residual, the Jacobian, the step and the exit test are FP64
+            residual64 = True
+        fp32_res = fp32 and not residual64   # the Jacobian too: FP32 only while the residual is
         t0 = time.perf_counter()
         if on_device:
-            assembler.assemble_rk_device(fp32, fp32_res)
+            assembler.assemble_rk_device(fp32_res, fp32_res)
             t1 = time.perf_counter()
             assembler.step_on_device(linear_solver)
             t2 = time.perf_counter()
             res.linear_iterations.append(linear_solver.last_iterations)
         elif L is not None:
-            K, rhs = assembler.assemble_rk(fp32, fp32_res)
+            K, rhs = assembler.assemble_rk(fp32_res, fp32_res)
             t1 = time.perf_counter()
             n3 = K.shape[0]
             G0 = cuvh - L @ np.concatenate(assembler.state())
@@ -864,7 +870,7 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
             assembler.set_shift(LT @ lam, LT @ dl)
             G1 = L @ d
         else:
-            K, rhs = assembler.assemble_rk(fp32, fp32_res)
+            K, rhs = assembler.assemble_rk(fp32_res, fp32_res)
             t1 = time.perf_counter()
             d = linear_solver(K, rhs)
             t2 = time.perf_counter()
@@ -923,6 +929,9 @@ def solve(assembler, CtrlVar, linear_solver=superlu_solve, fp32: bool = False, f
                         res.cauchy_steps += 1
             assembler.set_step(chosen)
         backtrack_ok = gamma > 0
+        # two FP32 steps in a row shorter than 0.25: the FP32 Jacobian no longer gives Newton-
+        # quality steps, and the next iteration switches to FP64
+        short32 = short32 + 1 if (fp32_res and gamma < 0.25) else 0
         if gamma > 0:
             assembler.accept(step)
             if L is not None:
