@@ -828,32 +828,24 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     double *buf = lds + NBK * NBK * (rp - rowptr_n[k0]);
     const int t0 = inc_ptr[k], t1 = inc_ptr[k + 1];
     const int deg = (int)(rowptr_n[k + 1] - rp);
+    const int dg = deg > 0 ? diag[k] : 0;   // loaded now, used after the sums
     int bad = 0;
-    if (gl < 2 * NBK) {
-        // lane 2*bi: T of block row bi, lane 2*bi+1: F
-        double v = 0;
-        for (int t = t0; t < t1; ++t) v += double(RE[(int64_t)t * 6 + gl]);
-        const int bi = gl >> 1;
-        if (gl & 1) {
-            if (Fext) Fext[(int64_t)bi * nn + k] = v;
-        } else {
-            if (Tint) Tint[(int64_t)bi * nn + k] = v;
-        }
-        const double other = __shfl_xor(v, 1, 64);
-        if (!(gl & 1)) {
-            const double r = v - other;
-            R[(int64_t)bi * nn + k] = r;
-            if (check_nan && isnan(r)) bad |= 1;
-        }
-    }
+    // lane 2*bi (< 2 NBK): T of block row bi, lane 2*bi+1: F, summed in ascending element order;
+    // their loads go out with the element rows of the same chunk, so that the node's residual
+    // rows cost no round trips of their own
+    const bool tf_lane = gl < 2 * NBK;
+    double tf = 0.0;
     for (int v = gl; v < NBK * NBK * deg; v += kRowLanes) buf[v] = 0.0;
-    // the node's element rows t0..t1-1 are contiguous (incidence order); up to 4 at a time:
+    // the node's element rows t0..t1-1 are contiguous (incidence order); CH at a time:
     // all loads first, then the adds in ascending element order. Value u of the NBK^2 x NOD
     // used ones is block (bi, bj) = (u / (NBK NOD), u / NOD mod NBK), column j = u mod NOD,
     // at (3 bi + bj) NOD + j of the element row
     constexpr int NV = NBK * NBK * NOD;                   // values used per element row
     constexpr int VPL = (NV + kRowLanes - 1) / kRowLanes; // values per lane
-    constexpr int CH = 4;
+    // element rows loaded per round: 4 for 3- and 6-node elements, 2 for 10-node elements
+    // (measured on 400k-element P1, 200k P2 and 500k P3 meshes: P1 and P2 rows within 2 %
+    // for 2-4; P3 rows 1.38 ms with 2 against 1.60 ms with 4 and 2.77 ms with 8)
+    constexpr int CH = NOD >= 10 ? 2 : 4;
     constexpr int PST = pos_inc_stride(NOD);
     if constexpr (NBK == 3 && sizeof(TE) == sizeof(double) && (9 * NOD) % 2 == 0) {
         // FP64 uvh rows of 6- and 10-node elements are 16-byte aligned (54 or 90 values): each
@@ -865,6 +857,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
             const int cnt = min(CH, t1 - c0);
             double2 vals[CH][VP2];
             uint64_t cw[CH][PW];
+            double re[CH];
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
                 if (c < cnt) {
@@ -875,8 +868,12 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                     const uint64_t *pw = reinterpret_cast<const uint64_t *>(pos + (int64_t)(c0 + c) * PST);
 #pragma unroll
                     for (int w = 0; w < PW; ++w) cw[c][w] = pw[w];
+                    if (tf_lane) re[c] = double(RE[(int64_t)(c0 + c) * 6 + gl]);
                 }
             }
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+                if (c < cnt && tf_lane) tf += re[c];
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
                 if (c >= cnt) continue;
@@ -900,6 +897,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
         const int cnt = min(CH, t1 - c0);
         TE vals[CH][VPL];
         int cols[CH][VPL];
+        double re[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
 #pragma unroll
@@ -912,7 +910,11 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                     cols[c][u] = pos[(int64_t)(c0 + c) * PST + j];
                 }
             }
+            if (c < cnt && tf_lane) re[c] = double(RE[(int64_t)(c0 + c) * 6 + gl]);
         }
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (c < cnt && tf_lane) tf += re[c];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
 #pragma unroll
@@ -926,6 +928,20 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
             }
         }
     }
+    if (tf_lane) {
+        const int bi = gl >> 1;
+        if (gl & 1) {
+            if (Fext) Fext[(int64_t)bi * nn + k] = tf;
+        } else {
+            if (Tint) Tint[(int64_t)bi * nn + k] = tf;
+        }
+        const double other = __shfl_xor(tf, 1, 64);
+        if (!(gl & 1)) {
+            const double r = tf - other;
+            R[(int64_t)bi * nn + k] = r;
+            if (check_nan && isnan(r)) bad |= 1;
+        }
+    }
 #pragma unroll
     for (int bi = 0; bi < NBK; ++bi) {
         double *dst = Kval + (int64_t)bi * NBK * nnzN + NBK * rp;
@@ -933,7 +949,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
         for (int v = gl; v < NBK * deg; v += kRowLanes) dst[v] = src[v];
     }
     // a node no element references has an empty row (R = 0, no diagonal to check)
-    if (check_nan && deg > 0 && gl < NBK && isnan(buf[gl * NBK * deg + gl * deg + diag[k]])) bad |= 2;
+    if (check_nan && deg > 0 && gl < NBK && isnan(buf[gl * NBK * deg + gl * deg + dg])) bad |= 2;
     if (bad) atomicOr(status, bad);
 }
 
