@@ -1029,6 +1029,35 @@ __global__ __launch_bounds__(256) void k_rows_gather(int nn, int64_t nnzN, const
     if (bad) atomicOr(status, bad);
 }
 
+// T and F of a node's block rows from its incident element residual rows t0 .. t1-1
+// (contiguous, 6 values each), added in ascending element order; the rows are loaded four at a
+// time before they are added, so that a node costs one memory round trip per four elements.
+template <typename TE>
+__device__ __forceinline__ void sum_residual_rows(const TE *__restrict__ RE, int t0, int t1, int nbk, double (&T)[3],
+                                                  double (&F)[3]) {
+    constexpr int CH = 4;
+    for (int c0 = t0; c0 < t1; c0 += CH) {
+        const int cnt = min(CH, t1 - c0);
+        TE v[CH][6];
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (c < cnt) {
+#pragma unroll
+                for (int x = 0; x < 6; ++x) v[c][x] = RE[(int64_t)(c0 + c) * 6 + x];
+            }
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (c < cnt) {
+#pragma unroll
+                for (int bi = 0; bi < 3; ++bi)
+                    if (bi < nbk) {
+                        T[bi] += double(v[c][2 * bi]);
+                        F[bi] += double(v[c][2 * bi + 1]);
+                    }
+            }
+    }
+}
+
 // Residual only: one thread per node sums its incident element residual rows (contiguous).
 template <typename TE>
 __global__ __launch_bounds__(256) void k_rows_r(int nn, int nbk, const int32_t *__restrict__ inc_ptr,
@@ -1039,15 +1068,7 @@ __global__ __launch_bounds__(256) void k_rows_r(int nn, int nbk, const int32_t *
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nn) return;
     double T[3] = {0, 0, 0}, F[3] = {0, 0, 0};
-    for (int t = inc_ptr[k]; t < inc_ptr[k + 1]; ++t) {
-        const TE *re = RE + (int64_t)t * 6;
-#pragma unroll
-        for (int bi = 0; bi < 3; ++bi)
-            if (bi < nbk) {
-                T[bi] += double(re[2 * bi]);
-                F[bi] += double(re[2 * bi + 1]);
-            }
-    }
+    sum_residual_rows(RE, inc_ptr[k], inc_ptr[k + 1], nbk, T, F);
     bool bad = false;
 #pragma unroll
     for (int bi = 0; bi < 3; ++bi)
@@ -1246,12 +1267,9 @@ __global__ __launch_bounds__(256) void k_rows_r_batch(const BatchMember *__restr
     const int k = w.y * 256 + threadIdx.x;
     if (k >= M.nn) return;
     const TE *RE = static_cast<const TE *>(M.RE);
-    double T0 = 0, T1 = 0, T2 = 0, F0 = 0, F1 = 0, F2 = 0;
-    for (int t = M.inc_ptr[k]; t < M.inc_ptr[k + 1]; ++t) {
-        const TE *re = RE + (int64_t)t * 6;
-        T0 += double(re[0]); F0 += double(re[1]); T1 += double(re[2]);
-        F1 += double(re[3]); T2 += double(re[4]); F2 += double(re[5]);
-    }
+    double T[3] = {0, 0, 0}, F[3] = {0, 0, 0};
+    sum_residual_rows(RE, M.inc_ptr[k], M.inc_ptr[k + 1], 3, T, F);
+    const double T0 = T[0], T1 = T[1], T2 = T[2], F0 = F[0], F1 = F[1], F2 = F[2];
     const int nn = M.nn;
     const double r0 = T0 - F0, r1 = T1 - F1, r2 = T2 - F2;
     M.R[k] = r0; M.R[nn + k] = r1; M.R[2 * (int64_t)nn + k] = r2;
