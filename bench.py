@@ -91,6 +91,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", action="store_true", help="64-mesh batched assembly")
+    ap.add_argument("--batch-newton", action="store_true", help="batched Newton solves with HIP-graph stages")
     ap.add_argument("--serve", action="store_true", help="request serving, latency over a load sweep")
     ap.add_argument("--newton", action="store_true",
                     help="full Newton solve (newton_p2 with the host solver; --workload with --device-solver)")
@@ -127,10 +128,12 @@ def main(argv=None):
               file=sys.stderr)
         return 2
 
-    if args.batch or args.serve or args.newton or args.uv:
+    if args.batch or args.serve or args.newton or args.uv or args.batch_newton:
         from measure import bench_modes
         log = lambda m: print(m, file=sys.stderr, flush=True)
-        if args.batch:
+        if args.batch_newton:
+            line = bench_modes.bench_batch_newton(log=log)
+        elif args.batch:
             line = bench_modes.bench_batch(args.steps, args.warmup, log=log)
         elif args.serve:
             line = bench_modes.bench_serve(log=log)

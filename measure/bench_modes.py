@@ -122,6 +122,47 @@ def bench_batch(steps: int, warmup: int, n_members: int = 64, log=print) -> dict
     }
 
 
+def bench_batch_newton(n_members: int = 16, log=print) -> dict:
+    """Batched Newton solves (``uasource_amd.batch_newton``): ``n_members`` independent uvh
+    problems (the Newton workload recipe on 100 km x 50 km P1/P2 meshes of 2k-7k elements,
+    reseeded per member), run to rForce < 1e-15 with the assembly and cost stages replayed
+    from HIP graphs and, for comparison, eagerly; the two runs give the same iterates bit for
+    bit (``tests/test_gpu_batch_newton.py``)."""
+    import dataclasses
+    from uasource_amd.batch_newton import BatchNewton
+    base = [WORKLOADS["cpu_reference"], WORKLOADS["medium_p2"]]
+    mem = []
+    for k in range(n_members):
+        w = dataclasses.replace(base[k % 2], name=f"member{k}", seed=100 + k)
+        M = build_mesh(w)
+        F0, F1, _, fixed = newton_problem(M, w)
+        mem.append((M, F0, F1, fixed))
+    cv = default_ctrlvar()
+    out = {}
+    for graphs in (True, False, True):
+        bn = BatchNewton(cv, mem, graphs=graphs)
+        t = time.perf_counter()
+        res = bn.solve()
+        dt = time.perf_counter() - t
+        replays = bn.n_graph_replays
+        bn.close()
+        key = "graphs" if graphs else "eager"
+        out[key] = {"seconds": dt, "graph_replays": replays, "iterations": [r.iterations for r in res],
+                    "converged": all(r.converged for r in res), "max_rForce": max(r.rForce for r in res)}
+        log(f"batch newton ({key}): {dt:.2f} s, iterations {out[key]['iterations']}, replays {replays}")
+    nele = sum(m[0].Nele for m in mem)
+    return {
+        "metric": "batched Newton uvh solves to rForce < 1e-15 (HIP-graph stages)", "value": out["graphs"]["seconds"],
+        "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": out["graphs"]["seconds"] * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded meshes, inflow conditions)",
+        "config": {"workload": "batch_newton", "members": n_members, "Nele_total": nele,
+                   "linear_solver": "block ILU(0) Krylov (GPU), per member", "parallelism": "1 GPU"},
+        "graphs": out["graphs"], "eager": out["eager"],
+        "note": "the first graphs run includes capture; the value is the second graphs run",
+    }
+
+
 def bench_serve(n_requests: int = 500, n_meshes: int = 24, slots: int = 8, log=print) -> dict:
     """500 R/K requests (Newton step : line search = 1:3) over meshes of 2k-300k elements
     (lognormal, P1/P2/P3 = 50/35/15 %), Poisson arrivals at fractions of the saturated rate."""
