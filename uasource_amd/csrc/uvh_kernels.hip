@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -449,46 +450,63 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
 }
 
 // Gather of node a of element e into the LDS staging record nv[NV_COUNT][NOD].
-template <int NOD>
+template <int NOD, bool SL = true>
 __device__ __forceinline__ void gather_node_at(int k, int a, const double *__restrict__ xn,
                                                const double *__restrict__ yn, const FieldPtrs &f,
                                                const double *__restrict__ bnod, const Params &p,
                                                double *__restrict__ nv);
 
-template <int NOD>
+template <int NOD, bool SL>
 __device__ __forceinline__ void gather_node(const int32_t *__restrict__ conn, const double *__restrict__ xn,
                                             const double *__restrict__ yn, const FieldPtrs &f,
                                             const double *__restrict__ bnod, const Params &p, int64_t e, int a,
                                             double *__restrict__ nv) {
-    gather_node_at<NOD>(conn[e * NOD + a], a, xn, yn, f, bnod, p, nv);
+    gather_node_at<NOD, SL>(conn[e * NOD + a], a, xn, yn, f, bnod, p, nv);
 }
 
 // Gather of node k, local node a of its element, into the LDS staging record nv[NV_COUNT][NOD].
-template <int NOD>
+// Every load is issued before any loaded value is used or any branch is taken, so the gather
+// costs one memory round trip: an optional field that is absent (a null pointer) is read
+// through the nodal b instead and replaced by zero afterwards. q, muk and V0 are read only
+// where the sliding laws other than Weertman are compiled in (SL).
+template <int NOD, bool SL>
 __device__ __forceinline__ void gather_node_at(int k, int a, const double *__restrict__ xn,
                                                const double *__restrict__ yn, const FieldPtrs &f,
                                                const double *__restrict__ bnod, const Params &p,
                                                double *__restrict__ nv) {
+    const double *pd = f.dadh ? f.dadh : bnod;
     double h = f.h[k], u = f.u[k], v = f.v[k], u0 = f.u0[k], v0 = f.v0[k];
     double as1 = f.as1[k], ab1 = f.ab1[k], as0 = f.as0[k], ab0 = f.ab0[k];
-    const double h0 = f.h0[k];
+    const double h0 = f.h0[k], S = f.S[k], B = f.B[k], rho = f.rho[k], C = f.C[k], m = f.m[k], n = f.n[k];
+    const double A = f.A[k], bl = bnod[k], x = xn[k], y = yn[k];
+    double dadh = pd[k], q = 0.0, muk = 0.0, V0 = 0.0;
+    if constexpr (SL) {
+        const double *pq = f.q ? f.q : bnod, *pm = f.muk ? f.muk : bnod, *pv = f.V0 ? f.V0 : bnod;
+        q = pq[k]; muk = pm[k]; V0 = pv[k];
+        if (!f.q) q = 0.0;
+        if (!f.muk) muk = 0.0;
+        if (!f.V0) V0 = 0.0;
+    }
+    if (!f.dadh) dadh = 0.0;
     if (p.zero_fields) {  // uvhMatrixAssembly.m:73-87
         u = 0.0; v = 0.0; u0 = 0.0; v0 = 0.0; h = h0;
         as1 = fabs(ab1) + 1.0; ab1 = fabs(ab1);
         as0 = fabs(ab0) + 1.0; ab0 = fabs(ab0);
     }
     nv[NV_H * NOD + a] = h; nv[NV_U * NOD + a] = u; nv[NV_V * NOD + a] = v;
-    nv[NV_S * NOD + a] = f.S[k]; nv[NV_B * NOD + a] = f.B[k]; nv[NV_RHO * NOD + a] = f.rho[k];
-    nv[NV_C * NOD + a] = f.C[k]; nv[NV_M * NOD + a] = f.m[k]; nv[NV_N * NOD + a] = f.n[k];
-    nv[NV_A * NOD + a] = f.A[k]; nv[NV_AS1 * NOD + a] = as1; nv[NV_AB1 * NOD + a] = ab1;
+    nv[NV_S * NOD + a] = S; nv[NV_B * NOD + a] = B; nv[NV_RHO * NOD + a] = rho;
+    nv[NV_C * NOD + a] = C; nv[NV_M * NOD + a] = m; nv[NV_N * NOD + a] = n;
+    nv[NV_A * NOD + a] = A; nv[NV_AS1 * NOD + a] = as1; nv[NV_AB1 * NOD + a] = ab1;
     nv[NV_H0 * NOD + a] = h0; nv[NV_U0 * NOD + a] = u0; nv[NV_V0 * NOD + a] = v0;
     nv[NV_AS0 * NOD + a] = as0; nv[NV_AB0 * NOD + a] = ab0;
-    nv[NV_DADH * NOD + a] = f.dadh ? f.dadh[k] : 0.0;
-    nv[NV_BL * NOD + a] = bnod[k];
-    nv[NV_X * NOD + a] = xn[k]; nv[NV_Y * NOD + a] = yn[k];
-    nv[NV_Q * NOD + a] = f.q ? f.q[k] : 0.0;
-    nv[NV_MUK * NOD + a] = f.muk ? f.muk[k] : 0.0;
-    nv[NV_JV0 * NOD + a] = f.V0 ? f.V0[k] : 0.0;
+    nv[NV_DADH * NOD + a] = dadh;
+    nv[NV_BL * NOD + a] = bl;
+    nv[NV_X * NOD + a] = x; nv[NV_Y * NOD + a] = y;
+    if constexpr (SL) {
+        nv[NV_Q * NOD + a] = q;
+        nv[NV_MUK * NOD + a] = muk;
+        nv[NV_JV0 * NOD + a] = V0;
+    }
 }
 
 // Row i, block row BI (0: x, 1: y, 2: h) of the element: T and F of that row and its
@@ -698,28 +716,45 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     double *work = lds + ttab + ((row_table_doubles<T>(tstride) + 1) & ~1);  // 16-byte aligned union:
                                             // nodal [epb][NV_COUNT][NOD] / coef [epb][nip][kNC]
     const int tid = threadIdx.x;
+    // The shape-table entry of this thread is loaded before the gathers and stored after them,
+    // so that the tile's global loads take two round trips in all (tables and connectivity,
+    // then the nodal values) instead of one per loop.
+    double vN = 0, vXi = 0, vEta = 0, vW = 0;
+    if (tid < tstride) {
+        const int q = tid / NOD, i = tid - q * NOD;
+        vN = tab->N[q][i]; vXi = tab->dxi[q][i]; vEta = tab->deta[q][i];
+    }
+    if (tid < nip) vW = tab->w[tid];
+    const int64_t e0 = (int64_t)lb * epb;
+
+    // Stages 1 and 2 leave the highest waves of the block idle (epb*NOD and epb*nip < nthr);
+    // which waves those are rotates with the tile, so that the several blocks a CU holds do
+    // not all leave the same SIMD idle. vt is the thread's index within the rotated order.
+    const int nwav = nthr >> 6;
+    const int vt = nwav > 1 ? (tid + 64 * (lb % nwav)) % nthr : tid;
+
+    // stage 1: gathers, one (element, node) per thread
+    if (vt < epb * NOD) {
+        const int el = vt / NOD, a = vt - el * NOD;
+        if (e0 + el < nele) gather_node<NOD, SL>(conn, xn, yn, f, bnod, p, e0 + el, a, work + el * nv_stride<NOD>());
+    }
     for (int k = tid; k < tstride; k += nthr) {
-        const int q = k / NOD, i = k - q * NOD;
-        const double vN = tab->N[q][i], vXi = tab->dxi[q][i], vEta = tab->deta[q][i];
+        if (k != tid) {
+            const int q = k / NOD, i = k - q * NOD;
+            vN = tab->N[q][i]; vXi = tab->dxi[q][i]; vEta = tab->deta[q][i];
+        }
         sN[k] = vN; sXi[k] = vXi; sEta[k] = vEta;
         if (sizeof(T) != sizeof(double)) {
             rN[k] = T(vN); rXi[k] = T(vXi); rEta[k] = T(vEta);
         }
     }
-    for (int k = tid; k < nip; k += nthr) sW[k] = tab->w[k];
-    const int64_t e0 = (int64_t)lb * epb;
-
-    // stage 1: gathers, one (element, node) per thread
-    if (tid < epb * NOD) {
-        const int el = tid / NOD, a = tid - el * NOD;
-        if (e0 + el < nele) gather_node<NOD>(conn, xn, yn, f, bnod, p, e0 + el, a, work + el * nv_stride<NOD>());
-    }
+    for (int k = tid; k < nip; k += nthr) sW[k] = k == tid ? vW : tab->w[k];
     __syncthreads();
 
     // stage 2: physics, one (element, integration point) per thread
     T c[kNC];
-    const int pel = tid / nip, pq = tid - pel * nip;
-    const bool phys = tid < epb * nip && e0 + pel < nele;
+    const int pel = vt / nip, pq = vt - pel * nip;
+    const bool phys = vt < epb * nip && e0 + pel < nele;
     if (phys) ip_physics<NOD, T, SL>(work + pel * nv_stride<NOD>(), sN + pq * NOD, sXi + pq * NOD, sEta + pq * NOD,
                                  sW[pq], p, c);
     __syncthreads();  // the nodal staging is dead from here; the coefficient records reuse it
