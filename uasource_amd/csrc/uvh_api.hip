@@ -19,7 +19,8 @@ hipError_t launch_nodal_s(int nn, const double *sfc, const double *h, double *b,
 hipError_t launch_element(int nod, int nele, int nip, int epb, int nthr, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
-                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, hipStream_t s);
+                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, int *status,
+                          hipStream_t s);
 hipError_t launch_rows(int nod, int nn, int64_t block_vals, bool fp32, const int32_t *inc_ptr, const uint8_t *pos,
                        const int64_t *rowptr_n, const uint8_t *diag, int64_t nnzN, const void *EB, const void *RE,
                        double *R, double *Tint, double *Fext, double *Kval, int check_nan, int *status,
@@ -764,7 +765,11 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
     const uvh::Params p = make_params(*ctrl, flags);
     const uvh::FieldPtrs f = to_ptrs(*df);
     const int check = (flags & UVH_NO_NAN_CHECK) ? 0 : 1;
-    hipError_t e = ctx->timed("nodal_b", s, [&] { return uvh::launch_nodal_b(m.nn, f, p, ctx->d_b, ctx->d_status, s); });
+    // the fused kernel reads b from a nodal pre-pass; the two-pass element kernel computes b in
+    // its gathers and clears the NaN flags itself
+    hipError_t e = hipSuccess;
+    if (fused)
+        e = ctx->timed("nodal_b", s, [&] { return uvh::launch_nodal_b(m.nn, f, p, ctx->d_b, ctx->d_status, s); });
     if (e == hipSuccess && fused) {
         uvh::CallArgs ca{};
         ca.f = f; ca.p = p; ca.R = dR; ca.Tint = dT; ca.Fext = dF; ca.Kval = ronly ? nullptr : dK;
@@ -796,8 +801,8 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
         return ctx->timed(el_name, st, [&] {
             return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32, full, ctx->d_conn,
                                        ctx->d_x, ctx->d_y,
-                                       f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, ctx->d_EB,
-                                       ctx->d_RE, lb0, lb1, st);
+                                       f, nullptr, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, ctx->d_EB,
+                                       ctx->d_RE, lb0, lb1, ctx->d_status, st);
         });
     };
     auto rows = [&](const int32_t *blocks, int nblk, hipStream_t st) {
@@ -875,11 +880,10 @@ int assemble_noalloc_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *
     const bool full = ctx->rows_mode == 1 || (ctx->rows_mode == 0 && fp32);
     const int t = fp32 ? 1 : 0;
     ctx->last_status = status;
-    hipError_t e = uvh::launch_nodal_b(m.nn, f, p, b, status, s);
-    if (e == hipSuccess)
-        e = uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32, full, ctx->d_conn,
-                                ctx->d_x, ctx->d_y, f, b,
-                                ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, EB, RE, 0, -1, s);
+    (void)b;   // b is computed in the element kernel's gathers
+    hipError_t e = uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32, full, ctx->d_conn,
+                                       ctx->d_x, ctx->d_y, f, nullptr,
+                                       ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, EB, RE, 0, -1, status, s);
     if (e == hipSuccess)
         e = uvh::launch_rows(m.nod, m.nn, m.max_block_deg, fp32, ctx->d_inc_ptr, ctx->d_pos_inc, ctx->d_rowptr_n,
                              ctx->d_diag, m.nnzN(), EB, RE, dR, dT, dF, ronly ? nullptr : dK, check, status, nullptr, 0, s);
@@ -999,7 +1003,7 @@ int uv_assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctr
                            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32,
                                                       full, ctx->d_conn,
                                                       ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p,
-                                                      ctx->d_inc_pos, ronly, ctx->d_EB, ctx->d_RE, 0, -1, s);
+                                                      ctx->d_inc_pos, ronly, ctx->d_EB, ctx->d_RE, 0, -1, nullptr, s);
                        });
     if (e == hipSuccess)
         e = ctx->timed(fp32 ? (ronly ? "uv_rows32_r" : "uv_rows32_rk") : (ronly ? "uv_rows_r" : "uv_rows_rk"), s, [&] {

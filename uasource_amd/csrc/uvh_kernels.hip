@@ -60,19 +60,26 @@ constexpr int kNC = C_COUNT + (C_COUNT & 1);  // even, for 16-byte LDS reads
 template <int NOD, typename T>
 constexpr int eb_stride() { return 9 * NOD; }
 
-__global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b, int *__restrict__ status) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0 && status) *status = 0;   // the assembly's NaN flags start clear
-    if (k >= nn) return;
-    double h = p.zero_fields ? f.h0[k] : f.h[k];
+// Lower surface b at a node from floatation (Calc_bs_From_hBS.m:71-79,109), h being the
+// node's thickness (h0 under ZeroFields). Every operation is rounded on its own (no fused
+// multiply-adds), so that the nodal kernels and the element kernels' gathers, which both use
+// it, give the same b bit for bit.
+__device__ __forceinline__ double lower_surface(double h, double S, double B, double rho, const Params &p) {
+#pragma clang fp contract(off)
     if (p.reset_thickness && h < p.thick_min) h = p.thick_min;
-    const double S = f.S[k], B = f.B[k], rho = f.rho[k];
     const double hf = p.rhow * (S - B) / rho;
     const double GF = 0.5 + 0.5 * tanh(p.kH * ((h - hf) - p.Hh0));
     const double bfloat = S - rho * h / p.rhow;
     double bb = GF * B + (1.0 - GF) * bfloat;
     if (p.enforce_b_above && bb < B) bb = B;
-    b[k] = bb;
+    return bb;
+}
+
+__global__ void k_nodal_b(int nn, FieldPtrs f, Params p, double *__restrict__ b, int *__restrict__ status) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0 && status) *status = 0;   // the assembly's NaN flags start clear
+    if (k >= nn) return;
+    b[k] = lower_surface(p.zero_fields ? f.h0[k] : f.h[k], f.S[k], f.B[k], f.rho[k], p);
 }
 
 // uv assembly: the lower surface at the nodes from the given upper surface, b = s - h (the
@@ -474,14 +481,16 @@ __device__ __forceinline__ void gather_node_at(int k, int a, const double *__res
                                                const double *__restrict__ yn, const FieldPtrs &f,
                                                const double *__restrict__ bnod, const Params &p,
                                                double *__restrict__ nv) {
-    const double *pd = f.dadh ? f.dadh : bnod;
+    const double *pb = bnod ? bnod : f.h;   // a valid pointer: b itself or a stand-in
+    const double *pd = f.dadh ? f.dadh : pb;
     double h = f.h[k], u = f.u[k], v = f.v[k], u0 = f.u0[k], v0 = f.v0[k];
     double as1 = f.as1[k], ab1 = f.ab1[k], as0 = f.as0[k], ab0 = f.ab0[k];
     const double h0 = f.h0[k], S = f.S[k], B = f.B[k], rho = f.rho[k], C = f.C[k], m = f.m[k], n = f.n[k];
-    const double A = f.A[k], bl = bnod[k], x = xn[k], y = yn[k];
+    const double A = f.A[k], x = xn[k], y = yn[k];
+    double bl = pb[k];
     double dadh = pd[k], q = 0.0, muk = 0.0, V0 = 0.0;
     if constexpr (SL) {
-        const double *pq = f.q ? f.q : bnod, *pm = f.muk ? f.muk : bnod, *pv = f.V0 ? f.V0 : bnod;
+        const double *pq = f.q ? f.q : pb, *pm = f.muk ? f.muk : pb, *pv = f.V0 ? f.V0 : pb;
         q = pq[k]; muk = pm[k]; V0 = pv[k];
         if (!f.q) q = 0.0;
         if (!f.muk) muk = 0.0;
@@ -493,6 +502,9 @@ __device__ __forceinline__ void gather_node_at(int k, int a, const double *__res
         as1 = fabs(ab1) + 1.0; ab1 = fabs(ab1);
         as0 = fabs(ab0) + 1.0; ab0 = fabs(ab0);
     }
+    // without a nodal b (the uvh assembly), b is computed here from the gathered values, as
+    // k_nodal_b does per node
+    if (!bnod) bl = lower_surface(h, S, B, rho, p);
     nv[NV_H * NOD + a] = h; nv[NV_U * NOD + a] = u; nv[NV_V * NOD + a] = v;
     nv[NV_S * NOD + a] = S; nv[NV_B * NOD + a] = B; nv[NV_RHO * NOD + a] = rho;
     nv[NV_C * NOD + a] = C; nv[NV_M * NOD + a] = m; nv[NV_N * NOD + a] = n;
@@ -825,10 +837,14 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
                                                  const ShapeTables *__restrict__ tab,
                                                  const typename RowTab<T>::type *__restrict__ rtab, Params p,
                                                  const int32_t *__restrict__ inc_pos,
-                                                 T *__restrict__ EB, T *__restrict__ RE, int lb0, int lb1) {
+                                                 T *__restrict__ EB, T *__restrict__ RE, int lb0, int lb1,
+                                                 int *__restrict__ status) {
     extern __shared__ double lds[];
     const int lb = lb0 + xcd_block(blockIdx.x, gridDim.x);   // tiles lb0 .. lb1-1 of this launch
     if (lb >= lb1) return;  // whole block
+    // without a nodal pre-pass (bnod null), the block of the first tile clears the assembly's
+    // NaN flags; the row kernels that set them run after this kernel
+    if (status && lb == 0 && threadIdx.x == 0) *status = 0;
     element_block<NOD, T, RONLY, FULL, SL>(lds, nele, nip, epb, lb, blockDim.x, conn, xn, yn, f, bnod, tab, rtab, p,
                                            inc_pos, EB, RE);
 }
@@ -1193,15 +1209,7 @@ __global__ __launch_bounds__(256) void k_nodal_b_batch(const BatchMember *__rest
     const int k = w.y * 256 + threadIdx.x;
     if (k >= M.nn) return;
     const FieldPtrs &f = M.f;
-    double h = p.zero_fields ? f.h0[k] : f.h[k];
-    if (p.reset_thickness && h < p.thick_min) h = p.thick_min;
-    const double S = f.S[k], B = f.B[k], rho = f.rho[k];
-    const double hf = p.rhow * (S - B) / rho;
-    const double GF = 0.5 + 0.5 * tanh(p.kH * ((h - hf) - p.Hh0));
-    const double bfloat = S - rho * h / p.rhow;
-    double bb = GF * B + (1.0 - GF) * bfloat;
-    if (p.enforce_b_above && bb < B) bb = B;
-    M.b[k] = bb;
+    M.b[k] = lower_surface(p.zero_fields ? f.h0[k] : f.h[k], f.S[k], f.B[k], f.rho[k], p);
 }
 
 // items: (member, element block of the member's epb elements), all members with NOD nodes
@@ -1583,7 +1591,7 @@ template <int NOD, typename T>
 hipError_t launch_element_t(int nele, int nip, int epb, int nthr, bool full, const int32_t *conn, const double *x,
                             const double *y, const FieldPtrs &f, const double *b, const ShapeTables *tab,
                             const ShapeTables32 *tab32, const Params &p, const int32_t *inc_pos, bool ronly,
-                            void *EB, void *RE, int lb0, int lb1, hipStream_t s) {
+                            void *EB, void *RE, int lb0, int lb1, int *status, hipStream_t s) {
     const typename RowTab<T>::type *rtab;
     if constexpr (sizeof(T) == sizeof(double)) rtab = tab;
     else rtab = tab32;
@@ -1601,7 +1609,7 @@ hipError_t launch_element_t(int nele, int nip, int epb, int nthr, bool full, con
                                          : k_element<NOD, T, false, false, true>);
     lds_attr(reinterpret_cast<const void *>(kern), lds);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(nthr), lds, s, nele, nip, epb, conn, x, y, f, b, tab, rtab, p, inc_pos,
-                       static_cast<T *>(EB), static_cast<T *>(RE), lb0, lb1);
+                       static_cast<T *>(EB), static_cast<T *>(RE), lb0, lb1, status);
     return hipGetLastError();
 }
 }  // namespace
@@ -1638,15 +1646,16 @@ hipError_t launch_nodal_s(int nn, const double *sfc, const double *h, double *b,
 hipError_t launch_element(int nod, int nele, int nip, int epb, int nthr, bool fp32, bool full, const int32_t *conn,
                           const double *x, const double *y, const FieldPtrs &f, const double *b,
                           const ShapeTables *tab, const ShapeTables32 *tab32, const Params &p,
-                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, hipStream_t s) {
+                          const int32_t *inc_pos, bool ronly, void *EB, void *RE, int lb0, int lb1, int *status,
+                          hipStream_t s) {
     if (nthr < 64 || nthr > 256 || nthr % 64 != 0) return hipErrorInvalidConfiguration;
     if (epb < 1 || epb * nip > nthr || epb * nod > nthr) return hipErrorInvalidConfiguration;
     const int ntiles = (nele + epb - 1) / epb;
     if (lb1 < 0 || lb1 > ntiles) lb1 = ntiles;
     if (lb0 < 0) lb0 = 0;
 #define UVH_EL(NODV)                                                                                           \
-    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, nthr, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s) \
-                : launch_element_t<NODV, double>(nele, nip, epb, nthr, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, s)
+    return fp32 ? launch_element_t<NODV, float>(nele, nip, epb, nthr, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, status, s) \
+                : launch_element_t<NODV, double>(nele, nip, epb, nthr, full, conn, x, y, f, b, tab, tab32, p, inc_pos, ronly, EB, RE, lb0, lb1, status, s)
     switch (nod) {
         case 3: UVH_EL(3);
         case 6: UVH_EL(6);
