@@ -140,6 +140,26 @@ struct Fn<float> {
     static __device__ __forceinline__ float abs(float x) { return ::fabsf(x); }
 };
 
+// Reciprocal for divisors that are finite and away from zero (1 + t in [1, 2], a density, a
+// Jacobian determinant, Glen's and Weertman's exponents, squared speeds and strain rates with
+// their regularisation): the hardware estimate refined by two Newton steps in FP64 (one
+// rounding error more than a correctly rounded division when multiplied), the hardware
+// estimate alone in FP32 (about one ulp). A zero or infinite divisor gives NaN, not Inf.
+template <typename T>
+__device__ __forceinline__ T rcp_nr(T x);
+template <>
+__device__ __forceinline__ double rcp_nr<double>(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+template <>
+__device__ __forceinline__ float rcp_nr<float>(float x) {
+    return __builtin_amdgcn_rcpf(x);
+}
+
 // Smooth step and its derivative at x = k (z - z0) (HeavisideApprox.m:101, DiracDelta.m:29):
 //   H = 1/2 + 1/2 tanh(x),   D = k/2 sech^2(x).
 // One expm1 gives both: with t = exp(-2|x|), tanh|x| = (1 - t)/(1 + t) and
@@ -147,10 +167,10 @@ struct Fn<float> {
 template <typename T>
 __device__ __forceinline__ void soft_step(T x, T k, T &H, T &D) {
     const T em = Fn<T>::expm1(T(-2) * Fn<T>::abs(x));  // t - 1, in (-1, 0]
-    const T den = T(2) + em;                             // 1 + t
-    const T th = copysign(-em / den, x);
+    const T rden = rcp_nr(T(2) + em);                   // 1 / (1 + t)
+    const T th = copysign(-em * rden, x);
     H = T(0.5) + T(0.5) * th;
-    D = T(0.5) * k * (T(4) * (T(1) + em) / (den * den));
+    D = T(2) * k * ((T(1) + em) * rden * rden);
 }
 
 // Floatation terms at one point (uvhAssemblyIntPointImplicitSUPG.m:209-245):
@@ -158,7 +178,7 @@ __device__ __forceinline__ void soft_step(T x, T k, T &H, T &D) {
 // H = S - B, the water depth d = HE rho h / rhow + He H+ and dd = d(d)/dh.
 template <typename T>
 __device__ __forceinline__ void floatation(T h, T hf, T H, T rho, const Params &p, T &He, T &delta, T &d, T &dd) {
-    const T kH = T(p.kH), Hh0 = T(p.Hh0), rrhow = T(1) / T(p.rhow);
+    const T kH = T(p.kH), Hh0 = T(p.Hh0), rrhow = rcp_nr(T(p.rhow));
     T HE, Delta, HH, unused;
     soft_step<T>(kH * ((h - hf) - Hh0), kH, He, delta);
     if (p.Hh0 == 0.0) {
@@ -309,7 +329,7 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
         J21 += deta * xi; J22 += deta * yi;
     }
     const T det = J11 * J22 - J12 * J21;
-    const T rdet = T(1) / det;
+    const T rdet = rcp_nr(det);
     const T i11 = J22 * rdet, i12 = -J12 * rdet, i21 = -J21 * rdet, i22 = J11 * rdet;
 
     // interpolation and gradients (uvhAssemblyIntPointImplicitSUPG.m:62-126,206-208,272-300);
@@ -358,7 +378,7 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     // floatation (uvhAssemblyIntPointImplicitSUPG.m:209-245), FP64, and the combinations of
     // the water depth d that enter the driving stress and its derivative
     const double Hd = Sd - Bd;
-    const double hfd = p.rhow * Hd / rhod;
+    const double hfd = p.rhow * Hd * rcp_nr(rhod);
     double He64, delta64, d64, dd64;
     floatation<double>(hd, hfd, Hd, rhod, p, He64, delta64, d64, dd64);
     const T He = T(He64), delta = T(delta64);
@@ -372,19 +392,19 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     //   A^(-1/n) e^((1-n)/n) = exp(-log(A)/n + (1-n)/n * log(e)),  E = (1-n)/(4n) * that / e^2
     const T eps0 = T(p.eps0);
     const T e2 = eps0 * eps0 + exx * exx + eyy * eyy + exx * eyy + exy * exy;
-    const T rn = T(1) / n;
+    const T rn = rcp_nr(n);
     const T visc = Fn<T>::exp(-Fn<T>::log(A) * rn + (T(1) - n) * rn * (T(0.5) * Fn<T>::log(e2)));
     const T eta = T(0.5) * visc + T(p.eta0);
-    const T E = T(0.25) * (T(1) - n) * rn * visc / e2;
+    const T E = T(0.25) * (T(1) - n) * rn * visc * rcp_nr(e2);
 
     // Weertman sliding (BasalDrag.m:54-77,410-443):
     //   beta2 = (C+C0)^(-1/m) U^(1/m-1),  Dbeta2 = (1/m-1) beta2 / U^2
     const T Hec = He < T(0) ? T(0) : He, dlc = delta < T(0) ? T(0) : delta;
     const T uz = T(p.uz);
     const T U2 = u * u + v * v + uz * uz;
-    const T rm = T(1) / m;
+    const T rm = rcp_nr(m);
     const T beta2 = Fn<T>::exp(-Fn<T>::log(C + T(p.czero)) * rm + (rm - T(1)) * (T(0.5) * Fn<T>::log(U2)));
-    const T Db = (rm - T(1)) * beta2 / U2;
+    const T Db = (rm - T(1)) * beta2 * rcp_nr(U2);
     T taux = Hec * beta2 * u, tauy = Hec * beta2 * v;
     T dtxdu = Hec * (beta2 + Db * u * u), dtydv = Hec * (beta2 + Db * v * v);
     T dtxdv = Hec * Db * u * v;
@@ -404,7 +424,7 @@ __device__ void ip_physics(const double *__restrict__ nv,  // [NV_COUNT][NOD] in
     const T l = Fn<T>::sqrt(T(2) * area);
     const T speed0 = Fn<T>::sqrt(u0 * u0 + v0 * v0 + uz * uz);
     const T taut = T(p.dt / 2.0 + 2.220446049250313e-16);
-    const T taus = l / (T(2) * speed0 + uz);
+    const T taus = l * rcp_nr(T(2) * speed0 + uz);
     T tau;
     switch (p.tau_opt) {
         case 1: tau = taut; break;
