@@ -24,7 +24,8 @@ class O2Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dt", "theta", "beta0", "tau_multiplier", "speed_zero", "eps_zero",
                                           "eta_zero", "czero", "kH", "Hh0", "g", "rhow", "alpha")] + \
                [("tau_option", C.c_int), ("ronly", C.c_int), ("zero_fields", C.c_int),
-                ("reset_thickness", C.c_int), ("thick_min", C.c_double), ("law", C.c_int), ("nzero", C.c_double)]
+                ("reset_thickness", C.c_int), ("thick_min", C.c_double), ("law", C.c_int), ("nzero", C.c_double),
+                ("enforce_b_above", C.c_int)]
 
 # CtrlVar.SlidingLaw (UaMain/BasalDrag.m:182-257) -> the restatement's law code
 SLIDING_LAWS = {"Weertman": 0, "W": 0, "Budd": 1, "W-N0": 1, "Coulomb": 2, "C": 2, "Tsai": 3, "minCW-N0": 3,
@@ -105,6 +106,7 @@ def params_from(CtrlVar, F1, ronly=None, zero_fields=None) -> O2Params:
     P.thick_min = CtrlVar.ThickMin
     P.law = SLIDING_LAWS[str(CtrlVar.get("SlidingLaw", "Weertman"))]
     P.nzero = float(CtrlVar.get("Nzero", 1e-20))
+    P.enforce_b_above = int(bool(CtrlVar.get("Enforce_bAboveB", False)))
     return P
 
 
@@ -149,17 +151,20 @@ class O2Mesh:
 
 
 def assemble(CtrlVar, MUA, F0, F1, mesh: O2Mesh | None = None, nthreads: int = 0,
-             ronly=None, zero_fields=None, want_K=True):
-    """Run O2. Returns dict(R, Tint, Fext, Kval, K (scipy CSR or None), status, mesh)."""
+             ronly=None, zero_fields=None, want_K=True, dadh=None):
+    """Run O2. Returns dict(R, Tint, Fext, Kval, K (scipy CSR or None), status, mesh).
+    ``dadh`` (Nn, optional): d(as + ab)/dh at the nodes, the mass-balance feedback term of the
+    Jacobian (uvhMatrixAssembly.m:151-158); zero when absent."""
     if mesh is None:
         mesh = O2Mesh(MUA)
     P = params_from(CtrlVar, F1, ronly, zero_fields)
     nele, nod, nn = MUA.Nele, MUA.nod, MUA.Nnodes
     f1 = [np.ascontiguousarray(getattr(F1, k), dtype=np.float64)
           for k in ("h", "ub", "vb", "S", "B", "rho", "C", "m", "n", "AGlen", "as_", "ab")]
-    f1 += [_optional(F1, "q"), _optional(F1, "muk"), _optional(F1, "V0")]
+    f1 += [_optional(F1, "q"), _optional(F1, "muk"), _optional(F1, "V0"),
+           None if dadh is None else np.ascontiguousarray(dadh, dtype=np.float64)]
     f0 = [np.ascontiguousarray(getattr(F0, k), dtype=np.float64) for k in ("h", "ub", "vb", "as_", "ab")]
-    f1p = (C.c_void_p * 15)(*[None if a is None else a.ctypes.data for a in f1])
+    f1p = (C.c_void_p * 16)(*[None if a is None else a.ctypes.data for a in f1])
     f0p = (C.c_void_p * 5)(*[a.ctypes.data for a in f0])
     R = np.zeros(3 * nn)
     Tint = np.zeros(3 * nn)

@@ -379,7 +379,7 @@ def _int_point(Iint, MUA, nodal, CtrlVar, rhow, g, Ronly, ca, sa, dt):
 BLOCK_NAMES = ("Kxu", "Kxv", "Kxh", "Kyu", "Kyv", "Kyh", "Khu", "Khv", "Khh")
 
 
-def uvh_matrix_assembly(CtrlVar, MUA, F0, F1, BCs1=None, return_blocks=False):
+def uvh_matrix_assembly(CtrlVar, MUA, F0, F1, BCs1=None, return_blocks=False, dadh=None):
     """Residual and Jacobian as Úa's ``uvhMatrixAssembly`` returns them.
 
     Returns a dict with ``R`` (3*Nn), ``K`` (scipy CSR, 3Nn x 3Nn, duplicates summed,
@@ -409,7 +409,11 @@ def uvh_matrix_assembly(CtrlVar, MUA, F0, F1, BCs1=None, return_blocks=False):
     if not cv.ResetThicknessInNonLinLoop:
         cv.ResetThicknessToMinThickness = False
     b1, _ = calc_bs_from_hBS(cv, f1["h"], f1["S"], f1["B"], f1["rho"], rhow)
-    dadh = np.zeros(MUA.Nnodes)
+    # d(as + ab)/dh: F1.dasdh + F1.dabdh where the mass balance feeds back on the geometry
+    # (MassBalanceGeometryFeedback 3, uvhMatrixAssembly.m:151-158), zero otherwise; the
+    # caller passes it, the mass-balance callback being outside the assembly
+    # (none under ZeroFields, :143, :168)
+    dadh = np.zeros(MUA.Nnodes) if dadh is None or ZeroFields else np.asarray(dadh, dtype=np.float64)
 
     conn = MUA.connectivity
     Nn = MUA.Nnodes

@@ -44,10 +44,12 @@ typedef struct {
     int law;             /* CtrlVar.SlidingLaw: 0 Weertman, 1 Budd, 2 Coulomb, 3 Tsai, 4 Cornford,
                             5 Umbi, 6 Joughin */
     double nzero;        /* CtrlVar.Nzero */
+    int enforce_b_above; /* CtrlVar.Enforce_bAboveB */
 } o2_params;
 
-/* field order in the `f1` array of pointers: h ub vb S B rho C m n AGlen as ab ; f0: h ub vb as ab */
-enum { F1_H, F1_U, F1_V, F1_S, F1_B, F1_RHO, F1_C, F1_M, F1_N, F1_A, F1_AS, F1_AB, F1_Q, F1_MUK, F1_V0, NF1 };
+/* field order in the `f1` array of pointers: h ub vb S B rho C m n AGlen as ab, then the optional
+ * q muk V0 and dadh (null when absent); f0: h ub vb as ab */
+enum { F1_H, F1_U, F1_V, F1_S, F1_B, F1_RHO, F1_C, F1_M, F1_N, F1_A, F1_AS, F1_AB, F1_Q, F1_MUK, F1_V0, F1_DADH, NF1 };
 enum { F0_H, F0_U, F0_V, F0_AS, F0_AB, NF0 };
 
 #define MAXNOD 10
@@ -137,6 +139,8 @@ static void nodal_b(const o2_params *P, int64_t nn, const double *h, const doubl
         double GF = soft_step(P->kH, hk - hf, P->Hh0);
         double bfloat = S[k] - rho[k] * hk / P->rhow;
         b[k] = GF * B[k] + (1.0 - GF) * bfloat;
+        /* UaMain/Calc_bs_From_hBS.m:82-93: no lower surface below the bed */
+        if (P->enforce_b_above && b[k] < B[k]) b[k] = B[k];
     }
 }
 
@@ -457,6 +461,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
         f1[F1_U] = zero; f1[F1_V] = zero; f0[F0_U] = zero; f0[F0_V] = zero;
         f1[F1_H] = f0in[F0_H];
         f1[F1_AS] = as1; f1[F1_AB] = ab1; f0[F0_AS] = as0; f0[F0_AB] = ab0;
+        f1[F1_DADH] = NULL;   /* no mass-balance feedback term under ZeroFields (uvhMatrixAssembly.m:143) */
     }
     nodal_b(P, nn, f1[F1_H], f1[F1_S], f1[F1_B], f1[F1_RHO], bwork);
     const int corners[3][3] = {{0, 1, 2}, {0, 2, 4}, {0, 3, 6}};
@@ -473,7 +478,7 @@ int o2_assemble(const o2_params *P, int nele, int nod, int64_t nn, const int32_t
             nv[9][a] = f1[F1_A][k]; nv[10][a] = f1[F1_AS][k]; nv[11][a] = f1[F1_AB][k];
             nv[12][a] = f0[F0_H][k]; nv[13][a] = f0[F0_U][k]; nv[14][a] = f0[F0_V][k];
             nv[15][a] = f0[F0_AS][k]; nv[16][a] = f0[F0_AB][k];
-            nv[17][a] = 0.0;              /* dadh */
+            nv[17][a] = f1[F1_DADH] ? f1[F1_DADH][k] : 0.0;   /* dadh, uvhMatrixAssembly.m:151-158 */
             nv[18][a] = bwork[k];
             nv[19][a] = f1[F1_Q] ? f1[F1_Q][k] : 0.0;
             nv[20][a] = f1[F1_MUK] ? f1[F1_MUK][k] : 0.0;

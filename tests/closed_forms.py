@@ -168,3 +168,29 @@ def measured_glen_moments(M, Tint):
     n = M.Nnodes
     x, y = M.coordinates[:, 0], M.coordinates[:, 1]
     return float(np.dot(x, Tint[:n])), float(np.dot(y, Tint[:n])), float(np.dot(y, Tint[n:2 * n]))
+
+
+# Mass-balance feedback term of the Jacobian (uvhAssemblyIntPointImplicitSUPG.m:309-316,
+# uvhNodalLoopSSTREAM.m:206: Khh gains -dt theta rho dadh psi_i N_j). With rho and dadh = c
+# constant it is the only part of K that changes with dadh, and R does not change at all:
+#   K(c) - K(0) on the h-h block = -dt theta rho c  int psi_i N_j.
+# Summed over rows (sum_i psi_i = 1 on straight-sided elements) and weighted by the column
+# node's coordinate (sum_j x_j N_j = x):
+#   sum_ij x_j dKhh_ij = -dt theta rho c sum_e A_e xbar_e,   sum_ij dKhh_ij = -dt theta rho c sum_e A_e.
+# A flipped sign, a missing theta or rho, or dadh weighting the row instead of the column
+# misses by 100 % or more.
+DADH = 0.37
+
+
+def dadh_moments(M, cv, rho, dt):
+    A, xb, _ = element_areas_centroids(M)
+    f = -dt * cv.theta * rho * DADH
+    return f * np.sum(A * xb), f * np.sum(A)
+
+
+def measured_dadh_moments(M, K0, K1):
+    """K0, K1: scipy sparse Jacobians without and with dadh (3Nn x 3Nn, unknowns u, v, h)."""
+    n = M.Nnodes
+    d = (K1 - K0).tocsr()[2 * n:, 2 * n:]
+    x = M.coordinates[:, 0]
+    return float(np.sum(d @ x)), float(np.sum(d @ np.ones(n)))

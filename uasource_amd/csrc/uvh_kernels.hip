@@ -517,10 +517,11 @@ __device__ __forceinline__ void gather_node_at(int k, int a, const double *__res
         if (!f.V0) V0 = 0.0;
     }
     if (!f.dadh) dadh = 0.0;
-    if (p.zero_fields) {  // uvhMatrixAssembly.m:73-87
+    if (p.zero_fields) {  // uvhMatrixAssembly.m:73-87; no mass-balance feedback term (:143, :168)
         u = 0.0; v = 0.0; u0 = 0.0; v0 = 0.0; h = h0;
         as1 = fabs(ab1) + 1.0; ab1 = fabs(ab1);
         as0 = fabs(ab0) + 1.0; ab0 = fabs(ab0);
+        dadh = 0.0;
     }
     // without a nodal b (the uvh assembly), b is computed here from the gathered values, as
     // k_nodal_b does per node
