@@ -641,6 +641,64 @@ __device__ __forceinline__ void element_row_b(int i, int nip, const T *__restric
     Tr = Ts; Fr = Fs;
 }
 
+// Block rows x and y of row i in one pass: T and F of both and their 6 x NOD Jacobian blocks,
+// with the inverse Jacobian folded into the per-(row, point) coefficients as in element_row_b
+// (the same regrouped terms); the prefactors that depend only on the row's point values
+// (Nx_i, Ny_i, a_i, b_i) are formed once for both rows. Without K (RONLY) only T and F.
+template <int NOD, typename T, bool RONLY>
+__device__ __forceinline__ void element_row_uv(int i, int nip, const T *__restrict__ cr, const T *__restrict__ sN,
+                                               const T *__restrict__ sXi, const T *__restrict__ sEta,
+                                               const typename RowTab<T>::type *__restrict__ g, T (&TF)[4],
+                                               T (&K)[RONLY ? 1 : 6][NOD]) {
+    T Tx = 0, Fx = 0, Ty = 0, Fy = 0;
+    if (!RONLY) {
+#pragma unroll
+        for (int b = 0; b < 6; ++b)
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) K[b][j] = T(0);
+    }
+    for (int q = 0; q < nip; ++q) {
+        const T *r = cr + q * kNC;
+        const T *qN = g->N[q], *qXi = g->dxi[q], *qEta = g->deta[q];
+        const T I11 = r[C_I11], I12 = r[C_I12], I21 = r[C_I21], I22 = r[C_I22];
+        const T Ni = sN[q * NOD + i];
+        const T Nxi = I11 * sXi[q * NOD + i] + I12 * sEta[q * NOD + i];
+        const T Nyi = I21 * sXi[q * NOD + i] + I22 * sEta[q * NOD + i];
+        const T S1 = r[C_S1], S2 = r[C_S2], EXY = r[C_EXY];
+        const T wheta = r[C_WHETA];
+        const T ai = S1 * Nxi + EXY * Nyi;
+        const T bi = S2 * Nyi + EXY * Nxi;
+        Tx += T(2) * wheta * ai + r[C_WTX] * Ni;
+        Fx += r[C_WF1X] * Ni + r[C_WF2] * Nxi;
+        Ty += T(2) * wheta * bi + r[C_WTY] * Ni;
+        Fy += r[C_WF1Y] * Ni + r[C_WF2] * Nyi;
+        if (!RONLY) {
+            const T whE2 = r[C_WHE2];
+            const T wa = whE2 * ai, wb = whE2 * bi;
+            const T xxx = T(4) * wheta * Nxi + wa * S1, xmix = wheta * Nyi + wa * EXY, xyy = T(2) * wheta * Nxi + wa * S2;
+            const T yxx = T(2) * wheta * Nyi + wb * S1, ymix = wheta * Nxi + wb * EXY, yyy = T(4) * wheta * Nyi + wb * S2;
+            const T dxu = r[C_WDTXDU] * Ni, dxv = r[C_WDTXDV] * Ni, dyv = r[C_WDTYDV] * Ni;
+            const T xh = r[C_PX] * Nxi + r[C_PXY] * Nyi + r[C_PCX] * Ni;
+            const T yh = r[C_PY] * Nyi + r[C_PXY] * Nxi + r[C_PCY] * Ni;
+            const T a0x = xxx * I11 + xmix * I21, a0e = xxx * I12 + xmix * I22;   // Kxu
+            const T a1x = xyy * I21 + xmix * I11, a1e = xyy * I22 + xmix * I12;   // Kxv
+            const T b0x = yxx * I11 + ymix * I21, b0e = yxx * I12 + ymix * I22;   // Kyu
+            const T b1x = yyy * I21 + ymix * I11, b1e = yyy * I22 + ymix * I12;   // Kyv
+#pragma unroll
+            for (int j = 0; j < NOD; ++j) {
+                const T Nj = qN[j], Xj = qXi[j], Ej = qEta[j];
+                K[0][j] = fma(a0x, Xj, fma(a0e, Ej, fma(dxu, Nj, K[0][j])));
+                K[1][j] = fma(a1x, Xj, fma(a1e, Ej, fma(dxv, Nj, K[1][j])));
+                K[2][j] = fma(xh, Nj, K[2][j]);
+                K[3][j] = fma(b0x, Xj, fma(b0e, Ej, fma(dxv, Nj, K[3][j])));
+                K[4][j] = fma(b1x, Xj, fma(b1e, Ej, fma(dyv, Nj, K[4][j])));
+                K[5][j] = fma(yh, Nj, K[5][j]);
+            }
+        }
+    }
+    TF[0] = Tx; TF[1] = Fx; TF[2] = Ty; TF[3] = Fy;
+}
+
 // Row i of the element in one pass: T and F of the three block rows and all nine Jacobian
 // blocks, with the column derivatives Nx_j, Ny_j formed once per (j, point) for all blocks
 // (the same regrouped terms as element_row_b). Used where the 9 x NOD accumulators fit the
@@ -823,12 +881,58 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
         }
         return;
     }
-    // stage 3: element rows, as (block row, element, row) items dealt round-robin to the
-    // threads; each item accumulates 3 x NOD Jacobian terms and its T and F over the points
+    // stage 3: element rows. FP64 3- and 6-node elements: (element, row) items for the x and y
+    // block rows together, then (element, row) items for the h block row; else (block row,
+    // element, row) items. Each accumulates its Jacobian terms and T, F over the points in
+    // registers. Each kind of item takes a segment of the threads' item space that starts at a
+    // wave boundary (seg: nrow rounded up to whole waves), so that no wave runs two kinds of row
+    // arithmetic in one round, unless that takes the block one more round of items.
     const int nbr = p.uv ? 2 : 3;   // the uv assembly has no thickness rows
-    for (int it = tid; it < nbr * nrow; it += nthr) {
-        const int bi = it / nrow;
-        const int rr = it - bi * nrow;
+    constexpr bool kUV2 = NOD <= 6 && sizeof(T) == sizeof(double);
+    const int nkind = kUV2 ? (p.uv ? 1 : 2) : nbr;
+    const int wseg = ((nrow + 63) / 64) * 64;
+    const int seg = (nkind * wseg + nthr - 1) / nthr <= (nkind * nrow + nthr - 1) / nthr ? wseg : nrow;
+    if constexpr (kUV2) {
+        for (int v = tid; v < nkind * seg; v += nthr) {
+            const int kind = v / seg, rr = v - kind * seg;
+            if (rr >= nrow) continue;
+            const int el = rr / NOD, i = rr - el * NOD;
+            const int64_t e = e0 + el;
+            if (e >= nele) continue;
+            const int64_t row = inc_pos[e * NOD + i];
+            const T *crr = rec + el * rec_stride(nip);
+            if (kind == 0) {
+                T TF[4], Kb[RONLY ? 1 : 6][NOD];
+                element_row_uv<NOD, T, RONLY>(i, nip, crr, rN, rXi, rEta, rtab, TF, Kb);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) RE[row * 6 + k] = TF[k];
+                if (!RONLY) {
+                    T *eb = EB + row * eb_stride<NOD, T>();
+#pragma unroll
+                    for (int b = 0; b < (RONLY ? 1 : 6); ++b)
+#pragma unroll
+                        for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = Kb[b][j];
+                }
+            } else {
+                T Tr, Fr, Kb[3][NOD];
+                element_row_b<NOD, 2, T>(i, nip, crr, rN, rXi, rEta, rtab, Tr, Fr, Kb);
+                RE[row * 6 + 4] = Tr;
+                RE[row * 6 + 5] = Fr;
+                if (!RONLY) {
+                    T *eb = EB + row * eb_stride<NOD, T>() + 6 * NOD;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+#pragma unroll
+                        for (int j = 0; j < NOD; ++j) eb[b * NOD + j] = Kb[b][j];
+                }
+            }
+        }
+        return;
+    }
+    for (int v = tid; v < nbr * seg; v += nthr) {
+        const int bi = v / seg;
+        const int rr = v - bi * seg;
+        if (rr >= nrow) continue;
         const int el = rr / NOD, i = rr - el * NOD;
         const int64_t e = e0 + el;
         if (e >= nele) continue;
