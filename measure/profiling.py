@@ -3,9 +3,12 @@
 Two subcommands, both reading rocprofv3 CSV output and writing short text under
 ``profiles/``:
 
-``stats <trace_dir> <workload> [--fp32] [--ronly] [--command CMD]``
+``stats <trace_dir> <workload> [--fp32] [--ronly] [--command CMD] [--phases W,T,E]``
     turns ``rocprofv3 --kernel-trace --stats`` output (``run_kernel_stats.csv``) into
-    ``profiles/kernel_stats_<workload>[_ronly].md``.
+    ``profiles/kernel_stats_<workload>[_ronly].md``. With ``--phases`` the per-call trace
+    (``run_kernel_trace.csv``) of every kernel called W+T+E times is split in dispatch order into
+    ``bench.py``'s W warm-up calls, T timed calls and E calls bracketed by HIP events, and the
+    average duration of each phase is listed as well.
 
 ``traffic <dir> <workload> [--fp32] [--ronly]``
     turns four ``rocprofv3 --pmc`` passes into ``profiles/traffic_<workload>[_ronly].json``
@@ -134,6 +137,32 @@ def stats_md(trace_dir: str, workload: str, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def phases_md(trace_dir: str, phases: tuple) -> str:
+    """Average duration per bench phase of each kernel called exactly sum(phases) times."""
+    with open(os.path.join(trace_dir, "run_kernel_trace.csv")) as fh:
+        rows = list(csv.DictReader(fh))
+    calls = collections.defaultdict(list)
+    for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+        calls[signature_name(r["Kernel_Name"])].append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    w, t, e = phases
+    lines = ["", "Per-call durations from the same run's kernel trace (`run_kernel_trace.csv`), split in",
+             f"dispatch order into the {w} warm-up assemblies, the {t} timed ones and the {e} that `bench.py`",
+             "brackets with HIP events for its per-kernel times (`kernels_us`, `roofline.kernel_us`). The",
+             "first table's averages run over all calls, including any change of a kernel's duration over",
+             "the first calls after start-up; the last column lists the final five calls.", "",
+             "| Kernel | Calls | warm-up avg (us) | timed avg (us) | event-bracketed avg (us) | last 5 (us) |",
+             "|---|---|---|---|---|---|"]
+    for name, d in calls.items():
+        if len(d) != w + t + e:
+            continue
+        avg = lambda xs: sum(xs) / len(xs) if xs else float("nan")
+        last = ", ".join(f"{x:.0f}" for x in d[-5:])
+        lines.append(f"| `{name}` | {len(d)} | {avg(d[:w]):.1f} | {avg(d[w:w + t]):.1f} | "
+                     f"{avg(d[w + t:]):.1f} | {last} |")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
     if len(argv) < 3 or argv[0] not in ("stats", "traffic"):
@@ -153,6 +182,8 @@ def main(argv=None) -> int:
     if "--command" in argv:
         command = argv[argv.index("--command") + 1]
     text = stats_md(d, workload, command)
+    if "--phases" in argv:
+        text += phases_md(d, tuple(int(x) for x in argv[argv.index("--phases") + 1].split(",")))
     fn = os.path.join(PROFILES, f"kernel_stats_{workload}{suffix}.md")
     with open(fn, "w") as fh:
         fh.write(text)
