@@ -227,13 +227,15 @@ int uvh_uv_assemble_device(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctrl
  *   NULL. Analyses the pattern once. Every node must belong to an element (a node in none has
  *   an empty row, K is singular there): UVH_E_ARG otherwise.
  * uvh_lin_factor: block ILU(0) of the device K values `dKval` (the context's CSR order, as
- *   uvh_assemble_device writes them). UVH_E_NOCONV on a zero pivot.
+ *   uvh_assemble_device writes them), computed in FP64; the triangular solves read the factors
+ *   rounded to FP32 (sums and vectors in FP64), or the FP64 factors when the environment
+ *   variable UVH_LIN_FACTOR is "fp64" at uvh_lin_create. UVH_E_NOCONV on a zero pivot.
  * uvh_lin_solve: dx = K^-1 drhs (device vectors of 3*Nn in [ub; vb; h] order) to
  *   ||rhs - K dx|| <= rtol ||rhs|| within maxit iterations; *iterations and *relres (the true
  *   relative residual of dx; either may be NULL) are set on every return; iterations counts
  *   BiCGStab iterations plus GMRES steps. UVH_E_NOCONV if the tolerance is not reached; dx then
- *   holds the best iterate seen. Blocks the host once per reduction (a few times per
- *   iteration). */
+ *   holds the best iterate seen. BiCGStab keeps its scalars on the device and blocks the host
+ *   once per iteration, with the next iteration already queued; GMRES blocks once per step. */
 typedef struct uvh_lin uvh_lin;
 
 /* Krylov methods (uvh_lin_set_krylov): BiCGStab; GMRES(restart), the reference branch's

@@ -7,7 +7,9 @@ to 1e-8 (the solve's residual bound times the conditioning of these systems) and
 residual, recomputed on the host, is below 2e-12; fixed degrees of freedom are exactly zero;
 a solve is bit for bit reproducible; a zero right-hand side gives zero in no iterations; the
 iteration limit and the call order are enforced. Each Krylov method (BiCGStab, GMRES(m) as in
-the reference branch, and the default BiCGStab with a GMRES fall-back) meets the same bars. The Newton iteration with the device solver
+the reference branch, and the default BiCGStab with a GMRES fall-back) meets the same bars, and
+so do the triangular solves on the FP32-rounded factors (the default) and on the FP64 factors
+(UVH_LIN_FACTOR=fp64). The Newton iteration with the device solver
 (LinSolveTol = 1e-8, Úa's default) converges like the oracle's Newton with a direct solve:
 iteration counts within one, rForce < 1e-15, the same u, v, h."""
 import numpy as np
@@ -97,6 +99,28 @@ def test_each_krylov_method_matches_direct_solve(name, method):
         y = torch.empty_like(drhs)
         solver.solve(drhs, y)
         assert torch.equal(x, y)
+    finally:
+        solver.close()
+
+
+@pytest.mark.parametrize("factor", ["fp32", "fp64"])
+@pytest.mark.parametrize("name", ["tiny_p3", "medium_p2"])
+def test_factor_precisions_match_direct_solve(name, factor, monkeypatch):
+    import torch
+    print(f"[gpu] linear solve {name}, {factor} factors", flush=True)
+    monkeypatch.setenv("UVH_LIN_FACTOR", factor)
+    M, fixed, ctx, K, rhs, dK, drhs = system(name)
+    solver = GpuLinearSolver(ctx, fixed, rtol=1e-12, maxit=20000, method="bicgstab")
+    try:
+        x = torch.empty_like(drhs)
+        solver.factor(dK)
+        its, relres = solver.solve(drhs, x)
+        xg = x.cpu().numpy()
+        xd, free = direct(K, rhs, fixed)
+        err = np.linalg.norm(xg - xd) / np.linalg.norm(xd)
+        print(f"   {its} iterations, relres {relres:.2e}, err {err:.2e} {solver.last_message}", flush=True)
+        assert solver.last_message == "" and relres < 2e-12 and err < 1e-8
+        assert np.all(xg[fixed] == 0.0)
     finally:
         solver.close()
 

@@ -22,13 +22,19 @@
 //     triangular solves run colour by colour, one thread per block row, all rows of a colour
 //     at once (k_ilu_color, k_lsolve_color, k_usolve_color); the diagonal blocks of U are kept
 //     inverted. Reductions have a fixed shape (fixed grid, fixed tree): a solve is bit for bit
-//     reproducible.
+//     reproducible;
+//   * the factorization is FP64; the triangular solves read its factors rounded to FP32 (half
+//     the bytes of the preconditioner, which is most of an iteration's traffic), with FP64 sums
+//     and vectors, and the products with K stay FP64;
+//   * BiCGStab keeps its scalars on the device, so an iteration needs one host read of a stop
+//     word, and the next iteration is queued before that read.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,6 +46,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kRedBlocks = 512;   // fixed reduction grid: the summation order depends only on n
+
+// index of value q = 3 b + c of block p (blocks row major, one after another)
+__device__ __forceinline__ int64_t bidx(int64_t p, int q) { return p * 9 + q; }
 
 // Block (n, m) of the node graph entry k: value (b, c) = K[(b, n), (c, m)], read from the
 // assembly's CSR order (row (b, n) at b*3*nnzN + 3*rowptr[n], its column block c at
@@ -57,7 +66,7 @@ __global__ void k_to_blocks(int64_t total, int nn, int64_t nnzN, const int64_t *
     const int64_t r0 = rowptr[n], deg = rowptr[n + 1] - r0;
     double v = K[int64_t(b) * 3 * nnzN + 3 * r0 + c * deg + (k - r0)];
     if (fixed && (fixed[int64_t(b) * nn + n] | fixed[int64_t(c) * nn + m])) v = (n == m && b == c) ? 1.0 : 0.0;
-    A[int64_t(slot[k]) * 9 + bc] = v;
+    A[bidx(slot[k], bc)] = v;
 }
 
 // Row kernels below give kRowLanes lanes to one block row: lane l takes the row's blocks
@@ -74,12 +83,16 @@ __device__ __forceinline__ void row_sum(double &t0, double &t1, double &t2) {
     }
 }
 
-// t += A_p v_ci[p] over the row's blocks p0 .. p1-1 taken by this lane
+// t += A_p v_ci[p] over the row's blocks p0 .. p1-1 taken by this lane (blocks of type TM,
+// arithmetic in FP64)
+template <typename TM>
 __device__ __forceinline__ void row_dot(int p0, int p1, int lane, const int32_t *__restrict__ ci,
-                                        const double *__restrict__ A, const double *__restrict__ v, double &t0,
+                                        const TM *__restrict__ A, const double *__restrict__ v, double &t0,
                                         double &t1, double &t2) {
     for (int p = p0 + lane; p < p1; p += kRowLanes) {
-        const double *a = A + int64_t(p) * 9;
+        double a[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a[q] = double(A[bidx(p, q)]);
         const int64_t m = 3 * int64_t(ci[p]);
         const double v0 = v[m], v1 = v[m + 1], v2 = v[m + 2];
         t0 += a[0] * v0 + a[1] * v1 + a[2] * v2;
@@ -153,45 +166,6 @@ __global__ __launch_bounds__(kThreads) void k_dots(int64_t n, const double *__re
     block_partials<NP>(v, part);
 }
 
-// p = r + beta (p - omega v)
-__global__ void k_update_p(int64_t n, const double *__restrict__ r, const double *__restrict__ v, double beta,
-                           double omega, double *__restrict__ p) {
-    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
-        p[i] = r[i] + beta * (p[i] - omega * v[i]);
-}
-
-// s = r - alpha v, x += alpha y, partials of (s, s)
-__global__ __launch_bounds__(kThreads) void k_update_s(int64_t n, const double *__restrict__ r,
-                                                       const double *__restrict__ v, const double *__restrict__ y,
-                                                       double alpha, double *__restrict__ s, double *__restrict__ x,
-                                                       double *__restrict__ part) {
-    double acc[1] = {0.0};
-    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
-        const double si = r[i] - alpha * v[i];
-        s[i] = si;
-        x[i] += alpha * y[i];
-        acc[0] += si * si;
-    }
-    block_partials<1>(acc, part);
-}
-
-// x += omega z, r = s - omega t, partials of (rhat, r) and (r, r)
-__global__ __launch_bounds__(kThreads) void k_update_xr(int64_t n, const double *__restrict__ s,
-                                                        const double *__restrict__ t, const double *__restrict__ z,
-                                                        const double *__restrict__ rhat, double omega,
-                                                        double *__restrict__ x, double *__restrict__ r,
-                                                        double *__restrict__ part) {
-    double acc[2] = {0.0, 0.0};
-    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
-        x[i] += omega * z[i];
-        const double ri = s[i] - omega * t[i];
-        r[i] = ri;
-        acc[0] += rhat[i] * ri;
-        acc[1] += ri * ri;
-    }
-    block_partials<2>(acc, part);
-}
-
 // res = b - w (in place in w), partials of (res, res)
 __global__ __launch_bounds__(kThreads) void k_residual(int64_t n, const double *__restrict__ b,
                                                        double *__restrict__ w, double *__restrict__ part) {
@@ -227,6 +201,11 @@ __global__ void k_fill(int64_t n, double v, double *__restrict__ x) {
         x[i] = v;
 }
 
+__global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+        b[i] = float(a[i]);
+}
+
 __global__ void k_copy(int64_t n, const double *__restrict__ a, double *__restrict__ b) {
     for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
         b[i] = a[i];
@@ -244,6 +223,12 @@ __device__ __forceinline__ void mm3(const double *a, const double *b, double *c)
 // in ascending k, L_ik = A_ik D_k^-1 and A_ij -= L_ik U_kj for every j > k in both rows;
 // then D_i^-1 replaces the diagonal block. Rows k < r0 are final (earlier colours); rows of
 // this colour do not read each other.
+template <typename TM>
+__device__ __forceinline__ void load_block(const TM *__restrict__ LU, int p, double (&a)[9]) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) a[q] = double(LU[bidx(p, q)]);
+}
+
 __global__ void k_ilu_color(int r0, int r1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                             const int32_t *__restrict__ dslot, double *__restrict__ LU, int *__restrict__ status) {
     const int i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -251,12 +236,12 @@ __global__ void k_ilu_color(int r0, int r1, const int32_t *__restrict__ rp, cons
     const int e1 = rp[i + 1], di = dslot[i];
     for (int p = rp[i]; p < di; ++p) {
         const int k = ci[p];
-        double a[9], l[9];
+        double a[9], dk[9], l[9];
+        load_block(LU, p, a);
+        load_block(LU, dslot[k], dk);
+        mm3(a, dk, l);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) a[q] = LU[int64_t(p) * 9 + q];
-        mm3(a, LU + int64_t(dslot[k]) * 9, l);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) LU[int64_t(p) * 9 + q] = l[q];
+        for (int q = 0; q < 9; ++q) LU[bidx(p, q)] = l[q];
         // merge the rest of row i with the upper part of row k (both sorted by column)
         int q = p + 1, r = dslot[k] + 1;
         const int k1 = rp[k + 1];
@@ -264,11 +249,10 @@ __global__ void k_ilu_color(int r0, int r1, const int32_t *__restrict__ rp, cons
             const int jq = ci[q], jr = ci[r];
             if (jq == jr) {
                 double u[9], lu[9];
-#pragma unroll
-                for (int t = 0; t < 9; ++t) u[t] = LU[int64_t(r) * 9 + t];
+                load_block(LU, r, u);
                 mm3(l, u, lu);
 #pragma unroll
-                for (int t = 0; t < 9; ++t) LU[int64_t(q) * 9 + t] -= lu[t];
+                for (int t = 0; t < 9; ++t) LU[bidx(q, t)] -= lu[t];
                 ++q;
                 ++r;
             } else if (jq < jr) {
@@ -280,8 +264,7 @@ __global__ void k_ilu_color(int r0, int r1, const int32_t *__restrict__ rp, cons
     }
     // invert the pivot block (cofactors over the determinant)
     double d[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) d[q] = LU[int64_t(di) * 9 + q];
+    load_block(LU, di, d);
     const double c0 = d[4] * d[8] - d[5] * d[7], c1 = d[5] * d[6] - d[3] * d[8], c2 = d[3] * d[7] - d[4] * d[6];
     const double det = d[0] * c0 + d[1] * c1 + d[2] * c2;
     if (!(det != 0.0) || !isfinite(det)) {
@@ -294,43 +277,235 @@ __global__ void k_ilu_color(int r0, int r1, const int32_t *__restrict__ rp, cons
     v[3] = c1 * r; v[4] = (d[0] * d[8] - d[2] * d[6]) * r; v[5] = (d[2] * d[3] - d[0] * d[5]) * r;
     v[6] = c2 * r; v[7] = (d[1] * d[6] - d[0] * d[7]) * r; v[8] = (d[0] * d[4] - d[1] * d[3]) * r;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) LU[int64_t(di) * 9 + q] = v[q];
+    for (int q = 0; q < 9; ++q) LU[bidx(di, q)] = v[q];
 }
 
-// y_i = x_i - sum over lower blocks L_ik y_k, rows of one colour (L has identity diagonal blocks)
-__global__ void k_lsolve_color(int r0, int r1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                               const int32_t *__restrict__ dslot, const double *__restrict__ LU,
-                               const double *__restrict__ x, double *__restrict__ y) {
+// The preconditioner M^-1 = U^-1 L^-1 of the block ILU(0), colour by colour. Rows of colour 0
+// have no lower blocks (every neighbour has a larger new number), so L^-1 x equals x there
+// and the lower solves read x for those columns instead of a copy; rows of the last colour
+// have no upper blocks, so their upper solve x_i = D_i^-1 y_i is done in the lower-solve launch
+// of that colour.
+
+// y_i = x_i - sum over lower blocks L_ik y_k (y_k = x_k for k < c1, the rows of colour 0), rows
+// of one colour (L has identity diagonal blocks); with last, also z_i = D_i^-1 y_i
+template <typename TM>
+__global__ void k_lsolve_color(int r0, int r1, int c1, int last, const int32_t *__restrict__ rp,
+                               const int32_t *__restrict__ ci, const int32_t *__restrict__ dslot,
+                               const TM *__restrict__ LU, const double *__restrict__ x,
+                               double *__restrict__ y, double *__restrict__ z) {
     const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int i = r0 + int(g / kRowLanes), lane = int(g % kRowLanes);
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    if (i < r1) row_dot(rp[i], dslot[i], lane, ci, LU, y, t0, t1, t2);
+    if (i < r1) {
+        for (int p = rp[i] + lane, p1 = dslot[i]; p < p1; p += kRowLanes) {
+            double a[9];
+            load_block(LU, p, a);
+            const int k = ci[p];
+            const double *src = k < c1 ? x : y;
+            const int64_t m = 3 * int64_t(k);
+            const double v0 = src[m], v1 = src[m + 1], v2 = src[m + 2];
+            t0 += a[0] * v0 + a[1] * v1 + a[2] * v2;
+            t1 += a[3] * v0 + a[4] * v1 + a[5] * v2;
+            t2 += a[6] * v0 + a[7] * v1 + a[8] * v2;
+        }
+    }
     row_sum(t0, t1, t2);
     if (i < r1 && lane == 0) {
-        y[3 * int64_t(i)] = x[3 * int64_t(i)] - t0;
-        y[3 * int64_t(i) + 1] = x[3 * int64_t(i) + 1] - t1;
-        y[3 * int64_t(i) + 2] = x[3 * int64_t(i) + 2] - t2;
+        t0 = x[3 * int64_t(i)] - t0;
+        t1 = x[3 * int64_t(i) + 1] - t1;
+        t2 = x[3 * int64_t(i) + 2] - t2;
+        if (last) {
+            double d[9];
+            load_block(LU, dslot[i], d);
+            z[3 * int64_t(i)] = d[0] * t0 + d[1] * t1 + d[2] * t2;
+            z[3 * int64_t(i) + 1] = d[3] * t0 + d[4] * t1 + d[5] * t2;
+            z[3 * int64_t(i) + 2] = d[6] * t0 + d[7] * t1 + d[8] * t2;
+        } else {
+            y[3 * int64_t(i)] = t0;
+            y[3 * int64_t(i) + 1] = t1;
+            y[3 * int64_t(i) + 2] = t2;
+        }
     }
 }
 
-// x_i = D_i^-1 (y_i - sum over upper blocks U_ij x_j), rows of one colour
-__global__ void k_usolve_color(int r0, int r1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                               const int32_t *__restrict__ dslot, const double *__restrict__ LU,
-                               const double *__restrict__ y, double *__restrict__ x) {
+// z_i = D_i^-1 (y_i - sum over upper blocks U_ij z_j), rows of one colour; y_i is read from x
+// for the rows of colour 0 (i < c1)
+template <typename TM>
+__global__ void k_usolve_color(int r0, int r1, int c1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const int32_t *__restrict__ dslot, const TM *__restrict__ LU,
+                               const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ z) {
     const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int i = r0 + int(g / kRowLanes), lane = int(g % kRowLanes);
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    if (i < r1) row_dot(dslot[i] + 1, rp[i + 1], lane, ci, LU, x, t0, t1, t2);
+    if (i < r1) row_dot(dslot[i] + 1, rp[i + 1], lane, ci, LU, z, t0, t1, t2);
     row_sum(t0, t1, t2);
     if (i < r1 && lane == 0) {
-        t0 = y[3 * int64_t(i)] - t0;
-        t1 = y[3 * int64_t(i) + 1] - t1;
-        t2 = y[3 * int64_t(i) + 2] - t2;
-        const double *d = LU + int64_t(dslot[i]) * 9;
-        x[3 * int64_t(i)] = d[0] * t0 + d[1] * t1 + d[2] * t2;
-        x[3 * int64_t(i) + 1] = d[3] * t0 + d[4] * t1 + d[5] * t2;
-        x[3 * int64_t(i) + 2] = d[6] * t0 + d[7] * t1 + d[8] * t2;
+        const double *yi = (i < c1 ? x : y) + 3 * int64_t(i);
+        t0 = yi[0] - t0;
+        t1 = yi[1] - t1;
+        t2 = yi[2] - t2;
+        double d[9];
+        load_block(LU, dslot[i], d);
+        z[3 * int64_t(i)] = d[0] * t0 + d[1] * t1 + d[2] * t2;
+        z[3 * int64_t(i) + 1] = d[3] * t0 + d[4] * t1 + d[5] * t2;
+        z[3 * int64_t(i) + 2] = d[6] * t0 + d[7] * t1 + d[8] * t2;
     }
+}
+
+// BiCGStab with its scalars on the device. Each iteration's kernels read rho, alpha, omega and
+// the dot products from `sc` and decide breakdown, convergence and the best iterate there, so
+// the host reads one stop word per iteration and can launch the next iteration before it has
+// read the last one's word: every kernel that changes the solver state returns at once while
+// the stop word is set, which makes an iteration launched after a stop a no-op. The arithmetic
+// and its order are those of the host-driven form (the same IEEE operations on the same values).
+enum : int { S_RHO, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RHAT2, S_RR, S_BEST, S_THR, S_RV, S_VV, S_SS, S_TS, S_TT,
+             S_RHATR, S_RRN, S_COUNT };
+enum : int { F_STOP, F_COPY, F_COUNT };   // stop word; x is to be copied to xbest
+enum : int { B_RUN, B_BREAK_V, B_CONV_S, B_CONV_R, B_BREAK_R, B_OMEGA0 };   // stop word values
+constexpr double kBreak = 1e-12;   // near-breakdown threshold of the inner products
+
+__global__ void k_bi_init(double *__restrict__ sc, int *__restrict__ flag, double bnorm, double thr) {
+    sc[S_RHO] = sc[S_RHAT2] = sc[S_RR] = bnorm * bnorm;
+    sc[S_RHO_OLD] = sc[S_ALPHA] = sc[S_OMEGA] = 1.0;
+    sc[S_BEST] = bnorm;
+    sc[S_THR] = thr;
+    flag[F_STOP] = B_RUN;
+    flag[F_COPY] = 0;
+}
+
+// restart with the current residual as the shadow vector (its copy and p = v = 0 are separate
+// launches): rho = (rhat, rhat) = (r, r)
+__global__ void k_bi_restart(double *__restrict__ sc, int *__restrict__ flag) {
+    sc[S_RHO] = sc[S_RHAT2] = sc[S_RR];
+    sc[S_RHO_OLD] = sc[S_ALPHA] = sc[S_OMEGA] = 1.0;
+    flag[F_STOP] = B_RUN;
+}
+
+// xbest = x if the last residual check asked for it; then, unless stopped,
+// p = r + beta (p - omega v), beta = (rho / rho_old) (alpha / omega)
+__global__ void k_bi_p(int64_t n, const double *__restrict__ r, const double *__restrict__ v,
+                       double *__restrict__ p, const double *__restrict__ x, double *__restrict__ xbest,
+                       const double *__restrict__ sc, const int *__restrict__ flag) {
+    const bool cp = flag[F_COPY] != 0, stop = flag[F_STOP] != B_RUN;
+    const double beta = (sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]), omega = sc[S_OMEGA];
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+        if (cp) xbest[i] = x[i];
+        if (!stop) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    }
+}
+
+// unless stopped or (rhat, v) near zero: alpha = rho / (rhat, v), s = r - alpha v,
+// x += alpha y, partials of (s, s)
+__global__ __launch_bounds__(kThreads) void k_bi_s(int64_t n, const double *__restrict__ r,
+                                                   const double *__restrict__ v, const double *__restrict__ y,
+                                                   double *__restrict__ s, double *__restrict__ x,
+                                                   double *__restrict__ part, double *__restrict__ sc,
+                                                   int *__restrict__ flag) {
+    const double rv = sc[S_RV];
+    // the same values in every thread, so every block takes the same branch
+    const bool brk = std::fabs(rv) <= kBreak * std::sqrt(sc[S_RHAT2] * sc[S_VV]);
+    if (flag[F_STOP] != B_RUN || brk) {
+        if (brk && blockIdx.x == 0 && threadIdx.x == 0 && flag[F_STOP] == B_RUN) flag[F_STOP] = B_BREAK_V;
+        return;
+    }
+    const double alpha = sc[S_RHO] / rv;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHA] = alpha;
+    double acc[1] = {0.0};
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+        const double si = r[i] - alpha * v[i];
+        s[i] = si;
+        x[i] += alpha * y[i];
+        acc[0] += si * si;
+    }
+    block_partials<1>(acc, part);
+}
+
+// unless stopped: xbest = x if asked for, omega = (t, s) / (t, t), x += omega z,
+// r = s - omega t, partials of (rhat, r) and (r, r)
+__global__ __launch_bounds__(kThreads) void k_bi_xr(int64_t n, const double *__restrict__ s,
+                                                    const double *__restrict__ t, const double *__restrict__ z,
+                                                    const double *__restrict__ rhat, double *__restrict__ x,
+                                                    double *__restrict__ r, double *__restrict__ xbest,
+                                                    double *__restrict__ part, double *__restrict__ sc,
+                                                    const int *__restrict__ flag) {
+    if (flag[F_STOP] != B_RUN) return;
+    const double tt = sc[S_TT];
+    const double omega = tt > 0.0 ? sc[S_TS] / tt : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_OMEGA] = omega;
+    const bool cp = flag[F_COPY] != 0;
+    double acc[2] = {0.0, 0.0};
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+        if (cp) xbest[i] = x[i];
+        x[i] += omega * z[i];
+        const double ri = s[i] - omega * t[i];
+        r[i] = ri;
+        acc[0] += rhat[i] * ri;
+        acc[1] += ri * ri;
+    }
+    block_partials<2>(acc, part);
+}
+
+// sc[slot + p] = sum of part[p * nb ..] in the fixed tree order of k_finish (np <= 2); then
+// mode 1 (after k_bi_s): stop at convergence, else note a better iterate (copied in k_bi_xr);
+// mode 2 (after k_bi_xr): rr, stop at convergence, note a better iterate (copied in the next
+// k_bi_p), rho_old = rho, rho = (rhat, r), stop for a restart when omega = 0 or (rhat, r) is
+// near zero
+__global__ __launch_bounds__(kThreads) void k_bi_finish(int nb, int np, const double *__restrict__ part,
+                                                        double *__restrict__ sc, int slot, int mode,
+                                                        int *__restrict__ flag) {
+    __shared__ double red[kThreads];
+    double tot[2] = {0.0, 0.0};
+    for (int p = 0; p < np; ++p) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < nb; i += kThreads) v += part[p * nb + i];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = kThreads / 2; w > 0; w >>= 1) {
+            if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        tot[p] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int p = 0; p < np; ++p) sc[slot + p] = tot[p];
+    if (mode == 0) return;
+    flag[F_COPY] = 0;
+    if (flag[F_STOP] != B_RUN) return;
+    if (mode == 1) {
+        const double nrm = std::sqrt(tot[0]);
+        if (nrm <= sc[S_THR]) {
+            flag[F_STOP] = B_CONV_S;
+        } else if (nrm < sc[S_BEST]) {
+            sc[S_BEST] = nrm;
+            flag[F_COPY] = 1;
+        }
+        return;
+    }
+    const double rr = tot[1], nrm = std::sqrt(rr);
+    sc[S_RR] = rr;
+    if (nrm <= sc[S_THR]) {
+        flag[F_STOP] = B_CONV_R;
+        return;
+    }
+    if (nrm < sc[S_BEST]) {
+        sc[S_BEST] = nrm;
+        flag[F_COPY] = 1;
+    }
+    sc[S_RHO_OLD] = sc[S_RHO];
+    sc[S_RHO] = tot[0];
+    if (sc[S_OMEGA] == 0.0)
+        flag[F_STOP] = B_OMEGA0;
+    else if (std::fabs(tot[0]) <= kBreak * std::sqrt(sc[S_RHAT2] * rr))
+        flag[F_STOP] = B_BREAK_R;
+}
+
+// xbest = x if the last residual check asked for it
+__global__ void k_copy_if(int64_t n, const double *__restrict__ x, double *__restrict__ xbest,
+                          const int *__restrict__ flag) {
+    if (!flag[F_COPY]) return;
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+        xbest[i] = x[i];
 }
 
 // GMRES helpers over a basis V of k vectors of length n, stored one after another
@@ -383,12 +558,18 @@ struct uvh_lin {
     int ncolors = 0;
     uint8_t *d_fixed = nullptr;
     double *d_A = nullptr, *d_LU = nullptr;
+    float *d_LUf = nullptr;                           // the factors rounded to FP32 (lu32)
+    bool lu32 = true;                                 // triangular solves read d_LUf instead of d_LU
     // BiCGStab vectors (node-interleaved) and reduction scratch
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_rhat = nullptr, *d_p = nullptr, *d_v = nullptr;
     double *d_s = nullptr, *d_t = nullptr, *d_y = nullptr, *d_z = nullptr, *d_w = nullptr;
     double *d_part = nullptr, *d_out = nullptr;       // reduction partials and results (kMaxKrylov + 2 values)
     double *h_out = nullptr;                          // pinned: 3 (kMaxKrylov + 2) values
     double *d_xbest = nullptr;                        // BiCGStab: iterate with the smallest residual so far
+    double *d_sc = nullptr;                           // BiCGStab scalars (S_COUNT values)
+    int *d_flag = nullptr;                            // BiCGStab stop word and copy flag (F_COUNT values)
+    int *h_flag = nullptr;                            // pinned: the stop words of two consecutive iterations
+    hipEvent_t ev[2] = {nullptr, nullptr};            // recorded after each of those copies
     // Krylov method (uvh_lin_set_krylov) and the GMRES basis (allocated on first use)
     int method = UVH_LIN_AUTO, restart = 30;
     double *d_V = nullptr, *d_coef = nullptr;
@@ -413,6 +594,7 @@ hipError_t lalloc(T **p, size_t n) {
 
 int grid_of(int64_t n) { return int((n + kThreads - 1) / kThreads); }
 
+
 // reduce NP partial sums on the stream and return them on the host
 int finish(uvh_lin *L, int np, hipStream_t s, double *out) {
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, s, kRedBlocks, np, L->d_part, L->d_out);
@@ -423,22 +605,34 @@ int finish(uvh_lin *L, int np, hipStream_t s, double *out) {
     return UVH_OK;
 }
 
-// y = U^-1 L^-1 x: colours ascending for L, descending for U; w holds L^-1 x (x and y must
-// not be w)
-void precondition(uvh_lin *L, const double *x, double *y, hipStream_t s) {
+// y = U^-1 L^-1 x: colours ascending for L, descending for U; w holds L^-1 x on the rows of
+// colours 1 .. nc-1 (x and y must not be w). Colour 0 needs no lower solve and the last colour
+// no separate upper solve, so nc colours take 2 nc - 2 launches (one when nc = 1). The factors
+// are read as TM (float: the FP64 factors rounded once after the factorization; every sum and
+// every vector stays FP64, so the preconditioner is a fixed linear operator and the Krylov
+// method reaches the same tolerances).
+template <typename TM>
+void precondition_t(uvh_lin *L, const TM *LU, const double *x, double *y, hipStream_t s) {
     const int nc = int(L->cstart.size()) - 1;
-    for (int c = 0; c < nc; ++c) {
+    const int c1 = L->cstart[1];
+    for (int c = nc == 1 ? 0 : 1; c < nc; ++c) {
         const int r0 = L->cstart[c], r1 = L->cstart[c + 1];
-        hipLaunchKernelGGL(k_lsolve_color, dim3(grid_of(int64_t(r1 - r0) * kRowLanes)), dim3(kThreads), 0, s, r0, r1,
-                           L->d_rp, L->d_ci,
-                           L->d_dslot, L->d_LU, x, L->d_w);
+        hipLaunchKernelGGL(k_lsolve_color<TM>, dim3(grid_of(int64_t(r1 - r0) * kRowLanes)), dim3(kThreads), 0, s, r0, r1,
+                           c == 0 ? 0 : c1, int(c == nc - 1), L->d_rp, L->d_ci, L->d_dslot, LU, x,
+                           L->d_w, y);
     }
-    for (int c = nc - 1; c >= 0; --c) {
+    for (int c = nc - 2; c >= 0; --c) {
         const int r0 = L->cstart[c], r1 = L->cstart[c + 1];
-        hipLaunchKernelGGL(k_usolve_color, dim3(grid_of(int64_t(r1 - r0) * kRowLanes)), dim3(kThreads), 0, s, r0, r1,
-                           L->d_rp, L->d_ci,
-                           L->d_dslot, L->d_LU, L->d_w, y);
+        hipLaunchKernelGGL(k_usolve_color<TM>, dim3(grid_of(int64_t(r1 - r0) * kRowLanes)), dim3(kThreads), 0, s, r0, r1,
+                           c1, L->d_rp, L->d_ci, L->d_dslot, LU, x, L->d_w, y);
     }
+}
+
+void precondition(uvh_lin *L, const double *x, double *y, hipStream_t s) {
+    if (L->lu32)
+        precondition_t(L, L->d_LUf, x, y, s);
+    else
+        precondition_t(L, L->d_LU, x, y, s);
 }
 
 void block_mv(uvh_lin *L, const double *x, double *y, hipStream_t s) {
@@ -450,79 +644,86 @@ void block_mv(uvh_lin *L, const double *x, double *y, hipStream_t s) {
 // r or v, or omega = 0) restarts with the current residual as the new shadow vector, at most
 // kMaxRestarts times. The iterate with the smallest residual is kept and returned when the
 // tolerance is not reached, so a diverging run cannot hand back a worse vector than it saw.
+// The scalars live on the device (k_bi_*); the host keeps one iteration in flight ahead of the
+// one whose stop word it reads, and relaunches that iteration after a restart (it ran as a
+// no-op behind the stop word).
 int bicgstab(uvh_lin *L, double rtol, int maxit, double bnorm, hipStream_t s, int &it, bool &converged,
              std::string &why) {
     const int64_t n = L->n3;
     const dim3 G(kRedBlocks), B(kThreads);
-    constexpr double kBreak = 1e-12;
     constexpr int kMaxRestarts = 20;
-    double red[2];
-    int st;
     hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_r);
     hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_rhat);
     hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_p);
     hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_v);
     hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_xbest);
-    double rho = bnorm * bnorm, rho_old = 1.0, alpha = 1.0, omega = 1.0;   // rhat = r = b
-    double rhat2 = rho, rr = rho, best = bnorm;
-    int restarts = 0;
-    auto restart = [&]() {
-        hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_r, L->d_rhat);
-        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_p);
-        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_v);
-        rho = rhat2 = rr;
-        rho_old = alpha = omega = 1.0;
-        ++restarts;
-    };
-    auto keep_if_best = [&](double norm) {
-        if (norm < best) {
-            best = norm;
-            hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_x, L->d_xbest);
-        }
-    };
-    converged = false;
-    for (it = 1; it <= maxit; ++it) {
-        if (std::fabs(rho) <= kBreak * std::sqrt(rhat2 * rr)) {
-            if (restarts == kMaxRestarts) { why = "BiCGStab breakdown: (rhat, r) = 0"; break; }
-            restart();
-        }
-        const double beta = (rho / rho_old) * (alpha / omega);
-        hipLaunchKernelGGL(k_update_p, G, B, 0, s, n, L->d_r, L->d_v, beta, omega, L->d_p);
+    hipLaunchKernelGGL(k_bi_init, dim3(1), dim3(1), 0, s, L->d_sc, L->d_flag, bnorm, rtol * bnorm);
+    double *sc = L->d_sc;
+    int *flag = L->d_flag;
+    int launched = 0, done = 0, restarts = 0;
+    auto launch = [&]() {
+        hipLaunchKernelGGL(k_bi_p, G, B, 0, s, n, L->d_r, L->d_v, L->d_p, L->d_x, L->d_xbest, sc, flag);
         precondition(L, L->d_p, L->d_y, s);
         block_mv(L, L->d_y, L->d_v, s);
         hipLaunchKernelGGL(k_dots<2>, G, B, 0, s, n, L->d_rhat, L->d_v, L->d_v, L->d_v, L->d_part);
-        if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
-        if (std::fabs(red[0]) <= kBreak * std::sqrt(rhat2 * red[1])) {
-            if (restarts == kMaxRestarts) { why = "BiCGStab breakdown: (rhat, v) = 0"; break; }
-            restart();   // p and v are discarded; this iteration starts again
-            continue;
-        }
-        alpha = rho / red[0];
-        hipLaunchKernelGGL(k_update_s, G, B, 0, s, n, L->d_r, L->d_v, L->d_y, alpha, L->d_s, L->d_x, L->d_part);
-        if ((st = finish(L, 1, s, red)) != UVH_OK) return st;
-        if (std::sqrt(red[0]) <= rtol * bnorm) { converged = true; break; }
-        keep_if_best(std::sqrt(red[0]));
+        hipLaunchKernelGGL(k_bi_finish, dim3(1), B, 0, s, kRedBlocks, 2, L->d_part, sc, int(S_RV), 0, flag);
+        hipLaunchKernelGGL(k_bi_s, G, B, 0, s, n, L->d_r, L->d_v, L->d_y, L->d_s, L->d_x, L->d_part, sc, flag);
+        hipLaunchKernelGGL(k_bi_finish, dim3(1), B, 0, s, kRedBlocks, 1, L->d_part, sc, int(S_SS), 1, flag);
         precondition(L, L->d_s, L->d_z, s);
         block_mv(L, L->d_z, L->d_t, s);
         hipLaunchKernelGGL(k_dots<2>, G, B, 0, s, n, L->d_t, L->d_s, L->d_t, L->d_t, L->d_part);
-        if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
-        omega = red[1] > 0.0 ? red[0] / red[1] : 0.0;
-        hipLaunchKernelGGL(k_update_xr, G, B, 0, s, n, L->d_s, L->d_t, L->d_z, L->d_rhat, omega, L->d_x, L->d_r,
-                           L->d_part);
-        if ((st = finish(L, 2, s, red)) != UVH_OK) return st;
-        rr = red[1];
-        if (std::sqrt(rr) <= rtol * bnorm) { converged = true; break; }
-        keep_if_best(std::sqrt(rr));
-        rho_old = rho;
-        rho = red[0];
-        if (omega == 0.0) {
-            if (restarts == kMaxRestarts) { why = "BiCGStab breakdown: omega = 0"; break; }
-            restart();
+        hipLaunchKernelGGL(k_bi_finish, dim3(1), B, 0, s, kRedBlocks, 2, L->d_part, sc, int(S_TS), 0, flag);
+        hipLaunchKernelGGL(k_bi_xr, G, B, 0, s, n, L->d_s, L->d_t, L->d_z, L->d_rhat, L->d_x, L->d_r, L->d_xbest,
+                           L->d_part, sc, flag);
+        hipLaunchKernelGGL(k_bi_finish, dim3(1), B, 0, s, kRedBlocks, 2, L->d_part, sc, int(S_RHATR), 2, flag);
+        ++launched;
+        const int slot = launched & 1;
+        hipError_t e = hipMemcpyAsync(L->h_flag + slot, flag + F_STOP, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(L->ev[slot], s);
+        return e;
+    };
+    auto fail = [&](hipError_t e) {
+        return L->fail(UVH_E_HIP, std::string("uvh_lin_solve: BiCGStab: ") + hipGetErrorString(e));
+    };
+    converged = false;
+    hipError_t e = launch();
+    if (e != hipSuccess) return fail(e);
+    for (;;) {
+        const int cur = done + 1;   // launched >= cur
+        if (launched == cur && launched < maxit && (e = launch()) != hipSuccess) return fail(e);
+        if ((e = hipEventSynchronize(L->ev[cur & 1])) != hipSuccess) return fail(e);
+        const int word = L->h_flag[cur & 1];
+        done = cur;
+        if (word == B_RUN) {
+            if (done == maxit) break;
+            continue;
         }
+        if (word == B_CONV_S || word == B_CONV_R) {
+            converged = true;
+            break;
+        }
+        // a restart: after a breakdown inside iteration `done` (it counts as done), or before
+        // the next one; not past the last iteration unless omega = 0 ended this one
+        if (word == B_BREAK_R && done == maxit) break;
+        if (restarts == kMaxRestarts) {
+            why = word == B_BREAK_V ? "BiCGStab breakdown: (rhat, v) = 0"
+                  : word == B_BREAK_R ? "BiCGStab breakdown: (rhat, r) = 0"
+                                      : "BiCGStab breakdown: omega = 0";
+            break;
+        }
+        hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_r, L->d_rhat);
+        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_p);
+        hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_v);
+        hipLaunchKernelGGL(k_bi_restart, dim3(1), dim3(1), 0, s, sc, flag);
+        ++restarts;
+        launched = done;   // an iteration launched ahead ran as a no-op: launch it again
+        if (done == maxit) break;
+        if ((e = launch()) != hipSuccess) return fail(e);
     }
-    if (it > maxit) it = maxit;
+    it = done;
     if (!converged) {
         if (why.empty()) why = "BiCGStab iteration limit";
+        hipLaunchKernelGGL(k_copy_if, G, B, 0, s, n, L->d_x, L->d_xbest, flag);   // a better last iterate
         hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_xbest, L->d_x);
     }
     return UVH_OK;
@@ -616,11 +817,14 @@ void destroy(uvh_lin *L) {
     hipSetDevice(L->device);
     for (void *p : {(void *)L->d_rowptr64, (void *)L->d_colind, (void *)L->d_newnode, (void *)L->d_slot,
                     (void *)L->d_rp, (void *)L->d_ci, (void *)L->d_node_of, (void *)L->d_fixed,
-                    (void *)L->d_A, (void *)L->d_LU, (void *)L->d_b, (void *)L->d_x, (void *)L->d_r, (void *)L->d_rhat,
+                    (void *)L->d_A, (void *)L->d_LU, (void *)L->d_LUf, (void *)L->d_b, (void *)L->d_x, (void *)L->d_r, (void *)L->d_rhat,
                     (void *)L->d_p, (void *)L->d_v, (void *)L->d_s, (void *)L->d_t, (void *)L->d_y, (void *)L->d_z,
                     (void *)L->d_w, (void *)L->d_part, (void *)L->d_out, (void *)L->d_dslot, (void *)L->d_status,
-                    (void *)L->d_xbest, (void *)L->d_V, (void *)L->d_coef})
+                    (void *)L->d_xbest, (void *)L->d_V, (void *)L->d_coef, (void *)L->d_sc, (void *)L->d_flag})
         if (p) hipFree(p);
+    for (hipEvent_t e : L->ev)
+        if (e) hipEventDestroy(e);
+    if (L->h_flag) hipHostFree(L->h_flag);
     if (L->h_out) hipHostFree(L->h_out);
     if (L->h_status) hipHostFree(L->h_status);
     delete L;
@@ -639,6 +843,9 @@ int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **out) {
     auto *L = new uvh_lin();
     L->device = uvh::ctx_device(ctx);
     hipSetDevice(L->device);
+    // UVH_LIN_FACTOR=fp64 keeps the triangular solves on the FP64 factors
+    const char *fac = std::getenv("UVH_LIN_FACTOR");
+    L->lu32 = !(fac && std::string(fac) == "fp64");
     L->nn = M->nn;
     L->nnzN = M->nnzN();
     L->n3 = 3 * int64_t(M->nn);
@@ -696,10 +903,15 @@ int uvh_lin_create(uvh_ctx *ctx, const uint8_t *fixed, uvh_lin **out) {
               lalloc(&L->d_status, 1) == hipSuccess &&
               hipHostMalloc(reinterpret_cast<void **>(&L->h_status), sizeof(int), 0) == hipSuccess &&
               lalloc(&L->d_A, 9 * L->nnzN) == hipSuccess && lalloc(&L->d_LU, 9 * L->nnzN) == hipSuccess &&
+              (!L->lu32 || lalloc(&L->d_LUf, 9 * L->nnzN) == hipSuccess) &&
               lalloc(&L->d_part, (kMaxKrylov + 2) * kRedBlocks) == hipSuccess &&
               lalloc(&L->d_out, kMaxKrylov + 2) == hipSuccess && lalloc(&L->d_coef, kMaxKrylov + 2) == hipSuccess &&
               hipHostMalloc(reinterpret_cast<void **>(&L->h_out), 3 * (kMaxKrylov + 2) * sizeof(double), 0) ==
-                  hipSuccess;
+                  hipSuccess &&
+              lalloc(&L->d_sc, S_COUNT) == hipSuccess && lalloc(&L->d_flag, F_COUNT) == hipSuccess &&
+              hipHostMalloc(reinterpret_cast<void **>(&L->h_flag), 2 * sizeof(int), 0) == hipSuccess &&
+              hipEventCreateWithFlags(&L->ev[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&L->ev[1], hipEventDisableTiming) == hipSuccess;
     for (double **v : {&L->d_b, &L->d_x, &L->d_r, &L->d_rhat, &L->d_p, &L->d_v, &L->d_s, &L->d_t, &L->d_y, &L->d_z,
                        &L->d_w, &L->d_xbest})
         ok = ok && lalloc(v, L->n3) == hipSuccess;
@@ -754,6 +966,11 @@ int uvh_lin_factor(uvh_lin *L, const double *dK, void *stream) {
         L->factored = false;
         return L->fail(UVH_E_NOCONV, "uvh_lin_factor: singular pivot block in (renumbered) block row " +
                                          std::to_string(*L->h_status));
+    }
+    if (L->lu32) {
+        hipLaunchKernelGGL(k_to_f32, dim3(kRedBlocks * 4), dim3(kThreads), 0, s, total, L->d_LU, L->d_LUf);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return L->fail(UVH_E_HIP, std::string("uvh_lin_factor: ") + hipGetErrorString(e));
     }
     L->factored = true;
     return UVH_OK;
