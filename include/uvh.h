@@ -252,6 +252,30 @@ int uvh_lin_factor(uvh_lin *lin, const double *dKval, void *stream);
 int uvh_lin_solve(uvh_lin *lin, const double *drhs, double *dx, double rtol, int32_t maxit, int32_t *iterations,
                   double *relres, void *stream);
 
+/* Batched solves. The reference solves one system per call (UaMain/solveKApe.m:117-151); an
+ * ensemble of members each takes that branch in turn. Here the BiCGStab iterations of n
+ * members' systems advance together, one launch per step of the iteration for all of them
+ * (flat work lists of the members' vector chunks and block rows, and per colour index the
+ * members' rows of that colour), each member being a uvh_lin of its own mesh, factored with
+ * uvh_lin_factor before the solve. The convergence test, the near-breakdown restarts and the
+ * iteration limit are per member and on the device; the host waits once per iteration, with
+ * the next iteration already queued, until every member has stopped. For callers that solve
+ * many members' Newton steps together (the batched Newton form).
+ *
+ * uvhb_lin_create: members[0..n-1], all on one device; destroy the batch before any member.
+ * uvhb_lin_solve: dx[m] = K_m^-1 drhs[m] (device vectors of 3*Nn_m in [ub; vb; h] order) for
+ *   every member with active[m] != 0 (active NULL: all), to ||rhs - K dx|| <= rtol ||rhs||
+ *   within maxit iterations; iterations[m], relres[m] (true relative residual) and converged[m]
+ *   are set for every member, 0 for inactive ones (each array may be NULL). UVH_E_NOCONV if an
+ *   active member missed the tolerance; its dx then holds its best iterate. */
+typedef struct uvhb_lin uvhb_lin;
+
+int uvhb_lin_create(uvh_lin *const *members, int32_t n, uvhb_lin **batch);
+void uvhb_lin_destroy(uvhb_lin *batch);
+const char *uvhb_lin_last_error(const uvhb_lin *batch);
+int uvhb_lin_solve(uvhb_lin *batch, const double *const *drhs, double *const *dx, const uint8_t *active, double rtol,
+                   int32_t maxit, int32_t *iterations, double *relres, int32_t *converged, void *stream);
+
 /* ---------------------------------------------------------------- batched assembly
  * Many independent meshes (ensemble members with perturbed C and AGlen, mixed 3-, 6- and
  * 10-node elements) assembled together: per element type, one element launch and one row

@@ -126,8 +126,9 @@ def bench_batch_newton(n_members: int = 16, log=print) -> dict:
     """Batched Newton solves (``uasource_amd.batch_newton``): ``n_members`` independent uvh
     problems (the Newton workload recipe on 100 km x 50 km P1/P2 meshes of 2k-7k elements,
     reseeded per member), run to rForce < 1e-15 with the assembly and cost stages replayed
-    from HIP graphs and, for comparison, eagerly; the two runs give the same iterates bit for
-    bit (``tests/test_gpu_batch_newton.py``)."""
+    from HIP graphs and the members' Newton steps in one batched Krylov solve; for comparison
+    eagerly (the same iterates bit for bit, ``tests/test_gpu_batch_newton.py``) and with the
+    members' steps solved one after another."""
     import dataclasses
     from uasource_amd.batch_newton import BatchNewton
     base = [WORKLOADS["cpu_reference"], WORKLOADS["medium_p2"]]
@@ -139,16 +140,17 @@ def bench_batch_newton(n_members: int = 16, log=print) -> dict:
         mem.append((M, F0, F1, fixed))
     cv = default_ctrlvar()
     out = {}
-    for graphs in (True, False, True):
-        bn = BatchNewton(cv, mem, graphs=graphs)
+    for graphs, batched, key in ((True, True, "graphs"), (False, True, "eager"), (True, True, "graphs"),
+                                 (True, False, "graphs_member_solves")):
+        bn = BatchNewton(cv, mem, graphs=graphs, batched_solve=batched)
         t = time.perf_counter()
         res = bn.solve()
         dt = time.perf_counter() - t
         replays = bn.n_graph_replays
         bn.close()
-        key = "graphs" if graphs else "eager"
         out[key] = {"seconds": dt, "graph_replays": replays, "iterations": [r.iterations for r in res],
-                    "converged": all(r.converged for r in res), "max_rForce": max(r.rForce for r in res)}
+                    "converged": all(r.converged for r in res), "max_rForce": max(r.rForce for r in res),
+                    "krylov_iterations": int(sum(sum(r.linear_iterations) for r in res))}
         log(f"batch newton ({key}): {dt:.2f} s, iterations {out[key]['iterations']}, replays {replays}")
     nele = sum(m[0].Nele for m in mem)
     return {
@@ -157,8 +159,8 @@ def bench_batch_newton(n_members: int = 16, log=print) -> dict:
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded meshes, inflow conditions)",
         "config": {"workload": "batch_newton", "members": n_members, "Nele_total": nele,
-                   "linear_solver": "block ILU(0) Krylov (GPU), per member", "parallelism": "1 GPU"},
-        "graphs": out["graphs"], "eager": out["eager"],
+                   "linear_solver": "block ILU(0) BiCGStab (GPU), all members batched", "parallelism": "1 GPU"},
+        "graphs": out["graphs"], "eager": out["eager"], "graphs_member_solves": out["graphs_member_solves"],
         "note": "the first graphs run includes capture; the value is the second graphs run",
     }
 

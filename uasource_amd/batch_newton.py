@@ -8,8 +8,12 @@ and the backtracking of ``BackTracking.m``), but the members step together:
 
 * graph A: the trial state of every member set to its state, then the R and K of every member
   in one batched assembly (``uvhb_assemble_device``);
-* the step ``K d = -R`` of every member on the device (block ILU(0) and BiCGStab/GMRES,
-  ``uvh_lin_*``), outside the graphs: the Krylov loops test their convergence on the host;
+* the step ``K d = -R`` of every member on the device: each member's block ILU(0)
+  (``uvh_lin_factor``), then one batched BiCGStab for all members (``uvhb_lin_solve``: one
+  launch per step of the iteration for every member, convergence tested per member on the
+  device, the host waiting once per iteration until all have stopped), outside the graphs; a
+  member that misses the tolerance is solved again alone with BiCGStab and the GMRES fall-back,
+  as a single mesh is;
 * graph C: the cost at gamma = 0 of every member (|w R|^2 and -w R . d, ``uvhb_cost_device``);
 * graph B: the trial state x + gamma_m d_m of every member, with each member's gamma in a
   device vector, its R (one batched R-only assembly) and its cost.
@@ -25,15 +29,16 @@ import numpy as np
 
 from . import newton, uvh
 from .batch import UvhBatch
-from .linsolve import GpuLinearSolver
+from .linsolve import GpuBatchLinearSolver, GpuLinearSolver
 
 EPS = np.finfo(float).eps
 
 
 class BatchNewton:
-    def __init__(self, CtrlVar, members, device: int = 0, graphs: bool = True):
+    def __init__(self, CtrlVar, members, device: int = 0, graphs: bool = True, batched_solve: bool = True):
         """``members``: list of (MUA, F0, F1, fixed) with ``fixed`` indices into [ub; vb; h]
-        (or None). The members share CtrlVar and F1's scalars g, rhow and alpha."""
+        (or None). The members share CtrlVar and F1's scalars g, rhow and alpha. With
+        ``batched_solve=False`` the members' Newton steps are solved one after another."""
         import torch
         self.torch = torch
         self.cv = CtrlVar
@@ -88,12 +93,16 @@ class BatchNewton:
         self.rr = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
         self.d2 = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
         self.solvers = [GpuLinearSolver(mv, fx, rtol=CtrlVar.LinSolveTol) for mv, fx in zip(self.batch.members, self.fixed)]
+        self.bsolver = GpuBatchLinearSolver(self.solvers, rtol=CtrlVar.LinSolveTol) if batched_solve else None
         self.stream = torch.cuda.Stream(self.dev)   # default priority
         self.sptr = self.stream.cuda_stream
         self.n_graph_replays = 0
         self._graphs = None
 
     def close(self):
+        if getattr(self, "bsolver", None) is not None:
+            self.bsolver.close()
+            self.bsolver = None
         for s in self.solvers:
             s.close()
         self.solvers = []
@@ -199,11 +208,21 @@ class BatchNewton:
             it[act] += 1
             self._run("A")
             self._status()
-            for m in act:   # the Newton steps (device Krylov solves, host-tested)
+            for m in act:   # the Newton steps: every member's factorization, then the solves
                 torch.neg(self.R[m], out=self.rhs[m])
                 self.solvers[m].factor(self.K[m], self.sptr)
-                self.solvers[m].solve(self.rhs[m], self.d[m], self.sptr)
-                res[m].linear_iterations.append(self.solvers[m].last_iterations)
+            if self.bsolver is not None:
+                its, _, conv = self.bsolver.solve(self.rhs, self.d, active=active, stream=self.sptr)
+                for m in act:
+                    n_it = int(its[m])
+                    if not conv[m]:   # BiCGStab again alone, then GMRES from its best iterate
+                        self.solvers[m].solve(self.rhs[m], self.d[m], self.sptr)
+                        n_it += self.solvers[m].last_iterations
+                    res[m].linear_iterations.append(n_it)
+            else:
+                for m in act:
+                    self.solvers[m].solve(self.rhs[m], self.d[m], self.sptr)
+                    res[m].linear_iterations.append(self.solvers[m].last_iterations)
             for m in np.nonzero(~active)[0]:
                 self.d[m].zero_()   # members that are done stay where they are
             self._run("C")

@@ -153,6 +153,27 @@ __device__ __forceinline__ void block_partials(double (&v)[NP], double *__restri
     }
 }
 
+// Block-level sum of NP values per thread into part[p * stride] (part already offset to this
+// block's slot).
+template <int NP>
+__device__ __forceinline__ void block_partials_at(double (&v)[NP], double *__restrict__ part, int stride) {
+    __shared__ double red[NP][kThreads];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) red[p][threadIdx.x] = v[p];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) red[p][threadIdx.x] += red[p][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) part[p * stride] = red[p][0];
+    }
+}
+
 // partials of (a0, b0) and (a1, b1) (the second pair optional)
 template <int NP>
 __global__ __launch_bounds__(kThreads) void k_dots(int64_t n, const double *__restrict__ a0,
@@ -286,15 +307,14 @@ __global__ void k_ilu_color(int r0, int r1, const int32_t *__restrict__ rp, cons
 // have no upper blocks, so their upper solve x_i = D_i^-1 y_i is done in the lower-solve launch
 // of that colour.
 
-// y_i = x_i - sum over lower blocks L_ik y_k (y_k = x_k for k < c1, the rows of colour 0), rows
-// of one colour (L has identity diagonal blocks); with last, also z_i = D_i^-1 y_i
+// y_i = x_i - sum over lower blocks L_ik y_k (y_k = x_k for k < c1, the rows of colour 0) for
+// row i < r1 of one colour (L has identity diagonal blocks); with last, z_i = D_i^-1 y_i is
+// written instead. All kRowLanes lanes of the row's group take part (the row sum shuffles).
 template <typename TM>
-__global__ void k_lsolve_color(int r0, int r1, int c1, int last, const int32_t *__restrict__ rp,
-                               const int32_t *__restrict__ ci, const int32_t *__restrict__ dslot,
-                               const TM *__restrict__ LU, const double *__restrict__ x,
-                               double *__restrict__ y, double *__restrict__ z) {
-    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int i = r0 + int(g / kRowLanes), lane = int(g % kRowLanes);
+__device__ __forceinline__ void lsolve_row(int i, int r1, int lane, int c1, int last, const int32_t *__restrict__ rp,
+                                           const int32_t *__restrict__ ci, const int32_t *__restrict__ dslot,
+                                           const TM *__restrict__ LU, const double *__restrict__ x,
+                                           double *__restrict__ y, double *__restrict__ z) {
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
     if (i < r1) {
         for (int p = rp[i] + lane, p1 = dslot[i]; p < p1; p += kRowLanes) {
@@ -328,14 +348,13 @@ __global__ void k_lsolve_color(int r0, int r1, int c1, int last, const int32_t *
     }
 }
 
-// z_i = D_i^-1 (y_i - sum over upper blocks U_ij z_j), rows of one colour; y_i is read from x
-// for the rows of colour 0 (i < c1)
+// z_i = D_i^-1 (y_i - sum over upper blocks U_ij z_j) for row i < r1 of one colour; y_i is read
+// from x for the rows of colour 0 (i < c1)
 template <typename TM>
-__global__ void k_usolve_color(int r0, int r1, int c1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                               const int32_t *__restrict__ dslot, const TM *__restrict__ LU,
-                               const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ z) {
-    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int i = r0 + int(g / kRowLanes), lane = int(g % kRowLanes);
+__device__ __forceinline__ void usolve_row(int i, int r1, int lane, int c1, const int32_t *__restrict__ rp,
+                                           const int32_t *__restrict__ ci, const int32_t *__restrict__ dslot,
+                                           const TM *__restrict__ LU, const double *__restrict__ x,
+                                           const double *__restrict__ y, double *__restrict__ z) {
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
     if (i < r1) row_dot(dslot[i] + 1, rp[i + 1], lane, ci, LU, z, t0, t1, t2);
     row_sum(t0, t1, t2);
@@ -350,6 +369,24 @@ __global__ void k_usolve_color(int r0, int r1, int c1, const int32_t *__restrict
         z[3 * int64_t(i) + 1] = d[3] * t0 + d[4] * t1 + d[5] * t2;
         z[3 * int64_t(i) + 2] = d[6] * t0 + d[7] * t1 + d[8] * t2;
     }
+}
+
+// the rows r0 .. r1-1 of one colour, kRowLanes lanes per row
+template <typename TM>
+__global__ void k_lsolve_color(int r0, int r1, int c1, int last, const int32_t *__restrict__ rp,
+                               const int32_t *__restrict__ ci, const int32_t *__restrict__ dslot,
+                               const TM *__restrict__ LU, const double *__restrict__ x,
+                               double *__restrict__ y, double *__restrict__ z) {
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    lsolve_row(r0 + int(g / kRowLanes), r1, int(g % kRowLanes), c1, last, rp, ci, dslot, LU, x, y, z);
+}
+
+template <typename TM>
+__global__ void k_usolve_color(int r0, int r1, int c1, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const int32_t *__restrict__ dslot, const TM *__restrict__ LU,
+                               const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ z) {
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    usolve_row(r0 + int(g / kRowLanes), r1, int(g % kRowLanes), c1, rp, ci, dslot, LU, x, y, z);
 }
 
 // BiCGStab with its scalars on the device. Each iteration's kernels read rho, alpha, omega and
@@ -508,6 +545,271 @@ __global__ void k_copy_if(int64_t n, const double *__restrict__ x, double *__res
         xbest[i] = x[i];
 }
 
+// Batched BiCGStab (uvhb_lin_*): the systems of many members advance together, one launch per
+// step of the iteration for all of them, over flat work lists of (member, vector chunk),
+// (member, 32 block rows) and, per colour index, (member, 32 rows of that colour). Each member
+// keeps its own scalars and stop word; restarts, the iteration count and the iteration limit
+// are handled on the device too, so the host only waits until every member has stopped.
+enum : int { F_ITS = F_COUNT, F_RESTARTS, F_MAXIT, F_BWORDS = 8 };   // the batched form's flag words
+enum : int { B_FAIL = 6, B_MAXIT = 7, B_IDLE = 8 };                  // its further stop word values
+constexpr int kMaxRestarts = 20;                                      // near-breakdown restarts per solve
+constexpr int kChunk = 4 * kThreads;                                  // vector values per work item
+constexpr int kRowsPerItem = kThreads / kRowLanes;                    // block rows per work item
+
+struct LinMember {
+    const int32_t *rp, *ci, *dslot;
+    const double *A;
+    const void *LU;   // float when lu32, else double
+    double *r, *rhat, *p, *v, *s, *t, *y, *z, *w, *x, *xbest, *part, *sc;
+    int *flag;
+    int64_t n3;
+    int nchunk, lu32;
+};
+
+__device__ __forceinline__ bool bb_terminal(int w) { return w == B_CONV_S || w == B_CONV_R || w >= B_FAIL; }
+__device__ __forceinline__ bool bb_restart(int w) { return w == B_BREAK_V || w == B_BREAK_R || w == B_OMEGA0; }
+
+// xbest = x if asked for; then p = r + beta (p - omega v), or, when the last iteration asked
+// for a restart, rhat = p = r and v = 0 (the restarted scalars are set in k_bb_finish)
+__global__ __launch_bounds__(kThreads) void k_bb_p(const LinMember *__restrict__ M, const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const LinMember &L = M[it.x];
+    const int w = L.flag[F_STOP];
+    const bool cp = L.flag[F_COPY] != 0;
+    const int64_t i0 = int64_t(it.y) * kChunk, i1 = min(L.n3, i0 + kChunk);
+    if (bb_terminal(w)) {
+        if (cp)
+            for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) L.xbest[i] = L.x[i];
+        return;
+    }
+    if (bb_restart(w)) {
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+            if (cp) L.xbest[i] = L.x[i];
+            L.rhat[i] = L.r[i];
+            L.p[i] = L.r[i];
+            L.v[i] = 0.0;
+        }
+        return;
+    }
+    const double *sc = L.sc;
+    const double beta = (sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]), omega = sc[S_OMEGA];
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        if (cp) L.xbest[i] = L.x[i];
+        L.p[i] = L.r[i] + beta * (L.p[i] - omega * L.v[i]);
+    }
+}
+
+// z = M^-1 x, lower solves: one colour index per launch (which = 0: x = p, z = y; 1: x = s,
+// z = z); item (member, first row, end of the colour's rows, last + 2 c1)
+__global__ __launch_bounds__(kThreads) void k_bb_lsolve(const LinMember *__restrict__ M, const int4 *__restrict__ items,
+                                                        int which) {
+    const int4 it = items[blockIdx.x];
+    const LinMember &L = M[it.x];
+    if (bb_terminal(L.flag[F_STOP])) return;
+    const int i = it.y + int(threadIdx.x) / kRowLanes, lane = int(threadIdx.x) % kRowLanes;
+    const double *x = which ? L.s : L.p;
+    double *z = which ? L.z : L.y;
+    if (L.lu32)
+        lsolve_row(i, it.z, lane, it.w >> 1, it.w & 1, L.rp, L.ci, L.dslot, static_cast<const float *>(L.LU), x, L.w,
+                   z);
+    else
+        lsolve_row(i, it.z, lane, it.w >> 1, it.w & 1, L.rp, L.ci, L.dslot, static_cast<const double *>(L.LU), x,
+                   L.w, z);
+}
+
+// upper solves, one colour index per launch; item (member, first row, end of the colour's rows, c1)
+__global__ __launch_bounds__(kThreads) void k_bb_usolve(const LinMember *__restrict__ M, const int4 *__restrict__ items,
+                                                        int which) {
+    const int4 it = items[blockIdx.x];
+    const LinMember &L = M[it.x];
+    if (bb_terminal(L.flag[F_STOP])) return;
+    const int i = it.y + int(threadIdx.x) / kRowLanes, lane = int(threadIdx.x) % kRowLanes;
+    const double *x = which ? L.s : L.p;
+    double *z = which ? L.z : L.y;
+    if (L.lu32)
+        usolve_row(i, it.z, lane, it.w, L.rp, L.ci, L.dslot, static_cast<const float *>(L.LU), x, L.w, z);
+    else
+        usolve_row(i, it.z, lane, it.w, L.rp, L.ci, L.dslot, static_cast<const double *>(L.LU), x, L.w, z);
+}
+
+// which = 0: v = A y; 1: t = A z; item (member, first block row)
+__global__ __launch_bounds__(kThreads) void k_bb_mv(const LinMember *__restrict__ M, const int2 *__restrict__ items,
+                                                    const int *__restrict__ nrows, int which) {
+    const int2 it = items[blockIdx.x];
+    const LinMember &L = M[it.x];
+    if (bb_terminal(L.flag[F_STOP])) return;
+    const int i = it.y + int(threadIdx.x) / kRowLanes, lane = int(threadIdx.x) % kRowLanes;
+    const int nn = nrows[it.x];
+    const double *x = which ? L.z : L.y;
+    double *y = which ? L.t : L.v;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (i < nn) row_dot(L.rp[i], L.rp[i + 1], lane, L.ci, L.A, x, t0, t1, t2);
+    row_sum(t0, t1, t2);
+    if (i < nn && lane == 0) {
+        y[3 * int64_t(i)] = t0;
+        y[3 * int64_t(i) + 1] = t1;
+        y[3 * int64_t(i) + 2] = t2;
+    }
+}
+
+// partials of (rhat, v) and (v, v) (which = 0) or (t, s) and (t, t) (which = 1) into
+// part[p * nchunk + chunk]
+__global__ __launch_bounds__(kThreads) void k_bb_dots(const LinMember *__restrict__ M, const int2 *__restrict__ items,
+                                                      int which) {
+    const int2 it = items[blockIdx.x];
+    const LinMember &L = M[it.x];
+    if (bb_terminal(L.flag[F_STOP])) return;
+    const double *a0 = which ? L.t : L.rhat, *b0 = which ? L.s : L.v, *a1 = which ? L.t : L.v;
+    const int64_t i0 = int64_t(it.y) * kChunk, i1 = min(L.n3, i0 + kChunk);
+    double acc[2] = {0.0, 0.0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        acc[0] += a0[i] * b0[i];
+        acc[1] += a1[i] * a1[i];
+    }
+    block_partials_at<2>(acc, L.part + it.y, L.nchunk);
+}
+
+// as k_bi_s, per member chunk
+__global__ __launch_bounds__(kThreads) void k_bb_s(const LinMember *__restrict__ M, const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const LinMember &L = M[it.x];
+    double *sc = L.sc;
+    int *flag = L.flag;
+    const double rv = sc[S_RV];
+    const bool brk = std::fabs(rv) <= kBreak * std::sqrt(sc[S_RHAT2] * sc[S_VV]);
+    if (flag[F_STOP] != B_RUN || brk) {
+        if (brk && it.y == 0 && threadIdx.x == 0 && flag[F_STOP] == B_RUN) flag[F_STOP] = B_BREAK_V;
+        return;
+    }
+    const double alpha = sc[S_RHO] / rv;
+    if (it.y == 0 && threadIdx.x == 0) sc[S_ALPHA] = alpha;
+    const int64_t i0 = int64_t(it.y) * kChunk, i1 = min(L.n3, i0 + kChunk);
+    double acc[1] = {0.0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        const double si = L.r[i] - alpha * L.v[i];
+        L.s[i] = si;
+        L.x[i] += alpha * L.y[i];
+        acc[0] += si * si;
+    }
+    block_partials_at<1>(acc, L.part + it.y, L.nchunk);
+}
+
+// as k_bi_xr, per member chunk
+__global__ __launch_bounds__(kThreads) void k_bb_xr(const LinMember *__restrict__ M, const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const LinMember &L = M[it.x];
+    if (L.flag[F_STOP] != B_RUN) return;
+    double *sc = L.sc;
+    const double tt = sc[S_TT];
+    const double omega = tt > 0.0 ? sc[S_TS] / tt : 0.0;
+    if (it.y == 0 && threadIdx.x == 0) sc[S_OMEGA] = omega;
+    const bool cp = L.flag[F_COPY] != 0;
+    const int64_t i0 = int64_t(it.y) * kChunk, i1 = min(L.n3, i0 + kChunk);
+    double acc[2] = {0.0, 0.0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        if (cp) L.xbest[i] = L.x[i];
+        L.x[i] += omega * L.z[i];
+        const double ri = L.s[i] - omega * L.t[i];
+        L.r[i] = ri;
+        acc[0] += L.rhat[i] * ri;
+        acc[1] += ri * ri;
+    }
+    block_partials_at<2>(acc, L.part + it.y, L.nchunk);
+}
+
+// one workgroup per member: sc[slot + p] = the fixed-order sum of its chunk partials, then
+// mode 0 (first dot products of an iteration): a requested restart (rho = (rhat, rhat) =
+//   (r, r), the other scalars 1; past kMaxRestarts the member fails) and the iteration count;
+// mode 1 (after k_bb_s) and mode 2 (after k_bb_xr): the decisions of k_bi_finish, and at the
+//   end of an iteration the iteration limit; mode 3: the sums only
+__global__ __launch_bounds__(kThreads) void k_bb_finish(const LinMember *__restrict__ M, int np, int slot, int mode) {
+    const LinMember &L = M[blockIdx.x];
+    int *flag = L.flag;
+    double *sc = L.sc;
+    const int w0 = flag[F_STOP];
+    if (bb_terminal(w0)) {
+        if (threadIdx.x == 0 && mode != 0 && mode != 3) flag[F_COPY] = 0;
+        return;
+    }
+    __shared__ double red[kThreads];
+    double tot[2] = {0.0, 0.0};
+    for (int p = 0; p < np; ++p) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < L.nchunk; i += kThreads) v += L.part[p * L.nchunk + i];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = kThreads / 2; w > 0; w >>= 1) {
+            if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        tot[p] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        if (bb_restart(w0)) {
+            if (flag[F_RESTARTS] == kMaxRestarts) {
+                flag[F_STOP] = B_FAIL;
+                return;
+            }
+            ++flag[F_RESTARTS];
+            sc[S_RHO] = sc[S_RHAT2] = sc[S_RR];
+            sc[S_RHO_OLD] = sc[S_ALPHA] = sc[S_OMEGA] = 1.0;
+            flag[F_STOP] = B_RUN;
+        }
+        ++flag[F_ITS];
+    }
+    for (int p = 0; p < np; ++p) sc[slot + p] = tot[p];
+    if (mode == 0 || mode == 3) return;
+    flag[F_COPY] = 0;
+    if (mode == 1) {
+        if (flag[F_STOP] != B_RUN) return;
+        const double nrm = std::sqrt(tot[0]);
+        if (nrm <= sc[S_THR]) {
+            flag[F_STOP] = B_CONV_S;
+        } else if (nrm < sc[S_BEST]) {
+            sc[S_BEST] = nrm;
+            flag[F_COPY] = 1;
+        }
+        return;
+    }
+    if (flag[F_STOP] == B_RUN) {
+        const double rr = tot[1], nrm = std::sqrt(rr);
+        sc[S_RR] = rr;
+        if (nrm <= sc[S_THR]) {
+            flag[F_STOP] = B_CONV_R;
+            return;
+        }
+        if (nrm < sc[S_BEST]) {
+            sc[S_BEST] = nrm;
+            flag[F_COPY] = 1;
+        }
+        sc[S_RHO_OLD] = sc[S_RHO];
+        sc[S_RHO] = tot[0];
+        if (sc[S_OMEGA] == 0.0)
+            flag[F_STOP] = B_OMEGA0;
+        else if (std::fabs(tot[0]) <= kBreak * std::sqrt(sc[S_RHAT2] * rr))
+            flag[F_STOP] = B_BREAK_R;
+    }
+    if (flag[F_ITS] >= flag[F_MAXIT] && !bb_terminal(flag[F_STOP])) flag[F_STOP] = B_MAXIT;
+}
+
+// words[m] = stop word of member m
+__global__ void k_bb_words(const LinMember *__restrict__ M, int n, int *__restrict__ words) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) words[m] = M[m].flag[F_STOP];
+}
+
+// a member's counters at the start of a solve; word = B_RUN for an active member (with
+// k_bi_init), B_IDLE for one left out, B_CONV_R for a zero right-hand side
+__global__ void k_bb_start(int *__restrict__ flag, int word, int maxit) {
+    flag[F_STOP] = word;
+    flag[F_COPY] = 0;
+    flag[F_ITS] = 0;
+    flag[F_RESTARTS] = 0;
+    flag[F_MAXIT] = maxit;
+}
+
 // GMRES helpers over a basis V of k vectors of length n, stored one after another
 constexpr int kMaxKrylov = 64;   // longest GMRES cycle
 
@@ -651,7 +953,6 @@ int bicgstab(uvh_lin *L, double rtol, int maxit, double bnorm, hipStream_t s, in
              std::string &why) {
     const int64_t n = L->n3;
     const dim3 G(kRedBlocks), B(kThreads);
-    constexpr int kMaxRestarts = 20;
     hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_r);
     hipLaunchKernelGGL(k_copy, G, B, 0, s, n, L->d_b, L->d_rhat);
     hipLaunchKernelGGL(k_fill, G, B, 0, s, n, 0.0, L->d_p);
@@ -1039,6 +1340,277 @@ int uvh_lin_solve(uvh_lin *L, const double *drhs, double *dx, double rtol, int32
     if (!converged)
         return L->fail(UVH_E_NOCONV, "uvh_lin_solve: relative residual " + std::to_string(res) + " after " +
                                          std::to_string(it) + " iterations" + (why.empty() ? "" : " (" + why + ")"));
+    return UVH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Batched solves: the BiCGStab iterations of many members' systems (each a uvh_lin, factored
+// on its own) in one launch per step, over the flat work lists built at creation.
+struct uvhb_lin {
+    std::string err;
+    int device = 0, n = 0, maxcol = 0;
+    std::vector<uvh_lin *> mem;
+    LinMember *d_members = nullptr;
+    int2 *d_vitems = nullptr, *d_ritems = nullptr;
+    int4 *d_litems = nullptr, *d_uitems = nullptr;
+    int *d_nrows = nullptr, *d_flags = nullptr, *d_words = nullptr;
+    int nv = 0, nr = 0;
+    std::vector<int> loff, uoff;   // colour c's lower / upper items: [off[c], off[c+1])
+    int *h_words = nullptr;        // pinned: 2 n stop words (two iterations in flight)
+    double *h_red = nullptr;       // pinned: n values
+    int *h_its = nullptr;          // pinned: n iteration counts
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int fail(int code, const std::string &m) {
+        err = m;
+        return code;
+    }
+};
+
+namespace {
+
+void bdestroy(uvhb_lin *B) {
+    if (!B) return;
+    hipSetDevice(B->device);
+    for (void *p : {(void *)B->d_members, (void *)B->d_vitems, (void *)B->d_ritems, (void *)B->d_litems,
+                    (void *)B->d_uitems, (void *)B->d_nrows, (void *)B->d_flags, (void *)B->d_words})
+        if (p) hipFree(p);
+    for (hipEvent_t e : B->ev)
+        if (e) hipEventDestroy(e);
+    for (void *p : {(void *)B->h_words, (void *)B->h_red, (void *)B->h_its})
+        if (p) hipHostFree(p);
+    delete B;
+}
+
+// one BiCGStab iteration of every member, then the stop words into pinned slot `slot`
+hipError_t biter(uvhb_lin *B, int slot, hipStream_t s) {
+    const dim3 T(kThreads);
+    const int n = B->n;
+    hipLaunchKernelGGL(k_bb_p, dim3(B->nv), T, 0, s, B->d_members, B->d_vitems);
+    for (int which = 0; which < 2; ++which) {
+        if (which == 1) {
+            hipLaunchKernelGGL(k_bb_s, dim3(B->nv), T, 0, s, B->d_members, B->d_vitems);
+            hipLaunchKernelGGL(k_bb_finish, dim3(n), T, 0, s, B->d_members, 1, int(S_SS), 1);
+        }
+        for (int c = 0; c < B->maxcol; ++c)
+            if (B->loff[c + 1] > B->loff[c])
+                hipLaunchKernelGGL(k_bb_lsolve, dim3(B->loff[c + 1] - B->loff[c]), T, 0, s, B->d_members,
+                                   B->d_litems + B->loff[c], which);
+        for (int c = B->maxcol - 1; c >= 0; --c)
+            if (B->uoff[c + 1] > B->uoff[c])
+                hipLaunchKernelGGL(k_bb_usolve, dim3(B->uoff[c + 1] - B->uoff[c]), T, 0, s, B->d_members,
+                                   B->d_uitems + B->uoff[c], which);
+        hipLaunchKernelGGL(k_bb_mv, dim3(B->nr), T, 0, s, B->d_members, B->d_ritems, B->d_nrows, which);
+        hipLaunchKernelGGL(k_bb_dots, dim3(B->nv), T, 0, s, B->d_members, B->d_vitems, which);
+        hipLaunchKernelGGL(k_bb_finish, dim3(n), T, 0, s, B->d_members, 2, which ? int(S_TS) : int(S_RV),
+                           which ? 3 : 0);
+    }
+    hipLaunchKernelGGL(k_bb_xr, dim3(B->nv), T, 0, s, B->d_members, B->d_vitems);
+    hipLaunchKernelGGL(k_bb_finish, dim3(n), T, 0, s, B->d_members, 2, int(S_RHATR), 2);
+    hipLaunchKernelGGL(k_bb_words, dim3((n + kThreads - 1) / kThreads), T, 0, s, B->d_members, n, B->d_words);
+    hipError_t e = hipMemcpyAsync(B->h_words + slot * n, B->d_words, n * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(B->ev[slot], s);
+    return e;
+}
+
+bool word_terminal(int w) { return w == B_CONV_S || w == B_CONV_R || w >= B_FAIL; }
+
+}  // namespace
+
+extern "C" {
+
+int uvhb_lin_create(uvh_lin *const *members, int32_t n, uvhb_lin **out) {
+    if (!out || !members || n < 1) return UVH_E_ARG;
+    *out = nullptr;
+    auto *B = new uvhb_lin();
+    B->n = n;
+    B->mem.assign(members, members + n);
+    for (uvh_lin *L : B->mem)
+        if (!L || L->device != B->mem[0]->device) {
+            delete B;
+            return UVH_E_ARG;
+        }
+    B->device = B->mem[0]->device;
+    hipSetDevice(B->device);
+    std::vector<LinMember> tab(n);
+    std::vector<int2> vit, rit;
+    std::vector<int> nrows(n);
+    for (int m = 0; m < n; ++m) B->maxcol = std::max(B->maxcol, int(B->mem[m]->cstart.size()) - 1);
+    std::vector<std::vector<int4>> lit(B->maxcol), uit(B->maxcol);
+    bool ok = lalloc(&B->d_flags, int64_t(n) * F_BWORDS) == hipSuccess;
+    for (int m = 0; m < n && ok; ++m) {
+        uvh_lin *L = B->mem[m];
+        LinMember &T = tab[m];
+        T.rp = L->d_rp;
+        T.ci = L->d_ci;
+        T.dslot = L->d_dslot;
+        T.A = L->d_A;
+        T.LU = L->lu32 ? static_cast<const void *>(L->d_LUf) : static_cast<const void *>(L->d_LU);
+        T.r = L->d_r; T.rhat = L->d_rhat; T.p = L->d_p; T.v = L->d_v; T.s = L->d_s; T.t = L->d_t;
+        T.y = L->d_y; T.z = L->d_z; T.w = L->d_w; T.x = L->d_x; T.xbest = L->d_xbest;
+        T.part = L->d_part;
+        T.sc = L->d_sc;
+        T.flag = B->d_flags + int64_t(m) * F_BWORDS;
+        T.n3 = L->n3;
+        T.nchunk = int((L->n3 + kChunk - 1) / kChunk);
+        T.lu32 = L->lu32 ? 1 : 0;
+        if (2 * int64_t(T.nchunk) > int64_t(kMaxKrylov + 2) * kRedBlocks) ok = false;   // partial slots
+        nrows[m] = L->nn;
+        for (int c = 0; c < T.nchunk; ++c) vit.push_back(make_int2(m, c));
+        for (int r0 = 0; r0 < L->nn; r0 += kRowsPerItem) rit.push_back(make_int2(m, r0));
+        const std::vector<int32_t> &cs = L->cstart;
+        const int nc = int(cs.size()) - 1, c1 = cs[1];
+        for (int c = nc == 1 ? 0 : 1; c < nc; ++c)
+            for (int r0 = cs[c]; r0 < cs[c + 1]; r0 += kRowsPerItem)
+                lit[c].push_back(make_int4(m, r0, cs[c + 1], int(c == nc - 1) + 2 * (c == 0 ? 0 : c1)));
+        for (int c = nc - 2; c >= 0; --c)
+            for (int r0 = cs[c]; r0 < cs[c + 1]; r0 += kRowsPerItem) uit[c].push_back(make_int4(m, r0, cs[c + 1], c1));
+    }
+    std::vector<int4> lflat, uflat;
+    B->loff.assign(1, 0);
+    B->uoff.assign(1, 0);
+    for (int c = 0; c < B->maxcol; ++c) {
+        lflat.insert(lflat.end(), lit[c].begin(), lit[c].end());
+        uflat.insert(uflat.end(), uit[c].begin(), uit[c].end());
+        B->loff.push_back(int(lflat.size()));
+        B->uoff.push_back(int(uflat.size()));
+    }
+    B->nv = int(vit.size());
+    B->nr = int(rit.size());
+    ok = ok && lalloc(&B->d_members, n) == hipSuccess && lalloc(&B->d_vitems, vit.size()) == hipSuccess &&
+         lalloc(&B->d_ritems, rit.size()) == hipSuccess && lalloc(&B->d_litems, lflat.size()) == hipSuccess &&
+         lalloc(&B->d_uitems, uflat.size()) == hipSuccess && lalloc(&B->d_nrows, n) == hipSuccess &&
+         lalloc(&B->d_words, n) == hipSuccess &&
+         hipHostMalloc(reinterpret_cast<void **>(&B->h_words), 2 * n * sizeof(int), 0) == hipSuccess &&
+         hipHostMalloc(reinterpret_cast<void **>(&B->h_red), n * sizeof(double), 0) == hipSuccess &&
+         hipHostMalloc(reinterpret_cast<void **>(&B->h_its), n * sizeof(int), 0) == hipSuccess &&
+         hipEventCreateWithFlags(&B->ev[0], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&B->ev[1], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        bdestroy(B);
+        return UVH_E_NOMEM;
+    }
+    hipError_t e = hipMemcpy(B->d_members, tab.data(), n * sizeof(LinMember), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_vitems, vit.data(), vit.size() * sizeof(int2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_ritems, rit.data(), rit.size() * sizeof(int2), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !lflat.empty())
+        e = hipMemcpy(B->d_litems, lflat.data(), lflat.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !uflat.empty())
+        e = hipMemcpy(B->d_uitems, uflat.data(), uflat.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_nrows, nrows.data(), n * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        bdestroy(B);
+        return UVH_E_HIP;
+    }
+    *out = B;
+    return UVH_OK;
+}
+
+void uvhb_lin_destroy(uvhb_lin *B) { bdestroy(B); }
+
+const char *uvhb_lin_last_error(const uvhb_lin *B) { return B ? B->err.c_str() : "null batch solver"; }
+
+int uvhb_lin_solve(uvhb_lin *B, const double *const *drhs, double *const *dx, const uint8_t *active, double rtol,
+                   int32_t maxit, int32_t *iterations, double *relres, int32_t *converged, void *stream) {
+    if (!B) return UVH_E_ARG;
+    if (!drhs || !dx) return B->fail(UVH_E_ARG, "uvhb_lin_solve: null right-hand sides or solutions");
+    if (!(rtol > 0.0) || maxit < 1) return B->fail(UVH_E_ARG, "uvhb_lin_solve: rtol must be > 0 and maxit >= 1");
+    const int n = B->n;
+    for (int m = 0; m < n; ++m) {
+        if (active && !active[m]) continue;
+        if (!drhs[m] || !dx[m]) return B->fail(UVH_E_ARG, "uvhb_lin_solve: null vector of member " + std::to_string(m));
+        if (!B->mem[m]->factored)
+            return B->fail(UVH_E_STATE, "uvhb_lin_solve: member " + std::to_string(m) + " has no factorization");
+    }
+    hipSetDevice(B->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 G(kRedBlocks), T(kThreads);
+    // right-hand sides in the solvers' order and their norms
+    for (int m = 0; m < n; ++m) {
+        if (active && !active[m]) continue;
+        uvh_lin *L = B->mem[m];
+        hipLaunchKernelGGL(k_gather_rhs, dim3(grid_of(L->n3)), T, 0, s, L->nn, L->d_newnode, L->d_fixed, drhs[m], L->d_b);
+        hipLaunchKernelGGL(k_fill, G, T, 0, s, L->n3, 0.0, L->d_x);
+        hipLaunchKernelGGL(k_dots<1>, G, T, 0, s, L->n3, L->d_b, L->d_b, nullptr, nullptr, L->d_part);
+        hipLaunchKernelGGL(k_finish, dim3(1), T, 0, s, kRedBlocks, 1, L->d_part, L->d_out);
+        hipMemcpyAsync(B->h_red + m, L->d_out, sizeof(double), hipMemcpyDeviceToHost, s);
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
+    std::vector<double> bnorm(n, 0.0);
+    for (int m = 0; m < n; ++m) {
+        uvh_lin *L = B->mem[m];
+        int *flag = B->d_flags + int64_t(m) * F_BWORDS;
+        const bool act = !active || active[m];
+        bnorm[m] = act ? std::sqrt(B->h_red[m]) : 0.0;
+        if (!act) {
+            hipLaunchKernelGGL(k_bb_start, dim3(1), dim3(1), 0, s, flag, int(B_IDLE), maxit);
+        } else if (bnorm[m] == 0.0) {
+            hipLaunchKernelGGL(k_bb_start, dim3(1), dim3(1), 0, s, flag, int(B_CONV_R), maxit);
+        } else {
+            hipLaunchKernelGGL(k_copy, G, T, 0, s, L->n3, L->d_b, L->d_r);
+            hipLaunchKernelGGL(k_copy, G, T, 0, s, L->n3, L->d_b, L->d_rhat);
+            hipLaunchKernelGGL(k_fill, G, T, 0, s, L->n3, 0.0, L->d_p);
+            hipLaunchKernelGGL(k_fill, G, T, 0, s, L->n3, 0.0, L->d_v);
+            hipLaunchKernelGGL(k_fill, G, T, 0, s, L->n3, 0.0, L->d_xbest);
+            hipLaunchKernelGGL(k_bb_start, dim3(1), dim3(1), 0, s, flag, int(B_RUN), maxit);
+            hipLaunchKernelGGL(k_bi_init, dim3(1), dim3(1), 0, s, L->d_sc, flag, bnorm[m], rtol * bnorm[m]);
+        }
+    }
+    // iterations, one in flight ahead of the stop words being read; every member stops on the
+    // device (tolerance, breakdown past the restarts, or the iteration limit), so at most
+    // maxit + kMaxRestarts + 1 iterations are needed
+    const int cap = maxit + kMaxRestarts + 2;
+    int launched = 0, done = 0;
+    if ((e = biter(B, 1, s)) != hipSuccess) return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
+    launched = 1;
+    for (;;) {
+        const int cur = done + 1;
+        if (launched == cur && launched < cap) {
+            if ((e = biter(B, (launched + 1) & 1, s)) != hipSuccess)
+                return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
+            ++launched;
+        }
+        if ((e = hipEventSynchronize(B->ev[cur & 1])) != hipSuccess)
+            return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
+        done = cur;
+        bool all = true;
+        for (int m = 0; m < n && all; ++m) all = word_terminal(B->h_words[(cur & 1) * n + m]);
+        if (all || done >= cap) break;
+    }
+    const int *words = B->h_words + (done & 1) * n;
+    std::vector<int> word(words, words + n);
+    // best iterate where the tolerance was missed, the true residual, the solution in [ub; vb; h]
+    for (int m = 0; m < n; ++m) {
+        if (active && !active[m]) continue;
+        uvh_lin *L = B->mem[m];
+        int *flag = B->d_flags + int64_t(m) * F_BWORDS;
+        if (bnorm[m] > 0.0) {
+            if (word[m] != B_CONV_S && word[m] != B_CONV_R) {
+                hipLaunchKernelGGL(k_copy_if, G, T, 0, s, L->n3, L->d_x, L->d_xbest, flag);
+                hipLaunchKernelGGL(k_copy, G, T, 0, s, L->n3, L->d_xbest, L->d_x);
+            }
+            block_mv(L, L->d_x, L->d_w, s);
+            hipLaunchKernelGGL(k_residual, G, T, 0, s, L->n3, L->d_b, L->d_w, L->d_part);
+            hipLaunchKernelGGL(k_finish, dim3(1), T, 0, s, kRedBlocks, 1, L->d_part, L->d_out);
+            hipMemcpyAsync(B->h_red + m, L->d_out, sizeof(double), hipMemcpyDeviceToHost, s);
+            hipMemcpyAsync(B->h_its + m, flag + F_ITS, sizeof(int), hipMemcpyDeviceToHost, s);
+        }
+        hipLaunchKernelGGL(k_scatter_sol, dim3(grid_of(L->n3)), T, 0, s, L->nn, L->d_newnode, L->d_x, dx[m]);
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+        return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
+    int nfail = 0;
+    for (int m = 0; m < n; ++m) {
+        const bool act = !active || active[m];
+        const bool conv = act && (bnorm[m] == 0.0 || word[m] == B_CONV_S || word[m] == B_CONV_R);
+        if (iterations) iterations[m] = act && bnorm[m] > 0.0 ? B->h_its[m] : 0;
+        if (relres) relres[m] = act && bnorm[m] > 0.0 ? std::sqrt(B->h_red[m]) / bnorm[m] : 0.0;
+        if (converged) converged[m] = conv ? 1 : 0;
+        if (act && !conv) ++nfail;
+    }
+    if (nfail) return B->fail(UVH_E_NOCONV, "uvhb_lin_solve: " + std::to_string(nfail) + " member(s) missed the tolerance");
     return UVH_OK;
 }
 

@@ -15,7 +15,7 @@ import numpy as np
 from . import uvh
 
 EXPORTED_LIN = ("uvh_lin_create", "uvh_lin_destroy", "uvh_lin_last_error", "uvh_lin_set_krylov", "uvh_lin_factor",
-                "uvh_lin_solve")
+                "uvh_lin_solve", "uvhb_lin_create", "uvhb_lin_destroy", "uvhb_lin_last_error", "uvhb_lin_solve")
 
 UVH_E_NOCONV = 8
 METHODS = {"bicgstab": 0, "gmres": 1, "auto": 2}
@@ -40,6 +40,14 @@ def _bind(lib):
     lib.uvh_lin_factor.restype = C.c_int
     lib.uvh_lin_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), vp]
     lib.uvh_lin_solve.restype = C.c_int
+    lib.uvhb_lin_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    lib.uvhb_lin_create.restype = C.c_int
+    lib.uvhb_lin_destroy.argtypes = [vp]
+    lib.uvhb_lin_destroy.restype = None
+    lib.uvhb_lin_last_error.argtypes = [vp]
+    lib.uvhb_lin_last_error.restype = C.c_char_p
+    lib.uvhb_lin_solve.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int32, vp, vp, vp, vp]
+    lib.uvhb_lin_solve.restype = C.c_int
     _bound = True
     return lib
 
@@ -125,3 +133,53 @@ class GpuLinearSolver:
         if st != uvh.UVH_OK and (st != UVH_E_NOCONV or self.raise_on_noconv):
             raise self._err(st, "uvh_lin_solve")
         return its.value, rel.value
+
+
+class GpuBatchLinearSolver:
+    """BiCGStab on many members' systems at once (``uvhb_lin_*``): one launch per step of the
+    iteration for all members, each member a ``GpuLinearSolver`` of its own mesh (factored with
+    its ``factor`` before ``solve``). The solvers must outlive this object."""
+
+    def __init__(self, solvers, rtol: float = 1e-8, maxit: int = 5000):
+        self.lib = _bind(uvh.load_library())
+        self.solvers = list(solvers)
+        self.n = len(self.solvers)
+        arr = (C.c_void_p * self.n)(*[sv.h.value for sv in self.solvers])
+        h = C.c_void_p()
+        st = self.lib.uvhb_lin_create(arr, self.n, C.byref(h))
+        if st != uvh.UVH_OK:
+            raise uvh.UvhLibraryError(f"uvhb_lin_create failed with status {st}")
+        self.h = h
+        self.rtol, self.maxit = float(rtol), int(maxit)
+        self.last_iterations = np.zeros(self.n, dtype=np.int32)
+        self.last_relres = np.zeros(self.n)
+        self.last_converged = np.zeros(self.n, dtype=np.int32)
+        self.last_message = ""
+
+    def close(self):
+        if self.h:
+            self.lib.uvhb_lin_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, rhs, x, active=None, stream=None, rtol: float | None = None, maxit: int | None = None):
+        """x[m] = K_m^-1 rhs[m] for the active members (device vectors); returns (iterations,
+        true relative residuals, converged) per member. A member that misses the tolerance keeps
+        its best iterate, as the single solver does (no exception)."""
+        vp = C.c_void_p
+        r = (vp * self.n)(*[uvh._ptr(v) for v in rhs])
+        xx = (vp * self.n)(*[uvh._ptr(v) for v in x])
+        act = None if active is None else np.ascontiguousarray(np.asarray(active, dtype=np.uint8))
+        st = self.lib.uvhb_lin_solve(self.h, r, xx, None if act is None else act.ctypes.data,
+                                     self.rtol if rtol is None else rtol, self.maxit if maxit is None else maxit,
+                                     self.last_iterations.ctypes.data, self.last_relres.ctypes.data,
+                                     self.last_converged.ctypes.data, stream)
+        self.last_message = "" if st == uvh.UVH_OK else self.lib.uvhb_lin_last_error(self.h).decode()
+        if st != uvh.UVH_OK and st != UVH_E_NOCONV:
+            raise uvh.UvhLibraryError(f"uvhb_lin_solve: status {st}: {self.last_message}")
+        return self.last_iterations.copy(), self.last_relres.copy(), self.last_converged.copy()
