@@ -267,7 +267,15 @@ int uvh_lin_solve(uvh_lin *lin, const double *drhs, double *dx, double rtol, int
  *   every member with active[m] != 0 (active NULL: all), to ||rhs - K dx|| <= rtol ||rhs||
  *   within maxit iterations; iterations[m], relres[m] (true relative residual) and converged[m]
  *   are set for every member, 0 for inactive ones (each array may be NULL). UVH_E_NOCONV if an
- *   active member missed the tolerance; its dx then holds its best iterate. */
+ *   active member missed the tolerance; its dx then holds its best iterate.
+ * The same solve in steps, for callers that capture the iterations in a HIP graph:
+ * uvhb_lin_begin: the right-hand sides and their norms (one host synchronisation) and every
+ *   member's start; uvhb_lin_iterate: `iterations` iterations of every member, launches only (no
+ *   synchronisation, no allocation: capturable), then the members' stop words copied to pinned
+ *   host memory; iterations after a member has stopped do nothing for it; uvhb_lin_stopped: 1
+ *   when every stop word read by the last uvhb_lin_iterate (the caller synchronises first)
+ *   says stopped, else 0; uvhb_lin_end: best iterates, true residuals and the solutions, as
+ *   uvhb_lin_solve returns them. */
 typedef struct uvhb_lin uvhb_lin;
 
 int uvhb_lin_create(uvh_lin *const *members, int32_t n, uvhb_lin **batch);
@@ -275,6 +283,12 @@ void uvhb_lin_destroy(uvhb_lin *batch);
 const char *uvhb_lin_last_error(const uvhb_lin *batch);
 int uvhb_lin_solve(uvhb_lin *batch, const double *const *drhs, double *const *dx, const uint8_t *active, double rtol,
                    int32_t maxit, int32_t *iterations, double *relres, int32_t *converged, void *stream);
+int uvhb_lin_begin(uvhb_lin *batch, const double *const *drhs, const uint8_t *active, double rtol, int32_t maxit,
+                   void *stream);
+int uvhb_lin_iterate(uvhb_lin *batch, int32_t iterations, void *stream);
+int uvhb_lin_stopped(const uvhb_lin *batch);
+int uvhb_lin_end(uvhb_lin *batch, double *const *dx, int32_t *iterations, double *relres, int32_t *converged,
+                 void *stream);
 
 /* ---------------------------------------------------------------- batched assembly
  * Many independent meshes (ensemble members with perturbed C and AGlen, mixed 3-, 6- and

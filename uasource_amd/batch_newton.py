@@ -9,11 +9,12 @@ and the backtracking of ``BackTracking.m``), but the members step together:
 * graph A: the trial state of every member set to its state, then the R and K of every member
   in one batched assembly (``uvhb_assemble_device``);
 * the step ``K d = -R`` of every member on the device: each member's block ILU(0)
-  (``uvh_lin_factor``), then one batched BiCGStab for all members (``uvhb_lin_solve``: one
-  launch per step of the iteration for every member, convergence tested per member on the
-  device, the host waiting once per iteration until all have stopped), outside the graphs; a
-  member that misses the tolerance is solved again alone with BiCGStab and the GMRES fall-back,
-  as a single mesh is;
+  (``uvh_lin_factor``), then one batched BiCGStab for all members (``uvhb_lin_*``: one launch
+  per step of the iteration for every member, convergence, restarts and the iteration limit
+  per member on the device); graph S holds ``KRYLOV_CHUNK`` iterations of every member and is
+  replayed until every member has stopped (an iteration after a member's stop does nothing
+  for it); a member that misses the tolerance is solved again alone with BiCGStab and the
+  GMRES fall-back, as a single mesh is;
 * graph C: the cost at gamma = 0 of every member (|w R|^2 and -w R . d, ``uvhb_cost_device``);
 * graph B: the trial state x + gamma_m d_m of every member, with each member's gamma in a
   device vector, its R (one batched R-only assembly) and its cost.
@@ -32,6 +33,7 @@ from .batch import UvhBatch
 from .linsolve import GpuBatchLinearSolver, GpuLinearSolver
 
 EPS = np.finfo(float).eps
+KRYLOV_CHUNK = 8   # BiCGStab iterations per replay of graph S (per eager call in the eager run)
 
 
 class BatchNewton:
@@ -150,6 +152,28 @@ class BatchNewton:
             self._graphs[name] = g
         self.stream.synchronize()
 
+    def _krylov(self, active):
+        """The batched BiCGStab of the active members' steps: graph S (captured at the first
+        call, inside a begun solve) or the same iterations eagerly, until every member stops."""
+        torch, bs = self.torch, self.bsolver
+        bs.begin(self.rhs, active=active, stream=self.sptr)
+        with torch.cuda.stream(self.stream):
+            if self.graphs and "S" not in self._graphs:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=self.stream):
+                    bs.iterate(KRYLOV_CHUNK, self.sptr)
+                self._graphs["S"] = g
+            while True:
+                if self.graphs:
+                    self._graphs["S"].replay()
+                    self.n_graph_replays += 1
+                else:
+                    bs.iterate(KRYLOV_CHUNK, self.sptr)
+                self.stream.synchronize()
+                if bs.stopped():
+                    break
+        return bs.end(self.d, stream=self.sptr)
+
     def _status(self):
         self.stream.synchronize()
         st = self.batch.sync_status()
@@ -212,7 +236,7 @@ class BatchNewton:
                 torch.neg(self.R[m], out=self.rhs[m])
                 self.solvers[m].factor(self.K[m], self.sptr)
             if self.bsolver is not None:
-                its, _, conv = self.bsolver.solve(self.rhs, self.d, active=active, stream=self.sptr)
+                its, _, conv = self._krylov(active)
                 for m in act:
                     n_it = int(its[m])
                     if not conv[m]:   # BiCGStab again alone, then GMRES from its best iterate

@@ -1361,7 +1361,12 @@ struct uvhb_lin {
     int *h_words = nullptr;        // pinned: 2 n stop words (two iterations in flight)
     double *h_red = nullptr;       // pinned: n values
     int *h_its = nullptr;          // pinned: n iteration counts
+    int *h_flags = nullptr;        // pinned: the members' flag words at the end of a solve
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // the solve in progress (uvhb_lin_begin .. uvhb_lin_end)
+    std::vector<double> bnorm;
+    std::vector<uint8_t> act;
+    bool begun = false;
     int fail(int code, const std::string &m) {
         err = m;
         return code;
@@ -1378,13 +1383,13 @@ void bdestroy(uvhb_lin *B) {
         if (p) hipFree(p);
     for (hipEvent_t e : B->ev)
         if (e) hipEventDestroy(e);
-    for (void *p : {(void *)B->h_words, (void *)B->h_red, (void *)B->h_its})
+    for (void *p : {(void *)B->h_words, (void *)B->h_red, (void *)B->h_its, (void *)B->h_flags})
         if (p) hipHostFree(p);
     delete B;
 }
 
-// one BiCGStab iteration of every member, then the stop words into pinned slot `slot`
-hipError_t biter(uvhb_lin *B, int slot, hipStream_t s) {
+// one BiCGStab iteration of every member (launches only)
+void biter_launch(uvhb_lin *B, hipStream_t s) {
     const dim3 T(kThreads);
     const int n = B->n;
     hipLaunchKernelGGL(k_bb_p, dim3(B->nv), T, 0, s, B->d_members, B->d_vitems);
@@ -1408,10 +1413,36 @@ hipError_t biter(uvhb_lin *B, int slot, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_bb_xr, dim3(B->nv), T, 0, s, B->d_members, B->d_vitems);
     hipLaunchKernelGGL(k_bb_finish, dim3(n), T, 0, s, B->d_members, 2, int(S_RHATR), 2);
-    hipLaunchKernelGGL(k_bb_words, dim3((n + kThreads - 1) / kThreads), T, 0, s, B->d_members, n, B->d_words);
-    hipError_t e = hipMemcpyAsync(B->h_words + slot * n, B->d_words, n * sizeof(int), hipMemcpyDeviceToHost, s);
+}
+
+// the members' stop words into pinned slot `slot`
+hipError_t bwords(uvhb_lin *B, int slot, hipStream_t s) {
+    const int n = B->n;
+    hipLaunchKernelGGL(k_bb_words, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, s, B->d_members, n,
+                       B->d_words);
+    return hipMemcpyAsync(B->h_words + slot * n, B->d_words, n * sizeof(int), hipMemcpyDeviceToHost, s);
+}
+
+// one iteration of every member, then its stop words into slot `slot` and that slot's event
+hipError_t biter(uvhb_lin *B, int slot, hipStream_t s) {
+    biter_launch(B, s);
+    hipError_t e = bwords(B, slot, s);
     if (e == hipSuccess) e = hipEventRecord(B->ev[slot], s);
     return e;
+}
+
+// where a member missed the tolerance: x = xbest, after a copy of x itself if its last residual
+// check asked for one; nothing for a converged or idle member
+__global__ void k_bb_best(int64_t n, double *__restrict__ x, double *__restrict__ xbest, const int *__restrict__ flag) {
+    const int w = flag[F_STOP];
+    if (w == B_CONV_S || w == B_CONV_R || w == B_IDLE) return;
+    const bool cp = flag[F_COPY] != 0;
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+        if (cp)
+            xbest[i] = x[i];
+        else
+            x[i] = xbest[i];
+    }
 }
 
 bool word_terminal(int w) { return w == B_CONV_S || w == B_CONV_R || w >= B_FAIL; }
@@ -1485,6 +1516,7 @@ int uvhb_lin_create(uvh_lin *const *members, int32_t n, uvhb_lin **out) {
          hipHostMalloc(reinterpret_cast<void **>(&B->h_words), 2 * n * sizeof(int), 0) == hipSuccess &&
          hipHostMalloc(reinterpret_cast<void **>(&B->h_red), n * sizeof(double), 0) == hipSuccess &&
          hipHostMalloc(reinterpret_cast<void **>(&B->h_its), n * sizeof(int), 0) == hipSuccess &&
+         hipHostMalloc(reinterpret_cast<void **>(&B->h_flags), size_t(n) * F_BWORDS * sizeof(int), 0) == hipSuccess &&
          hipEventCreateWithFlags(&B->ev[0], hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&B->ev[1], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
@@ -1511,24 +1543,28 @@ void uvhb_lin_destroy(uvhb_lin *B) { bdestroy(B); }
 
 const char *uvhb_lin_last_error(const uvhb_lin *B) { return B ? B->err.c_str() : "null batch solver"; }
 
-int uvhb_lin_solve(uvhb_lin *B, const double *const *drhs, double *const *dx, const uint8_t *active, double rtol,
-                   int32_t maxit, int32_t *iterations, double *relres, int32_t *converged, void *stream) {
+int uvhb_lin_begin(uvhb_lin *B, const double *const *drhs, const uint8_t *active, double rtol, int32_t maxit,
+                   void *stream) {
     if (!B) return UVH_E_ARG;
-    if (!drhs || !dx) return B->fail(UVH_E_ARG, "uvhb_lin_solve: null right-hand sides or solutions");
-    if (!(rtol > 0.0) || maxit < 1) return B->fail(UVH_E_ARG, "uvhb_lin_solve: rtol must be > 0 and maxit >= 1");
+    if (!drhs) return B->fail(UVH_E_ARG, "uvhb_lin_begin: null right-hand sides");
+    if (!(rtol > 0.0) || maxit < 1) return B->fail(UVH_E_ARG, "uvhb_lin_begin: rtol must be > 0 and maxit >= 1");
     const int n = B->n;
+    B->act.assign(n, 1);
     for (int m = 0; m < n; ++m) {
-        if (active && !active[m]) continue;
-        if (!drhs[m] || !dx[m]) return B->fail(UVH_E_ARG, "uvhb_lin_solve: null vector of member " + std::to_string(m));
+        if (active && !active[m]) {
+            B->act[m] = 0;
+            continue;
+        }
+        if (!drhs[m]) return B->fail(UVH_E_ARG, "uvhb_lin_begin: null right-hand side of member " + std::to_string(m));
         if (!B->mem[m]->factored)
-            return B->fail(UVH_E_STATE, "uvhb_lin_solve: member " + std::to_string(m) + " has no factorization");
+            return B->fail(UVH_E_STATE, "uvhb_lin_begin: member " + std::to_string(m) + " has no factorization");
     }
     hipSetDevice(B->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 G(kRedBlocks), T(kThreads);
     // right-hand sides in the solvers' order and their norms
     for (int m = 0; m < n; ++m) {
-        if (active && !active[m]) continue;
+        if (!B->act[m]) continue;
         uvh_lin *L = B->mem[m];
         hipLaunchKernelGGL(k_gather_rhs, dim3(grid_of(L->n3)), T, 0, s, L->nn, L->d_newnode, L->d_fixed, drhs[m], L->d_b);
         hipLaunchKernelGGL(k_fill, G, T, 0, s, L->n3, 0.0, L->d_x);
@@ -1537,16 +1573,15 @@ int uvhb_lin_solve(uvhb_lin *B, const double *const *drhs, double *const *dx, co
         hipMemcpyAsync(B->h_red + m, L->d_out, sizeof(double), hipMemcpyDeviceToHost, s);
     }
     hipError_t e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
-    std::vector<double> bnorm(n, 0.0);
+    if (e != hipSuccess) return B->fail(UVH_E_HIP, std::string("uvhb_lin_begin: ") + hipGetErrorString(e));
+    B->bnorm.assign(n, 0.0);
     for (int m = 0; m < n; ++m) {
         uvh_lin *L = B->mem[m];
         int *flag = B->d_flags + int64_t(m) * F_BWORDS;
-        const bool act = !active || active[m];
-        bnorm[m] = act ? std::sqrt(B->h_red[m]) : 0.0;
-        if (!act) {
+        B->bnorm[m] = B->act[m] ? std::sqrt(B->h_red[m]) : 0.0;
+        if (!B->act[m]) {
             hipLaunchKernelGGL(k_bb_start, dim3(1), dim3(1), 0, s, flag, int(B_IDLE), maxit);
-        } else if (bnorm[m] == 0.0) {
+        } else if (B->bnorm[m] == 0.0) {
             hipLaunchKernelGGL(k_bb_start, dim3(1), dim3(1), 0, s, flag, int(B_CONV_R), maxit);
         } else {
             hipLaunchKernelGGL(k_copy, G, T, 0, s, L->n3, L->d_b, L->d_r);
@@ -1555,63 +1590,108 @@ int uvhb_lin_solve(uvhb_lin *B, const double *const *drhs, double *const *dx, co
             hipLaunchKernelGGL(k_fill, G, T, 0, s, L->n3, 0.0, L->d_v);
             hipLaunchKernelGGL(k_fill, G, T, 0, s, L->n3, 0.0, L->d_xbest);
             hipLaunchKernelGGL(k_bb_start, dim3(1), dim3(1), 0, s, flag, int(B_RUN), maxit);
-            hipLaunchKernelGGL(k_bi_init, dim3(1), dim3(1), 0, s, L->d_sc, flag, bnorm[m], rtol * bnorm[m]);
+            hipLaunchKernelGGL(k_bi_init, dim3(1), dim3(1), 0, s, L->d_sc, flag, B->bnorm[m], rtol * B->bnorm[m]);
         }
     }
+    if ((e = hipGetLastError()) != hipSuccess)
+        return B->fail(UVH_E_HIP, std::string("uvhb_lin_begin: ") + hipGetErrorString(e));
+    B->begun = true;
+    return UVH_OK;
+}
+
+int uvhb_lin_iterate(uvhb_lin *B, int32_t iterations, void *stream) {
+    if (!B) return UVH_E_ARG;
+    if (!B->begun) return B->fail(UVH_E_STATE, "uvhb_lin_iterate: no solve begun (uvhb_lin_begin)");
+    if (iterations < 1) return B->fail(UVH_E_ARG, "uvhb_lin_iterate: iterations must be >= 1");
+    hipSetDevice(B->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int k = 0; k < iterations; ++k) biter_launch(B, s);
+    hipError_t e = bwords(B, 0, s);
+    if (e != hipSuccess) return B->fail(UVH_E_HIP, std::string("uvhb_lin_iterate: ") + hipGetErrorString(e));
+    return UVH_OK;
+}
+
+int uvhb_lin_stopped(const uvhb_lin *B) {
+    if (!B) return UVH_E_ARG;
+    for (int m = 0; m < B->n; ++m)
+        if (!word_terminal(B->h_words[m])) return 0;
+    return 1;
+}
+
+int uvhb_lin_end(uvhb_lin *B, double *const *dx, int32_t *iterations, double *relres, int32_t *converged,
+                 void *stream) {
+    if (!B) return UVH_E_ARG;
+    if (!B->begun) return B->fail(UVH_E_STATE, "uvhb_lin_end: no solve begun (uvhb_lin_begin)");
+    if (!dx) return B->fail(UVH_E_ARG, "uvhb_lin_end: null solutions");
+    const int n = B->n;
+    for (int m = 0; m < n; ++m)
+        if (B->act[m] && !dx[m]) return B->fail(UVH_E_ARG, "uvhb_lin_end: null solution of member " + std::to_string(m));
+    hipSetDevice(B->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 G(kRedBlocks), T(kThreads);
+    // best iterate where the tolerance was missed, the true residual, the solution in [ub; vb; h]
+    for (int m = 0; m < n; ++m) {
+        if (!B->act[m]) continue;
+        uvh_lin *L = B->mem[m];
+        int *flag = B->d_flags + int64_t(m) * F_BWORDS;
+        if (B->bnorm[m] > 0.0) {
+            hipLaunchKernelGGL(k_bb_best, G, T, 0, s, L->n3, L->d_x, L->d_xbest, flag);
+            block_mv(L, L->d_x, L->d_w, s);
+            hipLaunchKernelGGL(k_residual, G, T, 0, s, L->n3, L->d_b, L->d_w, L->d_part);
+            hipLaunchKernelGGL(k_finish, dim3(1), T, 0, s, kRedBlocks, 1, L->d_part, L->d_out);
+            hipMemcpyAsync(B->h_red + m, L->d_out, sizeof(double), hipMemcpyDeviceToHost, s);
+        }
+        hipLaunchKernelGGL(k_scatter_sol, dim3(grid_of(L->n3)), T, 0, s, L->nn, L->d_newnode, L->d_x, dx[m]);
+    }
+    hipMemcpyAsync(B->h_flags, B->d_flags, size_t(n) * F_BWORDS * sizeof(int), hipMemcpyDeviceToHost, s);
+    hipError_t e = hipStreamSynchronize(s);
+    B->begun = false;
+    if (e != hipSuccess) return B->fail(UVH_E_HIP, std::string("uvhb_lin_end: ") + hipGetErrorString(e));
+    int nfail = 0;
+    for (int m = 0; m < n; ++m) {
+        const int *fl = B->h_flags + int64_t(m) * F_BWORDS;
+        const bool live = B->act[m] && B->bnorm[m] > 0.0;
+        const bool conv = B->act[m] && (B->bnorm[m] == 0.0 || fl[F_STOP] == B_CONV_S || fl[F_STOP] == B_CONV_R);
+        if (iterations) iterations[m] = live ? fl[F_ITS] : 0;
+        if (relres) relres[m] = live ? std::sqrt(B->h_red[m]) / B->bnorm[m] : 0.0;
+        if (converged) converged[m] = conv ? 1 : 0;
+        if (B->act[m] && !conv) ++nfail;
+    }
+    if (nfail)
+        return B->fail(UVH_E_NOCONV, "uvhb_lin_end: " + std::to_string(nfail) + " member(s) missed the tolerance");
+    return UVH_OK;
+}
+
+int uvhb_lin_solve(uvhb_lin *B, const double *const *drhs, double *const *dx, const uint8_t *active, double rtol,
+                   int32_t maxit, int32_t *iterations, double *relres, int32_t *converged, void *stream) {
+    if (!B) return UVH_E_ARG;
+    if (!dx) return B->fail(UVH_E_ARG, "uvhb_lin_solve: null solutions");
+    int st = uvhb_lin_begin(B, drhs, active, rtol, maxit, stream);
+    if (st != UVH_OK) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
     // iterations, one in flight ahead of the stop words being read; every member stops on the
     // device (tolerance, breakdown past the restarts, or the iteration limit), so at most
     // maxit + kMaxRestarts + 1 iterations are needed
-    const int cap = maxit + kMaxRestarts + 2;
-    int launched = 0, done = 0;
-    if ((e = biter(B, 1, s)) != hipSuccess) return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
-    launched = 1;
-    for (;;) {
+    const int n = B->n, cap = maxit + kMaxRestarts + 2;
+    int launched = 1, done = 0;
+    hipError_t e = biter(B, 1, s);
+    while (e == hipSuccess) {
         const int cur = done + 1;
         if (launched == cur && launched < cap) {
-            if ((e = biter(B, (launched + 1) & 1, s)) != hipSuccess)
-                return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
+            if ((e = biter(B, (launched + 1) & 1, s)) != hipSuccess) break;
             ++launched;
         }
-        if ((e = hipEventSynchronize(B->ev[cur & 1])) != hipSuccess)
-            return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
+        if ((e = hipEventSynchronize(B->ev[cur & 1])) != hipSuccess) break;
         done = cur;
         bool all = true;
         for (int m = 0; m < n && all; ++m) all = word_terminal(B->h_words[(cur & 1) * n + m]);
         if (all || done >= cap) break;
     }
-    const int *words = B->h_words + (done & 1) * n;
-    std::vector<int> word(words, words + n);
-    // best iterate where the tolerance was missed, the true residual, the solution in [ub; vb; h]
-    for (int m = 0; m < n; ++m) {
-        if (active && !active[m]) continue;
-        uvh_lin *L = B->mem[m];
-        int *flag = B->d_flags + int64_t(m) * F_BWORDS;
-        if (bnorm[m] > 0.0) {
-            if (word[m] != B_CONV_S && word[m] != B_CONV_R) {
-                hipLaunchKernelGGL(k_copy_if, G, T, 0, s, L->n3, L->d_x, L->d_xbest, flag);
-                hipLaunchKernelGGL(k_copy, G, T, 0, s, L->n3, L->d_xbest, L->d_x);
-            }
-            block_mv(L, L->d_x, L->d_w, s);
-            hipLaunchKernelGGL(k_residual, G, T, 0, s, L->n3, L->d_b, L->d_w, L->d_part);
-            hipLaunchKernelGGL(k_finish, dim3(1), T, 0, s, kRedBlocks, 1, L->d_part, L->d_out);
-            hipMemcpyAsync(B->h_red + m, L->d_out, sizeof(double), hipMemcpyDeviceToHost, s);
-            hipMemcpyAsync(B->h_its + m, flag + F_ITS, sizeof(int), hipMemcpyDeviceToHost, s);
-        }
-        hipLaunchKernelGGL(k_scatter_sol, dim3(grid_of(L->n3)), T, 0, s, L->nn, L->d_newnode, L->d_x, dx[m]);
-    }
-    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+    if (e != hipSuccess) {
+        B->begun = false;
         return B->fail(UVH_E_HIP, std::string("uvhb_lin_solve: ") + hipGetErrorString(e));
-    int nfail = 0;
-    for (int m = 0; m < n; ++m) {
-        const bool act = !active || active[m];
-        const bool conv = act && (bnorm[m] == 0.0 || word[m] == B_CONV_S || word[m] == B_CONV_R);
-        if (iterations) iterations[m] = act && bnorm[m] > 0.0 ? B->h_its[m] : 0;
-        if (relres) relres[m] = act && bnorm[m] > 0.0 ? std::sqrt(B->h_red[m]) / bnorm[m] : 0.0;
-        if (converged) converged[m] = conv ? 1 : 0;
-        if (act && !conv) ++nfail;
     }
-    if (nfail) return B->fail(UVH_E_NOCONV, "uvhb_lin_solve: " + std::to_string(nfail) + " member(s) missed the tolerance");
-    return UVH_OK;
+    return uvhb_lin_end(B, dx, iterations, relres, converged, stream);
 }
 
 }  // extern "C"

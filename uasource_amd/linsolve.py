@@ -15,7 +15,8 @@ import numpy as np
 from . import uvh
 
 EXPORTED_LIN = ("uvh_lin_create", "uvh_lin_destroy", "uvh_lin_last_error", "uvh_lin_set_krylov", "uvh_lin_factor",
-                "uvh_lin_solve", "uvhb_lin_create", "uvhb_lin_destroy", "uvhb_lin_last_error", "uvhb_lin_solve")
+                "uvh_lin_solve", "uvhb_lin_create", "uvhb_lin_destroy", "uvhb_lin_last_error", "uvhb_lin_solve",
+                "uvhb_lin_begin", "uvhb_lin_iterate", "uvhb_lin_stopped", "uvhb_lin_end")
 
 UVH_E_NOCONV = 8
 METHODS = {"bicgstab": 0, "gmres": 1, "auto": 2}
@@ -48,6 +49,14 @@ def _bind(lib):
     lib.uvhb_lin_last_error.restype = C.c_char_p
     lib.uvhb_lin_solve.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int32, vp, vp, vp, vp]
     lib.uvhb_lin_solve.restype = C.c_int
+    lib.uvhb_lin_begin.argtypes = [vp, vp, vp, C.c_double, C.c_int32, vp]
+    lib.uvhb_lin_begin.restype = C.c_int
+    lib.uvhb_lin_iterate.argtypes = [vp, C.c_int32, vp]
+    lib.uvhb_lin_iterate.restype = C.c_int
+    lib.uvhb_lin_stopped.argtypes = [vp]
+    lib.uvhb_lin_stopped.restype = C.c_int
+    lib.uvhb_lin_end.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.uvhb_lin_end.restype = C.c_int
     _bound = True
     return lib
 
@@ -182,4 +191,32 @@ class GpuBatchLinearSolver:
         self.last_message = "" if st == uvh.UVH_OK else self.lib.uvhb_lin_last_error(self.h).decode()
         if st != uvh.UVH_OK and st != UVH_E_NOCONV:
             raise uvh.UvhLibraryError(f"uvhb_lin_solve: status {st}: {self.last_message}")
+        return self.last_iterations.copy(), self.last_relres.copy(), self.last_converged.copy()
+
+    # the same solve in steps (uvhb_lin_begin / _iterate / _stopped / _end), so that a caller can
+    # capture ``iterate`` in a HIP graph and replay it until ``stopped``
+    def begin(self, rhs, active=None, stream=None, rtol: float | None = None, maxit: int | None = None):
+        vp = C.c_void_p
+        self._rhs = (vp * self.n)(*[uvh._ptr(v) for v in rhs])
+        act = None if active is None else np.ascontiguousarray(np.asarray(active, dtype=np.uint8))
+        st = self.lib.uvhb_lin_begin(self.h, self._rhs, None if act is None else act.ctypes.data,
+                                     self.rtol if rtol is None else rtol, self.maxit if maxit is None else maxit, stream)
+        if st != uvh.UVH_OK:
+            raise uvh.UvhLibraryError(f"uvhb_lin_begin: status {st}: {self.lib.uvhb_lin_last_error(self.h).decode()}")
+
+    def iterate(self, iterations: int, stream=None):
+        st = self.lib.uvhb_lin_iterate(self.h, int(iterations), stream)
+        if st != uvh.UVH_OK:
+            raise uvh.UvhLibraryError(f"uvhb_lin_iterate: status {st}: {self.lib.uvhb_lin_last_error(self.h).decode()}")
+
+    def stopped(self) -> bool:
+        return self.lib.uvhb_lin_stopped(self.h) == 1
+
+    def end(self, x, stream=None):
+        xx = (C.c_void_p * self.n)(*[uvh._ptr(v) for v in x])
+        st = self.lib.uvhb_lin_end(self.h, xx, self.last_iterations.ctypes.data, self.last_relres.ctypes.data,
+                                   self.last_converged.ctypes.data, stream)
+        self.last_message = "" if st == uvh.UVH_OK else self.lib.uvhb_lin_last_error(self.h).decode()
+        if st != uvh.UVH_OK and st != UVH_E_NOCONV:
+            raise uvh.UvhLibraryError(f"uvhb_lin_end: status {st}: {self.last_message}")
         return self.last_iterations.copy(), self.last_relres.copy(), self.last_converged.copy()
