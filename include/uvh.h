@@ -262,7 +262,8 @@ int uvh_lin_solve(uvh_lin *lin, const double *drhs, double *dx, double rtol, int
  * the next iteration already queued, until every member has stopped. For callers that solve
  * many members' Newton steps together (the batched Newton form).
  *
- * uvhb_lin_create: members[0..n-1], all on one device; destroy the batch before any member.
+ * uvhb_lin_create: members[0..n-1], all on one device, each of at most 17,301,504 unknowns
+ *   (UVH_E_ARG otherwise); destroy the batch before any member.
  * uvhb_lin_solve: dx[m] = K_m^-1 drhs[m] (device vectors of 3*Nn_m in [ub; vb; h] order) for
  *   every member with active[m] != 0 (active NULL: all), to ||rhs - K dx|| <= rtol ||rhs||
  *   within maxit iterations; iterations[m], relres[m] (true relative residual) and converged[m]

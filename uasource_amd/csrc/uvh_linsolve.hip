@@ -1486,7 +1486,10 @@ int uvhb_lin_create(uvh_lin *const *members, int32_t n, uvhb_lin **out) {
         T.n3 = L->n3;
         T.nchunk = int((L->n3 + kChunk - 1) / kChunk);
         T.lu32 = L->lu32 ? 1 : 0;
-        if (2 * int64_t(T.nchunk) > int64_t(kMaxKrylov + 2) * kRedBlocks) ok = false;   // partial slots
+        if (2 * int64_t(T.nchunk) > int64_t(kMaxKrylov + 2) * kRedBlocks) {   // the member's partial slots
+            bdestroy(B);
+            return UVH_E_ARG;
+        }
         nrows[m] = L->nn;
         for (int c = 0; c < T.nchunk; ++c) vit.push_back(make_int2(m, c));
         for (int r0 = 0; r0 < L->nn; r0 += kRowsPerItem) rit.push_back(make_int2(m, r0));
