@@ -178,6 +178,54 @@ def test_batched_solve_matches_direct_solves():
             sv.close()
 
 
+def test_batched_solve_limit_order_and_steps():
+    """The batched solve's iteration limit (every member stops at maxit with its best iterate and
+    is reported as not converged), its call order (iterate/end before begin refused), and the
+    solve in steps (begin, iterate replayed until stopped, end) equal to one uvhb_lin_solve
+    bit for bit."""
+    import torch
+    from uasource_amd import uvh as U
+    from uasource_amd.linsolve import GpuBatchLinearSolver
+    names = ["tiny_p2", "medium_p2"]
+    print(f"[gpu] batched linear solve: limit, call order, steps {names}", flush=True)
+    systems = [system(nm) for nm in names]
+    solvers = [GpuLinearSolver(sy[2], sy[1], rtol=1e-10, maxit=20000, method="bicgstab") for sy in systems]
+    batch = None
+    try:
+        for sv, sy in zip(solvers, systems):
+            sv.factor(sy[5])
+        batch = GpuBatchLinearSolver(solvers, rtol=1e-10, maxit=20000)
+        rhs = [sy[6] for sy in systems]
+        with pytest.raises(U.UvhLibraryError):
+            batch.iterate(4)
+        xs = [torch.empty_like(r) for r in rhs]
+        its, rel, conv = batch.solve(rhs, xs, maxit=3)
+        print(f"   maxit 3: iterations {its.tolist()}, relres {rel.tolist()}, {batch.last_message}", flush=True)
+        assert its.tolist() == [3, 3] and conv.tolist() == [0, 0] and batch.last_message != ""
+        assert np.all(rel > 1e-10) and np.all(np.isfinite(rel))
+        ref = [torch.empty_like(r) for r in rhs]
+        its1, rel1, conv1 = batch.solve(rhs, ref)
+        assert conv1.tolist() == [1, 1]
+        ys = [torch.empty_like(r) for r in rhs]
+        batch.begin(rhs)
+        replays = 0
+        while True:
+            batch.iterate(8)
+            torch.cuda.synchronize()
+            replays += 1
+            if batch.stopped():
+                break
+        its2, rel2, conv2 = batch.end(ys)
+        print(f"   steps: {replays} calls of 8 iterations, iterations {its2.tolist()}", flush=True)
+        assert its2.tolist() == its1.tolist() and rel2.tolist() == rel1.tolist() and conv2.tolist() == [1, 1]
+        assert all(torch.equal(a, b) for a, b in zip(ref, ys))
+    finally:
+        if batch is not None:
+            batch.close()
+        for sv in solvers:
+            sv.close()
+
+
 def test_krylov_settings_are_checked():
     print("[gpu] linear solve settings", flush=True)
     M, fixed, ctx, K, rhs, dK, drhs = system("tiny_p2")
