@@ -29,6 +29,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "uvh_internal.h"
 
@@ -1023,21 +1024,24 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     // for 2-4; P3 rows 1.38 ms with 2 against 1.60 ms with 4 and 2.77 ms with 8)
     constexpr int CH = NOD >= 10 ? 2 : 4;
     constexpr int PST = pos_inc_stride(NOD);
-    if constexpr (NBK == 3 && sizeof(TE) == sizeof(double) && (9 * NOD) % 2 == 0) {
-        // FP64 uvh rows of 6- and 10-node elements are 16-byte aligned (54 or 90 values): each
-        // lane loads pairs of values (2 gl + 2 kRowLanes u), and the row's columns with one 8-byte
-        // load per 8 columns, 4 element rows at a time
+    if constexpr (NBK == 3 && (9 * NOD) % 2 == 0) {
+        // uvh rows of 6- and 10-node elements hold an even number of values (54 or 90), so each
+        // row is aligned to a pair (16 bytes in FP64, 8 in FP32): each lane loads pairs of values
+        // (2 gl + 2 kRowLanes u), and the row's columns with one 8-byte load per 8 columns, 4
+        // element rows at a time. Every LDS address still takes its adds in ascending element
+        // order, so the sums are those of the one-value-per-load form bit for bit.
+        using V2 = typename std::conditional<sizeof(TE) == sizeof(double), double2, float2>::type;
         constexpr int VP2 = (NV + 2 * kRowLanes - 1) / (2 * kRowLanes);   // pairs per lane
         constexpr int PW = PST / 8;                                      // column words per row
         for (int c0 = t0; c0 < t1; c0 += CH) {
             const int cnt = min(CH, t1 - c0);
-            double2 vals[CH][VP2];
+            V2 vals[CH][VP2];
             uint64_t cw[CH][PW];
             double re[CH];
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
                 if (c < cnt) {
-                    const double2 *row = reinterpret_cast<const double2 *>(EB + (int64_t)(c0 + c) * eb_stride<NOD, TE>());
+                    const V2 *row = reinterpret_cast<const V2 *>(EB + (int64_t)(c0 + c) * eb_stride<NOD, TE>());
 #pragma unroll
                     for (int u = 0; u < VP2; ++u)
                         if (2 * (gl + kRowLanes * u) < NV) vals[c][u] = row[gl + kRowLanes * u];
@@ -1062,7 +1066,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                             const int bk = v / NOD, j = v - bk * NOD;
                             const int bi = bk / 3, bj = bk - bi * 3;
                             const int col = int((cw[c][j >> 3] >> (8 * (j & 7))) & 0xFF);
-                            lds_add(buf + bi * 3 * deg + bj * deg + col, h ? vals[c][u].y : vals[c][u].x);
+                            lds_add(buf + bi * 3 * deg + bj * deg + col, double(h ? vals[c][u].y : vals[c][u].x));
                         }
                     }
                 }
