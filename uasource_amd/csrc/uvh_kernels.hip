@@ -1024,12 +1024,14 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     // for 2-4; P3 rows 1.38 ms with 2 against 1.60 ms with 4 and 2.77 ms with 8)
     constexpr int CH = NOD >= 10 ? 2 : 4;
     constexpr int PST = pos_inc_stride(NOD);
-    if constexpr (NBK == 3 && (9 * NOD) % 2 == 0) {
+    if constexpr (NBK == 3 && (9 * NOD) % 2 == 0 && (sizeof(TE) == sizeof(double) || NOD <= 6)) {
         // uvh rows of 6- and 10-node elements hold an even number of values (54 or 90), so each
         // row is aligned to a pair (16 bytes in FP64, 8 in FP32): each lane loads pairs of values
         // (2 gl + 2 kRowLanes u), and the row's columns with one 8-byte load per 8 columns, 4
         // element rows at a time. Every LDS address still takes its adds in ascending element
-        // order, so the sums are those of the one-value-per-load form bit for bit.
+        // order, so the sums are those of the one-value-per-load form bit for bit. (FP32 rows of
+        // 10-node elements keep one value per load: 90 values in pairs fill 45 of 64 lane slots,
+        // one by one 90 of 96, measured 1.46 against 1.40 ms at 500k P3 elements.)
         using V2 = typename std::conditional<sizeof(TE) == sizeof(double), double2, float2>::type;
         constexpr int VP2 = (NV + 2 * kRowLanes - 1) / (2 * kRowLanes);   // pairs per lane
         constexpr int PW = PST / 8;                                      // column words per row
