@@ -1024,14 +1024,18 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     // for 2-4; P3 rows 1.38 ms with 2 against 1.60 ms with 4 and 2.77 ms with 8)
     constexpr int CH = NOD >= 10 ? 2 : 4;
     constexpr int PST = pos_inc_stride(NOD);
-    if constexpr (NBK == 3 && (9 * NOD) % 2 == 0 && (sizeof(TE) == sizeof(double) || NOD <= 6)) {
-        // uvh rows of 6- and 10-node elements hold an even number of values (54 or 90), so each
-        // row is aligned to a pair (16 bytes in FP64, 8 in FP32): each lane loads pairs of values
-        // (2 gl + 2 kRowLanes u), and the row's columns with one 8-byte load per 8 columns, 4
-        // element rows at a time. Every LDS address still takes its adds in ascending element
-        // order, so the sums are those of the one-value-per-load form bit for bit. (FP32 rows of
-        // 10-node elements keep one value per load: 90 values in pairs fill 45 of 64 lane slots,
-        // one by one 90 of 96, measured 1.46 against 1.40 ms at 500k P3 elements.)
+    constexpr bool kPairs = NBK == 3 ? (9 * NOD) % 2 == 0 && (sizeof(TE) == sizeof(double) || NOD <= 6)
+                                     : NOD == 6 && sizeof(TE) == sizeof(double);
+    if constexpr (kPairs) {
+        // element rows of 6- and 10-node elements hold an even number of values (54 or 90), so
+        // each row is aligned to a pair (16 bytes in FP64, 8 in FP32), and so is each block row
+        // of the (u, v) blocks the uv system reads (NBK = 2, 6-node elements): each lane loads
+        // pairs of values (2 gl + 2 kRowLanes u of the NV used ones), and the row's columns with
+        // one 8-byte load per 8 columns, 4 element rows at a time. Every LDS address still takes
+        // its adds in ascending element order, so the sums are those of the one-value-per-load
+        // form bit for bit. (FP32 rows of 10-node elements keep one value per load: 90 values in
+        // pairs fill 45 of 64 lane slots, one by one 90 of 96, measured 1.46 against 1.40 ms at
+        // 500k P3 elements; so do the uv system's FP32 rows: 134 against 120 us at 200k P2.)
         using V2 = typename std::conditional<sizeof(TE) == sizeof(double), double2, float2>::type;
         constexpr int VP2 = (NV + 2 * kRowLanes - 1) / (2 * kRowLanes);   // pairs per lane
         constexpr int PW = PST / 8;                                      // column words per row
@@ -1045,8 +1049,16 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                 if (c < cnt) {
                     const V2 *row = reinterpret_cast<const V2 *>(EB + (int64_t)(c0 + c) * eb_stride<NOD, TE>());
 #pragma unroll
-                    for (int u = 0; u < VP2; ++u)
-                        if (2 * (gl + kRowLanes * u) < NV) vals[c][u] = row[gl + kRowLanes * u];
+                    for (int u = 0; u < VP2; ++u) {
+                        const int v = 2 * (gl + kRowLanes * u);
+                        if (v < NV) {
+                            // value v of the used ones: block (bk / NBK, bk mod NBK), column j,
+                            // at (3 bi + bj) NOD + j of the row (v itself when NBK = 3)
+                            const int bk = v / NOD, j = v - bk * NOD;
+                            const int src = ((bk / NBK) * 3 + bk % NBK) * NOD + j;
+                            vals[c][u] = row[src >> 1];
+                        }
+                    }
                     const uint64_t *pw = reinterpret_cast<const uint64_t *>(pos + (int64_t)(c0 + c) * PST);
 #pragma unroll
                     for (int w = 0; w < PW; ++w) cw[c][w] = pw[w];
@@ -1066,9 +1078,9 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                         const int v = 2 * (gl + kRowLanes * u) + h;
                         if (v < NV) {
                             const int bk = v / NOD, j = v - bk * NOD;
-                            const int bi = bk / 3, bj = bk - bi * 3;
+                            const int bi = bk / NBK, bj = bk - bi * NBK;
                             const int col = int((cw[c][j >> 3] >> (8 * (j & 7))) & 0xFF);
-                            lds_add(buf + bi * 3 * deg + bj * deg + col, double(h ? vals[c][u].y : vals[c][u].x));
+                            lds_add(buf + bi * NBK * deg + bj * deg + col, double(h ? vals[c][u].y : vals[c][u].x));
                         }
                     }
                 }
