@@ -780,77 +780,96 @@ __device__ __forceinline__ int row_table_doubles(int tstride) {
     return sizeof(T) == sizeof(double) ? 0 : (3 * tstride + 1) / 2;
 }
 
-// Lane-quad exchange of a double through DPP: quad_perm control CTRL (two bits per lane of the
-// quad naming its source lane). Every lane of the quad must be active.
+// Exchange of a double between lanes through DPP, control CTRL. Every lane of the lane's row
+// of 16 must be active.
 template <int CTRL>
-__device__ __forceinline__ double quad_dpp(double x) {
+__device__ __forceinline__ double lane_dpp(double x) {
     const long long b = __double_as_longlong(x);
     const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
     const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Stores each lane's NV-value row piece v at EB[row * stride + off] (FP64 element rows), the
-// four lanes of a quad together. The pieces are cut into 16-byte units; each group of four
-// units is transposed across the quad (two butterfly stages of DPP swaps), and then one store
-// instruction per source lane t writes units 4m..4m+3 of lane t's row from the quad's four
-// lanes, 64 contiguous bytes, instead of four lanes writing 16 bytes each to four rows. Units
-// past the last whole group are stored by their own lane. A lane with row < 0 has nothing
-// stored. Every lane of the quad must be active, and row * stride + off must be even.
-template <int NV>
-__device__ __forceinline__ void quad_store_rows(double *__restrict__ EB, int stride, int row, int off,
+// The value of lane ^ MASK (MASK 1, 2 or 4): quad_perm swaps for 1 and 2; for 4 the half-row
+// mirror (lane i of 8 takes lane 7 - i) followed by the quad reversal (i takes i ^ 3).
+template <int MASK>
+__device__ __forceinline__ double xor_dpp(double x) {
+    if constexpr (MASK == 1) return lane_dpp<0xB1>(x);
+    else if constexpr (MASK == 2) return lane_dpp<0x4E>(x);
+    else return lane_dpp<0x1B>(lane_dpp<0x141>(x));
+}
+
+// One butterfly stage of the transpose below: for each slot pair (a, a + MASK), the lane with
+// bit MASK set keeps the upper slot and takes its partner's lower slot into its lower one, the
+// other lane keeps the lower slot and takes its partner's upper slot into its upper one.
+template <int G, int MASK>
+__device__ __forceinline__ void row_butterfly(double (&s)[G][2], int q) {
+    const bool h = (q & MASK) != 0;
+#pragma unroll
+    for (int a = 0; a < G; ++a) {
+        if (a & MASK) continue;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const double r = xor_dpp<MASK>(h ? s[a][w] : s[a + MASK][w]);
+            s[a][w] = h ? r : s[a][w];
+            s[a + MASK][w] = h ? s[a + MASK][w] : r;
+        }
+    }
+}
+
+// Stores each lane's NV-value row piece v at EB[row * stride + off] (FP64 element rows), the G
+// lanes (4 or 8) of an aligned lane group together. The pieces are cut into 16-byte units; each
+// run of G units is transposed across the group (log2 G butterfly stages of DPP exchanges), and
+// then one store instruction per source lane t writes units G m .. G m + G - 1 of lane t's row
+// from the group's G lanes, 16 G contiguous bytes, instead of G lanes writing 16 bytes each to
+// G rows. Units past the last whole run are stored by their own lane. A lane with row < 0 has
+// nothing stored. Every lane of the group must be active, and row * stride + off must be even.
+template <int G, int NV>
+__device__ __forceinline__ void group_store_rows(double *__restrict__ EB, int stride, int row, int off,
                                                 const double *v) {
-    static_assert(NV % 2 == 0, "16-byte units");
-    constexpr int NU = NV / 2, NC = NU / 4;
-    const int q = __lane_id() & 3;
-    int rows[4];
-    rows[0] = __builtin_amdgcn_mov_dpp(row, 0x00, 0xF, 0xF, false);
-    rows[1] = __builtin_amdgcn_mov_dpp(row, 0x55, 0xF, 0xF, false);
-    rows[2] = __builtin_amdgcn_mov_dpp(row, 0xAA, 0xF, 0xF, false);
-    rows[3] = __builtin_amdgcn_mov_dpp(row, 0xFF, 0xF, 0xF, false);
-    const bool h1 = (q & 1) != 0, h2 = (q & 2) != 0;
+    static_assert(NV % 2 == 0 && (G == 4 || G == 8), "16-byte units, groups of 4 or 8 lanes");
+    constexpr int NU = NV / 2, NC = NU / G;
+    const int lane = __lane_id(), q = lane & (G - 1);
+    int rows[G];
+    if constexpr (G == 4) {
+        rows[0] = __builtin_amdgcn_mov_dpp(row, 0x00, 0xF, 0xF, false);
+        rows[1] = __builtin_amdgcn_mov_dpp(row, 0x55, 0xF, 0xF, false);
+        rows[2] = __builtin_amdgcn_mov_dpp(row, 0xAA, 0xF, 0xF, false);
+        rows[3] = __builtin_amdgcn_mov_dpp(row, 0xFF, 0xF, 0xF, false);
+    } else {
+#pragma unroll
+        for (int t = 0; t < G; ++t) rows[t] = __shfl(row, (lane & ~(G - 1)) + t, 64);
+    }
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
-        double s[4][2];
+        double s[G][2];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            s[c][0] = v[8 * m + 2 * c];
-            s[c][1] = v[8 * m + 2 * c + 1];
+        for (int c = 0; c < G; ++c) {
+            s[c][0] = v[2 * G * m + 2 * c];
+            s[c][1] = v[2 * G * m + 2 * c + 1];
         }
-        // stage 1, partner lane ^ 1: unit pairs (0, 1) and (2, 3); stage 2, partner lane ^ 2:
-        // pairs (0, 2) and (1, 3). The lane with the partner bit set keeps the pair's upper
-        // unit and takes the partner's lower unit into the lower slot, the other the reverse.
+        row_butterfly<G, 1>(s, q);
+        row_butterfly<G, 2>(s, q);
+        if constexpr (G == 8) row_butterfly<G, 4>(s, q);
+        // slot t now holds unit G m + q of lane t's piece
 #pragma unroll
-        for (int a = 0; a < 4; a += 2)
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const double r = quad_dpp<0xB1>(h1 ? s[a][w] : s[a + 1][w]);
-                s[a][w] = h1 ? r : s[a][w];
-                s[a + 1][w] = h1 ? s[a + 1][w] : r;
-            }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const double r = quad_dpp<0x4E>(h2 ? s[a][w] : s[a + 2][w]);
-                s[a][w] = h2 ? r : s[a][w];
-                s[a + 2][w] = h2 ? s[a + 2][w] : r;
-            }
-        // slot t now holds unit 4m + q of lane t's piece
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < G; ++t)
             if (rows[t] >= 0) {
-                *reinterpret_cast<double2 *>(EB + (int64_t)rows[t] * stride + off + 8 * m + 2 * q) =
+                *reinterpret_cast<double2 *>(EB + (int64_t)rows[t] * stride + off + 2 * G * m + 2 * q) =
                     make_double2(s[t][0], s[t][1]);
             }
     }
     if (row >= 0) {
 #pragma unroll
-        for (int k = 8 * NC; k < NV; k += 2) {
+        for (int k = 2 * G * NC; k < NV; k += 2) {
             *reinterpret_cast<double2 *>(EB + (int64_t)row * stride + off + k) = make_double2(v[k], v[k + 1]);
         }
     }
 }
+
+// lanes per group of the element kernel's row stores (group_store_rows); groups of 8 (128-byte
+// runs, a third exchange stage) measured slower at 7 points per element
+constexpr int kRowGroup = 4;
 
 // Per-element kernel: EPB elements per 256-thread block (runtime), dense thread mappings
 // for the three stages: (element, node) gathers, (element, point) physics, (element, row)
@@ -966,11 +985,11 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
     const int wseg = ((nrow + 63) / 64) * 64;
     const int seg = (nkind * wseg + nthr - 1) / nthr <= (nkind * nrow + nthr - 1) / nthr ? wseg : nrow;
     if constexpr (kUV2 && NOD == 6 && !RONLY) {
-        // 6-node rows stored a lane quad at a time (quad_store_rows): the four lanes of a quad
-        // share a kind of item (seg is a multiple of 4) and a trip of the loop (each wave runs
-        // the same trips); a lane without an item computes the tile's first row and stores
-        // nothing
-        if ((seg & 3) == 0 && (nthr & 63) == 0) {
+        // 6-node rows stored a lane group at a time (group_store_rows): the lanes of a group
+        // share a kind of item (seg is a multiple of the group) and a trip of the loop (each
+        // wave runs the same trips); a lane without an item computes the tile's first row and
+        // stores nothing
+        if (seg % kRowGroup == 0 && (nthr & 63) == 0) {
             for (int vb = tid & ~63; vb < nkind * seg; vb += nthr) {
                 const int v = vb + (tid & 63);
                 const int kind = v / seg, rr = v - kind * seg;
@@ -986,7 +1005,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
 #pragma unroll
                         for (int k = 0; k < 4; ++k) RE[(int64_t)row * 6 + k] = TF[k];
                     }
-                    quad_store_rows<6 * NOD>(EB, eb_stride<NOD, T>(), row, 0, &Kb[0][0]);
+                    group_store_rows<kRowGroup, 6 * NOD>(EB, eb_stride<NOD, T>(), row, 0, &Kb[0][0]);
                 } else {
                     T Tr, Fr, Kb[3][NOD];
                     element_row_b<NOD, 2, T>(i, nip, crr, rN, rXi, rEta, rtab, Tr, Fr, Kb);
@@ -994,7 +1013,7 @@ __device__ __forceinline__ void element_block(double *lds, int nele, int nip, in
                         RE[(int64_t)row * 6 + 4] = Tr;
                         RE[(int64_t)row * 6 + 5] = Fr;
                     }
-                    quad_store_rows<3 * NOD>(EB, eb_stride<NOD, T>(), row, 6 * NOD, &Kb[0][0]);
+                    group_store_rows<kRowGroup, 3 * NOD>(EB, eb_stride<NOD, T>(), row, 6 * NOD, &Kb[0][0]);
                 }
             }
             return;
