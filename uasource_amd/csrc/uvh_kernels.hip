@@ -1262,11 +1262,48 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
             if (check_nan && isnan(r)) bad |= 1;
         }
     }
+    // K. The wave's nodes are consecutive, so their parts of one block row of K are one
+    // contiguous range; when the wave holds all its kRowNodesPerWave nodes (uniform over the
+    // wave, every lane group is here) its 64 lanes store that range together, 64 consecutive
+    // values per instruction, else each lane group stores its node's part. In-order LDS
+    // operations within the wave make the other groups' sums visible to these reads.
+    const int kw = k0 + wave * kRowNodesPerWave;
+    if (kw + kRowNodesPerWave <= nn) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int64_t rpw = __shfl(rp, 0, 64);
+        int start[kRowNodesPerWave + 1], dgs[kRowNodesPerWave];
 #pragma unroll
-    for (int bi = 0; bi < NBK; ++bi) {
-        double *dst = Kval + (int64_t)bi * NBK * nnzN + NBK * rp;
-        const double *src = buf + bi * NBK * deg;
-        for (int v = gl; v < NBK * deg; v += kRowLanes) dst[v] = src[v];
+        for (int s = 0; s < kRowNodesPerWave; ++s) {
+            start[s] = NBK * (int)(__shfl(rp, s * kRowLanes, 64) - rpw);
+            dgs[s] = __shfl(deg, s * kRowLanes, 64);
+        }
+        start[kRowNodesPerWave] = start[kRowNodesPerWave - 1] + NBK * dgs[kRowNodesPerWave - 1];
+        const double *wbuf = lds + NBK * NBK * (rpw - rowptr_n[k0]);   // the wave's first node's buffer
+#pragma unroll
+        for (int bi = 0; bi < NBK; ++bi) {
+            double *dst = Kval + (int64_t)bi * NBK * nnzN + NBK * rpw;
+            for (int d = lane; d < start[kRowNodesPerWave]; d += 64) {
+                int s = 0;
+#pragma unroll
+                for (int t = 1; t < kRowNodesPerWave; ++t) s += d >= start[t];
+                int st = start[0], dgv = dgs[0];
+#pragma unroll
+                for (int t = 1; t < kRowNodesPerWave; ++t)
+                    if (s == t) {
+                        st = start[t];
+                        dgv = dgs[t];
+                    }
+                // node s's buffer starts NBK * start[s] values after the wave's first
+                dst[d] = wbuf[NBK * st + bi * NBK * dgv + (d - st)];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int bi = 0; bi < NBK; ++bi) {
+            double *dst = Kval + (int64_t)bi * NBK * nnzN + NBK * rp;
+            const double *src = buf + bi * NBK * deg;
+            for (int v = gl; v < NBK * deg; v += kRowLanes) dst[v] = src[v];
+        }
     }
     // a node no element references has an empty row (R = 0, no diagonal to check)
     if (check_nan && deg > 0 && gl < NBK && isnan(buf[gl * NBK * deg + gl * deg + dg])) bad |= 2;
