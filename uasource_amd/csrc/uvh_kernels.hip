@@ -1320,10 +1320,13 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
                                               double *__restrict__ Kval, int check_nan, int *__restrict__ status,
                                               const int32_t *__restrict__ blocks, int nblk) {
     extern __shared__ double lds[];
-    // node blocks of this launch: blocks[0 .. nblk-1], or 0 .. nblk-1 when blocks is null
+    // node blocks of this launch: blocks[0 .. nblk-1], or 0 .. nblk-1 when blocks is null, the
+    // latter taken in descending order: the element kernel writes the element rows with its
+    // tiles in ascending node order, so the rows it wrote last, the likeliest to be still in
+    // the memory-side Infinity Cache, are read first (config 2: 210 against 215 us)
     const int i = xcd_block(blockIdx.x, gridDim.x);
     if (i >= nblk) return;
-    const int k0 = (blocks ? blocks[i] : i) * row_nodes_per_block(NOD);
+    const int k0 = (blocks ? blocks[i] : nblk - 1 - i) * row_nodes_per_block(NOD);
     rows_block<NOD, TE, NBK>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval,
                              check_nan, status);
 }
