@@ -1267,6 +1267,9 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     // wave, every lane group is here) its 64 lanes store that range together, 64 consecutive
     // values per instruction, else each lane group stores its node's part. In-order LDS
     // operations within the wave make the other groups' sums visible to these reads.
+    // The stores are non-temporal: K is not read again by this assembly, and kept out of the
+    // caches it leaves the element rows not yet read in the Infinity Cache (config 2: rows
+    // 200 against 208 us).
     const int kw = k0 + wave * kRowNodesPerWave;
     if (kw + kRowNodesPerWave <= nn) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1294,7 +1297,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                         dgv = dgs[t];
                     }
                 // node s's buffer starts NBK * start[s] values after the wave's first
-                dst[d] = wbuf[NBK * st + bi * NBK * dgv + (d - st)];
+                __builtin_nontemporal_store(wbuf[NBK * st + bi * NBK * dgv + (d - st)], &dst[d]);
             }
         }
     } else {
@@ -1302,7 +1305,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
         for (int bi = 0; bi < NBK; ++bi) {
             double *dst = Kval + (int64_t)bi * NBK * nnzN + NBK * rp;
             const double *src = buf + bi * NBK * deg;
-            for (int v = gl; v < NBK * deg; v += kRowLanes) dst[v] = src[v];
+            for (int v = gl; v < NBK * deg; v += kRowLanes) __builtin_nontemporal_store(src[v], &dst[v]);
         }
     }
     // a node no element references has an empty row (R = 0, no diagonal to check)
