@@ -1111,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_element(int nele, int nip, int epb, con
 
 // NBK = 3: the uvh system (u, v, h rows and columns); NBK = 2: the uv system, which reads
 // the (u, v) part of the same element rows and writes R (2 Nn) and the 2 x 2-block K.
-template <int NOD, typename TE, int NBK = 3>
+template <int NOD, typename TE, int NBK = 3, bool KNT = false>
 __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const int32_t *__restrict__ inc_ptr,
                                            const uint8_t *__restrict__ pos, const int64_t *__restrict__ rowptr_n,
                                            const uint8_t *__restrict__ diag, int64_t nnzN,
@@ -1267,9 +1267,10 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     // wave, every lane group is here) its 64 lanes store that range together, 64 consecutive
     // values per instruction, else each lane group stores its node's part. In-order LDS
     // operations within the wave make the other groups' sums visible to these reads.
-    // The stores are non-temporal: K is not read again by this assembly, and kept out of the
-    // caches it leaves the element rows not yet read in the Infinity Cache (config 2: rows
-    // 200 against 208 us).
+    // With KNT the stores are non-temporal: K is not read again by the assembly, and kept out
+    // of the caches it leaves the element rows not yet read in the Infinity Cache (config 2:
+    // rows 200 against 208 us; the batched row kernels, whose element rows far exceed that
+    // cache, store K plainly: 64 meshes in FP64 2.75-2.78 against 2.80 ms).
     const int kw = k0 + wave * kRowNodesPerWave;
     if (kw + kRowNodesPerWave <= nn) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1297,7 +1298,9 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
                         dgv = dgs[t];
                     }
                 // node s's buffer starts NBK * start[s] values after the wave's first
-                __builtin_nontemporal_store(wbuf[NBK * st + bi * NBK * dgv + (d - st)], &dst[d]);
+                const double x = wbuf[NBK * st + bi * NBK * dgv + (d - st)];
+                if constexpr (KNT) __builtin_nontemporal_store(x, &dst[d]);
+                else dst[d] = x;
             }
         }
     } else {
@@ -1305,7 +1308,10 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
         for (int bi = 0; bi < NBK; ++bi) {
             double *dst = Kval + (int64_t)bi * NBK * nnzN + NBK * rp;
             const double *src = buf + bi * NBK * deg;
-            for (int v = gl; v < NBK * deg; v += kRowLanes) __builtin_nontemporal_store(src[v], &dst[v]);
+            for (int v = gl; v < NBK * deg; v += kRowLanes) {
+                if constexpr (KNT) __builtin_nontemporal_store(src[v], &dst[v]);
+                else dst[v] = src[v];
+            }
         }
     }
     // a node no element references has an empty row (R = 0, no diagonal to check)
@@ -1330,7 +1336,7 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
     const int i = xcd_block(blockIdx.x, gridDim.x);
     if (i >= nblk) return;
     const int k0 = (blocks ? blocks[i] : nblk - 1 - i) * row_nodes_per_block(NOD);
-    rows_block<NOD, TE, NBK>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval,
+    rows_block<NOD, TE, NBK, true>(lds, nn, k0, inc_ptr, pos, rowptr_n, diag, nnzN, EB, RE, R, Tint, Fext, Kval,
                              check_nan, status);
 }
 
