@@ -36,6 +36,7 @@ hipError_t launch_batch(const BatchMember *mem, int nmem, const int2 *nodal_item
                         const size_t *elem_lds, const size_t *row_lds, bool fp32, bool ronly, int check_nan,
                         int *ctr, int *status, const Params &p, bool flat, hipStream_t s);
 int element_epb_default(int nod, int nip, bool fp32, int nthr);
+int element_epb_uvh(int nod, int nip, bool fp32, int nthr);
 size_t element_row_stride(int nod, bool fp32);
 hipError_t launch_fused(int nod, int nip, int npatch, const FusedLayout &L, const MeshArgs *MA, const CallArgs *CA,
                         int64_t nnzN, int nn, bool sl, bool ronly, int check_nan, hipStream_t s);
@@ -121,7 +122,8 @@ struct uvh_ctx {
     // work
     double *d_b = nullptr, *d_RE = nullptr, *d_EB = nullptr;
     double *d_zero = nullptr;   // uv assembly: zero nodal field (lazily allocated)
-    int epb[2] = {0, 0};    // elements per block of the element kernel: [FP64, FP32]
+    int epb[2] = {0, 0};    // elements per block of the element kernel, uvh R+K: [FP64, FP32]
+    int epb_wide[2] = {0, 0};   // the same for the uv assembly and uvh R-only calls
     // threads per element block (UVH_ELEMENT_THREADS=64/128/192: one- to three-wave tiles, each
     // with as many elements as its stages fit; measured slower than 256 on 6- and 10-node meshes)
     int el_threads[2] = {256, 256};
@@ -585,12 +587,13 @@ int uvh_set_mesh(uvh_ctx *ctx, int32_t nele, int32_t nod, int32_t nnodes, const 
         return ctx->hip_fail(e, "uvh_set_mesh: upload");
     }
     for (int t = 0; t < 2; ++t) {
-        ctx->epb[t] = uvh::element_epb_default(nod, nip, t == 1, ctx->el_threads[t]);
+        ctx->epb[t] = uvh::element_epb_uvh(nod, nip, t == 1, ctx->el_threads[t]);
+        ctx->epb_wide[t] = uvh::element_epb_default(nod, nip, t == 1, ctx->el_threads[t]);
         const int o = ctx->epb_override;
         if (o > 0 && o * nip <= ctx->el_threads[t] && o * nod <= ctx->el_threads[t] &&
             uvh::element_lds_bytes(nod, nip, o, t == 1) <= 156 * 1024)
-            ctx->epb[t] = o;
-        ctx->bepb[t] = uvh::element_epb_default(nod, nip, t == 1, 256);
+            ctx->epb[t] = ctx->epb_wide[t] = o;
+        ctx->bepb[t] = uvh::element_epb_default(nod, nip, t == 1, 256);   // batches: autotuned (uvhb_set_tile)
     }
     ctx->fused_ok = fused;
     ctx->has_mesh = true;
@@ -799,8 +802,8 @@ int assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *d
     const char *row_name = fp32 ? (ronly ? "rows32_r" : "rows32_rk") : (ronly ? "rows_r" : "rows_rk");
     auto elements = [&](int lb0, int lb1, hipStream_t st) {
         return ctx->timed(el_name, st, [&] {
-            return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32, full, ctx->d_conn,
-                                       ctx->d_x, ctx->d_y,
+            return uvh::launch_element(m.nod, m.nele, m.nip, ronly ? ctx->epb_wide[t] : ctx->epb[t],
+                                       ctx->el_threads[t], fp32, full, ctx->d_conn, ctx->d_x, ctx->d_y,
                                        f, nullptr, ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, ctx->d_EB,
                                        ctx->d_RE, lb0, lb1, ctx->d_status, st);
         });
@@ -881,7 +884,8 @@ int assemble_noalloc_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_fields *
     const int t = fp32 ? 1 : 0;
     ctx->last_status = status;
     (void)b;   // b is computed in the element kernel's gathers
-    hipError_t e = uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32, full, ctx->d_conn,
+    hipError_t e = uvh::launch_element(m.nod, m.nele, m.nip, ronly ? ctx->epb_wide[t] : ctx->epb[t],
+                                       ctx->el_threads[t], fp32, full, ctx->d_conn,
                                        ctx->d_x, ctx->d_y, f, nullptr,
                                        ctx->d_tab, ctx->d_tab32, p, ctx->d_inc_pos, ronly, EB, RE, 0, -1, status, s);
     if (e == hipSuccess)
@@ -1000,7 +1004,7 @@ int uv_assemble_device_impl(uvh_ctx *ctx, const uvh_ctrl *ctrl, const uvh_uv_ctr
     if (e == hipSuccess)
         e = ctx->timed(fp32 ? (ronly ? "uv_element32_r" : "uv_element32_rk") : (ronly ? "uv_element_r" : "uv_element_rk"),
                        s, [&] {
-                           return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb[t], ctx->el_threads[t], fp32,
+                           return uvh::launch_element(m.nod, m.nele, m.nip, ctx->epb_wide[t], ctx->el_threads[t], fp32,
                                                       full, ctx->d_conn,
                                                       ctx->d_x, ctx->d_y, f, ctx->d_b, ctx->d_tab, ctx->d_tab32, p,
                                                       ctx->d_inc_pos, ronly, ctx->d_EB, ctx->d_RE, 0, -1, nullptr, s);

@@ -1950,6 +1950,18 @@ int element_epb_default(int nod, int nip, bool fp32, int nthr) {
     return element_epb_max(nod, nip, fp32, (!fp32 && nod <= 6) ? kElementLdsSmall : kElementLds, nthr);
 }
 
+// Tiles of the uvh assembly: FP64 3- and 6-node tiles are further capped so that the row stage
+// (two items per element row: the x and y block rows together, then the h block row) takes a
+// single round of the block's threads, 21 6-node elements per 256 threads instead of 27
+// (config 2: element kernel 253-257 against 265-268 us, the assembly 482-489 against 498-503;
+// 16-20 and 24-30 elements measured slower). The uv assembly, one item per element row, and the
+// R-only calls, whose rows are cheap, keep the larger tiles (21 elements: uv 222 against 197
+// us, R-only 158 against 143 us).
+int element_epb_uvh(int nod, int nip, bool fp32, int nthr) {
+    const int epb = element_epb_default(nod, nip, fp32, nthr);
+    return (!fp32 && nod <= 6) ? std::min(epb, nthr / (2 * nod)) : epb;
+}
+
 hipError_t launch_nodal_b(int nn, const FieldPtrs &f, const Params &p, double *b, int *status, hipStream_t s) {
     hipLaunchKernelGGL(k_nodal_b, dim3((nn + 255) / 256), dim3(256), 0, s, nn, f, p, b, status);
     return hipGetLastError();
