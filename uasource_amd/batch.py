@@ -170,10 +170,16 @@ def tile_candidates(nod: int, nip: int, fp32: bool) -> list[int]:
         e = min(cap, (lds - tables) // per_el)
         if e >= 1:
             out.add(int(e))
-    for rounds in (1, 2, 3, 4):
-        e = (256 * rounds) // (3 * nod)
-        if 1 <= e <= cap and tables + e * per_el <= 156 * 1024:
-            out.add(int(e))
+    # row-stage items per element: one per block row (3 nod); FP64 3- and 6-node elements take
+    # the x and y block rows in one item (2 nod), FP32 ones all nine blocks in one (nod)
+    items = {3 * nod}
+    if nod <= 6:
+        items.add(nod if fp32 else 2 * nod)
+    for per in sorted(items):
+        for rounds in (1, 2, 3, 4):
+            e = (256 * rounds) // per
+            if 1 <= e <= cap and tables + e * per_el <= 156 * 1024:
+                out.add(int(e))
     return sorted(out)
 
 
