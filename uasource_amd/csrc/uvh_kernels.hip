@@ -1268,7 +1268,7 @@ __device__ __forceinline__ void rows_block(double *lds, int nn, int k0, const in
     // values per instruction, else each lane group stores its node's part. In-order LDS
     // operations within the wave make the other groups' sums visible to these reads.
     // With KNT the stores are non-temporal: K is not read again by the assembly, and kept out
-    // of the caches it leaves the element rows not yet read in the Infinity Cache (config 2:
+    // of the caches it leaves the element rows not yet read in the Infinity Cache (200k P2 elements:
     // rows 200 against 208 us; the batched row kernels, whose element rows far exceed that
     // cache, store K plainly: 64 meshes in FP64 2.75-2.78 against 2.80 ms).
     const int kw = k0 + wave * kRowNodesPerWave;
@@ -1332,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_rows(int nn, const int32_t *__restrict_
     // node blocks of this launch: blocks[0 .. nblk-1], or 0 .. nblk-1 when blocks is null, the
     // latter taken in descending order: the element kernel writes the element rows with its
     // tiles in ascending node order, so the rows it wrote last, the likeliest to be still in
-    // the memory-side Infinity Cache, are read first (config 2: 210 against 215 us)
+    // the memory-side Infinity Cache, are read first (200k P2 elements: 210 against 215 us)
     const int i = xcd_block(blockIdx.x, gridDim.x);
     if (i >= nblk) return;
     const int k0 = (blocks ? blocks[i] : nblk - 1 - i) * row_nodes_per_block(NOD);
@@ -1953,7 +1953,7 @@ int element_epb_default(int nod, int nip, bool fp32, int nthr) {
 // Tiles of the uvh assembly: FP64 3- and 6-node tiles are further capped so that the row stage
 // (two items per element row: the x and y block rows together, then the h block row) takes a
 // single round of the block's threads, 21 6-node elements per 256 threads instead of 27
-// (config 2: element kernel 253-257 against 265-268 us, the assembly 482-489 against 498-503;
+// (200k P2 elements: element kernel 253-257 against 265-268 us, the assembly 482-489 against 498-503;
 // 16-20 and 24-30 elements measured slower). The uv assembly, one item per element row, and the
 // R-only calls, whose rows are cheap, keep the larger tiles (21 elements: uv 222 against 197
 // us, R-only 158 against 143 us).
